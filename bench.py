@@ -1,0 +1,189 @@
+"""Benchmark of the racer hot path on one MI355X.
+
+Workload (one step = Runner.step of main.mojo:183-191 for every car: policy
+forward, Gaussian sample, RK4 physics with crash/respawn/progress, lidar):
+ramp ribbon of radius 10 m with road height 0.3 sin(3 theta), 0.025 m grids,
+4096 cars x 1081 beams over 270 degrees, random-init 2 x 256 tanh policy,
+spawn heading noise U(-0.1, 0.1). Inputs are resident in HBM before timing.
+
+Prints one JSON line with env-steps/s, the roofline of the dominant kernel
+(measured with HIP events on the library's stream during the timed steps) and
+the CPU baseline (the oracle's float32 restatement, timed on a bounded sample
+of the same workload on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "env-steps/s at 4096 cars x 1081 beams; PPO iter seconds; % HBM/MFMA peak"
+HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def lidar_bytes(iterations: float, cars: int, beams: int, layers: int) -> float:
+    """Algorithmic bytes of one lidar launch: one packed (clearance, height)
+    record of 8 B per layer per march iteration, the 4 B range store per beam,
+    and 6 floats of car state read per car."""
+    return iterations * 8.0 * layers + 4.0 * cars * beams + 24.0 * cars
+
+
+def policy_flops(cars: int, beams: int, hidden: int) -> float:
+    IN, HA = beams + 5, hidden + 1
+    return 2.0 * cars * (hidden * IN + hidden * HA + 3 * HA)
+
+
+def cpu_baseline(track, beams: int, hidden: int, threads: int, seconds: float) -> dict:
+    """The oracle's Runner step (policy forward, sample, physics, lidar) on the
+    host, bounded to about `seconds` of work."""
+    import numpy as np
+
+    import oracle as O
+
+    O.set_threads(threads)
+    cars = 256
+    sim = O.Sim(track, cars, 42, beams=beams, heading_noise=0.1)
+    sim.reset()
+    pol = O.Policy(42, beams, hidden)
+    logp, value = np.zeros(cars, np.float32), np.zeros(cars, np.float32)
+
+    def step(t):
+        _, _, out = pol.forward(sim.obs)
+        sim.sample(out, pol.log_std(), sim.actions, logp, value, False, t)
+        sim.step()
+
+    step(0)
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < seconds:
+        step(steps + 1)
+        steps += 1
+    dt = time.perf_counter() - t0
+    return {"value": cars * steps / dt, "unit": "env-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{cars} cars x {beams} beams, 2x{hidden} float32 policy, ramp10, {steps} Runner steps "
+                      f"({dt:.1f} s) on {threads} host threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cars", type=int, default=4096)
+    ap.add_argument("--beams", type=int, default=1081)
+    ap.add_argument("--hidden", type=int, default=256)
+    ap.add_argument("--track", default="ramp10")
+    ap.add_argument("--precision", choices=["bf16", "fp32"], default="bf16")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        return 2
+
+    import numpy as np
+
+    import warporacer3d_amd as R
+    from warporacer3d_amd import track as T
+
+    arr = T.cached(args.track)
+    layers = int(arr[7])
+    cfg = R.Config(beams=args.beams, hidden=args.hidden, precision=1 if args.precision == "bf16" else 0,
+                   heading_noise=0.1)
+    dev = R.Device(True)
+    log("device:", dev.name)
+    run = R.Runner(arr, args.cars, cfg, seed=42, device=dev)
+    for t in range(args.warmup):
+        run.step(False, t)
+    dev.synchronize()
+    # march iterations of the lidar, counted by the instrumented variant on the warm state (untimed)
+    run.sim.count_iterations(True)
+    counts = []
+    for t in range(3):
+        run.step(False, args.warmup + t)
+        counts.append(run.sim.last_iterations)
+    run.sim.count_iterations(False)
+    dev.synchronize()
+    log("lidar iterations per step:", counts)
+    dev.profile_reset()
+    dev.profile(True)
+    t0 = time.perf_counter()
+    for t in range(args.steps):
+        run.step(False, args.warmup + 3 + t)
+    dev.synchronize()
+    dt = time.perf_counter() - t0
+    dev.profile(False)
+    prof = dev.profile_results()
+    # untimed: iteration count on the final state, to bracket the timed region
+    run.sim.count_iterations(True)
+    run.step(False, args.warmup + 3 + args.steps)
+    counts.append(run.sim.last_iterations)
+    run.sim.count_iterations(False)
+    ms = dt * 1000.0 / args.steps
+    value = args.cars * 1000.0 / ms
+    log(f"{ms:.4f} ms/step, {value:.4g} env-steps/s")
+    for k, (n, tot) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+        log(f"  {k:20s} launches {n:6d} avg {tot / max(n, 1) * 1000:9.2f} us  total {tot:9.2f} ms")
+    assert np.isfinite(run.sim.obs.read()).all(), "non-finite observations"
+
+    lidar_n, lidar_ms = prof.get("lidar", (0, 0.0))
+    lidar_us = lidar_ms / max(lidar_n, 1) * 1000.0
+    iters = float(np.mean(counts))
+    lbytes = lidar_bytes(iters, args.cars, args.beams, layers)
+    gemm_ms = sum(tot for k, (n, tot) in prof.items() if k.startswith("gemm"))
+    flops = policy_flops(args.cars, args.beams, args.hidden)
+    traffic = None
+    tfile = ROOT / "profiles" / "lidar_traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    roofline = {
+        "bound": "hbm", "kernel": "lidar", "achieved": lbytes / (lidar_us * 1e-6) / 1e9, "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": lbytes / (lidar_us * 1e-6) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
+        "algorithmic_bytes_per_launch": lbytes, "march_iterations_per_launch": iters,
+        "avg_launch_us": lidar_us,
+    }
+    peak = MFMA_BF16_PEAK_TFLOPS if args.precision == "bf16" else FP32_PEAK_TFLOPS
+    gemm_tflops = flops * args.steps / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
+    policy = {"bound": "mfma" if args.precision == "bf16" else "fp32", "achieved": gemm_tflops, "peak": peak,
+              "unit": "TFLOP/s", "frac": gemm_tflops / peak if gemm_tflops else None,
+              "flops_per_step": flops}
+    kernels = {k: {"launches": n, "avg_us": tot / max(n, 1) * 1000.0} for k, (n, tot) in prof.items()}
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads ...")
+        cpu = cpu_baseline(arr, args.beams, args.hidden, threads, args.cpu_seconds)
+        log("cpu baseline:", cpu)
+    out = {
+        "metric": METRIC, "value": value, "unit": "env-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16+fp32" if args.precision == "bf16" else "fp32",
+        "data": "synthetic (procedural ramp ribbon, random-init policy)",
+        "config": {"workload": f"Runner step: {args.track} ribbon R=10 m, height 0.3 sin 3t, 0.025 m grid; "
+                               f"{args.cars} cars x {args.beams} beams; 2x{args.hidden} tanh policy "
+                               f"({args.precision} GEMMs, fp32 dynamics and lidar)",
+                   "cars": args.cars, "beams": args.beams, "hidden": args.hidden, "track": args.track,
+                   "policy_precision": args.precision, "parallelism": "single GPU"},
+        "roofline": roofline, "policy_roofline": policy, "kernels": kernels, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

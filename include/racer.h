@@ -1,0 +1,189 @@
+/*
+ * libracer_hip: the C ABI of the MI355X (gfx950) racer hot path.
+ *
+ * It replaces the reference's Mojo `Device`, `Sim`, `Policy`, `Activations`
+ * and `Trainer` (racer/device.mojo, racer/sim.mojo, racer/ppo.mojo) with
+ * opaque handles over device memory. Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Every buffer is float32 device memory, feature-major [rows, cols] as in
+ *     the reference (racer/sim.mojo:5-7).
+ *   - A racer_view is the reference's `mat[rows](buffer, cols, offset)`
+ *     (racer/device.mojo:29-32): `rows` x `cols` floats, row-major, starting
+ *     `offset` floats into `buf`.
+ *   - Functions return 0 on success and nonzero on failure; the message is in
+ *     racer_last_error() (per thread). The reference raises Error(msg).
+ *   - Calls are asynchronous on the device's one in-order stream, except those
+ *     documented as synchronous (alloc, read, write, upload, sync, the
+ *     scalar-returning rollout/update), as in the reference.
+ *   - A handle is not re-entrant: drive each from one host thread.
+ */
+#ifndef RACER_H
+#define RACER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct racer_device racer_device;
+typedef struct racer_buf racer_buf;
+typedef struct racer_sim racer_sim;
+typedef struct racer_policy racer_policy;
+typedef struct racer_acts racer_acts;
+typedef struct racer_trainer racer_trainer;
+
+typedef struct {
+    racer_buf* buf;
+    size_t offset; /* floats into buf */
+    int64_t rows;
+    int64_t cols;
+} racer_view;
+
+/* Shape parameters that are compile-time constants in the reference
+ * (racer/sim.mojo:28 BEAMS; racer/ppo.mojo:17-26 H, ROLLOUT, EPOCHS, MINIBATCHES). */
+typedef struct {
+    int beams;          /* lidar beams (reference default 108) */
+    int hidden;         /* H, hidden width of both tanh layers (64) */
+    int rollout;        /* ROLLOUT steps per iteration (32) */
+    int epochs;         /* EPOCHS (4) */
+    int minibatches;    /* MINIBATCHES (4) */
+    int precision;      /* policy GEMMs: 0 = float32, 1 = bf16 MFMA with float32 accumulation */
+    float heading_noise;/* spawn heading noise U(-a, a) in radians; 0 = reference behaviour */
+} racer_config;
+
+/* The reference defaults: 108 beams, H 64, ROLLOUT 32, 4 epochs, 4 minibatches, float32. */
+void racer_config_default(racer_config* cfg);
+
+/* Version string of the library. */
+const char* racer_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* racer_last_error(void);
+
+/* ---------------- Device: racer/device.mojo:35-134 ---------------- */
+/* Device(gpu) (device.mojo:39-46). gpu = 0 fails: this library is the GPU path only. */
+int racer_device_create(int gpu, racer_device** out);
+void racer_device_destroy(racer_device* dev);
+/* DeviceContext.name() (main.mojo:176). */
+const char* racer_device_name(racer_device* dev);
+/* ctx.synchronize() */
+int racer_sync(racer_device* dev);
+/* alloc(count, fill) (device.mojo:48-54): filled and waited for. */
+int racer_alloc(racer_device* dev, size_t count, float fill, racer_buf** out);
+void racer_free(racer_buf* buf);
+size_t racer_buf_len(const racer_buf* buf);
+/* Device address of the buffer's first float (for zero-copy interop). */
+void* racer_buf_ptr(racer_buf* buf);
+/* read(buffer) / write(buffer, values) / upload(values) (device.mojo:121-134): synchronous. */
+int racer_read(racer_buf* buf, float* host);
+int racer_write(racer_buf* buf, const float* host);
+int racer_read_range(racer_buf* buf, size_t offset, size_t count, float* host);
+int racer_write_range(racer_buf* buf, size_t offset, size_t count, const float* host);
+int racer_upload(racer_device* dev, const float* host, size_t count, racer_buf** out);
+/* ctx.enqueue_copy between sub-buffers (ppo.mojo:190, 210): asynchronous. */
+int racer_copy(racer_buf* dst, size_t dst_offset, racer_buf* src, size_t src_offset, size_t count);
+
+/* ---------------- Sim: racer/sim.mojo:269-348 ---------------- */
+/* Sim(device, track, n, seed) (sim.mojo:280-289): copies the track array;
+ * fails with "not a track array" when len < 12 (sim.mojo:97-98). */
+int racer_sim_create(racer_device* dev, const float* track, size_t len, int n, int seed, const racer_config* cfg,
+                     racer_sim** out);
+void racer_sim_destroy(racer_sim* sim);
+/* Sim.reset (sim.mojo:291-305): spawn every car, zero actions, one step. */
+int racer_sim_reset(racer_sim* sim);
+/* Sim.step (sim.mojo:307-321): physics, rewards, respawns, then every lidar
+ * beam. actions [2, n], reward [1, n], done [1, n], obs [beams+5, n]. */
+int racer_sim_step(racer_sim* sim, racer_view actions, racer_view reward, racer_view done, racer_view obs);
+/* Sim.sample (sim.mojo:323-348): out [3, n], log_std [1, 2]. */
+int racer_sim_sample(racer_sim* sim, racer_view out, racer_view log_std, racer_view actions, racer_view logp,
+                     racer_view value, int deterministic, int64_t step);
+racer_buf* racer_sim_state(racer_sim* sim);   /* [11, n] */
+racer_buf* racer_sim_obs(racer_sim* sim);     /* [beams+5, n] */
+racer_buf* racer_sim_actions(racer_sim* sim); /* [2, n] */
+racer_buf* racer_sim_reward(racer_sim* sim);  /* [n] */
+racer_buf* racer_sim_done(racer_sim* sim);    /* [n] */
+int racer_sim_cars(const racer_sim* sim);
+int racer_sim_beams(const racer_sim* sim);
+
+/* ---------------- Policy and Activations: racer/ppo.mojo:47-138 ---------------- */
+/* Activations(device, cols) (ppo.mojo:56-60): h1, h2 [H+1, cols] filled with 1, o [3, cols]. */
+int racer_acts_create(racer_device* dev, int cols, const racer_config* cfg, racer_acts** out);
+void racer_acts_destroy(racer_acts* acts);
+racer_buf* racer_acts_h1(racer_acts* acts);
+racer_buf* racer_acts_h2(racer_acts* acts);
+racer_buf* racer_acts_o(racer_acts* acts);
+/* Policy(device, seed) (ppo.mojo:68-90): Gaussian fan-in init, then transpose. */
+int racer_policy_create(racer_device* dev, int seed, const racer_config* cfg, racer_policy** out);
+void racer_policy_destroy(racer_policy* pol);
+racer_buf* racer_policy_theta(racer_policy* pol); /* [P] = W0[H,IN] | W1[H,H+1] | W2[3,H+1] | logstd[2] */
+size_t racer_policy_params(const racer_policy* pol);
+/* Policy.log_std() (ppo.mojo:105-106) */
+racer_view racer_policy_log_std(racer_policy* pol);
+/* Policy.transpose (ppo.mojo:92-103) */
+int racer_policy_transpose(racer_policy* pol);
+/* Policy.forward (ppo.mojo:108-111): x [beams+5, acts.cols]. */
+int racer_policy_forward(racer_policy* pol, racer_view x, racer_acts* acts);
+/* Policy.backward (ppo.mojo:113-123): d3 [3, cols] -> d2, d1 [H+1, cols], grad W0/W1/W2 entries. */
+int racer_policy_backward(racer_policy* pol, racer_view x, racer_acts* acts, racer_view d3, racer_buf* d2,
+                          racer_buf* d1, racer_buf* grad);
+
+/* ---------------- Trainer: racer/ppo.mojo:141-337 ---------------- */
+int racer_trainer_create(racer_device* dev, int n, const racer_config* cfg, racer_trainer** out);
+void racer_trainer_destroy(racer_trainer* tr);
+/* Trainer.rollout (ppo.mojo:187-232): returns the mean reward per step (synchronous). */
+int racer_trainer_rollout(racer_trainer* tr, racer_sim* sim, racer_policy* pol, racer_acts* acts, int iteration,
+                          float* mean_reward);
+/* Trainer.update (ppo.mojo:234-257): returns the last minibatch's KL estimate (synchronous). */
+int racer_trainer_update(racer_trainer* tr, racer_policy* pol, float* kl);
+/* Trainer.minibatch (ppo.mojo:259-315): gradient of minibatch m into grad, its squared norm into stats. */
+int racer_trainer_minibatch(racer_trainer* tr, racer_policy* pol, int m, float adv_mean, float adv_scale);
+/* Trainer.adam (ppo.mojo:317-337): uses and advances the trainer's step count. */
+int racer_trainer_adam(racer_trainer* tr, racer_policy* pol);
+int racer_trainer_step_count(const racer_trainer* tr);
+racer_buf* racer_trainer_obs(racer_trainer* tr);       /* [(ROLLOUT+1), beams+5, n] */
+racer_buf* racer_trainer_actions(racer_trainer* tr);   /* [ROLLOUT, 2, n] */
+racer_buf* racer_trainer_logp(racer_trainer* tr);      /* [ROLLOUT * n] */
+racer_buf* racer_trainer_value(racer_trainer* tr);     /* [ROLLOUT * n] */
+racer_buf* racer_trainer_reward(racer_trainer* tr);    /* [ROLLOUT * n] */
+racer_buf* racer_trainer_done(racer_trainer* tr);      /* [ROLLOUT * n] */
+racer_buf* racer_trainer_advantage(racer_trainer* tr); /* [ROLLOUT * n] */
+racer_buf* racer_trainer_grad(racer_trainer* tr);      /* [P] */
+racer_buf* racer_trainer_d3(racer_trainer* tr);        /* [6, batch] */
+racer_buf* racer_trainer_stats(racer_trainer* tr);     /* [5] */
+racer_acts* racer_trainer_acts(racer_trainer* tr);     /* minibatch activations */
+
+/* ---------------- Checkpoints: racer/ppo.mojo:341-355 ---------------- */
+/* save: one float32 .npy of [iteration, theta...]. */
+int racer_save(racer_policy* pol, const char* path, int iteration);
+/* load: fails with "checkpoint does not match the network size" unless 1+P values. */
+int racer_load(racer_policy* pol, const char* path, int* iteration);
+
+/* ---------------- Runner: main.mojo:165-235 ---------------- */
+/* Runner.step (main.mojo:183-191): policy forward on sim.obs, sample into
+ * sim.actions (logp/value into scratch [2, n]), then Sim.step. */
+int racer_runner_step(racer_sim* sim, racer_policy* pol, racer_acts* acts, racer_buf* scratch, int deterministic,
+                      int64_t t);
+/* Runner.benchmark (main.mojo:225-235): 10 untimed steps, then `steps` timed
+ * steps; writes milliseconds per step. */
+int racer_benchmark(racer_sim* sim, racer_policy* pol, racer_acts* acts, racer_buf* scratch, int steps,
+                    double* ms_per_step);
+
+/* ---------------- Instrumentation (no reference counterpart) ---------------- */
+/* Kernel timing with HIP events on the device's stream: when enabled, every
+ * launch of a named kernel is bracketed by events; totals are read back here. */
+int racer_profile_enable(racer_device* dev, int on);
+/* Number of kernels with samples, and the i-th: name, launch count, total ms. */
+int racer_profile_count(racer_device* dev);
+int racer_profile_get(racer_device* dev, int i, const char** name, int64_t* launches, double* total_ms);
+int racer_profile_reset(racer_device* dev);
+/* Lidar march iterations summed over every beam of the last step (counted by
+ * an instrumented variant of the lidar kernel when enabled; 0 otherwise). */
+int racer_sim_count_iterations(racer_sim* sim, int on);
+int64_t racer_sim_last_iterations(racer_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

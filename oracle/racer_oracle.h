@@ -1,0 +1,111 @@
+/*
+ * TEST INFRASTRUCTURE ONLY. The CPU oracle: a plain-C restatement of the
+ * reference's simulation and PPO (racer/sim.mojo, racer/ppo.mojo), used by the
+ * tests, by smoke() as the checker, and by bench.py's cpu_baseline leg. No
+ * product code links or calls it.
+ *
+ * Layout follows the reference: every buffer is feature-major float32
+ * [rows, cols]; a "view" is a base pointer whose rows are `cols` floats apart.
+ */
+#ifndef RACER_ORACLE_H
+#define RACER_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int nx, ny;
+    float x0, y0, cell;
+    int waypoints;
+    float spacing;
+    int layers;
+    float slope;
+    long route;
+} omap;
+
+typedef struct {
+    int layer;
+    float clearance;
+    float height;
+} ospot;
+
+typedef struct {
+    float height, sx, sy;
+} osurface;
+
+/* Map (sim.mojo:81-152) */
+int om_init(omap* m, const float* d, size_t len);
+float om_at(const omap* m, const float* d, int layer, int which, float x, float y);
+ospot om_probe(const omap* m, const float* d, float x, float y, float z);
+osurface om_surface(const omap* m, const float* d, int layer, float x, float y);
+float om_progress(const omap* m, const float* d, int layer, float x, float y);
+void o_slopes(osurface s, float heading, float* along, float* across);
+
+/* Dynamics (sim.mojo:155-167) */
+void o_derivative(const float p[4], float steer, float accel, float grip, float along, float across, float out[4]);
+
+/* Per-car and per-beam functions (sim.mojo:170-266) */
+void o_spawn(float* s, long n, const omap* m, const float* d, long i, int seed, float heading_noise);
+void o_advance(float* s, long n, const float* act, long act_cols, float* reward, float* done, float* obs, long obs_cols,
+               int beams, const omap* m, const float* d, long i, int seed, float heading_noise);
+float o_scan(const omap* m, const float* d, const float* s, long n, const float* obs, long obs_cols, long i, int beam,
+             int beams, long* iterations);
+
+/* Whole-sim steps (sim.mojo:291-348). Returns the number of lidar march
+ * iterations taken (summed over every beam), for the roofline byte count. */
+void o_sim_reset(float* s, float* act, float* reward, float* done, float* obs, long n, int beams, const float* d,
+                 size_t len, int seed, float heading_noise);
+long o_sim_step(float* s, long n, const float* act, long act_cols, float* reward, float* done, float* obs,
+                long obs_cols, int beams, const float* d, size_t len, int seed, float heading_noise);
+void o_sim_sample(long n, const float* out, long out_cols, const float* log_std, float* act, long act_cols,
+                  float* logp, float* value, int deterministic, int64_t step, int seed);
+
+/* Policy (ppo.mojo:63-138). Weights theta = W0[H,IN] | W1[H,HA] | W2[3,HA] | logstd[2]. */
+long o_param_count(int beams, int hidden);
+void o_policy_init(float* theta, int beams, int hidden, int seed);
+void o_forward(const float* theta, int beams, int hidden, const float* x, long x_cols, long cols, float* h1, float* h2,
+               float* o);
+void o_backward(const float* theta, int beams, int hidden, const float* x, long cols, const float* h1, const float* h2,
+                const float* d3, float* d2, float* d1, float* grad);
+
+/* Trainer pieces (ppo.mojo:187-337). */
+void o_gae(long n, int rollout, const float* reward, const float* done, const float* value, const float* last_value,
+           float* advantage);
+void o_moments(long samples, const float* advantage, double* sum, double* sum_sq);
+void o_minibatch(const float* theta, int beams, int hidden, long n, int rollout, int minibatches, const float* obs,
+                 const float* actions, const float* logp, const float* value, const float* advantage, int m,
+                 float adv_mean, float adv_scale, float* x, float* h1, float* h2, float* o, float* d3, float* d2,
+                 float* d1, float* grad, float* norm_sq);
+void o_adam(float* theta, long P, long logstd_at, const float* grad, float* first, float* second, float norm_sq,
+            int step);
+float o_bias_correction(float beta, int step);
+
+/* Flat wrappers of the Map lookups and of one lidar beam, for the tests. */
+int o_probe(const float* d, size_t len, float x, float y, float z, float out[3]);
+int o_surface(const float* d, size_t len, int layer, float x, float y, float out[3]);
+float o_progress(const float* d, size_t len, int layer, float x, float y);
+float o_scan_one(const float* d, size_t len, const float* s, long n, const float* obs, long obs_cols, long i, int beam,
+                 int beams, long* iterations);
+
+/* Threads used by the parallel loops (1 = serial). */
+void o_set_threads(int threads);
+int o_get_threads(void);
+
+/* Elementary functions and RNG, exported for their own tests. */
+float o_sin(float x);
+float o_cos(float x);
+float o_exp(float x);
+float o_log(float x);
+float o_tanh(float x);
+void o_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void o_normal4(uint64_t seed, uint64_t subsequence, uint64_t offset, float out[4]);
+void o_uniform4(uint64_t seed, uint64_t subsequence, uint64_t offset, float out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,65 @@
+"""Pins of the shared float32 elementary functions and Philox against
+published known answers and float64 numpy (the oracle and the kernels share
+them, so they are checked on their own here)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+# Random123 kat_vectors, Philox4x32-10 (Salmon et al., SC'11).
+KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", KAT)
+def test_philox_known_answers(ctr, key, expect):
+    assert O.philox(ctr, key) == expect
+
+
+def _max_ulps(name, ref, xs):
+    f = O.elementary(name)
+    got = np.array([f(float(x)) for x in xs], np.float32).astype(np.float64)
+    r = ref(xs.astype(np.float64))
+    ulp = np.spacing(np.abs(r).astype(np.float32)).astype(np.float64)
+    return float(np.max(np.abs(got - r) / ulp))
+
+
+RNG = np.random.default_rng(7)
+
+
+@pytest.mark.parametrize("name,ref,xs,bound", [
+    ("sin", np.sin, np.concatenate([RNG.uniform(-8, 8, 4000), RNG.uniform(-1e-3, 1e-3, 500)]), 2.0),
+    ("cos", np.cos, np.concatenate([RNG.uniform(-8, 8, 4000), RNG.uniform(-1e-3, 1e-3, 500)]), 2.0),
+    ("exp", np.exp, np.concatenate([RNG.uniform(-100, 88, 4000), RNG.uniform(-1, 1, 500)]), 1.5),
+    ("log", np.log, np.concatenate([RNG.uniform(1e-6, 1, 3000), 10 ** RNG.uniform(-37, 37, 1000),
+                                    np.float32(2.0) ** -np.arange(127, 149)]), 2.0),
+    ("tanh", np.tanh, np.concatenate([RNG.uniform(-10, 10, 4000), RNG.uniform(-0.6, 0.6, 2000)]), 2.0),
+])
+def test_elementary_accuracy(name, ref, xs, bound):
+    xs = xs.astype(np.float32)
+    assert _max_ulps(name, ref, xs) <= bound
+
+
+def test_special_values():
+    assert O.elementary("sin")(0.0) == 0.0 and O.elementary("cos")(0.0) == 1.0
+    assert O.elementary("exp")(0.0) == 1.0 and O.elementary("log")(1.0) == 0.0
+    assert O.elementary("tanh")(20.0) == 1.0 and O.elementary("tanh")(-20.0) == -1.0
+    assert np.isinf(O.elementary("exp")(100.0)) and O.elementary("exp")(-200.0) == 0.0
+    assert np.isneginf(O.elementary("log")(0.0)) and np.isnan(O.elementary("log")(-1.0))
+    assert np.isnan(O.elementary("tanh")(float("nan")))
+
+
+def test_uniform_and_normal_statistics():
+    u = np.array([O.uniform4(42, i, 3) for i in range(5000)]).ravel()
+    assert 0.0 <= u.min() and u.max() < 1.0
+    assert abs(u.mean() - 0.5) < 0.01 and abs(u.var() - 1 / 12) < 0.003
+    z = np.array([O.normal4(43, i, 0) for i in range(10000)]).ravel()
+    assert abs(z.mean()) < 0.02 and abs(z.std() - 1) < 0.02
+    assert abs(np.mean(z ** 3)) < 0.06 and abs(np.mean(z ** 4) - 3) < 0.15
+    # distinct (seed, subsequence, offset) give distinct streams
+    assert not np.array_equal(O.normal4(1, 0, 0), O.normal4(1, 1, 0))
+    assert not np.array_equal(O.normal4(1, 0, 0), O.normal4(1, 0, 1))

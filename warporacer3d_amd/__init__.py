@@ -1,0 +1,491 @@
+"""warporacer3d_amd: the racer hot path on MI355X (gfx950) behind the
+reference's Device / Sim / Policy / Activations / Trainer API
+(racer/device.mojo, racer/sim.mojo, racer/ppo.mojo, main.mojo).
+
+This module is a thin ctypes binding of libracer_hip.so (include/racer.h).
+There is no CPU fallback: importing works without a GPU, but creating a
+Device raises unless a HIP device is present, and a missing library raises.
+
+Buffers are float32 device memory, feature-major [rows, cols] as in the
+reference; `mat(buffer, rows, cols, offset)` is the reference's view.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libracer_hip.so"
+
+# Constants of racer/sim.mojo:14-56 and racer/ppo.mojo:17-37.
+STATE = 11
+X, Y, Z, HEADING, SPEED, STEER, STEPS, PROGRESS, EPISODE, FRICTION, RETURN = range(11)
+OUT = 3
+CRASH, TIMEOUT = 1, 2
+RANGE = 10.0
+FOV = 4.71238898
+HORIZON = 3000
+SPEED_MAX = 5.0
+STEER_MAX = 0.4189
+GAMMA, LAMBDA = 0.99, 0.95
+STATS = 5
+REWARD_SLOT, ADV_SLOT, NORM_SLOT, KL_SLOT = 0, 1, 3, 4
+
+
+class RacerError(RuntimeError):
+    """A failed libracer_hip call (the reference raises Error(msg))."""
+
+
+class Config(C.Structure):
+    """Shape parameters that are compile-time constants in the reference."""
+
+    _fields_ = [("beams", C.c_int), ("hidden", C.c_int), ("rollout", C.c_int), ("epochs", C.c_int),
+                ("minibatches", C.c_int), ("precision", C.c_int), ("heading_noise", C.c_float)]
+
+    def __init__(self, beams=108, hidden=64, rollout=32, epochs=4, minibatches=4, precision=0, heading_noise=0.0):
+        super().__init__(beams, hidden, rollout, epochs, minibatches, precision, heading_noise)
+
+    @property
+    def IN(self) -> int:
+        return self.beams + 5
+
+    @property
+    def HA(self) -> int:
+        return self.hidden + 1
+
+    def params(self) -> int:
+        return self.hidden * self.IN + self.hidden * self.HA + OUT * self.HA + 2
+
+
+class _View(C.Structure):
+    _fields_ = [("buf", C.c_void_p), ("offset", C.c_size_t), ("rows", C.c_int64), ("cols", C.c_int64)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded libracer_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RacerError(f"{LIB_PATH} is missing: run warporacer3d_amd/build.py (or __graft_entry__.build())")
+    L = C.CDLL(str(LIB_PATH))
+    vp, i, i64, f, sz = C.c_void_p, C.c_int, C.c_int64, C.c_float, C.c_size_t
+    cp = C.POINTER(C.c_void_p)
+    V = _View
+    sigs = {
+        "racer_version": (C.c_char_p, []),
+        "racer_last_error": (C.c_char_p, []),
+        "racer_config_default": (None, [C.POINTER(Config)]),
+        "racer_device_create": (i, [i, cp]),
+        "racer_device_destroy": (None, [vp]),
+        "racer_device_name": (C.c_char_p, [vp]),
+        "racer_sync": (i, [vp]),
+        "racer_alloc": (i, [vp, sz, f, cp]),
+        "racer_free": (None, [vp]),
+        "racer_buf_len": (sz, [vp]),
+        "racer_buf_ptr": (vp, [vp]),
+        "racer_read": (i, [vp, vp]),
+        "racer_write": (i, [vp, vp]),
+        "racer_read_range": (i, [vp, sz, sz, vp]),
+        "racer_write_range": (i, [vp, sz, sz, vp]),
+        "racer_upload": (i, [vp, vp, sz, cp]),
+        "racer_copy": (i, [vp, sz, vp, sz, sz]),
+        "racer_sim_create": (i, [vp, vp, sz, i, i, C.POINTER(Config), cp]),
+        "racer_sim_destroy": (None, [vp]),
+        "racer_sim_reset": (i, [vp]),
+        "racer_sim_step": (i, [vp, V, V, V, V]),
+        "racer_sim_sample": (i, [vp, V, V, V, V, V, i, i64]),
+        "racer_sim_state": (vp, [vp]),
+        "racer_sim_obs": (vp, [vp]),
+        "racer_sim_actions": (vp, [vp]),
+        "racer_sim_reward": (vp, [vp]),
+        "racer_sim_done": (vp, [vp]),
+        "racer_sim_cars": (i, [vp]),
+        "racer_sim_beams": (i, [vp]),
+        "racer_acts_create": (i, [vp, i, C.POINTER(Config), cp]),
+        "racer_acts_destroy": (None, [vp]),
+        "racer_acts_h1": (vp, [vp]),
+        "racer_acts_h2": (vp, [vp]),
+        "racer_acts_o": (vp, [vp]),
+        "racer_policy_create": (i, [vp, i, C.POINTER(Config), cp]),
+        "racer_policy_destroy": (None, [vp]),
+        "racer_policy_theta": (vp, [vp]),
+        "racer_policy_params": (sz, [vp]),
+        "racer_policy_log_std": (V, [vp]),
+        "racer_policy_transpose": (i, [vp]),
+        "racer_policy_forward": (i, [vp, V, vp]),
+        "racer_policy_backward": (i, [vp, V, vp, V, vp, vp, vp]),
+        "racer_trainer_create": (i, [vp, i, C.POINTER(Config), cp]),
+        "racer_trainer_destroy": (None, [vp]),
+        "racer_trainer_rollout": (i, [vp, vp, vp, vp, i, C.POINTER(C.c_float)]),
+        "racer_trainer_update": (i, [vp, vp, C.POINTER(C.c_float)]),
+        "racer_trainer_minibatch": (i, [vp, vp, i, f, f]),
+        "racer_trainer_adam": (i, [vp, vp]),
+        "racer_trainer_step_count": (i, [vp]),
+        "racer_save": (i, [vp, C.c_char_p, i]),
+        "racer_load": (i, [vp, C.c_char_p, C.POINTER(C.c_int)]),
+        "racer_runner_step": (i, [vp, vp, vp, vp, i, i64]),
+        "racer_benchmark": (i, [vp, vp, vp, vp, i, C.POINTER(C.c_double)]),
+        "racer_profile_enable": (i, [vp, i]),
+        "racer_profile_count": (i, [vp]),
+        "racer_profile_get": (i, [vp, i, C.POINTER(C.c_char_p), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+        "racer_profile_reset": (i, [vp]),
+        "racer_sim_count_iterations": (i, [vp, i]),
+        "racer_sim_last_iterations": (i64, [vp]),
+    }
+    for name in ("obs", "actions", "logp", "value", "reward", "done", "advantage", "grad", "d3", "stats", "acts"):
+        sigs[f"racer_trainer_{name}"] = (vp, [vp])
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise RacerError(lib().racer_last_error().decode())
+
+
+def exported_symbols() -> list[str]:
+    """Names of the C ABI entry points declared in include/racer.h."""
+    import re
+
+    text = (HERE.parent / "include" / "racer.h").read_text()
+    return sorted(set(re.findall(r"\b(racer_[a-z0-9_]+)\(", text)))
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Buffer:
+    """Flat float32 device memory (racer/device.mojo:20); owned unless wrapped."""
+
+    def __init__(self, device: "Device", handle: int, owned: bool = True):
+        self.device, self.handle, self.owned = device, handle, owned
+
+    def __len__(self) -> int:
+        return int(lib().racer_buf_len(self.handle))
+
+    @property
+    def ptr(self) -> int:
+        return int(lib().racer_buf_ptr(self.handle))
+
+    def read(self) -> np.ndarray:
+        return self.device.read(self)
+
+    def write(self, values) -> None:
+        self.device.write(self, values)
+
+    def free(self) -> None:
+        if self.owned and self.handle:
+            lib().racer_free(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class View:
+    """mat[rows](buffer, cols, offset) (racer/device.mojo:29-32)."""
+
+    def __init__(self, buffer: Buffer, rows: int, cols: int, offset: int = 0):
+        self.buffer, self.rows, self.cols, self.offset = buffer, rows, cols, offset
+
+    @property
+    def c(self) -> _View:
+        return _View(self.buffer.handle, self.offset, self.rows, self.cols)
+
+    def read(self) -> np.ndarray:
+        out = np.empty((self.rows, self.cols), np.float32)
+        _check(lib().racer_read_range(self.buffer.handle, self.offset, out.size, _ptr(out)))
+        return out
+
+
+def mat(buffer: Buffer, rows: int, cols: int, offset: int = 0) -> View:
+    return View(buffer, rows, cols, offset)
+
+
+class Device:
+    """Device(gpu) (racer/device.mojo:35-134). Only the GPU is provided."""
+
+    def __init__(self, gpu: bool = True):
+        h = C.c_void_p()
+        _check(lib().racer_device_create(int(gpu), C.byref(h)))
+        self.handle = h.value
+        self.gpu = True
+
+    @property
+    def name(self) -> str:
+        return lib().racer_device_name(self.handle).decode()
+
+    def alloc(self, count: int, fill: float = 0.0) -> Buffer:
+        h = C.c_void_p()
+        _check(lib().racer_alloc(self.handle, int(count), float(fill), C.byref(h)))
+        return Buffer(self, h.value)
+
+    def upload(self, values) -> Buffer:
+        a = np.ascontiguousarray(values, np.float32).ravel()
+        h = C.c_void_p()
+        _check(lib().racer_upload(self.handle, _ptr(a), a.size, C.byref(h)))
+        return Buffer(self, h.value)
+
+    def read(self, buffer: Buffer) -> np.ndarray:
+        out = np.empty(len(buffer), np.float32)
+        _check(lib().racer_read(buffer.handle, _ptr(out)))
+        return out
+
+    def write(self, buffer: Buffer, values) -> None:
+        a = np.ascontiguousarray(values, np.float32).ravel()
+        if a.size != len(buffer):
+            raise RacerError(f"write of {a.size} floats into a buffer of {len(buffer)}")
+        _check(lib().racer_write(buffer.handle, _ptr(a)))
+
+    def write_range(self, buffer: Buffer, offset: int, values) -> None:
+        a = np.ascontiguousarray(values, np.float32).ravel()
+        _check(lib().racer_write_range(buffer.handle, offset, a.size, _ptr(a)))
+
+    def copy(self, dst: Buffer, dst_offset: int, src: Buffer, src_offset: int, count: int) -> None:
+        _check(lib().racer_copy(dst.handle, dst_offset, src.handle, src_offset, count))
+
+    def synchronize(self) -> None:
+        _check(lib().racer_sync(self.handle))
+
+    # kernel timing with HIP events on the library's stream
+    def profile(self, on: bool = True) -> None:
+        _check(lib().racer_profile_enable(self.handle, int(on)))
+
+    def profile_reset(self) -> None:
+        _check(lib().racer_profile_reset(self.handle))
+
+    def profile_results(self) -> dict:
+        out = {}
+        for i in range(lib().racer_profile_count(self.handle)):
+            name, n, ms = C.c_char_p(), C.c_int64(), C.c_double()
+            _check(lib().racer_profile_get(self.handle, i, C.byref(name), C.byref(n), C.byref(ms)))
+            out[name.value.decode()] = (int(n.value), float(ms.value))
+        return out
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().racer_device_destroy(self.handle)
+        except Exception:
+            pass
+        self.handle = None
+
+
+def _wrap(device: Device, handle) -> Buffer:
+    return Buffer(device, handle, owned=False)
+
+
+class Sim:
+    """Sim(device, track, n, seed) (racer/sim.mojo:269-348)."""
+
+    def __init__(self, device: Device, track, n: int, seed: int, config: Config | None = None):
+        self.device = device
+        self.config = config or Config()
+        arr = np.ascontiguousarray(track, np.float32).ravel()
+        h = C.c_void_p()
+        _check(lib().racer_sim_create(device.handle, _ptr(arr), arr.size, int(n), int(seed),
+                                      C.byref(self.config), C.byref(h)))
+        self.handle = h.value
+        self.n, self.seed, self.beams = int(n), int(seed), self.config.beams
+        self.IN = self.beams + 5
+        L = lib()
+        self.state = _wrap(device, L.racer_sim_state(self.handle))
+        self.obs = _wrap(device, L.racer_sim_obs(self.handle))
+        self.actions = _wrap(device, L.racer_sim_actions(self.handle))
+        self.reward = _wrap(device, L.racer_sim_reward(self.handle))
+        self.done = _wrap(device, L.racer_sim_done(self.handle))
+
+    def reset(self) -> None:
+        _check(lib().racer_sim_reset(self.handle))
+
+    def step(self, actions: View | None = None, reward: View | None = None, done: View | None = None,
+             obs: View | None = None) -> None:
+        n = self.n
+        actions = actions or mat(self.actions, 2, n)
+        reward = reward or mat(self.reward, 1, n)
+        done = done or mat(self.done, 1, n)
+        obs = obs or mat(self.obs, self.IN, n)
+        _check(lib().racer_sim_step(self.handle, actions.c, reward.c, done.c, obs.c))
+
+    def sample(self, output: View, log_std: View, actions: View, logp: View, value: View, deterministic: bool,
+               step: int) -> None:
+        _check(lib().racer_sim_sample(self.handle, output.c, log_std.c, actions.c, logp.c, value.c,
+                                      int(deterministic), int(step)))
+
+    def count_iterations(self, on: bool = True) -> None:
+        _check(lib().racer_sim_count_iterations(self.handle, int(on)))
+
+    @property
+    def last_iterations(self) -> int:
+        return int(lib().racer_sim_last_iterations(self.handle))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().racer_sim_destroy(self.handle)
+        except Exception:
+            pass
+        self.handle = None
+
+
+class Activations:
+    """Activations(device, cols) (racer/ppo.mojo:47-60)."""
+
+    def __init__(self, device: Device, cols: int, config: Config | None = None, handle=None):
+        self.device = device
+        self.config = config or Config()
+        self.owned = handle is None
+        if handle is None:
+            h = C.c_void_p()
+            _check(lib().racer_acts_create(device.handle, int(cols), C.byref(self.config), C.byref(h)))
+            handle = h.value
+        self.handle = handle
+        self.cols = int(cols)
+        L = lib()
+        self.h1 = _wrap(device, L.racer_acts_h1(handle))
+        self.h2 = _wrap(device, L.racer_acts_h2(handle))
+        self.o = _wrap(device, L.racer_acts_o(handle))
+
+    def __del__(self):
+        try:
+            if self.owned and self.handle:
+                lib().racer_acts_destroy(self.handle)
+        except Exception:
+            pass
+        self.handle = None
+
+
+class Policy:
+    """Policy(device, seed) (racer/ppo.mojo:63-138)."""
+
+    def __init__(self, device: Device, seed: int, config: Config | None = None):
+        self.device = device
+        self.config = config or Config()
+        h = C.c_void_p()
+        _check(lib().racer_policy_create(device.handle, int(seed), C.byref(self.config), C.byref(h)))
+        self.handle = h.value
+        self.theta = _wrap(device, lib().racer_policy_theta(self.handle))
+        cfg = self.config
+        self.IN, self.HA, self.H = cfg.IN, cfg.HA, cfg.hidden
+        self.W0 = 0
+        self.W1 = cfg.hidden * cfg.IN
+        self.W2 = self.W1 + cfg.hidden * cfg.HA
+        self.LOGSTD = self.W2 + OUT * cfg.HA
+        self.P = int(lib().racer_policy_params(self.handle))
+
+    def log_std(self) -> View:
+        return mat(self.theta, 1, 2, self.LOGSTD)
+
+    def transpose(self) -> None:
+        _check(lib().racer_policy_transpose(self.handle))
+
+    def forward(self, x: View, acts: Activations) -> None:
+        _check(lib().racer_policy_forward(self.handle, x.c, acts.handle))
+
+    def backward(self, x: View, acts: Activations, d3: View, d2: Buffer, d1: Buffer, grad: Buffer) -> None:
+        _check(lib().racer_policy_backward(self.handle, x.c, acts.handle, d3.c, d2.handle, d1.handle, grad.handle))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().racer_policy_destroy(self.handle)
+        except Exception:
+            pass
+        self.handle = None
+
+
+class Trainer:
+    """Trainer(device, n) (racer/ppo.mojo:141-337)."""
+
+    def __init__(self, device: Device, n: int, config: Config | None = None):
+        self.device = device
+        self.config = config or Config()
+        h = C.c_void_p()
+        _check(lib().racer_trainer_create(device.handle, int(n), C.byref(self.config), C.byref(h)))
+        self.handle = h.value
+        self.n = int(n)
+        self.batch = self.n * self.config.rollout // self.config.minibatches
+        L = lib()
+        for name in ("obs", "actions", "logp", "value", "reward", "done", "advantage", "grad", "d3", "stats"):
+            setattr(self, name, _wrap(device, getattr(L, f"racer_trainer_{name}")(self.handle)))
+        self.acts = Activations(device, self.batch, self.config, handle=L.racer_trainer_acts(self.handle))
+
+    @property
+    def step(self) -> int:
+        return int(lib().racer_trainer_step_count(self.handle))
+
+    def rollout(self, sim: Sim, policy: Policy, acts: Activations, iteration: int) -> float:
+        r = C.c_float()
+        _check(lib().racer_trainer_rollout(self.handle, sim.handle, policy.handle, acts.handle, int(iteration),
+                                           C.byref(r)))
+        return float(r.value)
+
+    def update(self, policy: Policy) -> float:
+        kl = C.c_float()
+        _check(lib().racer_trainer_update(self.handle, policy.handle, C.byref(kl)))
+        return float(kl.value)
+
+    def minibatch(self, policy: Policy, m: int, adv_mean: float, adv_scale: float) -> None:
+        _check(lib().racer_trainer_minibatch(self.handle, policy.handle, int(m), float(adv_mean), float(adv_scale)))
+
+    def adam(self, policy: Policy) -> None:
+        _check(lib().racer_trainer_adam(self.handle, policy.handle))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().racer_trainer_destroy(self.handle)
+        except Exception:
+            pass
+        self.handle = None
+
+
+def save(policy: Policy, path: str | os.PathLike, iteration: int) -> None:
+    """save (racer/ppo.mojo:341-345): float32 .npy of [iteration, theta...]."""
+    _check(lib().racer_save(policy.handle, str(path).encode(), int(iteration)))
+
+
+def load(policy: Policy, path: str | os.PathLike) -> int:
+    """load (racer/ppo.mojo:348-355): returns the saved iteration."""
+    it = C.c_int()
+    _check(lib().racer_load(policy.handle, str(path).encode(), C.byref(it)))
+    return int(it.value)
+
+
+class Runner:
+    """A device, n cars on a track and a policy driving them (main.mojo:165-235)."""
+
+    def __init__(self, track, n: int, config: Config | None = None, seed: int = 42, device: Device | None = None):
+        self.device = device or Device(True)
+        self.config = config or Config()
+        self.sim = Sim(self.device, track, n, seed, self.config)
+        self.policy = Policy(self.device, seed, self.config)
+        self.acts = Activations(self.device, n, self.config)
+        self.scratch = self.device.alloc(2 * n)
+        self.sim.reset()
+
+    def step(self, deterministic: bool, t: int) -> None:
+        _check(lib().racer_runner_step(self.sim.handle, self.policy.handle, self.acts.handle, self.scratch.handle,
+                                       int(deterministic), int(t)))
+
+    def benchmark(self, steps: int) -> float:
+        """10 warm-up steps then `steps` timed steps; returns ms/step (main.mojo:225-235)."""
+        ms = C.c_double()
+        _check(lib().racer_benchmark(self.sim.handle, self.policy.handle, self.acts.handle, self.scratch.handle,
+                                     int(steps), C.byref(ms)))
+        return float(ms.value)

@@ -1,0 +1,195 @@
+// Matrix products of the policy (racer/ppo.mojo:108-138; the reference calls
+// MAX matmul through racer/device.mojo:82-98).
+//
+// Float32 path: every output accumulates fmaf(a, b, acc) over k = 0, 1, ...
+// in order, one accumulator per output, so forward results are bit-identical
+// to the CPU restatement. Tiles are 64 x 64 outputs per 256-thread workgroup,
+// 4 x 4 per thread, operands staged through LDS 16 k at a time.
+//
+// Weight gradients (K = the minibatch, up to 262144) split K into chunks of
+// RACER_SPLITK columns; chunk partials are summed in chunk order.
+#include "racer_internal.h"
+
+namespace {
+
+constexpr int BM = 64, BN = 64, BK = 16;
+
+template <int EPI>
+__device__ __forceinline__ float epilogue(float acc, const float* H, int64_t ldh, int m, int n) {
+    if constexpr (EPI == RACER_EPI_TANH) return racer_tanh(acc);
+    if constexpr (EPI == RACER_EPI_MASK) {
+        float h = H[(int64_t)m * ldh + n];
+        return acc * (1.0f - h * h);
+    }
+    return acc;
+}
+
+// C[M, N] = epi(A[M, K] B[K, N])
+template <int EPI>
+__global__ void __launch_bounds__(256) k_gemm_nn_f32(const float* __restrict__ A, int64_t lda,
+                                                     const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
+                                                     int64_t ldc, int M, int N, int K, const float* __restrict__ H,
+                                                     int64_t ldh) {
+    __shared__ float As[BK][BM + 1];
+    __shared__ float Bs[BK][BN];
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        {  // A tile: 64 rows x 16 k, 16 consecutive k per row
+            int kk = tid & 15, r = tid >> 4;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                int row = r + 16 * p;
+                int m = m0 + row, k = k0 + kk;
+                As[kk][row] = (m < M && k < K) ? A[(int64_t)m * lda + k] : 0.0f;
+            }
+        }
+        {  // B tile: 16 k x 64 columns, coalesced along the columns
+            int c = tid & 63, r = tid >> 6;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                int kk = r + 4 * p;
+                int k = k0 + kk, n = n0 + c;
+                Bs[kk][c] = (k < K && n < N) ? B[(int64_t)k * ldb + n] : 0.0f;
+            }
+        }
+        __syncthreads();
+        const int kv = min(BK, K - k0);
+        for (int kk = 0; kk < kv; ++kk) {
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int m = m0 + ty + 16 * i;
+        if (m >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int n = n0 + tx + 16 * j;
+            if (n < N) C[(int64_t)m * ldc + n] = epilogue<EPI>(acc[i][j], H, ldh, m, n);
+        }
+    }
+}
+
+// Partial C[chunk][M, N] = A[M, kchunk] B[N, kchunk]^T over one K chunk.
+__global__ void __launch_bounds__(256) k_gemm_nt_f32(const float* __restrict__ A, int64_t lda,
+                                                     const float* __restrict__ B, int64_t ldb,
+                                                     float* __restrict__ out, int64_t out_stride, int64_t ldc, int M,
+                                                     int N, int K, int chunk) {
+    __shared__ float As[BK][BM + 1];
+    __shared__ float Bs[BK][BN + 1];
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int kbeg = blockIdx.z * chunk;
+    const int kend = min(K, kbeg + chunk);
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        int kk = tid & 15, r = tid >> 4;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            int row = r + 16 * p;
+            int k = k0 + kk;
+            int m = m0 + row, n = n0 + row;
+            As[kk][row] = (m < M && k < kend) ? A[(int64_t)m * lda + k] : 0.0f;
+            Bs[kk][row] = (n < N && k < kend) ? B[(int64_t)n * ldb + k] : 0.0f;
+        }
+        __syncthreads();
+        const int kv = min(BK, kend - k0);
+        for (int q = 0; q < kv; ++q) {
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[q][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[q][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+    float* dst = out + (int64_t)blockIdx.z * out_stride;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int m = m0 + ty + 16 * i;
+        if (m >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int n = n0 + tx + 16 * j;
+            if (n < N) dst[(int64_t)m * ldc + n] = acc[i][j];
+        }
+    }
+}
+
+// C[m, n] = sum over chunks c in order of partial[c][m, n].
+__global__ void k_sum_chunks(const float* __restrict__ partial, int64_t stride, int chunks, float* __restrict__ C,
+                             int64_t ldc, int M, int N) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)M * N) return;
+    int m = (int)(idx / N), n = (int)(idx % N);
+    float acc = 0.0f;
+    for (int c = 0; c < chunks; ++c) acc += partial[c * stride + (int64_t)m * ldc + n];
+    C[(int64_t)m * ldc + n] = acc;
+}
+
+}  // namespace
+
+int racer_gemm_nn_f32(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                      int64_t ldc, int M, int N, int K, int epi, const float* H, int64_t ldh) {
+    if (M <= 0 || N <= 0) return 0;
+    dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM), block(256);
+    racer_timed timer(dev, epi == RACER_EPI_TANH ? "gemm_f32_tanh" : (epi == RACER_EPI_MASK ? "gemm_f32_mask" : "gemm_f32"));
+    if (epi == RACER_EPI_TANH)
+        hipLaunchKernelGGL(k_gemm_nn_f32<RACER_EPI_TANH>, grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc, M, N,
+                           K, H, ldh);
+    else if (epi == RACER_EPI_MASK)
+        hipLaunchKernelGGL(k_gemm_nn_f32<RACER_EPI_MASK>, grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc, M, N,
+                           K, H, ldh);
+    else
+        hipLaunchKernelGGL(k_gemm_nn_f32<RACER_EPI_NONE>, grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc, M, N,
+                           K, H, ldh);
+    return racer_launch_check("gemm_nn_f32");
+}
+
+int racer_gemm_nt_f32(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                      int64_t ldc, int M, int N, int K) {
+    if (M <= 0 || N <= 0) return 0;
+    const int chunk = 4096;
+    int chunks = K <= 0 ? 1 : (K + chunk - 1) / chunk;
+    int64_t stride = (int64_t)M * ldc;
+    if (racer_ensure_scratch(dev, (size_t)(stride * chunks))) return 1;
+    dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, chunks), block(256);
+    {
+        racer_timed timer(dev, "gemm_nt_f32");
+        hipLaunchKernelGGL(k_gemm_nt_f32, grid, block, 0, dev->stream, A, lda, B, ldb, dev->scratch, stride, ldc, M,
+                           N, K, chunk);
+    }
+    if (racer_launch_check("gemm_nt_f32")) return 1;
+    int64_t total = (int64_t)M * N;
+    {
+        racer_timed timer(dev, "sum_chunks");
+        hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, dev->stream,
+                           dev->scratch, stride, chunks, C, ldc, M, N);
+    }
+    return racer_launch_check("sum_chunks");
+}

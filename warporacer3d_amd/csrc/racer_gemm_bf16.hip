@@ -1,0 +1,177 @@
+// BF16 MFMA matrix products for the policy (gfx950 v_mfma_f32_32x32x16_bf16),
+// float32 accumulation.
+//
+// One 256-thread workgroup computes a 64 x 64 tile of C; each of its four
+// wavefronts owns a 32 x 32 block and issues two 32x32x16 MFMAs per 32-wide K
+// stage. Both operands are staged in LDS as [row][k] with k contiguous, so each
+// lane's MFMA fragment (8 consecutive k of one row or column) is one 16-byte
+// LDS read. Activations arrive as float32 and are rounded to bf16
+// (round-to-nearest-even) while staging; weights are pre-rounded once per
+// update into a row-padded bf16 copy (rows padded to a multiple of 8 values).
+#include "racer_internal.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;  // LDS row stride in bf16 (80 bytes)
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t u = racer_f2u(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// B operand modes: NN reads B[K][N] (k-major rows), NT reads B[N][K].
+template <bool A_BF16, bool B_NT, int EPI>
+__global__ void __launch_bounds__(256) k_gemm_bf16(const void* __restrict__ Ap, int64_t lda,
+                                                   const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
+                                                   int64_t ldc, int64_t chunk_stride, int M, int N, int K, int kchunk) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[TM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][LDK];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(K, kbeg + kchunk);
+    f32x16 acc;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0f;
+
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+        {  // A tile: 64 rows x 32 k, 8 consecutive k per thread
+            const int row = tid >> 2, kc = (tid & 3) * 8;
+            const int m = m0 + row;
+            uint16_t v[8];
+            if constexpr (A_BF16) {
+                // rows are padded to a multiple of 8 with zeros: whole 16-byte chunks are in range
+                const uint16_t* a = reinterpret_cast<const uint16_t*>(Ap);
+                if (m < M && k0 + kc < kend) {
+                    uint4 raw = *reinterpret_cast<const uint4*>(a + (int64_t)m * lda + k0 + kc);
+                    *reinterpret_cast<uint4*>(v) = raw;
+                } else {
+                    *reinterpret_cast<uint4*>(v) = make_uint4(0, 0, 0, 0);
+                }
+            } else {
+                const float* a = reinterpret_cast<const float*>(Ap);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    int k = k0 + kc + e;
+                    v[e] = (m < M && k < kend) ? to_bf16(a[(int64_t)m * lda + k]) : (uint16_t)0;
+                }
+            }
+            *reinterpret_cast<uint4*>(&As[row][kc]) = *reinterpret_cast<uint4*>(v);
+        }
+        if constexpr (B_NT) {  // B[N][K]: 8 consecutive k of one column per thread
+            const int col = tid >> 2, kc = (tid & 3) * 8;
+            const int n = n0 + col;
+            uint16_t v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                int k = k0 + kc + e;
+                v[e] = (n < N && k < kend) ? to_bf16(B[(int64_t)n * ldb + k]) : (uint16_t)0;
+            }
+            *reinterpret_cast<uint4*>(&Bs[col][kc]) = *reinterpret_cast<uint4*>(v);
+        } else {  // B[K][N]: 8 consecutive columns of one k per thread, coalesced along N
+            const int kk = tid >> 3, nc = (tid & 7) * 8;
+            const int k = k0 + kk;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                int n = n0 + nc + e;
+                Bs[nc + e][kk] = (k < kend && n < N) ? to_bf16(B[(int64_t)k * ldb + n]) : (uint16_t)0;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < TK; ks += 16) {
+            const int r = lane & 31, kb = (lane >> 5) * 8;
+            bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm * 32 + r][ks + kb]);
+            bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn * 32 + r][ks + kb]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D[i][j]: j = lane % 32, i = 8 (q / 4) + 4 (lane / 32) + q % 4
+    float* dst = C + (int64_t)blockIdx.z * chunk_stride;
+    const int j = n0 + wn * 32 + (lane & 31);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int i = m0 + wm * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+        if (i < M && j < N) {
+            float v = acc[q];
+            if constexpr (EPI == RACER_EPI_TANH) v = racer_tanh(v);
+            dst[(int64_t)i * ldc + j] = v;
+        }
+    }
+}
+
+__global__ void k_pack_bf16(const float* src, int64_t ld_src, uint16_t* dst, int64_t ld_dst, int rows, int cols) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)rows * ld_dst) return;
+    int r = (int)(idx / ld_dst), c = (int)(idx % ld_dst);
+    dst[idx] = c < cols ? to_bf16(src[(int64_t)r * ld_src + c]) : (uint16_t)0;
+}
+
+__global__ void k_sum_chunks_bf16(const float* __restrict__ partial, int64_t stride, int chunks, float* __restrict__ C,
+                                  int64_t ldc, int M, int N) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)M * N) return;
+    int m = (int)(idx / N), n = (int)(idx % N);
+    float acc = 0.0f;
+    for (int c = 0; c < chunks; ++c) acc += partial[c * stride + (int64_t)m * ldc + n];
+    C[(int64_t)m * ldc + n] = acc;
+}
+
+}  // namespace
+
+int racer_pack_bf16(racer_device* dev, const float* src, int64_t ld_src, uint16_t* dst, int64_t ld_dst, int rows,
+                    int cols) {
+    int64_t total = (int64_t)rows * ld_dst;
+    if (total <= 0) return 0;
+    {
+        racer_timed timer(dev, "pack_bf16");
+        hipLaunchKernelGGL(k_pack_bf16, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, dev->stream, src, ld_src,
+                           dst, ld_dst, rows, cols);
+    }
+    return racer_launch_check("pack_bf16");
+}
+
+int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                       int64_t ldc, int M, int N, int K, int epi) {
+    if (M <= 0 || N <= 0) return 0;
+    dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, 1), block(256);
+    racer_timed timer(dev, epi == RACER_EPI_TANH ? "gemm_bf16_tanh" : "gemm_bf16");
+    if (epi == RACER_EPI_TANH)
+        hipLaunchKernelGGL((k_gemm_bf16<true, false, RACER_EPI_TANH>), grid, block, 0, dev->stream, A, lda, B, ldb, C,
+                           ldc, (int64_t)0, M, N, K, K);
+    else
+        hipLaunchKernelGGL((k_gemm_bf16<true, false, RACER_EPI_NONE>), grid, block, 0, dev->stream, A, lda, B, ldb, C,
+                           ldc, (int64_t)0, M, N, K, K);
+    return racer_launch_check("gemm_nn_bf16");
+}
+
+int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                       int64_t ldc, int M, int N, int K) {
+    if (M <= 0 || N <= 0) return 0;
+    const int chunk = 8192;
+    int chunks = K <= 0 ? 1 : (K + chunk - 1) / chunk;
+    int64_t stride = (int64_t)M * ldc;
+    if (racer_ensure_scratch(dev, (size_t)(stride * chunks))) return 1;
+    dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, chunks), block(256);
+    {
+        racer_timed timer(dev, "gemm_nt_bf16");
+        hipLaunchKernelGGL((k_gemm_bf16<false, true, RACER_EPI_NONE>), grid, block, 0, dev->stream, A, lda, B, ldb,
+                           dev->scratch, ldc, stride, M, N, K, chunk);
+    }
+    if (racer_launch_check("gemm_nt_bf16")) return 1;
+    int64_t total = (int64_t)M * N;
+    {
+        racer_timed timer(dev, "sum_chunks");
+        hipLaunchKernelGGL(k_sum_chunks_bf16, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, dev->stream,
+                           dev->scratch, stride, chunks, C, ldc, M, N);
+    }
+    return racer_launch_check("sum_chunks");
+}

@@ -1,0 +1,160 @@
+// Internal structures of libracer_hip: device, buffers, sim, policy, trainer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/racer.h"
+#include "racer_contract.h"
+#include "racer_math.h"
+
+// ---------------------------------------------------------------- errors
+void racer_set_error(const std::string& msg);
+void racer_clear_error();
+
+#define RACER_TRY(expr)                                                              \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            racer_set_error(std::string(#expr) + ": " + hipGetErrorString(e_));      \
+            return 1;                                                                \
+        }                                                                            \
+    } while (0)
+
+#define RACER_REQUIRE(cond, msg)   \
+    do {                           \
+        if (!(cond)) {             \
+            racer_set_error(msg);  \
+            return 1;              \
+        }                          \
+    } while (0)
+
+// ---------------------------------------------------------------- device
+struct racer_profile_sample {
+    std::string name;
+    int64_t launches = 0;
+    double total_ms = 0.0;
+};
+
+struct racer_pending_event {
+    int kernel;
+    hipEvent_t start, stop;
+};
+
+struct racer_device {
+    int id = 0;
+    hipStream_t stream = nullptr;
+    std::string name;
+    int cus = 0;
+    bool profiling = false;
+    std::vector<racer_profile_sample> samples;
+    std::vector<racer_pending_event> pending;
+    std::vector<hipEvent_t> event_pool;
+    float* scratch = nullptr;  // small device scratch: reductions, counters
+    size_t scratch_len = 0;
+};
+
+struct racer_buf {
+    racer_device* dev = nullptr;
+    float* ptr = nullptr;
+    size_t len = 0;
+};
+
+// Launch bracket for the profiler: records events around a launch when enabled.
+struct racer_timed {
+    racer_device* dev;
+    int kernel = -1;
+    hipEvent_t start = nullptr, stop = nullptr;
+    racer_timed(racer_device* d, const char* name);
+    ~racer_timed();
+};
+
+int racer_check_view(const racer_view& v, int64_t rows, int64_t cols, const char* what);
+inline float* view_ptr(const racer_view& v) { return v.buf->ptr + v.offset; }
+int racer_ensure_scratch(racer_device* dev, size_t floats);
+int racer_launch_check(const char* what);
+
+// ---------------------------------------------------------------- map (device side)
+// Header fields of the track array (racer/sim.mojo:81-102) plus device pointers.
+struct racer_dmap {
+    const float* d;        // the track array as uploaded
+    const float* packed;   // per-cell records (clearance, height) for every layer, or null
+    int nx, ny, layers, waypoints;
+    float x0, y0, cell, spacing, slope;
+    int64_t route;         // offset of the waypoint table
+    int64_t plane;         // nx * ny
+};
+
+struct racer_sim {
+    racer_device* dev = nullptr;
+    int n = 0, seed = 0, beams = 0;
+    float heading_noise = 0.0f;
+    racer_buf *data = nullptr, *state = nullptr, *obs = nullptr, *actions = nullptr, *reward = nullptr,
+              *done = nullptr;
+    float* packed = nullptr;
+    float2* dirs = nullptr;  // (cos, sin) of every beam's azimuth
+    racer_dmap map{};
+    bool count_iterations = false;
+    unsigned long long* counter = nullptr;
+    int64_t last_iterations = 0;
+};
+
+// ---------------------------------------------------------------- policy
+struct racer_policy {
+    racer_device* dev = nullptr;
+    int beams = 0, hidden = 0, precision = 0;
+    int64_t IN = 0, HA = 0, P = 0, W1 = 0, W2 = 0, LOGSTD = 0;
+    racer_buf *theta = nullptr, *w1t = nullptr, *w2t = nullptr;
+    uint16_t* theta_bf16 = nullptr;  // bf16 W0 [H, pad8(IN)] then W1 [H, pad8(HA)] for the MFMA path
+    int64_t in_pad = 0, ha_pad = 0;
+    bool bf16_stale = true;
+};
+
+struct racer_acts {
+    racer_device* dev = nullptr;
+    int cols = 0, hidden = 0;
+    racer_buf *h1 = nullptr, *h2 = nullptr, *o = nullptr;
+};
+
+struct racer_trainer {
+    racer_device* dev = nullptr;
+    int n = 0, beams = 0, hidden = 0, rollout = 0, epochs = 0, minibatches = 0, precision = 0;
+    int64_t batch = 0, P = 0, IN = 0, HA = 0;
+    int step = 0;
+    racer_buf *obs = nullptr, *actions = nullptr, *logp = nullptr, *value = nullptr, *reward = nullptr,
+              *done = nullptr, *advantage = nullptr, *x = nullptr, *d3 = nullptr, *d2 = nullptr, *d1 = nullptr,
+              *grad = nullptr, *first = nullptr, *second = nullptr, *stats = nullptr;
+    racer_acts* acts = nullptr;
+    double* dstats = nullptr;  // float64 accumulators: advantage sum and sum of squares, reward sum
+};
+
+// ---------------------------------------------------------------- kernels (launchers)
+int racer_launch_reset(racer_sim* sim);
+int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done, float* obs, int64_t obs_cols);
+int racer_launch_sample(racer_sim* sim, const float* out, int64_t out_cols, const float* log_std, float* actions,
+                        float* logp, float* value, int deterministic, int64_t step);
+
+// GEMM epilogues
+enum { RACER_EPI_NONE = 0, RACER_EPI_TANH = 1, RACER_EPI_MASK = 2 };
+// C[M, N] = epi(A[M, K] B[K, N]); row-major with leading dimensions; mask epilogue
+// multiplies by (1 - h^2) with h = H[m, n].
+int racer_gemm_nn_f32(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                      int64_t ldc, int M, int N, int K, int epi, const float* H, int64_t ldh);
+// C[M, N] = A[M, K] B[N, K]^T (weight gradients); K is the long batch dimension.
+int racer_gemm_nt_f32(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                      int64_t ldc, int M, int N, int K);
+// bf16 MFMA variants (operands rounded to bf16, float32 accumulation)
+// A is a bf16 copy with rows padded to a multiple of 8 values (lda % 8 == 0, zeros beyond K).
+int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                       int64_t ldc, int M, int N, int K, int epi);
+int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                       int64_t ldc, int M, int N, int K);
+// dst[r, c] = bf16(src[r, c]) for c < cols, 0 for cols <= c < ld_dst.
+int racer_pack_bf16(racer_device* dev, const float* src, int64_t ld_src, uint16_t* dst, int64_t ld_dst, int rows,
+                    int cols);
+inline int64_t racer_pad8(int64_t v) { return (v + 7) / 8 * 8; }
+
+int racer_policy_forward_raw(racer_policy* pol, const float* x, int64_t x_cols, racer_acts* acts, int cols);

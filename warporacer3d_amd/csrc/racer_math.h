@@ -1,0 +1,244 @@
+/*
+ * Float32 elementary functions and the Philox4x32-10 counter RNG, written once
+ * and compiled unchanged for the host (gcc/g++) and for gfx950 (hipcc), so the
+ * CPU oracle and the HIP kernels produce bit-identical results.
+ *
+ * Build contract for bit-identical results on both sides:
+ *   - no floating-point contraction (-ffp-contract=off), no fast-math;
+ *   - IEEE float32 +, -, *, / and sqrtf (correctly rounded on x86-64 SSE and on
+ *     gfx950 with hipcc's default correctly-rounded divide/sqrt);
+ *   - every function below uses only those operations, floorf/fabsf and integer
+ *     bit manipulation.
+ *
+ * Methods (all from the mathematics, no library code):
+ *   sin/cos : Cody-Waite reduction by pi/2 (pi/2 split into 12-bit, 12-bit and
+ *             24-bit pieces, so k*piece is exact for |k| < 2^12), then the
+ *             Taylor series of sin to x^9 and of cos to x^10 on [-pi/4, pi/4]
+ *             (truncation error < 3e-9 relative), quadrant by k mod 4.
+ *   exp     : Cody-Waite reduction by ln 2 (same 3-piece split), Taylor series
+ *             of e^r to r^8 on |r| <= ln2/2, scaling by 2^k through the
+ *             exponent field in two factors (covers the subnormal range).
+ *   log     : x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
+ *             s = (m-1)/(m+1), odd series to s^13 (|s| <= 0.1716).
+ *   tanh    : odd Taylor series (tangent numbers) to x^19 for |x| < 0.55,
+ *             1 - 2/(e^{2|x|}+1) above, +-1 beyond 9.1.
+ *   philox  : Philox4x32-10 of Salmon, Moraes, Dror and Shaw, "Parallel random
+ *             numbers: as easy as 1, 2, 3" (SC'11): multipliers 0xD2511F53 and
+ *             0xCD9E8D57, Weyl key increments 0x9E3779B9 and 0xBB67AE85.
+ *   normal  : Box-Muller on two open-interval uniforms.
+ */
+#ifndef RACER_MATH_H
+#define RACER_MATH_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define RACER_FN __host__ __device__ static inline
+#else
+#include <math.h>
+#include <string.h>
+#define RACER_FN static inline
+#endif
+
+RACER_FN uint32_t racer_f2u(float f) {
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    return u;
+}
+
+RACER_FN float racer_u2f(uint32_t u) {
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+/* Comparisons written out so the -0/NaN behaviour is the same everywhere. */
+RACER_FN float racer_min(float a, float b) { return b < a ? b : a; }
+RACER_FN float racer_max(float a, float b) { return a < b ? b : a; }
+RACER_FN float racer_clamp(float x, float lo, float hi) { return racer_min(racer_max(x, lo), hi); }
+RACER_FN float racer_abs(float x) { return racer_u2f(racer_f2u(x) & 0x7fffffffu); }
+
+/* Round half away from zero to an integer-valued float (used for k in reductions). */
+RACER_FN float racer_rint(float x) {
+    return x >= 0.0f ? floorf(x + 0.5f) : -floorf(0.5f - x);
+}
+
+/* pi/2 = PIO2_1 + PIO2_2 + PIO2_3 (12 + 12 + 24 significant bits). */
+#define RACER_PIO2_1 0x1.922p+0f
+#define RACER_PIO2_2 (-0x1.2aep-18f)
+#define RACER_PIO2_3 (-0x1.de973ep-31f)
+#define RACER_2_OVER_PI 0x1.45f306p-1f
+#define RACER_PI 0x1.921fb6p+1f
+#define RACER_2PI 0x1.921fb6p+2f
+
+/* sin and cos of |r| <= pi/4 (plus rounding slack). */
+RACER_FN float racer_sin_kernel(float r) {
+    float r2 = r * r;
+    float p = (float)(1.0 / 362880.0);
+    p = p * r2 + (float)(-1.0 / 5040.0);
+    p = p * r2 + (float)(1.0 / 120.0);
+    p = p * r2 + (float)(-1.0 / 6.0);
+    return r + r * (r2 * p);
+}
+
+RACER_FN float racer_cos_kernel(float r) {
+    float r2 = r * r;
+    float p = (float)(-1.0 / 3628800.0);
+    p = p * r2 + (float)(1.0 / 40320.0);
+    p = p * r2 + (float)(-1.0 / 720.0);
+    p = p * r2 + (float)(1.0 / 24.0);
+    p = p * r2 + -0.5f;
+    return 1.0f + r2 * p;
+}
+
+/* Reduce x to r in [-pi/4, pi/4] with x = k pi/2 + r; returns k mod 4. */
+RACER_FN int racer_reduce_pio2(float x, float* r) {
+    float k = racer_rint(x * RACER_2_OVER_PI);
+    *r = ((x - k * RACER_PIO2_1) - k * RACER_PIO2_2) - k * RACER_PIO2_3;
+    return ((int)k) & 3;
+}
+
+RACER_FN float racer_sin(float x) {
+    float r;
+    int q = racer_reduce_pio2(x, &r);
+    float v = (q & 1) ? racer_cos_kernel(r) : racer_sin_kernel(r);
+    return (q & 2) ? -v : v;
+}
+
+RACER_FN float racer_cos(float x) {
+    float r;
+    int q = racer_reduce_pio2(x, &r);
+    float v = (q & 1) ? racer_sin_kernel(r) : racer_cos_kernel(r);
+    return ((q + 1) & 2) ? -v : v;
+}
+
+/* ln 2 = LN2_1 + LN2_2 + LN2_3 (12 + 12 + 24 significant bits). */
+#define RACER_LN2_1 0x1.62ep-1f
+#define RACER_LN2_2 0x1.0cp-15f
+#define RACER_LN2_3 (-0x1.05c61p-29f)
+#define RACER_INV_LN2 0x1.715476p+0f
+
+RACER_FN float racer_exp(float x) {
+    if (!(x == x)) return x;
+    if (x > 88.7228394f) return racer_u2f(0x7f800000u);
+    if (x < -103.972084f) return 0.0f;
+    float k = racer_rint(x * RACER_INV_LN2);
+    float r = ((x - k * RACER_LN2_1) - k * RACER_LN2_2) - k * RACER_LN2_3;
+    float p = (float)(1.0 / 40320.0);
+    p = p * r + (float)(1.0 / 5040.0);
+    p = p * r + (float)(1.0 / 720.0);
+    p = p * r + (float)(1.0 / 120.0);
+    p = p * r + (float)(1.0 / 24.0);
+    p = p * r + (float)(1.0 / 6.0);
+    p = p * r + 0.5f;
+    float e = 1.0f + (r + r * (r * p));
+    int ki = (int)k;
+    /* 2^k as two exact factors so k down to -150 stays representable. */
+    int k1 = ki / 2;
+    int k2 = ki - k1;
+    return (e * racer_u2f((uint32_t)(k1 + 127) << 23)) * racer_u2f((uint32_t)(k2 + 127) << 23);
+}
+
+RACER_FN float racer_log(float x) {
+    if (!(x > 0.0f)) {
+        if (x == 0.0f) return racer_u2f(0xff800000u);
+        return racer_u2f(0x7fc00000u);
+    }
+    if (x == racer_u2f(0x7f800000u)) return x;
+    uint32_t u = racer_f2u(x);
+    int e = 0;
+    if (u < 0x00800000u) { /* subnormal: scale by 2^24 first */
+        u = racer_f2u(x * 0x1p24f);
+        e = -24;
+    }
+    e += (int)(u >> 23) - 127;
+    float m = racer_u2f((u & 0x007fffffu) | 0x3f800000u); /* [1, 2) */
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float p = (float)(1.0 / 13.0);
+    p = p * s2 + (float)(1.0 / 11.0);
+    p = p * s2 + (float)(1.0 / 9.0);
+    p = p * s2 + (float)(1.0 / 7.0);
+    p = p * s2 + (float)(1.0 / 5.0);
+    p = p * s2 + (float)(1.0 / 3.0);
+    float logm = 2.0f * s + 2.0f * s * (s2 * p);
+    float fe = (float)e;
+    return (fe * RACER_LN2_3 + (fe * RACER_LN2_2 + logm)) + fe * RACER_LN2_1;
+}
+
+RACER_FN float racer_tanh(float x) {
+    float a = racer_abs(x);
+    float y;
+    if (a < 0.55f) {
+        float x2 = a * a;
+        float p = (float)(-443861162.0 / 1856156927625.0);
+        p = p * x2 + (float)(6404582.0 / 10854718875.0);
+        p = p * x2 + (float)(-929569.0 / 638512875.0);
+        p = p * x2 + (float)(21844.0 / 6081075.0);
+        p = p * x2 + (float)(-1382.0 / 155925.0);
+        p = p * x2 + (float)(62.0 / 2835.0);
+        p = p * x2 + (float)(-17.0 / 315.0);
+        p = p * x2 + (float)(2.0 / 15.0);
+        p = p * x2 + (float)(-1.0 / 3.0);
+        y = a + a * (x2 * p);
+    } else if (a < 9.1f) {
+        y = 1.0f - 2.0f / (racer_exp(2.0f * a) + 1.0f);
+    } else {
+        y = (a == a) ? 1.0f : a;
+    }
+    return x < 0.0f ? -y : y;
+}
+
+/* ---- Philox4x32-10 ---- */
+typedef struct {
+    uint32_t v[4];
+} racer_u32x4;
+
+RACER_FN racer_u32x4 racer_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+    for (int round = 0; round < 10; ++round) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    racer_u32x4 out = {{c0, c1, c2, c3}};
+    return out;
+}
+
+/* One block of the stream (seed, subsequence, offset): key = seed, counter =
+ * (offset lo, offset hi, subsequence lo, subsequence hi). */
+RACER_FN racer_u32x4 racer_random(uint64_t seed, uint64_t subsequence, uint64_t offset) {
+    return racer_philox((uint32_t)offset, (uint32_t)(offset >> 32), (uint32_t)subsequence,
+                        (uint32_t)(subsequence >> 32), (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+/* Uniform in [0, 1) from the top 24 bits. */
+RACER_FN float racer_uniform(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+
+/* Uniform in (0, 1), never 0 or 1. */
+RACER_FN float racer_uniform_open(uint32_t x) { return ((float)(x >> 8) + 0.5f) * 0x1p-24f; }
+
+/* Four standard normals from one Philox block by Box-Muller. */
+RACER_FN void racer_normal4(racer_u32x4 r, float out[4]) {
+    for (int h = 0; h < 2; ++h) {
+        float u = racer_uniform_open(r.v[2 * h]);
+        float w = racer_uniform_open(r.v[2 * h + 1]);
+        float radius = sqrtf(-2.0f * racer_log(u));
+        float angle = RACER_2PI * w;
+        out[2 * h] = radius * racer_cos(angle);
+        out[2 * h + 1] = radius * racer_sin(angle);
+    }
+}
+
+#endif

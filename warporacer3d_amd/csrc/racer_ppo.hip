@@ -1,0 +1,634 @@
+// Policy, activations and the PPO trainer on gfx950 (racer/ppo.mojo), the
+// .npy checkpoint format (racer/ppo.mojo:341-355, racer/device.mojo:137-143)
+// and the Runner step/benchmark loop (main.mojo:183-235).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include "racer_internal.h"
+
+int racer_fill(racer_device* dev, float* p, size_t n, float v);
+
+namespace {
+
+// Policy.__init__'s init kernel (racer/ppo.mojo:74-89), including the
+// reference's absolute-index bias-column tests for W1 and W2.
+__global__ void k_policy_init(float* theta, int64_t P, int64_t IN, int64_t OBS, int64_t H, int64_t HA, int64_t W1,
+                              int64_t W2, int64_t LOGSTD, int seed) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    float scale = 0.0f;
+    if (i < W1) {
+        scale = (i % IN < OBS) ? sqrtf(2.0f / (float)OBS) : 0.0f;
+    } else if (i < W2) {
+        scale = (i % HA < H) ? sqrtf(1.0f / (float)H) : 0.0f;
+    } else if (i < LOGSTD) {
+        scale = (i % HA < H) ? (((i - W2) / HA < 2) ? 0.00125f : 0.125f) : 0.0f;
+    } else {
+        theta[i] = -0.5f;
+        return;
+    }
+    float z[4];
+    racer_normal4(racer_random((uint64_t)(int64_t)seed, (uint64_t)i, 0), z);
+    theta[i] = scale * z[0];
+}
+
+// Policy.transpose (racer/ppo.mojo:92-103): w1t[k, j] = W1[j, k], w2t[k, j] = W2[j, k].
+__global__ void k_transpose(const float* theta, int64_t W1, int64_t W2, int H, int HA, float* w1t, float* w2t) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)HA * H) return;
+    int k = (int)(idx / H), j = (int)(idx % H);
+    w1t[(int64_t)k * H + j] = theta[W1 + (int64_t)j * HA + k];
+    if (j < RACER_OUT) w2t[(int64_t)k * RACER_OUT + j] = theta[W2 + (int64_t)j * HA + k];
+}
+
+// GAE (racer/ppo.mojo:218-230), one car per thread, the running advantage in float64.
+__global__ void k_gae(int n, int rollout, const float* reward, const float* done, const float* value,
+                      const float* last_value, float* advantage) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double next = last_value[i];
+    double running = 0.0;
+    for (int t = rollout - 1; t >= 0; --t) {
+        int64_t k = (int64_t)t * n + i;
+        double alive = done[k] == 0.0f ? 1.0 : 0.0;
+        double delta = (double)reward[k] + (double)RACER_GAMMA * alive * next - (double)value[k];
+        running = delta + (double)RACER_GAMMA * (double)RACER_LAMBDA * alive * running;
+        advantage[k] = (float)running;
+        next = value[k];
+    }
+}
+
+// Deterministic two-pass float64 sums of up to four rows (x, and x^2 when
+// `square`): pass 1 writes one partial per block, pass 2 sums them in order.
+constexpr int SUM_BLOCKS = 1024;
+
+__global__ void __launch_bounds__(256) k_sum_partial(const float* x, int64_t count, int64_t row_stride, int rows,
+                                                     int square, double* partial) {
+    __shared__ double red[256];
+    for (int r = 0; r < rows; ++r) {
+        const float* row = x + r * row_stride;
+        double acc = 0.0, acc2 = 0.0;
+        for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) {
+            double v = row[i];
+            acc += v;
+            acc2 += v * v;
+        }
+        for (int q = 0; q < 1 + square; ++q) {
+            red[threadIdx.x] = q ? acc2 : acc;
+            __syncthreads();
+            for (int s = 128; s > 0; s >>= 1) {
+                if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) partial[((int64_t)(r * (1 + square) + q)) * gridDim.x + blockIdx.x] = red[0];
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void k_sum_final(const double* partial, int blocks, int outputs, double* out, float* out_f32,
+                            int64_t f32_slot) {
+    int o = threadIdx.x;
+    if (o >= outputs) return;
+    double acc = 0.0;
+    for (int b = 0; b < blocks; ++b) acc += partial[(int64_t)o * blocks + b];
+    if (out) out[o] = acc;
+    if (out_f32) out_f32[f32_slot + o] = (float)acc;
+}
+
+// minibatch.gather (racer/ppo.mojo:276-279): x[r, b] = obs of sample s = b MB + m.
+__global__ void k_gather(const float* obs, int64_t n, int64_t IN, int64_t batch, int minibatches, int m, float* x) {
+    int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    int64_t s = b * minibatches + m;
+    const float* src = obs + (s / n) * IN * n + s % n;
+    for (int64_t r = 0; r < IN; ++r) x[r * batch + b] = src[r * n];
+}
+
+// minibatch.loss (racer/ppo.mojo:281-304): clipped surrogate and Huber value
+// derivatives, d log std per sample, and the per-sample KL estimate.
+__global__ void k_loss(int64_t batch, int64_t n, int minibatches, int m, const float* actions, const float* o,
+                       const float* logp, const float* value, const float* advantage, const float* log_std,
+                       float adv_mean, float adv_scale, float* d3) {
+    int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    int64_t s = b * minibatches + m;
+    int64_t t = s / n, i = s % n;
+    float new_logp = 0.0f;
+    float error[2], invvar[2];
+    for (int a = 0; a < 2; ++a) {
+        float ls = log_std[a];
+        error[a] = actions[(t * 2 + a) * n + i] - o[a * batch + b];
+        invvar[a] = racer_exp(-2.0f * ls);
+        new_logp = new_logp - ((((0.5f * error[a]) * error[a]) * invvar[a] + ls) + RACER_HALF_LOG_2PI);
+    }
+    float logratio = new_logp - logp[s];
+    float ratio = racer_exp(logratio);
+    float a_hat = (advantage[s] - adv_mean) * adv_scale;
+    float dlogp = (-a_hat * ratio) / (float)batch;
+    if ((a_hat >= 0.0f && ratio > 1.0f + RACER_CLIP) || (a_hat < 0.0f && ratio < 1.0f - RACER_CLIP)) dlogp = 0.0f;
+    for (int a = 0; a < 2; ++a) {
+        d3[a * batch + b] = (dlogp * error[a]) * invvar[a];
+        d3[(3 + a) * batch + b] = dlogp * ((error[a] * error[a]) * invvar[a] - 1.0f);
+    }
+    d3[2 * batch + b] = (0.5f * racer_clamp((o[2 * batch + b] - value[s]) - advantage[s], -1.0f, 1.0f)) / (float)batch;
+    d3[5 * batch + b] = (ratio - 1.0f) - logratio;
+}
+
+// Trainer.adam (racer/ppo.mojo:325-336). Bias corrections come from the host.
+__global__ void k_adam(float* theta, int64_t P, int64_t logstd_at, const float* grad, float* first, float* second,
+                       const float* stats, float c1, float c2) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    float clip = racer_min(1.0f, RACER_MAX_NORM / (sqrtf(stats[RACER_STAT_NORM]) + 1e-8f));
+    float g = grad[i] * clip;
+    float m = 0.9f * first[i] + 0.1f * g;
+    float v = 0.999f * second[i] + (0.001f * g) * g;
+    first[i] = m;
+    second[i] = v;
+    float m_hat = m / c1;
+    float v_hat = v / c2;
+    float value = theta[i] - (RACER_LR * m_hat) / (sqrtf(v_hat) + 1e-5f);
+    theta[i] = i >= logstd_at ? racer_min(racer_max(value, -2.0f), 0.0f) : value;
+}
+
+__global__ void k_store_norm(const double* pair, float* stats) { stats[RACER_STAT_NORM] = (float)pair[1]; }
+
+int row_sums(racer_device* dev, const float* x, int64_t count, int64_t row_stride, int rows, int square,
+             double* out64, float* out32, int64_t slot) {
+    int outputs = rows * (1 + square);
+    int blocks = (int)std::min<int64_t>(SUM_BLOCKS, std::max<int64_t>(1, (count + 255) / 256));
+    if (racer_ensure_scratch(dev, (size_t)outputs * blocks * 2)) return 1;
+    double* partial = reinterpret_cast<double*>(dev->scratch);
+    {
+        racer_timed timer(dev, "sum");
+        hipLaunchKernelGGL(k_sum_partial, dim3(blocks), dim3(256), 0, dev->stream, x, count, row_stride, rows, square,
+                           partial);
+        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, dev->stream, partial, blocks, outputs, out64, out32,
+                           slot);
+    }
+    return racer_launch_check("row sums");
+}
+
+float bias_correction(float beta, int step) { return (float)(1.0 - std::pow((double)beta, (double)step)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- activations
+extern "C" int racer_acts_create(racer_device* dev, int cols, const racer_config* cfg, racer_acts** out) {
+    racer_clear_error();
+    *out = nullptr;
+    racer_config def;
+    racer_config_default(&def);
+    if (!cfg) cfg = &def;
+    RACER_REQUIRE(cols > 0, "activations need at least one column");
+    auto* a = new racer_acts();
+    a->dev = dev;
+    a->cols = cols;
+    a->hidden = cfg->hidden;
+    size_t HA = (size_t)cfg->hidden + 1;
+    if (racer_alloc(dev, HA * cols, 1.0f, &a->h1) || racer_alloc(dev, HA * cols, 1.0f, &a->h2) ||
+        racer_alloc(dev, (size_t)RACER_OUT * cols, 0.0f, &a->o)) {
+        racer_acts_destroy(a);
+        return 1;
+    }
+    *out = a;
+    return 0;
+}
+
+extern "C" void racer_acts_destroy(racer_acts* a) {
+    if (!a) return;
+    racer_free(a->h1);
+    racer_free(a->h2);
+    racer_free(a->o);
+    delete a;
+}
+
+extern "C" racer_buf* racer_acts_h1(racer_acts* a) { return a->h1; }
+extern "C" racer_buf* racer_acts_h2(racer_acts* a) { return a->h2; }
+extern "C" racer_buf* racer_acts_o(racer_acts* a) { return a->o; }
+
+// ---------------------------------------------------------------- policy
+extern "C" int racer_policy_create(racer_device* dev, int seed, const racer_config* cfg, racer_policy** out) {
+    racer_clear_error();
+    *out = nullptr;
+    racer_config def;
+    racer_config_default(&def);
+    if (!cfg) cfg = &def;
+    RACER_REQUIRE(cfg->beams >= 2 && cfg->hidden >= 1, "bad network shape");
+    auto* p = new racer_policy();
+    p->dev = dev;
+    p->beams = cfg->beams;
+    p->hidden = cfg->hidden;
+    p->precision = cfg->precision;
+    p->IN = cfg->beams + 5;
+    p->HA = cfg->hidden + 1;
+    int64_t H = cfg->hidden;
+    p->W1 = H * p->IN;
+    p->W2 = p->W1 + H * p->HA;
+    p->LOGSTD = p->W2 + RACER_OUT * p->HA;
+    p->P = p->LOGSTD + 2;
+    if (racer_alloc(dev, (size_t)p->P, 0.0f, &p->theta) || racer_alloc(dev, (size_t)(p->HA * H), 0.0f, &p->w1t) ||
+        racer_alloc(dev, (size_t)(p->HA * RACER_OUT), 0.0f, &p->w2t)) {
+        racer_policy_destroy(p);
+        return 1;
+    }
+    p->in_pad = racer_pad8(p->IN);
+    p->ha_pad = racer_pad8(p->HA);
+    RACER_TRY(hipMalloc(&p->theta_bf16, (size_t)(H * (p->in_pad + p->ha_pad)) * sizeof(uint16_t)));
+    hipLaunchKernelGGL(k_policy_init, dim3((unsigned)((p->P + 255) / 256)), dim3(256), 0, dev->stream, p->theta->ptr,
+                       p->P, p->IN, p->IN - 1, H, p->HA, p->W1, p->W2, p->LOGSTD, seed);
+    if (racer_launch_check("policy init")) return 1;
+    if (racer_policy_transpose(p)) return 1;
+    RACER_TRY(hipStreamSynchronize(dev->stream));
+    *out = p;
+    return 0;
+}
+
+extern "C" void racer_policy_destroy(racer_policy* p) {
+    if (!p) return;
+    racer_free(p->theta);
+    racer_free(p->w1t);
+    racer_free(p->w2t);
+    if (p->theta_bf16) hipFree(p->theta_bf16);
+    delete p;
+}
+
+extern "C" racer_buf* racer_policy_theta(racer_policy* p) { return p->theta; }
+extern "C" size_t racer_policy_params(const racer_policy* p) { return (size_t)p->P; }
+
+extern "C" racer_view racer_policy_log_std(racer_policy* p) {
+    racer_view v;
+    v.buf = p->theta;
+    v.offset = (size_t)p->LOGSTD;
+    v.rows = 1;
+    v.cols = 2;
+    return v;
+}
+
+extern "C" int racer_policy_transpose(racer_policy* p) {
+    racer_clear_error();
+    int64_t total = p->HA * p->hidden;
+    {
+        racer_timed timer(p->dev, "transpose");
+        hipLaunchKernelGGL(k_transpose, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, p->dev->stream,
+                           p->theta->ptr, p->W1, p->W2, p->hidden, (int)p->HA, p->w1t->ptr, p->w2t->ptr);
+    }
+    p->bf16_stale = true;
+    return racer_launch_check("transpose");
+}
+
+int racer_policy_forward_raw(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols) {
+    const float* th = p->theta->ptr;
+    int H = p->hidden;
+    if (p->precision == 1) {
+        uint16_t* w0 = p->theta_bf16;
+        uint16_t* w1 = p->theta_bf16 + (int64_t)H * p->in_pad;
+        if (p->bf16_stale) {
+            if (racer_pack_bf16(p->dev, th, p->IN, w0, p->in_pad, H, (int)p->IN) ||
+                racer_pack_bf16(p->dev, th + p->W1, p->HA, w1, p->ha_pad, H, (int)p->HA))
+                return 1;
+            p->bf16_stale = false;
+        }
+        if (racer_gemm_nn_bf16(p->dev, w0, p->in_pad, x, x_cols, a->h1->ptr, cols, H, cols, (int)p->IN,
+                               RACER_EPI_TANH) ||
+            racer_gemm_nn_bf16(p->dev, w1, p->ha_pad, a->h1->ptr, cols, a->h2->ptr, cols, H, cols, (int)p->HA,
+                               RACER_EPI_TANH))
+            return 1;
+    } else {
+        if (racer_gemm_nn_f32(p->dev, th, p->IN, x, x_cols, a->h1->ptr, cols, H, cols, (int)p->IN, RACER_EPI_TANH,
+                              nullptr, 0) ||
+            racer_gemm_nn_f32(p->dev, th + p->W1, p->HA, a->h1->ptr, cols, a->h2->ptr, cols, H, cols, (int)p->HA,
+                              RACER_EPI_TANH, nullptr, 0))
+            return 1;
+    }
+    return racer_gemm_nn_f32(p->dev, th + p->W2, p->HA, a->h2->ptr, cols, a->o->ptr, cols, RACER_OUT, cols,
+                             (int)p->HA, RACER_EPI_NONE, nullptr, 0);
+}
+
+extern "C" int racer_policy_forward(racer_policy* p, racer_view x, racer_acts* a) {
+    racer_clear_error();
+    RACER_REQUIRE(a && a->hidden == p->hidden, "activations do not match the policy's hidden width");
+    if (racer_check_view(x, p->IN, a->cols, "policy input")) return 1;
+    return racer_policy_forward_raw(p, view_ptr(x), x.cols, a, a->cols);
+}
+
+static int backward_raw(racer_policy* p, const float* x, racer_acts* a, const float* d3, float* d2, float* d1,
+                        float* grad, int64_t cols) {
+    racer_device* dev = p->dev;
+    int H = p->hidden;
+    int64_t HA = p->HA, IN = p->IN;
+    const float* h1 = a->h1->ptr;
+    const float* h2 = a->h2->ptr;
+    // through (racer/ppo.mojo:125-138): d2 = (1 - h2^2) (w2t d3); d1 = (1 - h1^2) (w1t d2[:H])
+    if (racer_gemm_nn_f32(dev, p->w2t->ptr, RACER_OUT, d3, cols, d2, cols, (int)HA, (int)cols, RACER_OUT,
+                          RACER_EPI_MASK, h2, cols) ||
+        racer_gemm_nn_f32(dev, p->w1t->ptr, H, d2, cols, d1, cols, (int)HA, (int)cols, H, RACER_EPI_MASK, h1, cols))
+        return 1;
+    // weight gradients (racer/ppo.mojo:121-123)
+    if (p->precision == 1) {
+        if (racer_gemm_nt_bf16(dev, d1, cols, x, cols, grad, IN, H, (int)IN, (int)cols) ||
+            racer_gemm_nt_bf16(dev, d2, cols, h1, cols, grad + p->W1, HA, H, (int)HA, (int)cols))
+            return 1;
+    } else {
+        if (racer_gemm_nt_f32(dev, d1, cols, x, cols, grad, IN, H, (int)IN, (int)cols) ||
+            racer_gemm_nt_f32(dev, d2, cols, h1, cols, grad + p->W1, HA, H, (int)HA, (int)cols))
+            return 1;
+    }
+    return racer_gemm_nt_f32(dev, d3, cols, h2, cols, grad + p->W2, HA, RACER_OUT, (int)HA, (int)cols);
+}
+
+extern "C" int racer_policy_backward(racer_policy* p, racer_view x, racer_acts* a, racer_view d3, racer_buf* d2,
+                                     racer_buf* d1, racer_buf* grad) {
+    racer_clear_error();
+    int64_t cols = a->cols;
+    if (racer_check_view(x, p->IN, cols, "policy input") || racer_check_view(d3, RACER_OUT, cols, "d3")) return 1;
+    RACER_REQUIRE(d2 && d2->len >= (size_t)(p->HA * cols) && d1 && d1->len >= (size_t)(p->HA * cols) && grad &&
+                      grad->len >= (size_t)p->P,
+                  "backward buffers are too small");
+    return backward_raw(p, view_ptr(x), a, view_ptr(d3), d2->ptr, d1->ptr, grad->ptr, cols);
+}
+
+// ---------------------------------------------------------------- trainer
+extern "C" int racer_trainer_create(racer_device* dev, int n, const racer_config* cfg, racer_trainer** out) {
+    racer_clear_error();
+    *out = nullptr;
+    racer_config def;
+    racer_config_default(&def);
+    if (!cfg) cfg = &def;
+    RACER_REQUIRE(n > 0 && cfg->rollout > 0 && cfg->minibatches > 0 && cfg->epochs > 0, "bad trainer shape");
+    RACER_REQUIRE(((int64_t)n * cfg->rollout) % cfg->minibatches == 0,
+                  "cars x rollout must be a multiple of the minibatch count");
+    auto* tr = new racer_trainer();
+    tr->dev = dev;
+    tr->n = n;
+    tr->beams = cfg->beams;
+    tr->hidden = cfg->hidden;
+    tr->rollout = cfg->rollout;
+    tr->epochs = cfg->epochs;
+    tr->minibatches = cfg->minibatches;
+    tr->precision = cfg->precision;
+    tr->IN = cfg->beams + 5;
+    tr->HA = cfg->hidden + 1;
+    tr->batch = (int64_t)n * cfg->rollout / cfg->minibatches;
+    tr->P = cfg->hidden * tr->IN + cfg->hidden * tr->HA + RACER_OUT * tr->HA + 2;
+    size_t R = cfg->rollout, N = n, B = tr->batch;
+    if (racer_alloc(dev, (R + 1) * tr->IN * N, 0.0f, &tr->obs) || racer_alloc(dev, R * 2 * N, 0.0f, &tr->actions) ||
+        racer_alloc(dev, R * N, 0.0f, &tr->logp) || racer_alloc(dev, R * N, 0.0f, &tr->value) ||
+        racer_alloc(dev, R * N, 0.0f, &tr->reward) || racer_alloc(dev, R * N, 0.0f, &tr->done) ||
+        racer_alloc(dev, R * N, 0.0f, &tr->advantage) || racer_alloc(dev, tr->IN * B, 0.0f, &tr->x) ||
+        racer_alloc(dev, 6 * B, 0.0f, &tr->d3) || racer_alloc(dev, tr->HA * B, 0.0f, &tr->d2) ||
+        racer_alloc(dev, tr->HA * B, 0.0f, &tr->d1) || racer_alloc(dev, tr->P, 0.0f, &tr->grad) ||
+        racer_alloc(dev, tr->P, 0.0f, &tr->first) || racer_alloc(dev, tr->P, 0.0f, &tr->second) ||
+        racer_alloc(dev, RACER_STATS, 0.0f, &tr->stats) || racer_acts_create(dev, (int)B, cfg, &tr->acts)) {
+        racer_trainer_destroy(tr);
+        return 1;
+    }
+    RACER_TRY(hipMalloc(&tr->dstats, 8 * sizeof(double)));
+    *out = tr;
+    return 0;
+}
+
+extern "C" void racer_trainer_destroy(racer_trainer* tr) {
+    if (!tr) return;
+    racer_buf* bufs[] = {tr->obs, tr->actions, tr->logp, tr->value, tr->reward, tr->done, tr->advantage, tr->x,
+                         tr->d3, tr->d2, tr->d1, tr->grad, tr->first, tr->second, tr->stats};
+    for (auto* b : bufs) racer_free(b);
+    racer_acts_destroy(tr->acts);
+    if (tr->dstats) hipFree(tr->dstats);
+    delete tr;
+}
+
+static int check_compatible(racer_trainer* tr, racer_policy* p) {
+    RACER_REQUIRE(p->beams == tr->beams && p->hidden == tr->hidden, "trainer and policy shapes differ");
+    return 0;
+}
+
+extern "C" int racer_trainer_rollout(racer_trainer* tr, racer_sim* sim, racer_policy* p, racer_acts* acts,
+                                     int iteration, float* mean_reward) {
+    racer_clear_error();
+    if (check_compatible(tr, p)) return 1;
+    RACER_REQUIRE(sim->n == tr->n && sim->beams == tr->beams, "trainer and sim shapes differ");
+    RACER_REQUIRE(acts->cols == tr->n && acts->hidden == tr->hidden, "activations must have one column per car");
+    racer_device* dev = tr->dev;
+    int64_t n = tr->n, IN = tr->IN, R = tr->rollout;
+    // racer/ppo.mojo:190-210
+    RACER_TRY(hipMemcpyAsync(tr->obs->ptr, sim->obs->ptr, IN * n * sizeof(float), hipMemcpyDeviceToDevice,
+                             dev->stream));
+    for (int64_t t = 0; t < R; ++t) {
+        float* obs_t = tr->obs->ptr + t * IN * n;
+        if (racer_policy_forward_raw(p, obs_t, n, acts, (int)n)) return 1;
+        if (racer_launch_sample(sim, acts->o->ptr, n, p->theta->ptr + p->LOGSTD, tr->actions->ptr + t * 2 * n,
+                                tr->logp->ptr + t * n, tr->value->ptr + t * n, 0, (int64_t)iteration * R + t))
+            return 1;
+        if (racer_launch_step(sim, tr->actions->ptr + t * 2 * n, tr->reward->ptr + t * n, tr->done->ptr + t * n,
+                              tr->obs->ptr + (t + 1) * IN * n, n))
+            return 1;
+    }
+    RACER_TRY(hipMemcpyAsync(sim->obs->ptr, tr->obs->ptr + R * IN * n, IN * n * sizeof(float),
+                             hipMemcpyDeviceToDevice, dev->stream));
+    if (racer_policy_forward_raw(p, tr->obs->ptr + R * IN * n, n, acts, (int)n)) return 1;
+    {
+        racer_timed timer(dev, "gae");
+        hipLaunchKernelGGL(k_gae, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, dev->stream, (int)n, (int)R,
+                           tr->reward->ptr, tr->done->ptr, tr->value->ptr, acts->o->ptr + 2 * n, tr->advantage->ptr);
+    }
+    if (racer_launch_check("gae")) return 1;
+    if (row_sums(dev, tr->reward->ptr, R * n, 0, 1, 0, tr->dstats + 2, tr->stats->ptr, RACER_STAT_REWARD)) return 1;
+    double total = 0.0;
+    RACER_TRY(hipMemcpyAsync(&total, tr->dstats + 2, sizeof(double), hipMemcpyDeviceToHost, dev->stream));
+    RACER_TRY(hipStreamSynchronize(dev->stream));
+    *mean_reward = (float)(total / (double)(R * n));
+    return 0;
+}
+
+extern "C" int racer_trainer_minibatch(racer_trainer* tr, racer_policy* p, int m, float adv_mean, float adv_scale) {
+    racer_clear_error();
+    if (check_compatible(tr, p)) return 1;
+    RACER_REQUIRE(m >= 0 && m < tr->minibatches, "minibatch index out of range");
+    racer_device* dev = tr->dev;
+    int64_t B = tr->batch, n = tr->n;
+    unsigned blocks = (unsigned)((B + 255) / 256);
+    {
+        racer_timed timer(dev, "gather");
+        hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, dev->stream, tr->obs->ptr, n, tr->IN, B,
+                           tr->minibatches, m, tr->x->ptr);
+    }
+    if (racer_launch_check("gather")) return 1;
+    if (racer_policy_forward_raw(p, tr->x->ptr, B, tr->acts, (int)B)) return 1;
+    {
+        racer_timed timer(dev, "loss");
+        hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, dev->stream, B, n, tr->minibatches, m,
+                           tr->actions->ptr, tr->acts->o->ptr, tr->logp->ptr, tr->value->ptr, tr->advantage->ptr,
+                           p->theta->ptr + p->LOGSTD, adv_mean, adv_scale, tr->d3->ptr);
+    }
+    if (racer_launch_check("loss")) return 1;
+    if (backward_raw(p, tr->x->ptr, tr->acts, tr->d3->ptr, tr->d2->ptr, tr->d1->ptr, tr->grad->ptr, B)) return 1;
+    // log-std gradient: row sums of d3[3..4] (racer/ppo.mojo:313)
+    if (row_sums(dev, tr->d3->ptr + 3 * B, B, B, 2, 0, nullptr, tr->grad->ptr, p->LOGSTD)) return 1;
+    // squared gradient norm (racer/ppo.mojo:314-315)
+    if (row_sums(dev, tr->grad->ptr, tr->P, 0, 1, 1, tr->dstats + 4, nullptr, 0)) return 1;
+    // stats[NORM] = float(sum of squares), the second of the (sum, sum of squares) pair
+    hipLaunchKernelGGL(k_store_norm, dim3(1), dim3(1), 0, dev->stream, (const double*)(tr->dstats + 4),
+                       tr->stats->ptr);
+    return racer_launch_check("norm");
+}
+
+extern "C" int racer_trainer_adam(racer_trainer* tr, racer_policy* p) {
+    racer_clear_error();
+    if (check_compatible(tr, p)) return 1;
+    racer_device* dev = tr->dev;
+    float c1 = bias_correction(0.9f, tr->step), c2 = bias_correction(0.999f, tr->step);
+    {
+        racer_timed timer(dev, "adam");
+        hipLaunchKernelGGL(k_adam, dim3((unsigned)((tr->P + 255) / 256)), dim3(256), 0, dev->stream, p->theta->ptr,
+                           tr->P, p->LOGSTD, tr->grad->ptr, tr->first->ptr, tr->second->ptr, tr->stats->ptr, c1, c2);
+    }
+    if (racer_launch_check("adam")) return 1;
+    return racer_policy_transpose(p);
+}
+
+extern "C" int racer_trainer_update(racer_trainer* tr, racer_policy* p, float* kl) {
+    racer_clear_error();
+    if (check_compatible(tr, p)) return 1;
+    racer_device* dev = tr->dev;
+    int64_t samples = (int64_t)tr->rollout * tr->n;
+    // advantage moments in float64 (racer/ppo.mojo:242-250)
+    if (row_sums(dev, tr->advantage->ptr, samples, 0, 1, 1, tr->dstats, nullptr, 0)) return 1;
+    double mom[2];
+    RACER_TRY(hipMemcpyAsync(mom, tr->dstats, sizeof(mom), hipMemcpyDeviceToHost, dev->stream));
+    RACER_TRY(hipStreamSynchronize(dev->stream));
+    double mean = mom[0] / (double)samples;
+    double var = mom[1] / (double)samples - mean * mean;
+    float adv_mean = (float)mean;
+    float adv_scale = (float)(1.0 / std::sqrt(std::max(var, 1e-8)));
+    for (int epoch = 0; epoch < tr->epochs; ++epoch)
+        for (int m = 0; m < tr->minibatches; ++m) {
+            if (racer_trainer_minibatch(tr, p, (m + epoch) % tr->minibatches, adv_mean, adv_scale)) return 1;
+            tr->step += 1;
+            if (racer_trainer_adam(tr, p)) return 1;
+        }
+    // KL of the last minibatch (racer/ppo.mojo:256-257)
+    if (row_sums(dev, tr->d3->ptr + 5 * tr->batch, tr->batch, 0, 1, 0, tr->dstats + 6, tr->stats->ptr, RACER_STAT_KL))
+        return 1;
+    double klsum = 0.0;
+    RACER_TRY(hipMemcpyAsync(&klsum, tr->dstats + 6, sizeof(double), hipMemcpyDeviceToHost, dev->stream));
+    RACER_TRY(hipStreamSynchronize(dev->stream));
+    *kl = (float)(klsum / (double)tr->batch);
+    return 0;
+}
+
+extern "C" int racer_trainer_step_count(const racer_trainer* tr) { return tr->step; }
+extern "C" racer_buf* racer_trainer_obs(racer_trainer* tr) { return tr->obs; }
+extern "C" racer_buf* racer_trainer_actions(racer_trainer* tr) { return tr->actions; }
+extern "C" racer_buf* racer_trainer_logp(racer_trainer* tr) { return tr->logp; }
+extern "C" racer_buf* racer_trainer_value(racer_trainer* tr) { return tr->value; }
+extern "C" racer_buf* racer_trainer_reward(racer_trainer* tr) { return tr->reward; }
+extern "C" racer_buf* racer_trainer_done(racer_trainer* tr) { return tr->done; }
+extern "C" racer_buf* racer_trainer_advantage(racer_trainer* tr) { return tr->advantage; }
+extern "C" racer_buf* racer_trainer_grad(racer_trainer* tr) { return tr->grad; }
+extern "C" racer_buf* racer_trainer_d3(racer_trainer* tr) { return tr->d3; }
+extern "C" racer_buf* racer_trainer_stats(racer_trainer* tr) { return tr->stats; }
+extern "C" racer_acts* racer_trainer_acts(racer_trainer* tr) { return tr->acts; }
+
+// ---------------------------------------------------------------- checkpoints (.npy, float32, 1-D)
+extern "C" int racer_save(racer_policy* p, const char* path, int iteration) {
+    racer_clear_error();
+    std::vector<float> values((size_t)p->P + 1);
+    values[0] = (float)iteration;
+    if (racer_read(p->theta, values.data() + 1)) return 1;
+    std::string header = "{'descr': '<f4', 'fortran_order': False, 'shape': (" + std::to_string(values.size()) +
+                         ",), }";
+    size_t total = 10 + header.size() + 1;
+    size_t pad = (64 - total % 64) % 64;
+    header += std::string(pad, ' ') + "\n";
+    std::ofstream f(path, std::ios::binary);
+    RACER_REQUIRE(f.good(), std::string("cannot write ") + path);
+    const char magic[8] = {'\x93', 'N', 'U', 'M', 'P', 'Y', 1, 0};
+    f.write(magic, 8);
+    uint16_t hl = (uint16_t)header.size();
+    f.write(reinterpret_cast<const char*>(&hl), 2);
+    f.write(header.data(), (std::streamsize)header.size());
+    f.write(reinterpret_cast<const char*>(values.data()), (std::streamsize)(values.size() * sizeof(float)));
+    RACER_REQUIRE(f.good(), std::string("cannot write ") + path);
+    return 0;
+}
+
+extern "C" int racer_load(racer_policy* p, const char* path, int* iteration) {
+    racer_clear_error();
+    std::ifstream f(path, std::ios::binary);
+    RACER_REQUIRE(f.good(), std::string("cannot read ") + path);
+    char magic[8];
+    f.read(magic, 8);
+    RACER_REQUIRE(f.good() && std::memcmp(magic, "\x93NUMPY", 6) == 0, "not a .npy file");
+    uint32_t hl = 0;
+    if (magic[6] == 1) {
+        uint16_t h16 = 0;
+        f.read(reinterpret_cast<char*>(&h16), 2);
+        hl = h16;
+    } else {
+        f.read(reinterpret_cast<char*>(&hl), 4);
+    }
+    std::string header(hl, ' ');
+    f.read(&header[0], hl);
+    RACER_REQUIRE(f.good(), "truncated .npy header");
+    bool f4 = header.find("'<f4'") != std::string::npos, f8 = header.find("'<f8'") != std::string::npos;
+    RACER_REQUIRE(f4 || f8, "checkpoint must hold float32 or float64 values");
+    RACER_REQUIRE(header.find("'fortran_order': False") != std::string::npos, "checkpoint must be C-ordered");
+    size_t lp = header.find("'shape': ("), rp = header.find(')', lp);
+    RACER_REQUIRE(lp != std::string::npos && rp != std::string::npos, "checkpoint has no shape");
+    std::string dims = header.substr(lp + 10, rp - lp - 10);
+    size_t count = 1;
+    std::stringstream ss(dims);
+    std::string item;
+    while (std::getline(ss, item, ','))
+        if (item.find_first_not_of(' ') != std::string::npos) count *= (size_t)std::stoull(item);
+    RACER_REQUIRE(count == (size_t)p->P + 1, "checkpoint does not match the network size");
+    std::vector<float> values(count);
+    if (f4) {
+        f.read(reinterpret_cast<char*>(values.data()), (std::streamsize)(count * 4));
+    } else {
+        std::vector<double> tmp(count);
+        f.read(reinterpret_cast<char*>(tmp.data()), (std::streamsize)(count * 8));
+        for (size_t i = 0; i < count; ++i) values[i] = (float)tmp[i];
+    }
+    RACER_REQUIRE(f.good(), "truncated checkpoint");
+    if (racer_write(p->theta, values.data() + 1)) return 1;
+    if (racer_policy_transpose(p)) return 1;
+    if (iteration) *iteration = (int)values[0];
+    return 0;
+}
+
+// ---------------------------------------------------------------- runner
+extern "C" int racer_runner_step(racer_sim* sim, racer_policy* p, racer_acts* acts, racer_buf* scratch,
+                                 int deterministic, int64_t t) {
+    racer_clear_error();
+    int64_t n = sim->n;
+    RACER_REQUIRE(p->beams == sim->beams && acts->cols == sim->n && acts->hidden == p->hidden,
+                  "runner shapes differ");
+    RACER_REQUIRE(scratch && scratch->len >= (size_t)(2 * n), "scratch must hold 2 x cars floats");
+    if (racer_policy_forward_raw(p, sim->obs->ptr, n, acts, (int)n)) return 1;
+    if (racer_launch_sample(sim, acts->o->ptr, n, p->theta->ptr + p->LOGSTD, sim->actions->ptr, scratch->ptr,
+                            scratch->ptr + n, deterministic, t))
+        return 1;
+    return racer_launch_step(sim, sim->actions->ptr, sim->reward->ptr, sim->done->ptr, sim->obs->ptr, n);
+}
+
+extern "C" int racer_benchmark(racer_sim* sim, racer_policy* p, racer_acts* acts, racer_buf* scratch, int steps,
+                               double* ms_per_step) {
+    racer_clear_error();
+    for (int t = 0; t < 10; ++t)
+        if (racer_runner_step(sim, p, acts, scratch, 0, t)) return 1;
+    RACER_TRY(hipStreamSynchronize(sim->dev->stream));
+    auto start = std::chrono::steady_clock::now();
+    for (int t = 0; t < steps; ++t)
+        if (racer_runner_step(sim, p, acts, scratch, 0, t)) return 1;
+    RACER_TRY(hipStreamSynchronize(sim->dev->stream));
+    double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+    *ms_per_step = steps > 0 ? ms / steps : 0.0;
+    return 0;
+}

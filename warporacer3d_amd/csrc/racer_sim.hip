@@ -1,0 +1,568 @@
+// Simulation kernels for gfx950: track lookups, the RK4 slope bicycle with the
+// friction circle, crash/respawn/progress, the sphere-traced lidar, action
+// sampling (racer/sim.mojo), and the Sim C ABI.
+//
+// Float32 arithmetic is written in the reference's evaluation order and built
+// with -ffp-contract=off, using the shared float32 elementary functions, so
+// every result is bit-identical to the CPU restatement.
+#include <cstring>
+
+#include "racer_internal.h"
+
+namespace {
+
+struct Spot {
+    int layer;
+    float clearance;
+    float height;
+};
+
+struct Surf {
+    float height, sx, sy;
+};
+
+// Nearest cell of (x, y) (racer/sim.mojo:108-114); false off the map.
+__device__ __forceinline__ bool cell_of(const racer_dmap& m, float x, float y, int64_t* cell) {
+    float fx = floorf((x - m.x0) / m.cell + 0.5f);
+    float fy = floorf((y - m.y0) / m.cell + 0.5f);
+    if (!(fx >= 0.0f) || !(fy >= 0.0f) || fx >= (float)m.nx || fy >= (float)m.ny) return false;
+    *cell = (int64_t)fy * m.nx + (int64_t)fx;
+    return true;
+}
+
+__device__ __forceinline__ float grid_at(const racer_dmap& m, int layer, int which, int64_t cell) {
+    return m.d[RACER_HEADER + (int64_t)(3 * layer + which) * m.plane + cell];
+}
+
+// Map.at (racer/sim.mojo:108-114)
+__device__ __forceinline__ float map_at(const racer_dmap& m, int layer, int which, float x, float y) {
+    int64_t c;
+    return cell_of(m, x, y, &c) ? grid_at(m, layer, which, c) : -1.0f;
+}
+
+// Map.probe (racer/sim.mojo:116-124): the drivable layer whose height is nearest z.
+__device__ Spot map_probe(const racer_dmap& m, float x, float y, float z) {
+    int64_t c = 0;
+    bool on = cell_of(m, x, y, &c);
+    Spot best{0, -1.0f, on ? grid_at(m, 0, RACER_HEIGHT, c) : -1.0f};
+    float gap = 1e30f;
+    for (int l = 0; l < m.layers; ++l) {
+        float cl = on ? grid_at(m, l, RACER_CLEARANCE, c) : -1.0f;
+        float h = on ? grid_at(m, l, RACER_HEIGHT, c) : -1.0f;
+        if (cl > 0.0f && fabsf(h - z) < gap) {
+            best.layer = l;
+            best.clearance = cl;
+            best.height = h;
+            gap = fabsf(h - z);
+        }
+    }
+    return best;
+}
+
+// Map.surface (racer/sim.mojo:126-140): bilinear height and its gradient.
+__device__ Surf map_surface(const racer_dmap& m, int layer, float x, float y) {
+    float fx = (x - m.x0) / m.cell;
+    float fy = (y - m.y0) / m.cell;
+    float cx = floorf(fx), cy = floorf(fy);
+    int64_t ix = !(cx >= 0.0f) ? 0 : (cx > (float)(m.nx - 2) ? m.nx - 2 : (int64_t)cx);
+    int64_t iy = !(cy >= 0.0f) ? 0 : (cy > (float)(m.ny - 2) ? m.ny - 2 : (int64_t)cy);
+    float tx = racer_clamp(fx - (float)ix, 0.0f, 1.0f);
+    float ty = racer_clamp(fy - (float)iy, 0.0f, 1.0f);
+    const float* h = m.d + RACER_HEADER + (int64_t)(3 * layer + RACER_HEIGHT) * m.plane + iy * m.nx + ix;
+    float h0 = h[0], h1 = h[1], h2 = h[m.nx], h3 = h[m.nx + 1];
+    float w0 = (1.0f - tx) * (1.0f - ty);
+    float w1 = tx * (1.0f - ty);
+    float w2 = (1.0f - tx) * ty;
+    float w3 = tx * ty;
+    Surf s;
+    s.height = (h0 * w0 + h2 * w2) + (h1 * w1 + h3 * w3);  // 4-lane tree sum
+    s.sx = ((h1 - h0) * (1.0f - ty) + (h3 - h2) * ty) / m.cell;
+    s.sy = ((h2 - h0) * (1.0f - tx) + (h3 - h1) * tx) / m.cell;
+    return s;
+}
+
+// Surface.slopes (racer/sim.mojo:65-69): rise per metre ahead and to the left.
+__device__ __forceinline__ void slopes(const Surf& s, float heading, float* along, float* across) {
+    float c = racer_cos(heading);
+    float sn = racer_sin(heading);
+    *along = s.sx * c + s.sy * sn;
+    *across = -s.sx * sn + s.sy * c;
+}
+
+__device__ __forceinline__ float waypoint(const racer_dmap& m, int64_t w, int field) {
+    return m.d[m.route + 4 * w + field];
+}
+
+// Map.progress (racer/sim.mojo:146-152). Off the map w = -1 reads the floats
+// just before the route table, as the reference does.
+__device__ float map_progress(const racer_dmap& m, int layer, float x, float y) {
+    int64_t w = (int64_t)map_at(m, layer, RACER_NEAREST, x, y);
+    float heading = waypoint(m, w, 3);
+    float along = (x - waypoint(m, w, 0)) * racer_cos(heading) + (y - waypoint(m, w, 1)) * racer_sin(heading);
+    return (float)w + along / m.spacing;
+}
+
+// derivative (racer/sim.mojo:155-167): the kinematic bicycle on a slope.
+// tan(steer) is passed in as (sin, cos) so stages sharing a steer share them.
+__device__ __forceinline__ void derivative(const float p[4], float ssin, float scos, float accel, float grip,
+                                           float along, float across, float out[4]) {
+    float nz = 1.0f / sqrtf((1.0f + along * along) + across * across);
+    float load = (grip * RACER_G) * nz;
+    float v = racer_max(fabsf(p[3]), 0.5f) * (p[3] >= 0.0f ? 1.0f : -1.0f);
+    float slide = (RACER_G * across) * nz;
+    float lateral = racer_clamp((((v * p[3]) * ssin) / scos) / RACER_WHEELBASE + slide, -load, load);
+    float yaw = (lateral - slide) / v;
+    float longitudinal = sqrtf(racer_max(load * load - lateral * lateral, 0.0f));
+    float dv = racer_clamp(accel, -longitudinal, longitudinal) - (RACER_G * along) * nz;
+    out[0] = p[3] * racer_cos(p[2]);
+    out[1] = p[3] * racer_sin(p[2]);
+    out[2] = yaw;
+    out[3] = dv;
+}
+
+// spawn (racer/sim.mojo:170-181); heading_noise != 0 offsets the heading by
+// heading_noise * (2 u2 - 1) from the same Philox block.
+__device__ void spawn(const racer_dmap& m, float* s, int n, int i, int seed, float heading_noise) {
+    float episode = s[RACER_EPISODE * n + i] + 1.0f;
+    racer_u32x4 r = racer_random((uint64_t)(int64_t)seed, (uint64_t)i, (uint64_t)episode);
+    float u0 = racer_uniform(r.v[0]);
+    float u1 = racer_uniform(r.v[1]);
+    int64_t w = (int64_t)(u0 * (float)m.waypoints);
+    if (w > m.waypoints - 1) w = m.waypoints - 1;
+    for (int row = 0; row < RACER_STATE; ++row) s[row * n + i] = 0.0f;
+    s[RACER_X * n + i] = waypoint(m, w, 0);
+    s[RACER_Y * n + i] = waypoint(m, w, 1);
+    s[RACER_Z * n + i] = waypoint(m, w, 2);
+    float heading = waypoint(m, w, 3);
+    if (heading_noise != 0.0f) heading = heading + heading_noise * (2.0f * racer_uniform(r.v[2]) - 1.0f);
+    s[RACER_HEADING * n + i] = heading;
+    s[RACER_PROGRESS * n + i] = (float)w;
+    s[RACER_EPISODE * n + i] = episode;
+    s[RACER_FRICTION * n + i] = 0.85f + 0.3f * u1;
+}
+
+// advance (racer/sim.mojo:184-235): one 60 Hz step of car i.
+__device__ void advance(const racer_dmap& m, float* s, int n, const float* act, int64_t act_cols, float* reward,
+                        float* done, float* obs, int64_t obs_cols, int beams, int i, int seed, float heading_noise) {
+    const float DT = RACER_DT;
+    float p[4] = {s[RACER_X * n + i], s[RACER_Y * n + i], s[RACER_HEADING * n + i], s[RACER_SPEED * n + i]};
+    Spot spot = map_probe(m, p[0], p[1], s[RACER_Z * n + i]);
+    Surf ground = map_surface(m, spot.layer, p[0], p[1]);
+    float steer = s[RACER_STEER * n + i];
+    float rate = racer_clamp(act[i], -1.0f, 1.0f) * RACER_STEER_RATE;
+    float accel = racer_clamp(act[act_cols + i], -1.0f, 1.0f) * RACER_ACCEL_MAX;
+    float grip = RACER_GRIP * s[RACER_FRICTION * n + i];
+    bool crashed = false;
+    for (int sub = 0; sub < RACER_SUBSTEPS; ++sub) {
+        float along, across;
+        slopes(ground, p[2], &along, &across);
+        float half = steer + (rate * DT) / 2.0f;
+        float full = steer + rate * DT;
+        float s0 = racer_sin(steer), c0 = racer_cos(steer);
+        float s1 = racer_sin(half), c1 = racer_cos(half);
+        float s2 = racer_sin(full), c2 = racer_cos(full);
+        float k1[4], k2[4], k3[4], k4[4], q[4];
+        derivative(p, s0, c0, accel, grip, along, across, k1);
+        for (int j = 0; j < 4; ++j) q[j] = p[j] + k1[j] * (DT / 2.0f);
+        derivative(q, s1, c1, accel, grip, along, across, k2);
+        for (int j = 0; j < 4; ++j) q[j] = p[j] + k2[j] * (DT / 2.0f);
+        derivative(q, s1, c1, accel, grip, along, across, k3);
+        for (int j = 0; j < 4; ++j) q[j] = p[j] + k3[j] * DT;
+        derivative(q, s2, c2, accel, grip, along, across, k4);
+        for (int j = 0; j < 4; ++j) p[j] = p[j] + ((k1[j] + 2.0f * (k2[j] + k3[j])) + k4[j]) * (DT / 6.0f);
+        p[3] = racer_clamp(p[3], -RACER_SPEED_MAX, RACER_SPEED_MAX);
+        steer = racer_clamp(full, -RACER_STEER_MAX, RACER_STEER_MAX);
+        spot = map_probe(m, p[0], p[1], spot.height);
+        ground = map_surface(m, spot.layer, p[0], p[1]);
+        crashed = crashed || spot.clearance < RACER_HALF_DIAG ||
+                  ground.sx * ground.sx + ground.sy * ground.sy > 1.0f;
+    }
+    if (p[2] > RACER_PI) p[2] -= RACER_2PI;
+    if (p[2] < -RACER_PI) p[2] += RACER_2PI;
+    float progress = map_progress(m, spot.layer, p[0], p[1]);
+    float dp = progress - s[RACER_PROGRESS * n + i];
+    float halfw = (float)m.waypoints / 2.0f;
+    dp = dp > halfw ? dp - 2.0f * halfw : (dp < -halfw ? dp + 2.0f * halfw : dp);
+    float score = crashed ? -2.0f : (10.0f * dp) * m.spacing - (0.001f * steer) * steer;
+    float steps = s[RACER_STEPS * n + i] + 1.0f;
+    float reason = crashed ? (float)RACER_CRASH : (steps >= (float)RACER_HORIZON ? (float)RACER_TIMEOUT : 0.0f);
+    s[RACER_X * n + i] = p[0];
+    s[RACER_Y * n + i] = p[1];
+    s[RACER_Z * n + i] = spot.height;
+    s[RACER_HEADING * n + i] = p[2];
+    s[RACER_SPEED * n + i] = p[3];
+    s[RACER_STEER * n + i] = steer;
+    s[RACER_STEPS * n + i] = steps;
+    s[RACER_PROGRESS * n + i] = progress;
+    s[RACER_RETURN * n + i] += score;
+    reward[i] = score;
+    done[i] = reason;
+    if (reason > 0.0f) {
+        spawn(m, s, n, i, seed, heading_noise);
+        float x = s[RACER_X * n + i], y = s[RACER_Y * n + i];
+        ground = map_surface(m, map_probe(m, x, y, s[RACER_Z * n + i]).layer, x, y);
+    }
+    float along, across;
+    slopes(ground, s[RACER_HEADING * n + i], &along, &across);
+    obs[0 * obs_cols + i] = s[RACER_STEER * n + i] / RACER_STEER_MAX;
+    obs[1 * obs_cols + i] = s[RACER_SPEED * n + i] / RACER_SPEED_MAX;
+    obs[2 * obs_cols + i] = along;
+    obs[3 * obs_cols + i] = across;
+    obs[(int64_t)(4 + beams) * obs_cols + i] = 1.0f;
+}
+
+__global__ void __launch_bounds__(64) k_physics(racer_dmap m, float* s, int n, const float* act, int64_t act_cols,
+                                                float* reward, float* done, float* obs, int64_t obs_cols, int beams,
+                                                int seed, float heading_noise) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) advance(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, i, seed, heading_noise);
+}
+
+// Sim.reset's spawn kernel (racer/sim.mojo:299-304).
+__global__ void __launch_bounds__(256) k_reset(racer_dmap m, float* s, float* act, int n, int seed,
+                                               float heading_noise) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    spawn(m, s, n, i, seed, heading_noise);
+    act[i] = 0.0f;
+    act[n + i] = 0.0f;
+}
+
+// Per-cell records for the lidar: (clearance, height) of every layer side by
+// side, so one probe is one 8-byte (one layer) or 16-byte (two layers) load.
+__global__ void k_pack(const float* d, int64_t plane, int layers, float* packed) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= plane) return;
+    for (int l = 0; l < layers; ++l) {
+        packed[(c * layers + l) * 2 + 0] = d[RACER_HEADER + (int64_t)(3 * l + RACER_CLEARANCE) * plane + c];
+        packed[(c * layers + l) * 2 + 1] = d[RACER_HEADER + (int64_t)(3 * l + RACER_HEIGHT) * plane + c];
+    }
+}
+
+// Map.probe on the packed records, returning (clearance, height) only.
+template <int L>
+__device__ __forceinline__ void probe_packed(const racer_dmap& m, float x, float y, float z, float* clearance,
+                                             float* height) {
+    int64_t c = 0;
+    bool on = cell_of(m, x, y, &c);
+    if constexpr (L == 1) {
+        float2 r = on ? reinterpret_cast<const float2*>(m.packed)[c] : make_float2(-1.0f, -1.0f);
+        bool take = r.x > 0.0f && fabsf(r.y - z) < 1e30f;
+        *clearance = take ? r.x : -1.0f;
+        *height = r.y;
+    } else if constexpr (L == 2) {
+        float4 r = on ? reinterpret_cast<const float4*>(m.packed)[c] : make_float4(-1.0f, -1.0f, -1.0f, -1.0f);
+        float cl = -1.0f, h = r.y, gap = 1e30f;
+        if (r.x > 0.0f && fabsf(r.y - z) < gap) {
+            cl = r.x;
+            h = r.y;
+            gap = fabsf(r.y - z);
+        }
+        if (r.z > 0.0f && fabsf(r.w - z) < gap) {
+            cl = r.z;
+            h = r.w;
+        }
+        *clearance = cl;
+        *height = h;
+    } else {
+        Spot s = map_probe(m, x, y, z);
+        *clearance = s.clearance;
+        *height = s.height;
+    }
+}
+
+// scan (racer/sim.mojo:238-266) for every beam of one car per wavefront.
+// Each lane marches one beam; when a lane's beam ends, the lane stores the
+// range and takes the next unclaimed beam of the car (ballot + popcount), so
+// the wave stays full until the car's last beams.
+template <int L, bool COUNT>
+__global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __restrict__ s, int n,
+                                               float* __restrict__ obs, int64_t obs_cols, int beams,
+                                               const float2* __restrict__ dirs, unsigned long long* counter) {
+    const int lane = threadIdx.x & 63;
+    const int car = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    if (car >= n) return;
+    const float heading = s[RACER_HEADING * n + car];
+    const float along = obs[2 * obs_cols + car];
+    const float across = obs[3 * obs_cols + car];
+    const float ch = racer_cos(heading), sh = racer_sin(heading);
+    const float ox = (s[RACER_X * n + car] + RACER_MOUNT_FORWARD * ch) + 0.0f;
+    const float oy = (s[RACER_Y * n + car] + RACER_MOUNT_FORWARD * sh) + 0.0f;
+    const float oz = (s[RACER_Z * n + car] + RACER_MOUNT_FORWARD * along) + RACER_MOUNT_UP;
+    const float slope15 = 1.5f * m.slope;
+    const float half_cell = 0.5f * m.cell;
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f, descent = 0.0f, t = 0.0f, last = 0.0f, last_floor = 0.0f;
+    auto start = [&](int b) {
+        float2 cs = dirs[b];
+        float d0 = cs.x * ch + cs.y * (-sh);
+        float d1 = cs.x * sh + cs.y * ch;
+        float d2 = cs.x * along + cs.y * across;
+        float norm = sqrtf((d0 * d0 + d2 * d2) + (d1 * d1 + 0.0f));
+        dx = d0 / norm;
+        dy = d1 / norm;
+        dz = d2 / norm;
+        float horizontal = sqrtf(dx * dx + dy * dy);
+        descent = slope15 * horizontal + racer_max(-dz, 0.0f);
+        t = 0.0f;
+        last = 0.0f;
+        last_floor = 0.0f;
+    };
+    int b = lane;
+    if (b < beams) start(b);
+    int next = 64;
+    uint64_t iterations = 0;
+    while (__ballot(b < beams) != 0ull) {
+        bool finished = false;
+        float range = 0.0f;
+        if (b < beams) {
+            if (COUNT) ++iterations;
+            float px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
+            float clearance, height;
+            probe_packed<L>(m, px, py, pz, &clearance, &height);
+            float fl = (pz - height) - 0.03f;
+            if (fl < 0.0f) {
+                range = racer_min(last + ((t - last) * last_floor) / (last_floor - fl), RACER_RANGE);
+                finished = true;
+            } else {
+                float step = racer_min(clearance - m.cell, fl / descent);
+                if (step < half_cell) {
+                    range = racer_min(t, RACER_RANGE);
+                    finished = true;
+                } else {
+                    last = t;
+                    last_floor = fl;
+                    t += step;
+                    if (!(t < RACER_RANGE)) {
+                        range = racer_min(t, RACER_RANGE);
+                        finished = true;
+                    }
+                }
+            }
+            if (finished) obs[(int64_t)(4 + b) * obs_cols + car] = range / RACER_RANGE;
+        }
+        uint64_t done_mask = __ballot(finished);
+        if (finished) {
+            b = next + (int)__popcll(done_mask & ((1ull << lane) - 1ull));
+            if (b < beams) start(b);
+        }
+        next += (int)__popcll(done_mask);
+    }
+    if (COUNT) {
+        for (int off = 32; off > 0; off >>= 1) iterations += __shfl_xor(iterations, off);
+        if (lane == 0) atomicAdd(counter, (unsigned long long)iterations);
+    }
+}
+
+// Sim.sample's kernel (racer/sim.mojo:337-348).
+__global__ void __launch_bounds__(256) k_sample(int n, const float* out, int64_t out_cols, const float* log_std,
+                                                float* act, int64_t act_cols, float* logp, float* value,
+                                                int deterministic, int64_t step, int seed) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float noise[4];
+    racer_normal4(racer_random((uint64_t)(int64_t)seed + 1, (uint64_t)i, (uint64_t)step), noise);
+    float probability = 0.0f;
+    for (int a = 0; a < 2; ++a) {
+        float z = deterministic ? 0.0f : noise[a];
+        float ls = log_std[a];
+        act[a * act_cols + i] = out[a * out_cols + i] + racer_exp(ls) * z;
+        probability = probability - (((0.5f * z) * z + ls) + RACER_HALF_LOG_2PI);
+    }
+    logp[i] = probability;
+    value[i] = out[2 * out_cols + i];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+int racer_launch_reset(racer_sim* sim) {
+    int n = sim->n;
+    {
+        racer_timed timer(sim->dev, "reset");
+        hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, sim->dev->stream, sim->map,
+                           sim->state->ptr, sim->actions->ptr, n, sim->seed, sim->heading_noise);
+    }
+    return racer_launch_check("reset");
+}
+
+template <int L>
+static void launch_lidar(racer_sim* sim, float* obs, int64_t obs_cols) {
+    int n = sim->n;
+    dim3 grid((unsigned)((n + 3) / 4)), block(256);
+    if (sim->count_iterations)
+        hipLaunchKernelGGL((k_lidar<L, true>), grid, block, 0, sim->dev->stream, sim->map, sim->state->ptr, n, obs,
+                           obs_cols, sim->beams, sim->dirs, sim->counter);
+    else
+        hipLaunchKernelGGL((k_lidar<L, false>), grid, block, 0, sim->dev->stream, sim->map, sim->state->ptr, n, obs,
+                           obs_cols, sim->beams, sim->dirs, sim->counter);
+}
+
+int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done, float* obs, int64_t obs_cols) {
+    int n = sim->n;
+    {
+        racer_timed timer(sim->dev, "physics");
+        hipLaunchKernelGGL(k_physics, dim3((n + 63) / 64), dim3(64), 0, sim->dev->stream, sim->map, sim->state->ptr,
+                           n, actions, (int64_t)n, reward, done, obs, obs_cols, sim->beams, sim->seed,
+                           sim->heading_noise);
+    }
+    if (racer_launch_check("physics")) return 1;
+    if (sim->count_iterations) RACER_TRY(hipMemsetAsync(sim->counter, 0, sizeof(unsigned long long), sim->dev->stream));
+    {
+        racer_timed timer(sim->dev, "lidar");
+        if (sim->map.layers == 1)
+            launch_lidar<1>(sim, obs, obs_cols);
+        else if (sim->map.layers == 2)
+            launch_lidar<2>(sim, obs, obs_cols);
+        else
+            launch_lidar<0>(sim, obs, obs_cols);
+    }
+    if (racer_launch_check("lidar")) return 1;
+    if (sim->count_iterations) {
+        unsigned long long count = 0;
+        RACER_TRY(hipMemcpyAsync(&count, sim->counter, sizeof(count), hipMemcpyDeviceToHost, sim->dev->stream));
+        RACER_TRY(hipStreamSynchronize(sim->dev->stream));
+        sim->last_iterations = (int64_t)count;
+    }
+    return 0;
+}
+
+int racer_launch_sample(racer_sim* sim, const float* out, int64_t out_cols, const float* log_std, float* actions,
+                        float* logp, float* value, int deterministic, int64_t step) {
+    int n = sim->n;
+    {
+        racer_timed timer(sim->dev, "sample");
+        hipLaunchKernelGGL(k_sample, dim3((n + 255) / 256), dim3(256), 0, sim->dev->stream, n, out, out_cols,
+                           log_std, actions, (int64_t)n, logp, value, deterministic, step, sim->seed);
+    }
+    return racer_launch_check("sample");
+}
+
+// ---------------------------------------------------------------- Sim C ABI
+extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t len, int n, int seed,
+                                const racer_config* cfg, racer_sim** out) {
+    racer_clear_error();
+    *out = nullptr;
+    RACER_REQUIRE(dev, "null device");
+    RACER_REQUIRE(track && len >= RACER_HEADER, "not a track array");
+    RACER_REQUIRE(n > 0, "the car count must be positive");
+    racer_config def;
+    racer_config_default(&def);
+    if (!cfg) cfg = &def;
+    RACER_REQUIRE(cfg->beams >= 2, "at least two lidar beams are needed");
+    int nx = (int)track[0], ny = (int)track[1], layers = (int)track[7], wps = (int)track[5];
+    RACER_REQUIRE(nx >= 2 && ny >= 2 && layers >= 1 && wps >= 1, "not a track array");
+    int64_t plane = (int64_t)nx * ny;
+    int64_t route = RACER_HEADER + 3 * (int64_t)layers * plane;
+    RACER_REQUIRE((size_t)(route + 4 * (int64_t)wps) <= len, "not a track array");
+    RACER_REQUIRE(track[4] > 0.0f, "not a track array: the cell size must be positive");
+    // Route lookups index the waypoint table with the nearest-waypoint grids: keep them in range.
+    for (int l = 0; l < layers; ++l) {
+        const float* nearest = track + RACER_HEADER + (int64_t)(3 * l + RACER_NEAREST) * plane;
+        for (int64_t c = 0; c < plane; ++c)
+            RACER_REQUIRE(nearest[c] >= 0.0f && nearest[c] <= (float)(wps - 1),
+                          "not a track array: a nearest-waypoint cell is out of range");
+    }
+    auto* sim = new racer_sim();
+    sim->dev = dev;
+    sim->n = n;
+    sim->seed = seed;
+    sim->beams = cfg->beams;
+    sim->heading_noise = cfg->heading_noise;
+    int IN = cfg->beams + 5;
+    if (racer_upload(dev, track, len, &sim->data) || racer_alloc(dev, (size_t)RACER_STATE * n, 0.0f, &sim->state) ||
+        racer_alloc(dev, (size_t)IN * n, 0.0f, &sim->obs) || racer_alloc(dev, 2 * (size_t)n, 0.0f, &sim->actions) ||
+        racer_alloc(dev, (size_t)n, 0.0f, &sim->reward) || racer_alloc(dev, (size_t)n, 0.0f, &sim->done)) {
+        racer_sim_destroy(sim);
+        return 1;
+    }
+    racer_dmap& m = sim->map;
+    m.d = sim->data->ptr;
+    m.nx = nx;
+    m.ny = ny;
+    m.x0 = track[2];
+    m.y0 = track[3];
+    m.cell = track[4];
+    m.waypoints = wps;
+    m.spacing = track[6];
+    m.layers = layers;
+    m.slope = track[8];
+    m.route = route;
+    m.plane = plane;
+    m.packed = nullptr;
+    if (layers <= 2) {
+        RACER_TRY(hipMalloc(&sim->packed, (size_t)plane * layers * 2 * sizeof(float)));
+        hipLaunchKernelGGL(k_pack, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, dev->stream, m.d, plane,
+                           layers, sim->packed);
+        if (racer_launch_check("pack")) return 1;
+        m.packed = sim->packed;
+    }
+    // Beam directions: azimuth = -FOV/2 + b FOV/(BEAMS-1) (racer/sim.mojo:246).
+    std::vector<float2> dirs(cfg->beams);
+    for (int b = 0; b < cfg->beams; ++b) {
+        float az = -RACER_FOV / 2.0f + (float)b * (RACER_FOV / (float)(cfg->beams - 1));
+        dirs[b] = make_float2(racer_cos(az), racer_sin(az));
+    }
+    RACER_TRY(hipMalloc(&sim->dirs, dirs.size() * sizeof(float2)));
+    RACER_TRY(hipMemcpyAsync(sim->dirs, dirs.data(), dirs.size() * sizeof(float2), hipMemcpyHostToDevice,
+                             dev->stream));
+    RACER_TRY(hipMalloc(&sim->counter, sizeof(unsigned long long)));
+    RACER_TRY(hipStreamSynchronize(dev->stream));
+    *out = sim;
+    return 0;
+}
+
+extern "C" void racer_sim_destroy(racer_sim* sim) {
+    if (!sim) return;
+    if (sim->dev) hipStreamSynchronize(sim->dev->stream);
+    racer_free(sim->data);
+    racer_free(sim->state);
+    racer_free(sim->obs);
+    racer_free(sim->actions);
+    racer_free(sim->reward);
+    racer_free(sim->done);
+    if (sim->packed) hipFree(sim->packed);
+    if (sim->dirs) hipFree(sim->dirs);
+    if (sim->counter) hipFree(sim->counter);
+    delete sim;
+}
+
+extern "C" int racer_sim_reset(racer_sim* sim) {
+    racer_clear_error();
+    if (racer_launch_reset(sim)) return 1;
+    return racer_launch_step(sim, sim->actions->ptr, sim->reward->ptr, sim->done->ptr, sim->obs->ptr, sim->n);
+}
+
+extern "C" int racer_sim_step(racer_sim* sim, racer_view actions, racer_view reward, racer_view done, racer_view obs) {
+    racer_clear_error();
+    int n = sim->n;
+    if (racer_check_view(actions, 2, n, "actions") || racer_check_view(reward, 1, n, "reward") ||
+        racer_check_view(done, 1, n, "done") || racer_check_view(obs, sim->beams + 5, n, "obs"))
+        return 1;
+    return racer_launch_step(sim, view_ptr(actions), view_ptr(reward), view_ptr(done), view_ptr(obs), n);
+}
+
+extern "C" int racer_sim_sample(racer_sim* sim, racer_view out, racer_view log_std, racer_view actions,
+                                racer_view logp, racer_view value, int deterministic, int64_t step) {
+    racer_clear_error();
+    int n = sim->n;
+    if (racer_check_view(out, 3, n, "policy output") || racer_check_view(log_std, 1, 2, "log std") ||
+        racer_check_view(actions, 2, n, "actions") || racer_check_view(logp, 1, n, "logp") ||
+        racer_check_view(value, 1, n, "value"))
+        return 1;
+    return racer_launch_sample(sim, view_ptr(out), n, view_ptr(log_std), view_ptr(actions), view_ptr(logp),
+                               view_ptr(value), deterministic, step);
+}
+
+extern "C" racer_buf* racer_sim_state(racer_sim* sim) { return sim->state; }
+extern "C" racer_buf* racer_sim_obs(racer_sim* sim) { return sim->obs; }
+extern "C" racer_buf* racer_sim_actions(racer_sim* sim) { return sim->actions; }
+extern "C" racer_buf* racer_sim_reward(racer_sim* sim) { return sim->reward; }
+extern "C" racer_buf* racer_sim_done(racer_sim* sim) { return sim->done; }
+extern "C" int racer_sim_cars(const racer_sim* sim) { return sim->n; }
+extern "C" int racer_sim_beams(const racer_sim* sim) { return sim->beams; }
+
+extern "C" int racer_sim_count_iterations(racer_sim* sim, int on) {
+    sim->count_iterations = on != 0;
+    return 0;
+}
+extern "C" int64_t racer_sim_last_iterations(racer_sim* sim) { return sim->last_iterations; }
