@@ -88,6 +88,8 @@ def main() -> int:
     ap.add_argument("--precision", choices=["bf16", "fp32"], default="bf16")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="MiB copied once device to device before the run (known-size traffic for counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -104,6 +106,13 @@ def main() -> int:
                    heading_noise=0.1)
     dev = R.Device(True)
     log("device:", dev.name)
+    if args.calibrate_copy:
+        count = args.calibrate_copy * (1 << 20) // 4
+        src, dst = dev.alloc(count, 1.0), dev.alloc(count)
+        dev.copy(dst, 0, src, 0, count)
+        dev.synchronize()
+        log(f"calibration copy: {count * 4} bytes read and written")
+        del src, dst
     run = R.Runner(arr, args.cars, cfg, seed=42, device=dev)
     for t in range(args.warmup):
         run.step(False, t)

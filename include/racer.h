@@ -10,7 +10,7 @@
  *     the reference (racer/sim.mojo:5-7).
  *   - A racer_view is the reference's `mat[rows](buffer, cols, offset)`
  *     (racer/device.mojo:29-32): `rows` x `cols` floats, row-major, starting
- *     `offset` floats into `buf`.
+ *     `offset` floats into `buf`. Views are passed by pointer.
  *   - Functions return 0 on success and nonzero on failure; the message is in
  *     racer_last_error() (per thread). The reference raises Error(msg).
  *   - Calls are asynchronous on the device's one in-order stream, except those
@@ -95,10 +95,11 @@ void racer_sim_destroy(racer_sim* sim);
 int racer_sim_reset(racer_sim* sim);
 /* Sim.step (sim.mojo:307-321): physics, rewards, respawns, then every lidar
  * beam. actions [2, n], reward [1, n], done [1, n], obs [beams+5, n]. */
-int racer_sim_step(racer_sim* sim, racer_view actions, racer_view reward, racer_view done, racer_view obs);
+int racer_sim_step(racer_sim* sim, const racer_view* actions, const racer_view* reward, const racer_view* done,
+                   const racer_view* obs);
 /* Sim.sample (sim.mojo:323-348): out [3, n], log_std [1, 2]. */
-int racer_sim_sample(racer_sim* sim, racer_view out, racer_view log_std, racer_view actions, racer_view logp,
-                     racer_view value, int deterministic, int64_t step);
+int racer_sim_sample(racer_sim* sim, const racer_view* out, const racer_view* log_std, const racer_view* actions,
+                     const racer_view* logp, const racer_view* value, int deterministic, int64_t step);
 racer_buf* racer_sim_state(racer_sim* sim);   /* [11, n] */
 racer_buf* racer_sim_obs(racer_sim* sim);     /* [beams+5, n] */
 racer_buf* racer_sim_actions(racer_sim* sim); /* [2, n] */
@@ -119,15 +120,15 @@ int racer_policy_create(racer_device* dev, int seed, const racer_config* cfg, ra
 void racer_policy_destroy(racer_policy* pol);
 racer_buf* racer_policy_theta(racer_policy* pol); /* [P] = W0[H,IN] | W1[H,H+1] | W2[3,H+1] | logstd[2] */
 size_t racer_policy_params(const racer_policy* pol);
-/* Policy.log_std() (ppo.mojo:105-106) */
-racer_view racer_policy_log_std(racer_policy* pol);
+/* Policy.log_std() (ppo.mojo:105-106): writes the [1, 2] view of theta. */
+int racer_policy_log_std(racer_policy* pol, racer_view* out);
 /* Policy.transpose (ppo.mojo:92-103) */
 int racer_policy_transpose(racer_policy* pol);
 /* Policy.forward (ppo.mojo:108-111): x [beams+5, acts.cols]. */
-int racer_policy_forward(racer_policy* pol, racer_view x, racer_acts* acts);
+int racer_policy_forward(racer_policy* pol, const racer_view* x, racer_acts* acts);
 /* Policy.backward (ppo.mojo:113-123): d3 [3, cols] -> d2, d1 [H+1, cols], grad W0/W1/W2 entries. */
-int racer_policy_backward(racer_policy* pol, racer_view x, racer_acts* acts, racer_view d3, racer_buf* d2,
-                          racer_buf* d1, racer_buf* grad);
+int racer_policy_backward(racer_policy* pol, const racer_view* x, racer_acts* acts, const racer_view* d3,
+                          racer_buf* d2, racer_buf* d1, racer_buf* grad);
 
 /* ---------------- Trainer: racer/ppo.mojo:141-337 ---------------- */
 int racer_trainer_create(racer_device* dev, int n, const racer_config* cfg, racer_trainer** out);
@@ -182,6 +183,11 @@ int racer_profile_reset(racer_device* dev);
  * an instrumented variant of the lidar kernel when enabled; 0 otherwise). */
 int racer_sim_count_iterations(racer_sim* sim, int on);
 int64_t racer_sim_last_iterations(racer_sim* sim);
+/* Lidar launch schedule: 0 (default) = one wavefront per car with beam refill;
+ * 1 = one thread per (car, beam), beams fastest, the launch shape of the
+ * reference's 2-D Device.run (racer/device.mojo:67-79), for timing the plain
+ * baseline. Both give identical results. */
+int racer_sim_set_schedule(racer_sim* sim, int schedule);
 
 #ifdef __cplusplus
 }

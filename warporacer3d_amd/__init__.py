@@ -77,7 +77,7 @@ def lib() -> C.CDLL:
     L = C.CDLL(str(LIB_PATH))
     vp, i, i64, f, sz = C.c_void_p, C.c_int, C.c_int64, C.c_float, C.c_size_t
     cp = C.POINTER(C.c_void_p)
-    V = _View
+    VP = C.POINTER(_View)
     sigs = {
         "racer_version": (C.c_char_p, []),
         "racer_last_error": (C.c_char_p, []),
@@ -99,8 +99,8 @@ def lib() -> C.CDLL:
         "racer_sim_create": (i, [vp, vp, sz, i, i, C.POINTER(Config), cp]),
         "racer_sim_destroy": (None, [vp]),
         "racer_sim_reset": (i, [vp]),
-        "racer_sim_step": (i, [vp, V, V, V, V]),
-        "racer_sim_sample": (i, [vp, V, V, V, V, V, i, i64]),
+        "racer_sim_step": (i, [vp, VP, VP, VP, VP]),
+        "racer_sim_sample": (i, [vp, VP, VP, VP, VP, VP, i, i64]),
         "racer_sim_state": (vp, [vp]),
         "racer_sim_obs": (vp, [vp]),
         "racer_sim_actions": (vp, [vp]),
@@ -117,10 +117,10 @@ def lib() -> C.CDLL:
         "racer_policy_destroy": (None, [vp]),
         "racer_policy_theta": (vp, [vp]),
         "racer_policy_params": (sz, [vp]),
-        "racer_policy_log_std": (V, [vp]),
+        "racer_policy_log_std": (i, [vp, VP]),
         "racer_policy_transpose": (i, [vp]),
-        "racer_policy_forward": (i, [vp, V, vp]),
-        "racer_policy_backward": (i, [vp, V, vp, V, vp, vp, vp]),
+        "racer_policy_forward": (i, [vp, VP, vp]),
+        "racer_policy_backward": (i, [vp, VP, vp, VP, vp, vp, vp]),
         "racer_trainer_create": (i, [vp, i, C.POINTER(Config), cp]),
         "racer_trainer_destroy": (None, [vp]),
         "racer_trainer_rollout": (i, [vp, vp, vp, vp, i, C.POINTER(C.c_float)]),
@@ -138,6 +138,7 @@ def lib() -> C.CDLL:
         "racer_profile_reset": (i, [vp]),
         "racer_sim_count_iterations": (i, [vp, i]),
         "racer_sim_last_iterations": (i64, [vp]),
+        "racer_sim_set_schedule": (i, [vp, i]),
     }
     for name in ("obs", "actions", "logp", "value", "reward", "done", "advantage", "grad", "d3", "stats", "acts"):
         sigs[f"racer_trainer_{name}"] = (vp, [vp])
@@ -204,8 +205,10 @@ class View:
         self.buffer, self.rows, self.cols, self.offset = buffer, rows, cols, offset
 
     @property
-    def c(self) -> _View:
-        return _View(self.buffer.handle, self.offset, self.rows, self.cols)
+    def c(self):
+        """Pointer to the C racer_view (valid while this View lives)."""
+        self._c = _View(self.buffer.handle, self.offset, self.rows, self.cols)
+        return C.byref(self._c)
 
     def read(self) -> np.ndarray:
         out = np.empty((self.rows, self.cols), np.float32)
@@ -329,6 +332,10 @@ class Sim:
 
     def count_iterations(self, on: bool = True) -> None:
         _check(lib().racer_sim_count_iterations(self.handle, int(on)))
+
+    def set_schedule(self, schedule: int) -> None:
+        """0: wave per car with beam refill (default); 1: thread per (car, beam) as the reference launches it."""
+        _check(lib().racer_sim_set_schedule(self.handle, int(schedule)))
 
     @property
     def last_iterations(self) -> int:

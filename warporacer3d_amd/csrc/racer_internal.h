@@ -97,6 +97,7 @@ struct racer_sim {
     float* packed = nullptr;
     float2* dirs = nullptr;  // (cos, sin) of every beam's azimuth
     racer_dmap map{};
+    int schedule = 0;  // lidar launch: 0 wave per car with beam refill, 1 thread per (car, beam)
     bool count_iterations = false;
     unsigned long long* counter = nullptr;
     int64_t last_iterations = 0;

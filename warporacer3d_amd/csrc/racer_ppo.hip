@@ -261,13 +261,14 @@ extern "C" void racer_policy_destroy(racer_policy* p) {
 extern "C" racer_buf* racer_policy_theta(racer_policy* p) { return p->theta; }
 extern "C" size_t racer_policy_params(const racer_policy* p) { return (size_t)p->P; }
 
-extern "C" racer_view racer_policy_log_std(racer_policy* p) {
-    racer_view v;
-    v.buf = p->theta;
-    v.offset = (size_t)p->LOGSTD;
-    v.rows = 1;
-    v.cols = 2;
-    return v;
+extern "C" int racer_policy_log_std(racer_policy* p, racer_view* v) {
+    racer_clear_error();
+    RACER_REQUIRE(p && v, "null argument");
+    v->buf = p->theta;
+    v->offset = (size_t)p->LOGSTD;
+    v->rows = 1;
+    v->cols = 2;
+    return 0;
 }
 
 extern "C" int racer_policy_transpose(racer_policy* p) {
@@ -310,8 +311,10 @@ int racer_policy_forward_raw(racer_policy* p, const float* x, int64_t x_cols, ra
                              (int)p->HA, RACER_EPI_NONE, nullptr, 0);
 }
 
-extern "C" int racer_policy_forward(racer_policy* p, racer_view x, racer_acts* a) {
+extern "C" int racer_policy_forward(racer_policy* p, const racer_view* x_p, racer_acts* a) {
     racer_clear_error();
+    RACER_REQUIRE(p && x_p && a, "null argument");
+    const racer_view& x = *x_p;
     RACER_REQUIRE(a && a->hidden == p->hidden, "activations do not match the policy's hidden width");
     if (racer_check_view(x, p->IN, a->cols, "policy input")) return 1;
     return racer_policy_forward_raw(p, view_ptr(x), x.cols, a, a->cols);
@@ -342,9 +345,11 @@ static int backward_raw(racer_policy* p, const float* x, racer_acts* a, const fl
     return racer_gemm_nt_f32(dev, d3, cols, h2, cols, grad + p->W2, HA, RACER_OUT, (int)HA, (int)cols);
 }
 
-extern "C" int racer_policy_backward(racer_policy* p, racer_view x, racer_acts* a, racer_view d3, racer_buf* d2,
-                                     racer_buf* d1, racer_buf* grad) {
+extern "C" int racer_policy_backward(racer_policy* p, const racer_view* x_p, racer_acts* a, const racer_view* d3_p,
+                                     racer_buf* d2, racer_buf* d1, racer_buf* grad) {
     racer_clear_error();
+    RACER_REQUIRE(p && x_p && a && d3_p, "null argument");
+    const racer_view &x = *x_p, &d3 = *d3_p;
     int64_t cols = a->cols;
     if (racer_check_view(x, p->IN, cols, "policy input") || racer_check_view(d3, RACER_OUT, cols, "d3")) return 1;
     RACER_REQUIRE(d2 && d2->len >= (size_t)(p->HA * cols) && d1 && d1->len >= (size_t)(p->HA * cols) && grad &&

@@ -353,6 +353,52 @@ __global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __rest
     }
 }
 
+// The reference's own lidar launch (racer/sim.mojo:317-321 through the 2-D
+// Device.run of racer/device.mojo:67-79): one thread per (car, beam), beams
+// fastest, every thread recomputing its car's frame and its beam's azimuth.
+// Same arithmetic as k_lidar; kept as the schedule of the plain baseline.
+__global__ void __launch_bounds__(256) k_lidar_flat(racer_dmap m, const float* __restrict__ s, int n,
+                                                    float* __restrict__ obs, int64_t obs_cols, int beams) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * beams) return;
+    const int car = (int)(idx / beams), b = (int)(idx % beams);
+    const float heading = s[RACER_HEADING * n + car];
+    const float along = obs[2 * obs_cols + car];
+    const float across = obs[3 * obs_cols + car];
+    const float ch = racer_cos(heading), sh = racer_sin(heading);
+    const float az = -RACER_FOV / 2.0f + (float)b * (RACER_FOV / (float)(beams - 1));
+    const float ca = racer_cos(az), sa = racer_sin(az);
+    float d0 = ca * ch + sa * (-sh);
+    float d1 = ca * sh + sa * ch;
+    float d2 = ca * along + sa * across;
+    float norm = sqrtf((d0 * d0 + d2 * d2) + (d1 * d1 + 0.0f));
+    const float dx = d0 / norm, dy = d1 / norm, dz = d2 / norm;
+    const float ox = (s[RACER_X * n + car] + RACER_MOUNT_FORWARD * ch) + 0.0f;
+    const float oy = (s[RACER_Y * n + car] + RACER_MOUNT_FORWARD * sh) + 0.0f;
+    const float oz = (s[RACER_Z * n + car] + RACER_MOUNT_FORWARD * along) + RACER_MOUNT_UP;
+    const float horizontal = sqrtf(dx * dx + dy * dy);
+    const float descent = (1.5f * m.slope) * horizontal + racer_max(-dz, 0.0f);
+    float t = 0.0f, last = 0.0f, last_floor = 0.0f, range = 0.0f;
+    bool done = false;
+    while (t < RACER_RANGE) {
+        float pz = oz + t * dz;
+        Spot spot = map_probe(m, ox + t * dx, oy + t * dy, pz);
+        float fl = (pz - spot.height) - 0.03f;
+        if (fl < 0.0f) {
+            range = racer_min(last + ((t - last) * last_floor) / (last_floor - fl), RACER_RANGE);
+            done = true;
+            break;
+        }
+        float step = racer_min(spot.clearance - m.cell, fl / descent);
+        if (step < 0.5f * m.cell) break;
+        last = t;
+        last_floor = fl;
+        t += step;
+    }
+    if (!done) range = racer_min(t, RACER_RANGE);
+    obs[(int64_t)(4 + b) * obs_cols + car] = range / RACER_RANGE;
+}
+
 // Sim.sample's kernel (racer/sim.mojo:337-348).
 __global__ void __launch_bounds__(256) k_sample(int n, const float* out, int64_t out_cols, const float* log_std,
                                                 float* act, int64_t act_cols, float* logp, float* value,
@@ -407,7 +453,12 @@ int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done
     }
     if (racer_launch_check("physics")) return 1;
     if (sim->count_iterations) RACER_TRY(hipMemsetAsync(sim->counter, 0, sizeof(unsigned long long), sim->dev->stream));
-    {
+    if (sim->schedule == 1) {
+        int64_t threads = (int64_t)n * sim->beams;
+        racer_timed timer(sim->dev, "lidar_flat");
+        hipLaunchKernelGGL(k_lidar_flat, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, sim->dev->stream,
+                           sim->map, sim->state->ptr, n, obs, obs_cols, sim->beams);
+    } else {
         racer_timed timer(sim->dev, "lidar");
         if (sim->map.layers == 1)
             launch_lidar<1>(sim, obs, obs_cols);
@@ -532,8 +583,11 @@ extern "C" int racer_sim_reset(racer_sim* sim) {
     return racer_launch_step(sim, sim->actions->ptr, sim->reward->ptr, sim->done->ptr, sim->obs->ptr, sim->n);
 }
 
-extern "C" int racer_sim_step(racer_sim* sim, racer_view actions, racer_view reward, racer_view done, racer_view obs) {
+extern "C" int racer_sim_step(racer_sim* sim, const racer_view* actions_p, const racer_view* reward_p,
+                              const racer_view* done_p, const racer_view* obs_p) {
     racer_clear_error();
+    RACER_REQUIRE(sim && actions_p && reward_p && done_p && obs_p, "null argument");
+    const racer_view &actions = *actions_p, &reward = *reward_p, &done = *done_p, &obs = *obs_p;
     int n = sim->n;
     if (racer_check_view(actions, 2, n, "actions") || racer_check_view(reward, 1, n, "reward") ||
         racer_check_view(done, 1, n, "done") || racer_check_view(obs, sim->beams + 5, n, "obs"))
@@ -541,9 +595,13 @@ extern "C" int racer_sim_step(racer_sim* sim, racer_view actions, racer_view rew
     return racer_launch_step(sim, view_ptr(actions), view_ptr(reward), view_ptr(done), view_ptr(obs), n);
 }
 
-extern "C" int racer_sim_sample(racer_sim* sim, racer_view out, racer_view log_std, racer_view actions,
-                                racer_view logp, racer_view value, int deterministic, int64_t step) {
+extern "C" int racer_sim_sample(racer_sim* sim, const racer_view* out_p, const racer_view* log_std_p,
+                                const racer_view* actions_p, const racer_view* logp_p, const racer_view* value_p,
+                                int deterministic, int64_t step) {
     racer_clear_error();
+    RACER_REQUIRE(sim && out_p && log_std_p && actions_p && logp_p && value_p, "null argument");
+    const racer_view &out = *out_p, &log_std = *log_std_p, &actions = *actions_p, &logp = *logp_p,
+                     &value = *value_p;
     int n = sim->n;
     if (racer_check_view(out, 3, n, "policy output") || racer_check_view(log_std, 1, 2, "log std") ||
         racer_check_view(actions, 2, n, "actions") || racer_check_view(logp, 1, n, "logp") ||
@@ -566,3 +624,10 @@ extern "C" int racer_sim_count_iterations(racer_sim* sim, int on) {
     return 0;
 }
 extern "C" int64_t racer_sim_last_iterations(racer_sim* sim) { return sim->last_iterations; }
+
+extern "C" int racer_sim_set_schedule(racer_sim* sim, int schedule) {
+    racer_clear_error();
+    RACER_REQUIRE(sim && (schedule == 0 || schedule == 1), "schedule must be 0 (wave per car) or 1 (thread per beam)");
+    sim->schedule = schedule;
+    return 0;
+}
