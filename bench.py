@@ -76,6 +76,31 @@ def cpu_baseline(track, beams: int, hidden: int, threads: int, seconds: float) -
                       f"({dt:.1f} s) on {threads} host threads"}
 
 
+def plain_baseline(R, dev, track, args, steps: int) -> dict:
+    """The same Runner step in the reference's own GPU decomposition, on the
+    same GPU: one thread per car for physics, one thread per (car, beam) for the
+    lidar, three float32 products and a separate sample kernel."""
+    cfg = R.Config(beams=args.beams, hidden=args.hidden, precision=2, heading_noise=0.1)
+    run = R.Runner(track, args.cars, cfg, seed=42, device=dev)
+    run.sim.set_schedule(1)
+    for t in range(5):
+        run.step(False, t)
+    dev.synchronize()
+    dev.profile_reset()
+    dev.profile(True)
+    t0 = time.perf_counter()
+    for t in range(steps):
+        run.step(False, 5 + t)
+    dev.synchronize()
+    dt = time.perf_counter() - t0
+    dev.profile(False)
+    prof = dev.profile_results()
+    ms = dt * 1000.0 / steps
+    return {"value": args.cars * 1000.0 / ms, "unit": "env-steps/s", "ms_per_step": ms, "steps": steps,
+            "kernels": {k: {"launches": n, "avg_us": tot / max(n, 1) * 1000.0} for k, (n, tot) in prof.items()},
+            "schedule": "physics 1 thread/car; lidar 1 thread/(car, beam); 3 float32 products; sample kernel"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +113,7 @@ def main() -> int:
     ap.add_argument("--precision", choices=["bf16", "fp32"], default="bf16")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU decomposition baseline")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied once device to device before the run (known-size traffic for counters)")
     args = ap.parse_args()
@@ -117,6 +143,8 @@ def main() -> int:
     for t in range(args.warmup):
         run.step(False, t)
     dev.synchronize()
+    waves_per_car = run.sim.autotune()
+    log("autotuned lidar wavefronts per car:", waves_per_car)
     # march iterations of the lidar, counted by the instrumented variant on the warm state (untimed)
     run.sim.count_iterations(True)
     counts = []
@@ -151,7 +179,7 @@ def main() -> int:
     lidar_us = lidar_ms / max(lidar_n, 1) * 1000.0
     iters = float(np.mean(counts))
     lbytes = lidar_bytes(iters, args.cars, args.beams, layers)
-    gemm_ms = sum(tot for k, (n, tot) in prof.items() if k.startswith("gemm"))
+    gemm_ms = sum(tot for k, (n, tot) in prof.items() if k.startswith(("gemm", "policy", "head")))
     flops = policy_flops(args.cars, args.beams, args.hidden)
     traffic = None
     tfile = ROOT / "profiles" / "lidar_traffic.json"
@@ -172,6 +200,10 @@ def main() -> int:
               "unit": "TFLOP/s", "frac": gemm_tflops / peak if gemm_tflops else None,
               "flops_per_step": flops}
     kernels = {k: {"launches": n, "avg_us": tot / max(n, 1) * 1000.0} for k, (n, tot) in prof.items()}
+    plain = None
+    if not args.no_plain:
+        plain = plain_baseline(R, dev, arr, args, min(args.steps, 50))
+        log("plain GPU baseline:", plain)
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -187,8 +219,10 @@ def main() -> int:
                                f"{args.cars} cars x {args.beams} beams; 2x{args.hidden} tanh policy "
                                f"({args.precision} GEMMs, fp32 dynamics and lidar)",
                    "cars": args.cars, "beams": args.beams, "hidden": args.hidden, "track": args.track,
-                   "policy_precision": args.precision, "parallelism": "single GPU"},
+                   "policy_precision": args.precision, "parallelism": "single GPU",
+                   "lidar_waves_per_car": waves_per_car},
         "roofline": roofline, "policy_roofline": policy, "kernels": kernels, "cpu_baseline": cpu,
+        "plain_gpu_baseline": plain, "speedup_vs_plain_gpu": value / plain["value"] if plain else None,
     }
     print(json.dumps(out), flush=True)
     return 0

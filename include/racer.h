@@ -50,7 +50,9 @@ typedef struct {
     int rollout;        /* ROLLOUT steps per iteration (32) */
     int epochs;         /* EPOCHS (4) */
     int minibatches;    /* MINIBATCHES (4) */
-    int precision;      /* policy GEMMs: 0 = float32, 1 = bf16 MFMA with float32 accumulation */
+    int precision;      /* policy products: 0 = float32, 1 = bf16 MFMA with float32 accumulation,
+                           2 = float32 in the reference's decomposition (three products, then a
+                           separate sample kernel), kept to time the plain baseline */
     float heading_noise;/* spawn heading noise U(-a, a) in radians; 0 = reference behaviour */
 } racer_config;
 
@@ -188,6 +190,13 @@ int64_t racer_sim_last_iterations(racer_sim* sim);
  * reference's 2-D Device.run (racer/device.mojo:67-79), for timing the plain
  * baseline. Both give identical results. */
 int racer_sim_set_schedule(racer_sim* sim, int schedule);
+/* Wavefronts that share one car's beams in the default schedule (1, 2 or 4;
+ * results are identical for every choice). */
+int racer_sim_set_waves_per_car(racer_sim* sim, int waves);
+int racer_sim_waves_per_car(const racer_sim* sim);
+/* Autotuner: times the lidar with 1, 2 and 4 wavefronts per car on the current
+ * state and keeps the fastest; writes the choice. Call after reset or a step. */
+int racer_sim_autotune(racer_sim* sim, int* chosen);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,8 @@ def lib():
             "o_surface": (I, [P, C.c_size_t, I, F, F, P]),
             "o_progress": (F, [P, C.c_size_t, I, F, F]),
             "o_scan_one": (F, [P, C.c_size_t, P, L, P, L, L, I, I, P]),
+            "o_check_nearest_index": (L, [F, F, F]),
+            "o_check_min_quotient": (L, [L, C.c_uint64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(_lib, name)

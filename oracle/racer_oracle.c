@@ -590,3 +590,45 @@ float o_scan_one(const float* d, size_t len, const float* s, long n, const float
     if (om_init(&m, d, len)) return 0.0f;
     return o_scan(&m, d, s, n, obs, obs_cols, i, beam, beams, iterations);
 }
+
+/* ---------------- checks of the shared header's division shortcuts ---------------- */
+
+/* Every float a in [lo, hi] (walking the ordered bit patterns): number of a
+ * where racer_nearest_index(a, b, 1/b) differs from floorf(a / b + 0.5f). */
+long o_check_nearest_index(float b, float lo, float hi) {
+    float inv = 1.0f / b;
+    uint32_t ulo = racer_f2u(lo), uhi = racer_f2u(hi);
+    int64_t olo = (ulo & 0x80000000u) ? -(int64_t)(ulo & 0x7fffffffu) : (int64_t)ulo;
+    int64_t ohi = (uhi & 0x80000000u) ? -(int64_t)(uhi & 0x7fffffffu) : (int64_t)uhi;
+    long bad = 0;
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(+ : bad)
+    for (int64_t o = olo; o <= ohi; ++o) {
+        uint32_t u = o < 0 ? (uint32_t)(-o) | 0x80000000u : (uint32_t)o;
+        float a = racer_u2f(u);
+        float want = floorf(a / b + 0.5f);
+        float got = racer_nearest_index(a, b, inv);
+        if (racer_f2u(want) != racer_f2u(got)) ++bad;
+    }
+    return bad;
+}
+
+/* `count` seeded random (cap, num, den) triples: mismatches of
+ * racer_min_quotient against racer_min(cap, num / den). */
+long o_check_min_quotient(long count, uint64_t seed) {
+    long bad = 0;
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(+ : bad)
+    for (long k = 0; k < count; ++k) {
+        racer_u32x4 r = racer_random(seed, (uint64_t)k, 0);
+        float cap = (racer_uniform(r.v[0]) - 0.2f) * 3.0f;
+        float num = racer_uniform(r.v[1]) * 0.5f;
+        float den = racer_uniform(r.v[2]) * 0.3f;
+        if ((r.v[3] & 15u) == 0) den = 0.0f;
+        if ((r.v[3] & 240u) == 16u) num = 0.0f;
+        if ((r.v[3] & 3840u) == 256u && den > 0.0f) cap = num / den; /* ties at the boundary */
+        float inv = 1.0f / den;
+        float want = racer_min(cap, num / den);
+        float got = racer_min_quotient(cap, num, den, inv);
+        if (racer_f2u(want) != racer_f2u(got)) ++bad;
+    }
+    return bad;
+}

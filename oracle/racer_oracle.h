@@ -91,6 +91,10 @@ float o_progress(const float* d, size_t len, int layer, float x, float y);
 float o_scan_one(const float* d, size_t len, const float* s, long n, const float* obs, long obs_cols, long i, int beam,
                  int beams, long* iterations);
 
+/* Exhaustive / sampled checks of the shared header's division shortcuts. */
+long o_check_nearest_index(float b, float lo, float hi);
+long o_check_min_quotient(long count, uint64_t seed);
+
 /* Threads used by the parallel loops (1 = serial). */
 void o_set_threads(int threads);
 int o_get_threads(void);

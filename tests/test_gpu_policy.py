@@ -106,3 +106,54 @@ def test_backward_against_oracle(dev, precision):
         np.testing.assert_array_equal(gd2, rd2)
         scale = np.abs(rgrad[: o.LOGSTD]).max()
         assert np.max(np.abs(gg[: o.LOGSTD] - rgrad[: o.LOGSTD])) <= 2e-2 * scale
+
+
+def _bf16(a):
+    """Round float32 to bf16 (round to nearest even), returned as float64."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("beams,hidden,cols", [(1081, 256, 200), (108, 64, 77), (37, 130, 33)])
+def test_fused_bf16_forward_and_sample(dev, beams, hidden, cols):
+    """The fused bf16 forward (Runner.step's policy half) against a float64
+    evaluation of the same bf16-rounded operands; the float32 output layer and
+    the sampled actions against the oracle given the kernel's h2 / outputs."""
+    print(f"fused bf16 forward {beams}/{hidden}/{cols}", flush=True)
+    from helpers import track
+
+    cfg = R.Config(beams=beams, hidden=hidden, precision=1)
+    tr = track("ramp10")
+    sim = R.Sim(dev, tr, cols, 5, cfg)
+    pol = R.Policy(dev, 42, cfg)
+    acts = R.Activations(dev, cols, cfg)
+    scratch = dev.alloc(2 * cols)
+    sim.reset()
+    x = sim.obs.read().reshape(cfg.IN, cols).astype(np.float64)
+    R._check(R.lib().racer_runner_step(sim.handle, pol.handle, acts.handle, scratch.handle, 0, 9))
+    th = pol.theta.read()
+    H, IN, HA = hidden, cfg.IN, cfg.HA
+    W0 = th[: H * IN].reshape(H, IN)
+    W1 = th[H * IN : H * IN + H * HA].reshape(H, HA)
+    W2 = th[H * IN + H * HA : H * IN + H * HA + 3 * HA].reshape(3, HA).astype(np.float64)
+    h1 = acts.h1.read().reshape(HA, cols)
+    h2 = acts.h2.read().reshape(HA, cols)
+    o = acts.o.read().reshape(3, cols)
+    a0, xb = _bf16(W0), _bf16(x)
+    ref1 = np.tanh(a0 @ xb)
+    assert np.all(np.abs(h1[:H] - ref1) <= 1e-5 * (np.abs(a0) @ np.abs(xb)) + 2e-6)
+    assert np.all(h1[H] == 1) and np.all(h2[H] == 1)
+    a1, hb = _bf16(W1), _bf16(h1)
+    ref2 = np.tanh(a1 @ hb)
+    assert np.all(np.abs(h2[:H] - ref2) <= 1e-5 * (np.abs(a1) @ np.abs(hb)) + 2e-6)
+    ref_o = W2 @ h2.astype(np.float64)
+    assert np.all(np.abs(o - ref_o) <= 1e-6 * (np.abs(W2) @ np.abs(h2.astype(np.float64))) + 1e-7)
+    # the sampled actions follow from o exactly as the oracle's Sim.sample does
+    osim = O.Sim(tr, cols, 5, beams=beams)
+    oa, ol, ov = np.zeros((2, cols), np.float32), np.zeros(cols, np.float32), np.zeros(cols, np.float32)
+    osim.sample(o, th[-2:], oa, ol, ov, False, 9)
+    # Runner.step stepped the sim after sampling; its actions buffer holds the sampled actions
+    assert np.array_equal(sim.actions.read().reshape(2, cols), oa)
+    sc = scratch.read()
+    assert np.array_equal(sc[:cols], ol) and np.array_equal(sc[cols:], ov)

@@ -160,3 +160,27 @@ def test_reference_schedule_identical(dev):
         b.step()
     assert np.array_equal(a.obs.read(), b.obs.read())
     assert np.array_equal(a.state.read(), b.state.read())
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_waves_per_car_identical(dev, waves):
+    """Splitting a car's beams over 1, 2 or 4 wavefronts gives the oracle's bits."""
+    print(f"waves per car {waves}", flush=True)
+    O.set_threads(8)
+    g, o = _pair(dev, "bridge8", 40, 13, 1081)
+    g.set_waves_per_car(waves)
+    g.reset()
+    o.reset()
+    for _ in range(2):
+        g.step()
+        o.step()
+    _assert_same(g, o, f"waves {waves}")
+
+
+def test_autotune_keeps_results(dev):
+    g, o = _pair(dev, "ramp10", 64, 21, 1081)
+    g.reset()
+    o.reset()
+    chosen = g.autotune()
+    assert chosen in (1, 2, 4) and g.waves_per_car == chosen
+    _assert_same(g, o, "after autotune")

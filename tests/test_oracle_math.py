@@ -63,3 +63,15 @@ def test_uniform_and_normal_statistics():
     # distinct (seed, subsequence, offset) give distinct streams
     assert not np.array_equal(O.normal4(1, 0, 0), O.normal4(1, 1, 0))
     assert not np.array_equal(O.normal4(1, 0, 0), O.normal4(1, 0, 1))
+
+
+@pytest.mark.parametrize("cell", [0.025, 0.05])
+def test_nearest_index_shortcut_exhaustive(cell):
+    """racer_nearest_index equals floorf(a / cell + 0.5f) for every float a in [-40, 40]."""
+    O.set_threads(8)
+    assert O.lib().o_check_nearest_index(cell, -40.0, 40.0) == 0
+
+
+def test_min_quotient_shortcut_sampled():
+    O.set_threads(8)
+    assert O.lib().o_check_min_quotient(4_000_000, 11) == 0

@@ -139,6 +139,9 @@ def lib() -> C.CDLL:
         "racer_sim_count_iterations": (i, [vp, i]),
         "racer_sim_last_iterations": (i64, [vp]),
         "racer_sim_set_schedule": (i, [vp, i]),
+        "racer_sim_set_waves_per_car": (i, [vp, i]),
+        "racer_sim_waves_per_car": (i, [vp]),
+        "racer_sim_autotune": (i, [vp, C.POINTER(C.c_int)]),
     }
     for name in ("obs", "actions", "logp", "value", "reward", "done", "advantage", "grad", "d3", "stats", "acts"):
         sigs[f"racer_trainer_{name}"] = (vp, [vp])
@@ -336,6 +339,19 @@ class Sim:
     def set_schedule(self, schedule: int) -> None:
         """0: wave per car with beam refill (default); 1: thread per (car, beam) as the reference launches it."""
         _check(lib().racer_sim_set_schedule(self.handle, int(schedule)))
+
+    def set_waves_per_car(self, waves: int) -> None:
+        _check(lib().racer_sim_set_waves_per_car(self.handle, int(waves)))
+
+    @property
+    def waves_per_car(self) -> int:
+        return int(lib().racer_sim_waves_per_car(self.handle))
+
+    def autotune(self) -> int:
+        """Time the lidar with 1, 2 and 4 wavefronts per car; keep and return the fastest."""
+        w = C.c_int()
+        _check(lib().racer_sim_autotune(self.handle, C.byref(w)))
+        return int(w.value)
 
     @property
     def last_iterations(self) -> int:

@@ -108,6 +108,78 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const void* __restrict__ Ap, 
     }
 }
 
+// Forward products C[M, N] = epi(A[M, K] B[K, N]) with A a row-padded bf16
+// weight copy and B float32 activations (one column per sample). 64 x 64
+// tiles, K in stages of 64: while the four wavefronts run a stage's eight
+// MFMAs from LDS, the next stage's operands are already loading into
+// registers. Each thread gathers 16 consecutive k of one column (loads that
+// are contiguous across the wavefront), rounds them to bf16 and writes them to
+// LDS as two 16-byte vectors, so LDS holds B as [column][k].
+template <int EPI>
+__global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict__ A, int64_t lda,
+                                                      const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
+                                                      int64_t ldc, int M, int N, int K) {
+    constexpr int SK = 64, LD = SK + 8;
+    __shared__ __attribute__((aligned(16))) uint16_t As[TM][LD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][LD];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int arow = tid >> 2, akc = (tid & 3) * 16;  // A: 16 consecutive k of one row
+    const int bcol = tid & 63, bkc = (tid >> 6) * 16;  // B: 16 consecutive k of one column
+    uint4 ra[2];
+    float rb[16];
+    auto load = [&](int k0) {
+        const int m = m0 + arow;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = k0 + akc + 8 * h;
+            ra[h] = (m < M && k < K) ? *reinterpret_cast<const uint4*>(A + (int64_t)m * lda + k)
+                                     : make_uint4(0, 0, 0, 0);
+        }
+        const int n = n0 + bcol;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int k = k0 + bkc + e;
+            rb[e] = (n < N && k < K) ? B[(int64_t)k * ldb + n] : 0.0f;
+        }
+    };
+    f32x16 acc;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0f;
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += SK) {
+        *reinterpret_cast<uint4*>(&As[arow][akc]) = ra[0];
+        *reinterpret_cast<uint4*>(&As[arow][akc + 8]) = ra[1];
+        uint16_t v[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = to_bf16(rb[e]);
+        *reinterpret_cast<uint4*>(&Bs[bcol][bkc]) = *reinterpret_cast<uint4*>(v);
+        *reinterpret_cast<uint4*>(&Bs[bcol][bkc + 8]) = *reinterpret_cast<uint4*>(v + 8);
+        __syncthreads();
+        if (k0 + SK < K) load(k0 + SK);
+#pragma unroll
+        for (int ks = 0; ks < SK; ks += 16) {
+            const int r = lane & 31, kb = (lane >> 5) * 8;
+            bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm * 32 + r][ks + kb]);
+            bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn * 32 + r][ks + kb]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const int j = n0 + wn * 32 + (lane & 31);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int i = m0 + wm * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+        if (i < M && j < N) {
+            float v = acc[q];
+            if constexpr (EPI == RACER_EPI_TANH) v = racer_tanh(v);
+            C[(int64_t)i * ldc + j] = v;
+        }
+    }
+}
+
 __global__ void k_pack_bf16(const float* src, int64_t ld_src, uint16_t* dst, int64_t ld_dst, int rows, int cols) {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)rows * ld_dst) return;
@@ -145,11 +217,11 @@ int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const 
     dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, 1), block(256);
     racer_timed timer(dev, epi == RACER_EPI_TANH ? "gemm_bf16_tanh" : "gemm_bf16");
     if (epi == RACER_EPI_TANH)
-        hipLaunchKernelGGL((k_gemm_bf16<true, false, RACER_EPI_TANH>), grid, block, 0, dev->stream, A, lda, B, ldb, C,
-                           ldc, (int64_t)0, M, N, K, K);
+        hipLaunchKernelGGL(k_gemm_bf16_nn<RACER_EPI_TANH>, grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc, M, N,
+                           K);
     else
-        hipLaunchKernelGGL((k_gemm_bf16<true, false, RACER_EPI_NONE>), grid, block, 0, dev->stream, A, lda, B, ldb, C,
-                           ldc, (int64_t)0, M, N, K, K);
+        hipLaunchKernelGGL(k_gemm_bf16_nn<RACER_EPI_NONE>, grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc, M, N,
+                           K);
     return racer_launch_check("gemm_nn_bf16");
 }
 

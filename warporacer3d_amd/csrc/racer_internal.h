@@ -84,6 +84,7 @@ struct racer_dmap {
     const float* packed;   // per-cell records (clearance, height) for every layer, or null
     int nx, ny, layers, waypoints;
     float x0, y0, cell, spacing, slope;
+    float inv_cell;        // RN(1 / cell), for the division shortcut of the nearest-cell index
     int64_t route;         // offset of the waypoint table
     int64_t plane;         // nx * ny
 };
@@ -98,6 +99,7 @@ struct racer_sim {
     float2* dirs = nullptr;  // (cos, sin) of every beam's azimuth
     racer_dmap map{};
     int schedule = 0;  // lidar launch: 0 wave per car with beam refill, 1 thread per (car, beam)
+    int waves_per_car = 2;  // wavefronts sharing one car's beams (1, 2 or 4); see racer_sim_autotune
     bool count_iterations = false;
     unsigned long long* counter = nullptr;
     int64_t last_iterations = 0;
@@ -109,7 +111,7 @@ struct racer_policy {
     int beams = 0, hidden = 0, precision = 0;
     int64_t IN = 0, HA = 0, P = 0, W1 = 0, W2 = 0, LOGSTD = 0;
     racer_buf *theta = nullptr, *w1t = nullptr, *w2t = nullptr;
-    uint16_t* theta_bf16 = nullptr;  // bf16 W0 [H, pad8(IN)] then W1 [H, pad8(HA)] for the MFMA path
+    uint16_t* theta_bf16 = nullptr;  // bf16 W0 [H, pad32(IN)] then W1 [H, pad32(HA)], zero padded, for MFMA
     int64_t in_pad = 0, ha_pad = 0;
     bool bf16_stale = true;
 };
@@ -137,6 +139,9 @@ int racer_launch_reset(racer_sim* sim);
 int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done, float* obs, int64_t obs_cols);
 int racer_launch_sample(racer_sim* sim, const float* out, int64_t out_cols, const float* log_std, float* actions,
                         float* logp, float* value, int deterministic, int64_t step);
+int racer_launch_sample_raw(racer_device* dev, int n, const float* out, int64_t out_cols, const float* log_std,
+                            float* actions, int64_t act_cols, float* logp, float* value, int deterministic,
+                            int64_t step, int seed);
 
 // GEMM epilogues
 enum { RACER_EPI_NONE = 0, RACER_EPI_TANH = 1, RACER_EPI_MASK = 2 };
@@ -157,5 +162,9 @@ int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const flo
 int racer_pack_bf16(racer_device* dev, const float* src, int64_t ld_src, uint16_t* dst, int64_t ld_dst, int rows,
                     int cols);
 inline int64_t racer_pad8(int64_t v) { return (v + 7) / 8 * 8; }
+inline int64_t racer_pad32(int64_t v) { return (v + 31) / 32 * 32; }
+// The fused bf16 policy forward (and optionally Sim.sample) over `cols` columns of x.
+int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
+                       float* act, float* logp, float* value, int deterministic, int64_t step, int seed);
 
 int racer_policy_forward_raw(racer_policy* pol, const float* x, int64_t x_cols, racer_acts* acts, int cols);

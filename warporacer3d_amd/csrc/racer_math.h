@@ -194,6 +194,33 @@ RACER_FN float racer_tanh(float x) {
     return x < 0.0f ? -y : y;
 }
 
+/* floorf(a / b + 0.5f) with a / b correctly rounded, as a reciprocal multiply
+ * in the common case. inv_b = RN(1/b). q0 = RN(a * inv_b) differs from
+ * q = RN(a / b) by at most 3 ulp(q) (relative errors of 2^-24 from inv_b, 2^-24
+ * from the product, 2^-24 from q itself), and s0 = RN(q0 + 0.5) from
+ * RN(q + 0.5) by at most 3 ulp(q) + ulp(q + 0.5) <= 5 ulp(max(|q0|, 1)). So
+ * when s0 is more than (|q0| + 1) 2^-20 (> 8 such ulps) away from both
+ * neighbouring integers, both floors agree; otherwise the true division runs. */
+RACER_FN float racer_nearest_index(float a, float b, float inv_b) {
+    float q0 = a * inv_b;
+    float s0 = q0 + 0.5f;
+    float f0 = floorf(s0);
+    float margin = (racer_abs(q0) + 1.0f) * 0x1p-20f;
+    if (s0 - f0 > margin && (f0 + 1.0f) - s0 > margin) return f0;
+    return floorf(a / b + 0.5f);
+}
+
+/* racer_min(cap, num / den) with num / den correctly rounded, skipping the
+ * division when the reciprocal estimate is clearly above cap (then so is the
+ * exact quotient: |q0 - q| <= 2^-22 |q|). den >= 0; inv_den = RN(1/den),
+ * +inf when den is 0 or subnormal (then num / den is +inf or NaN, and
+ * racer_min returns cap either way when the estimate is +inf). */
+RACER_FN float racer_min_quotient(float cap, float num, float den, float inv_den) {
+    float q0 = num * inv_den;
+    if (q0 > cap + racer_abs(cap) * 0x1p-20f + 0x1p-126f) return cap;
+    return racer_min(cap, num / den);
+}
+
 /* ---- Philox4x32-10 ---- */
 typedef struct {
     uint32_t v[4];

@@ -158,6 +158,75 @@ __global__ void k_adam(float* theta, int64_t P, int64_t logstd_at, const float* 
 
 __global__ void k_store_norm(const double* pair, float* stats) { stats[RACER_STAT_NORM] = (float)pair[1]; }
 
+// Output layer o = W2 h2 (racer/ppo.mojo:111) for HEAD_COLS samples per
+// workgroup: the workgroup stages the [HA, HEAD_COLS] tile of h2 in LDS with
+// row-contiguous loads, then 3 x HEAD_COLS threads each accumulate one output
+// with fmaf over k = 0..HA-1 in order (the order of the float32 products).
+// When SAMPLE, Sim.sample's Gaussian action, log probability and value
+// (racer/sim.mojo:337-348) follow from those outputs.
+constexpr int HEAD_COLS = 32;
+
+template <bool SAMPLE>
+__global__ void __launch_bounds__(256) k_head(const float* __restrict__ W2, const float* __restrict__ h2, int HA,
+                                              int64_t cols, float* __restrict__ o, const float* log_std,
+                                              float* act, int64_t act_cols, float* logp, float* value,
+                                              int deterministic, int64_t step, int seed) {
+    extern __shared__ float tile[];  // [HA][HEAD_COLS], then W2 [3][HA], then out[3][HEAD_COLS]
+    float* wl = tile + HA * HEAD_COLS;
+    float* out = wl + 3 * HA;
+    const int64_t c0 = (int64_t)blockIdx.x * HEAD_COLS;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < HA * HEAD_COLS; idx += blockDim.x) {
+        int k = idx / HEAD_COLS, c = idx % HEAD_COLS;
+        tile[idx] = c0 + c < cols ? h2[(int64_t)k * cols + c0 + c] : 0.0f;
+    }
+    for (int idx = tid; idx < 3 * HA; idx += blockDim.x) wl[idx] = W2[idx];
+    __syncthreads();
+    if (tid < 3 * HEAD_COLS) {
+        const int j = tid / HEAD_COLS, c = tid % HEAD_COLS;
+        const float* w = wl + j * HA;
+        float acc = 0.0f;
+        for (int k = 0; k < HA; ++k) acc = fmaf(w[k], tile[k * HEAD_COLS + c], acc);
+        out[j * HEAD_COLS + c] = acc;
+        if (c0 + c < cols) o[j * cols + c0 + c] = acc;
+    }
+    if constexpr (SAMPLE) {
+        __syncthreads();
+        const int64_t col = c0 + tid;
+        if (tid < HEAD_COLS && col < cols) {
+            float noise[4];
+            racer_normal4(racer_random((uint64_t)(int64_t)seed + 1, (uint64_t)col, (uint64_t)step), noise);
+            float probability = 0.0f;
+            for (int a = 0; a < 2; ++a) {
+                float z = deterministic ? 0.0f : noise[a];
+                float ls = log_std[a];
+                act[a * act_cols + col] = out[a * HEAD_COLS + tid] + racer_exp(ls) * z;
+                probability = probability - (((0.5f * z) * z + ls) + RACER_HALF_LOG_2PI);
+            }
+            logp[col] = probability;
+            value[col] = out[2 * HEAD_COLS + tid];
+        }
+    }
+}
+
+int launch_head(racer_policy* p, const float* h2, int64_t cols, float* o, bool sample, float* act, float* logp,
+                float* value, int deterministic, int64_t step, int seed) {
+    const float* W2 = p->theta->ptr + p->W2;
+    const float* log_std = p->theta->ptr + p->LOGSTD;
+    unsigned blocks = (unsigned)((cols + HEAD_COLS - 1) / HEAD_COLS);
+    size_t shmem = ((size_t)(p->HA + 3) * HEAD_COLS + 3 * (size_t)p->HA) * sizeof(float);
+    if (sample) {
+        racer_timed timer(p->dev, "head_sample");
+        hipLaunchKernelGGL(k_head<true>, dim3(blocks), dim3(128), shmem, p->dev->stream, W2, h2, (int)p->HA, cols, o,
+                           log_std, act, cols, logp, value, deterministic, step, seed);
+    } else {
+        racer_timed timer(p->dev, "head");
+        hipLaunchKernelGGL(k_head<false>, dim3(blocks), dim3(128), shmem, p->dev->stream, W2, h2, (int)p->HA, cols,
+                           o, log_std, act, cols, logp, value, deterministic, step, seed);
+    }
+    return racer_launch_check("head");
+}
+
 int row_sums(racer_device* dev, const float* x, int64_t count, int64_t row_stride, int rows, int square,
              double* out64, float* out32, int64_t slot) {
     int outputs = rows * (1 + square);
@@ -220,6 +289,8 @@ extern "C" int racer_policy_create(racer_device* dev, int seed, const racer_conf
     racer_config_default(&def);
     if (!cfg) cfg = &def;
     RACER_REQUIRE(cfg->beams >= 2 && cfg->hidden >= 1, "bad network shape");
+    RACER_REQUIRE(cfg->hidden <= 460, "hidden width above 460 is not supported (the output layer stages h2 in LDS)");
+    RACER_REQUIRE(cfg->precision >= 0 && cfg->precision <= 2, "precision must be 0 (float32), 1 (bf16) or 2 (plain)");
     auto* p = new racer_policy();
     p->dev = dev;
     p->beams = cfg->beams;
@@ -237,8 +308,8 @@ extern "C" int racer_policy_create(racer_device* dev, int seed, const racer_conf
         racer_policy_destroy(p);
         return 1;
     }
-    p->in_pad = racer_pad8(p->IN);
-    p->ha_pad = racer_pad8(p->HA);
+    p->in_pad = racer_pad32(p->IN);
+    p->ha_pad = racer_pad32(p->HA);
     RACER_TRY(hipMalloc(&p->theta_bf16, (size_t)(H * (p->in_pad + p->ha_pad)) * sizeof(uint16_t)));
     hipLaunchKernelGGL(k_policy_init, dim3((unsigned)((p->P + 255) / 256)), dim3(256), 0, dev->stream, p->theta->ptr,
                        p->P, p->IN, p->IN - 1, H, p->HA, p->W1, p->W2, p->LOGSTD, seed);
@@ -283,18 +354,28 @@ extern "C" int racer_policy_transpose(racer_policy* p) {
     return racer_launch_check("transpose");
 }
 
-int racer_policy_forward_raw(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols) {
+// Hidden layers of the forward pass (racer/ppo.mojo:109-110).
+// The bf16 weight copies used by the MFMA products, refreshed after updates.
+static int refresh_bf16(racer_policy* p) {
+    if (!p->bf16_stale) return 0;
+    const float* th = p->theta->ptr;
+    int H = p->hidden;
+    uint16_t* w1 = p->theta_bf16 + (int64_t)H * p->in_pad;
+    if (racer_pack_bf16(p->dev, th, p->IN, p->theta_bf16, p->in_pad, H, (int)p->IN) ||
+        racer_pack_bf16(p->dev, th + p->W1, p->HA, w1, p->ha_pad, H, (int)p->HA))
+        return 1;
+    p->bf16_stale = false;
+    return 0;
+}
+
+// Hidden layers of the forward pass (racer/ppo.mojo:109-110) as two products.
+static int forward_hidden(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols) {
     const float* th = p->theta->ptr;
     int H = p->hidden;
     if (p->precision == 1) {
+        if (refresh_bf16(p)) return 1;
         uint16_t* w0 = p->theta_bf16;
         uint16_t* w1 = p->theta_bf16 + (int64_t)H * p->in_pad;
-        if (p->bf16_stale) {
-            if (racer_pack_bf16(p->dev, th, p->IN, w0, p->in_pad, H, (int)p->IN) ||
-                racer_pack_bf16(p->dev, th + p->W1, p->HA, w1, p->ha_pad, H, (int)p->HA))
-                return 1;
-            p->bf16_stale = false;
-        }
         if (racer_gemm_nn_bf16(p->dev, w0, p->in_pad, x, x_cols, a->h1->ptr, cols, H, cols, (int)p->IN,
                                RACER_EPI_TANH) ||
             racer_gemm_nn_bf16(p->dev, w1, p->ha_pad, a->h1->ptr, cols, a->h2->ptr, cols, H, cols, (int)p->HA,
@@ -307,8 +388,40 @@ int racer_policy_forward_raw(racer_policy* p, const float* x, int64_t x_cols, ra
                               RACER_EPI_TANH, nullptr, 0))
             return 1;
     }
-    return racer_gemm_nn_f32(p->dev, th + p->W2, p->HA, a->h2->ptr, cols, a->o->ptr, cols, RACER_OUT, cols,
-                             (int)p->HA, RACER_EPI_NONE, nullptr, 0);
+    return 0;
+}
+
+// The bf16 forward runs as one fused kernel when the hidden width fits its
+// workgroup (eight wavefronts of 32 rows).
+static bool use_fused(const racer_policy* p) { return p->precision == 1 && p->hidden <= 256; }
+
+int racer_policy_forward_raw(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols) {
+    if (use_fused(p)) {
+        if (refresh_bf16(p)) return 1;
+        return racer_policy_fused(p, x, x_cols, a, cols, false, nullptr, nullptr, nullptr, 0, 0, 0);
+    }
+    if (forward_hidden(p, x, x_cols, a, cols)) return 1;
+    if (p->precision == 2)  // the reference's decomposition: the output layer as a third product
+        return racer_gemm_nn_f32(p->dev, p->theta->ptr + p->W2, p->HA, a->h2->ptr, cols, a->o->ptr, cols, RACER_OUT,
+                                 cols, (int)p->HA, RACER_EPI_NONE, nullptr, 0);
+    return launch_head(p, a->h2->ptr, cols, a->o->ptr, false, nullptr, nullptr, nullptr, 0, 0, 0);
+}
+
+// Policy forward plus Sim.sample for one column per car (main.mojo:186-190,
+// racer/ppo.mojo:192-202).
+int racer_policy_act(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, float* act,
+                     float* logp, float* value, int deterministic, int64_t step, int seed) {
+    if (use_fused(p)) {
+        if (refresh_bf16(p)) return 1;
+        return racer_policy_fused(p, x, x_cols, a, cols, true, act, logp, value, deterministic, step, seed);
+    }
+    if (p->precision == 2) {  // three products, then a separate sample kernel (racer/sim.mojo:323-348)
+        if (racer_policy_forward_raw(p, x, x_cols, a, cols)) return 1;
+        return racer_launch_sample_raw(p->dev, cols, a->o->ptr, cols, p->theta->ptr + p->LOGSTD, act, cols, logp,
+                                       value, deterministic, step, seed);
+    }
+    if (forward_hidden(p, x, x_cols, a, cols)) return 1;
+    return launch_head(p, a->h2->ptr, cols, a->o->ptr, true, act, logp, value, deterministic, step, seed);
 }
 
 extern "C" int racer_policy_forward(racer_policy* p, const racer_view* x_p, racer_acts* a) {
@@ -426,9 +539,8 @@ extern "C" int racer_trainer_rollout(racer_trainer* tr, racer_sim* sim, racer_po
                              dev->stream));
     for (int64_t t = 0; t < R; ++t) {
         float* obs_t = tr->obs->ptr + t * IN * n;
-        if (racer_policy_forward_raw(p, obs_t, n, acts, (int)n)) return 1;
-        if (racer_launch_sample(sim, acts->o->ptr, n, p->theta->ptr + p->LOGSTD, tr->actions->ptr + t * 2 * n,
-                                tr->logp->ptr + t * n, tr->value->ptr + t * n, 0, (int64_t)iteration * R + t))
+        if (racer_policy_act(p, obs_t, n, acts, (int)n, tr->actions->ptr + t * 2 * n, tr->logp->ptr + t * n,
+                             tr->value->ptr + t * n, 0, (int64_t)iteration * R + t, sim->seed))
             return 1;
         if (racer_launch_step(sim, tr->actions->ptr + t * 2 * n, tr->reward->ptr + t * n, tr->done->ptr + t * n,
                               tr->obs->ptr + (t + 1) * IN * n, n))
@@ -616,9 +728,8 @@ extern "C" int racer_runner_step(racer_sim* sim, racer_policy* p, racer_acts* ac
     RACER_REQUIRE(p->beams == sim->beams && acts->cols == sim->n && acts->hidden == p->hidden,
                   "runner shapes differ");
     RACER_REQUIRE(scratch && scratch->len >= (size_t)(2 * n), "scratch must hold 2 x cars floats");
-    if (racer_policy_forward_raw(p, sim->obs->ptr, n, acts, (int)n)) return 1;
-    if (racer_launch_sample(sim, acts->o->ptr, n, p->theta->ptr + p->LOGSTD, sim->actions->ptr, scratch->ptr,
-                            scratch->ptr + n, deterministic, t))
+    if (racer_policy_act(p, sim->obs->ptr, n, acts, (int)n, sim->actions->ptr, scratch->ptr, scratch->ptr + n,
+                         deterministic, t, sim->seed))
         return 1;
     return racer_launch_step(sim, sim->actions->ptr, sim->reward->ptr, sim->done->ptr, sim->obs->ptr, n);
 }

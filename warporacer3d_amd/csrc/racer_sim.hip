@@ -23,8 +23,8 @@ struct Surf {
 
 // Nearest cell of (x, y) (racer/sim.mojo:108-114); false off the map.
 __device__ __forceinline__ bool cell_of(const racer_dmap& m, float x, float y, int64_t* cell) {
-    float fx = floorf((x - m.x0) / m.cell + 0.5f);
-    float fy = floorf((y - m.y0) / m.cell + 0.5f);
+    float fx = racer_nearest_index(x - m.x0, m.cell, m.inv_cell);
+    float fy = racer_nearest_index(y - m.y0, m.cell, m.inv_cell);
     if (!(fx >= 0.0f) || !(fy >= 0.0f) || fx >= (float)m.nx || fy >= (float)m.ny) return false;
     *cell = (int64_t)fy * m.nx + (int64_t)fx;
     return true;
@@ -141,13 +141,79 @@ __device__ void spawn(const racer_dmap& m, float* s, int n, int i, int seed, flo
     s[RACER_FRICTION * n + i] = 0.85f + 0.3f * u1;
 }
 
+// Map.probe at (x, y, z) and Map.surface of the probed layer at (x, y)
+// (racer/sim.mojo:116-140). With L = 1 or 2 layers both read the packed
+// (clearance, height) records: the probe cell and the four bilinear corners
+// are loaded together, every layer's heights at once, and the probed layer's
+// are selected afterwards, so the five loads are one round trip. L = 0 reads
+// the track array layer by layer.
+template <int L>
+__device__ __forceinline__ void probe_surface(const racer_dmap& m, float x, float y, float z, Spot* spot, Surf* g) {
+    if constexpr (L == 0) {
+        *spot = map_probe(m, x, y, z);
+        *g = map_surface(m, spot->layer, x, y);
+    } else {
+        const float2* rec = reinterpret_cast<const float2*>(m.packed);
+        int64_t c = 0;
+        const bool on = cell_of(m, x, y, &c);
+        float fx = (x - m.x0) / m.cell;
+        float fy = (y - m.y0) / m.cell;
+        float cx = floorf(fx), cy = floorf(fy);
+        int64_t ix = !(cx >= 0.0f) ? 0 : (cx > (float)(m.nx - 2) ? m.nx - 2 : (int64_t)cx);
+        int64_t iy = !(cy >= 0.0f) ? 0 : (cy > (float)(m.ny - 2) ? m.ny - 2 : (int64_t)cy);
+        float tx = racer_clamp(fx - (float)ix, 0.0f, 1.0f);
+        float ty = racer_clamp(fy - (float)iy, 0.0f, 1.0f);
+        const int64_t k = iy * m.nx + ix;
+        float2 pr[L], r0[L], r1[L], r2[L], r3[L];
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            pr[l] = on ? rec[c * L + l] : make_float2(-1.0f, -1.0f);
+            r0[l] = rec[k * L + l];
+            r1[l] = rec[(k + 1) * L + l];
+            r2[l] = rec[(k + m.nx) * L + l];
+            r3[l] = rec[(k + m.nx + 1) * L + l];
+        }
+        Spot best{0, -1.0f, pr[0].y};
+        float gap = 1e30f;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            if (pr[l].x > 0.0f && fabsf(pr[l].y - z) < gap) {
+                best.layer = l;
+                best.clearance = pr[l].x;
+                best.height = pr[l].y;
+                gap = fabsf(pr[l].y - z);
+            }
+        }
+        *spot = best;
+        float h0 = r0[0].y, h1 = r1[0].y, h2 = r2[0].y, h3 = r3[0].y;
+#pragma unroll
+        for (int l = 1; l < L; ++l) {
+            if (best.layer == l) {
+                h0 = r0[l].y;
+                h1 = r1[l].y;
+                h2 = r2[l].y;
+                h3 = r3[l].y;
+            }
+        }
+        float w0 = (1.0f - tx) * (1.0f - ty);
+        float w1 = tx * (1.0f - ty);
+        float w2 = (1.0f - tx) * ty;
+        float w3 = tx * ty;
+        g->height = (h0 * w0 + h2 * w2) + (h1 * w1 + h3 * w3);
+        g->sx = ((h1 - h0) * (1.0f - ty) + (h3 - h2) * ty) / m.cell;
+        g->sy = ((h2 - h0) * (1.0f - tx) + (h3 - h1) * tx) / m.cell;
+    }
+}
+
 // advance (racer/sim.mojo:184-235): one 60 Hz step of car i.
+template <int L>
 __device__ void advance(const racer_dmap& m, float* s, int n, const float* act, int64_t act_cols, float* reward,
                         float* done, float* obs, int64_t obs_cols, int beams, int i, int seed, float heading_noise) {
     const float DT = RACER_DT;
     float p[4] = {s[RACER_X * n + i], s[RACER_Y * n + i], s[RACER_HEADING * n + i], s[RACER_SPEED * n + i]};
-    Spot spot = map_probe(m, p[0], p[1], s[RACER_Z * n + i]);
-    Surf ground = map_surface(m, spot.layer, p[0], p[1]);
+    Spot spot;
+    Surf ground;
+    probe_surface<L>(m, p[0], p[1], s[RACER_Z * n + i], &spot, &ground);
     float steer = s[RACER_STEER * n + i];
     float rate = racer_clamp(act[i], -1.0f, 1.0f) * RACER_STEER_RATE;
     float accel = racer_clamp(act[act_cols + i], -1.0f, 1.0f) * RACER_ACCEL_MAX;
@@ -172,8 +238,8 @@ __device__ void advance(const racer_dmap& m, float* s, int n, const float* act, 
         for (int j = 0; j < 4; ++j) p[j] = p[j] + ((k1[j] + 2.0f * (k2[j] + k3[j])) + k4[j]) * (DT / 6.0f);
         p[3] = racer_clamp(p[3], -RACER_SPEED_MAX, RACER_SPEED_MAX);
         steer = racer_clamp(full, -RACER_STEER_MAX, RACER_STEER_MAX);
-        spot = map_probe(m, p[0], p[1], spot.height);
-        ground = map_surface(m, spot.layer, p[0], p[1]);
+        const float previous = spot.height;  // the re-probe uses the last spot's road height
+        probe_surface<L>(m, p[0], p[1], previous, &spot, &ground);
         crashed = crashed || spot.clearance < RACER_HALF_DIAG ||
                   ground.sx * ground.sx + ground.sy * ground.sy > 1.0f;
     }
@@ -200,7 +266,8 @@ __device__ void advance(const racer_dmap& m, float* s, int n, const float* act, 
     if (reason > 0.0f) {
         spawn(m, s, n, i, seed, heading_noise);
         float x = s[RACER_X * n + i], y = s[RACER_Y * n + i];
-        ground = map_surface(m, map_probe(m, x, y, s[RACER_Z * n + i]).layer, x, y);
+        Spot respawned;
+        probe_surface<L>(m, x, y, s[RACER_Z * n + i], &respawned, &ground);
     }
     float along, across;
     slopes(ground, s[RACER_HEADING * n + i], &along, &across);
@@ -211,11 +278,12 @@ __device__ void advance(const racer_dmap& m, float* s, int n, const float* act, 
     obs[(int64_t)(4 + beams) * obs_cols + i] = 1.0f;
 }
 
+template <int L>
 __global__ void __launch_bounds__(64) k_physics(racer_dmap m, float* s, int n, const float* act, int64_t act_cols,
                                                 float* reward, float* done, float* obs, int64_t obs_cols, int beams,
                                                 int seed, float heading_noise) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) advance(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, i, seed, heading_noise);
+    if (i < n) advance<L>(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, i, seed, heading_noise);
 }
 
 // Sim.reset's spawn kernel (racer/sim.mojo:299-304).
@@ -240,18 +308,25 @@ __global__ void k_pack(const float* d, int64_t plane, int layers, float* packed)
 }
 
 // Map.probe on the packed records, returning (clearance, height) only.
+// Map.probe on the packed records, returning (clearance, height) only. The
+// records are read with 32-bit buffer offsets (the descriptor's size makes any
+// out-of-range offset read zeros instead of faulting).
 template <int L>
-__device__ __forceinline__ void probe_packed(const racer_dmap& m, float x, float y, float z, float* clearance,
-                                             float* height) {
-    int64_t c = 0;
-    bool on = cell_of(m, x, y, &c);
+__device__ __forceinline__ void probe_packed(const racer_dmap& m, __amdgpu_buffer_rsrc_t rec, float x, float y,
+                                             float z, float* clearance, float* height) {
+    float fx = racer_nearest_index(x - m.x0, m.cell, m.inv_cell);
+    float fy = racer_nearest_index(y - m.y0, m.cell, m.inv_cell);
+    bool on = fx >= 0.0f && fy >= 0.0f && fx < (float)m.nx && fy < (float)m.ny;
+    int c = on ? (int)fy * m.nx + (int)fx : 0;
     if constexpr (L == 1) {
-        float2 r = on ? reinterpret_cast<const float2*>(m.packed)[c] : make_float2(-1.0f, -1.0f);
+        auto raw = __builtin_amdgcn_raw_buffer_load_b64(rec, c * 8, 0, 0);
+        float2 r = on ? *reinterpret_cast<float2*>(&raw) : make_float2(-1.0f, -1.0f);
         bool take = r.x > 0.0f && fabsf(r.y - z) < 1e30f;
         *clearance = take ? r.x : -1.0f;
         *height = r.y;
     } else if constexpr (L == 2) {
-        float4 r = on ? reinterpret_cast<const float4*>(m.packed)[c] : make_float4(-1.0f, -1.0f, -1.0f, -1.0f);
+        auto raw = __builtin_amdgcn_raw_buffer_load_b128(rec, c * 16, 0, 0);
+        float4 r = on ? *reinterpret_cast<float4*>(&raw) : make_float4(-1.0f, -1.0f, -1.0f, -1.0f);
         float cl = -1.0f, h = r.y, gap = 1e30f;
         if (r.x > 0.0f && fabsf(r.y - z) < gap) {
             cl = r.x;
@@ -271,17 +346,43 @@ __device__ __forceinline__ void probe_packed(const racer_dmap& m, float x, float
     }
 }
 
+constexpr int LIDAR_RECORD = 5;               // dx, dy, dz, descent, 1/descent
+constexpr int LIDAR_RING = 2 * LIDAR_RECORD * 64;  // two chunks of 64 beam records
+
 // scan (racer/sim.mojo:238-266) for every beam of one car per wavefront.
-// Each lane marches one beam; when a lane's beam ends, the lane stores the
-// range and takes the next unclaimed beam of the car (ballot + popcount), so
-// the wave stays full until the car's last beams.
+//
+// Each lane marches one beam at a time; a lane whose beam ends takes the next
+// unclaimed beam of the car (ballot + popcount), so the wavefront stays full
+// until the car's last beams. Beam set-up (direction, normalisation, floor
+// descent rate and its reciprocal) is done 64 beams at a time by the whole
+// wavefront into a two-chunk LDS ring, so a refill is five LDS reads rather
+// than a divergent pass of divisions and square roots. Ranges collect in LDS
+// and are scaled and stored in one full-lane pass at the end.
 template <int L, bool COUNT>
 __global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __restrict__ s, int n,
-                                               float* __restrict__ obs, int64_t obs_cols, int beams,
-                                               const float2* __restrict__ dirs, unsigned long long* counter) {
+                                               float* __restrict__ obs, int64_t obs_cols, int total_beams,
+                                               int per_wave, int waves_per_car, const float2* __restrict__ dirs,
+                                               unsigned long long* counter) {
+    extern __shared__ float lds[];
     const int lane = threadIdx.x & 63;
-    const int car = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    const int wave = threadIdx.x >> 6;
+    // Workgroups go round-robin over the 8 XCDs; give each XCD a contiguous run
+    // of cars so the range stores of neighbouring cars (adjacent floats of one
+    // obs row) meet in one L2 and leave it as whole lines.
+    int block = blockIdx.x;
+    if ((gridDim.x & 7) == 0) block = (block & 7) * (gridDim.x >> 3) + (block >> 3);
+    const int global_wave = (int)(((int64_t)block * blockDim.x + threadIdx.x) >> 6);
+    const int car = global_wave / waves_per_car;
     if (car >= n) return;
+    // This wavefront's share of the car's beams: [lo, lo + beams).
+    const int lo = (global_wave % waves_per_car) * per_wave;
+    const int beams = min(total_beams, lo + per_wave) - lo;
+    if (beams <= 0) return;
+    float* ring = lds + wave * (LIDAR_RING + per_wave);
+    float* ranges = ring + LIDAR_RING;
+    __amdgpu_buffer_rsrc_t rec = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)m.packed, (short)0, (int)(m.plane * (L > 0 ? L : 1) * 8), 0x00020000);
+
     const float heading = s[RACER_HEADING * n + car];
     const float along = obs[2 * obs_cols + car];
     const float across = obs[3 * obs_cols + car];
@@ -291,40 +392,65 @@ __global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __rest
     const float oz = (s[RACER_Z * n + car] + RACER_MOUNT_FORWARD * along) + RACER_MOUNT_UP;
     const float slope15 = 1.5f * m.slope;
     const float half_cell = 0.5f * m.cell;
-    float dx = 0.0f, dy = 0.0f, dz = 0.0f, descent = 0.0f, t = 0.0f, last = 0.0f, last_floor = 0.0f;
-    auto start = [&](int b) {
-        float2 cs = dirs[b];
-        float d0 = cs.x * ch + cs.y * (-sh);
-        float d1 = cs.x * sh + cs.y * ch;
-        float d2 = cs.x * along + cs.y * across;
-        float norm = sqrtf((d0 * d0 + d2 * d2) + (d1 * d1 + 0.0f));
-        dx = d0 / norm;
-        dy = d1 / norm;
-        dz = d2 / norm;
-        float horizontal = sqrtf(dx * dx + dy * dy);
-        descent = slope15 * horizontal + racer_max(-dz, 0.0f);
+
+    // Set-up of beams 64 c .. 64 c + 63, one per lane, into ring slot c % 2.
+    auto setup_chunk = [&](int c) {
+        int b = c * 64 + lane;
+        if (b < beams) {
+            float2 cs = dirs[lo + b];
+            float d0 = cs.x * ch + cs.y * (-sh);
+            float d1 = cs.x * sh + cs.y * ch;
+            float d2 = cs.x * along + cs.y * across;
+            float norm = sqrtf((d0 * d0 + d2 * d2) + (d1 * d1 + 0.0f));
+            float dx = d0 / norm, dy = d1 / norm, dz = d2 / norm;
+            float horizontal = sqrtf(dx * dx + dy * dy);
+            float descent = slope15 * horizontal + racer_max(-dz, 0.0f);
+            float* slot = ring + (c & 1) * (LIDAR_RECORD * 64);
+            slot[0 * 64 + lane] = dx;
+            slot[1 * 64 + lane] = dy;
+            slot[2 * 64 + lane] = dz;
+            slot[3 * 64 + lane] = descent;
+            slot[4 * 64 + lane] = 1.0f / descent;
+        }
+    };
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f, descent = 0.0f, inv_descent = 0.0f;
+    float t = 0.0f, last = 0.0f, last_floor = 0.0f;
+    auto take = [&](int b) {
+        const float* slot = ring + ((b >> 6) & 1) * (LIDAR_RECORD * 64);
+        int j = b & 63;
+        dx = slot[0 * 64 + j];
+        dy = slot[1 * 64 + j];
+        dz = slot[2 * 64 + j];
+        descent = slot[3 * 64 + j];
+        inv_descent = slot[4 * 64 + j];
         t = 0.0f;
         last = 0.0f;
         last_floor = 0.0f;
     };
+    setup_chunk(0);
+    setup_chunk(1);
+    int chunks = 2;  // beams below 64 * chunks have their set-up in the ring
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int b = lane;
-    if (b < beams) start(b);
+    if (b < beams) take(b);
     int next = 64;
     uint64_t iterations = 0;
     while (__ballot(b < beams) != 0ull) {
         bool finished = false;
-        float range = 0.0f;
         if (b < beams) {
             if (COUNT) ++iterations;
             float px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
             float clearance, height;
-            probe_packed<L>(m, px, py, pz, &clearance, &height);
+            probe_packed<L>(m, rec, px, py, pz, &clearance, &height);
             float fl = (pz - height) - 0.03f;
+            float range = 0.0f;
             if (fl < 0.0f) {
                 range = racer_min(last + ((t - last) * last_floor) / (last_floor - fl), RACER_RANGE);
                 finished = true;
             } else {
-                float step = racer_min(clearance - m.cell, fl / descent);
+                float step = racer_min_quotient(clearance - m.cell, fl, descent, inv_descent);
                 if (step < half_cell) {
                     range = racer_min(t, RACER_RANGE);
                     finished = true;
@@ -338,15 +464,31 @@ __global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __rest
                     }
                 }
             }
-            if (finished) obs[(int64_t)(4 + b) * obs_cols + car] = range / RACER_RANGE;
+            if (finished) ranges[b] = range;
         }
-        uint64_t done_mask = __ballot(finished);
-        if (finished) {
-            b = next + (int)__popcll(done_mask & ((1ull << lane) - 1ull));
-            if (b < beams) start(b);
+        const uint64_t done_mask = __ballot(finished);
+        if (done_mask != 0ull) {
+            const int count = (int)__popcll(done_mask);
+            // The ring slot reused here held chunk `chunks - 2`, whose beams
+            // are all below `next` and already taken.
+            if (next + count > chunks * 64 && chunks * 64 < beams) {
+                setup_chunk(chunks);
+                ++chunks;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            if (finished) {
+                b = next + (int)__popcll(done_mask & ((1ull << lane) - 1ull));
+                if (b < beams) take(b);
+            }
+            next += count;
         }
-        next += (int)__popcll(done_mask);
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int j = lane; j < beams; j += 64) obs[(int64_t)(4 + lo + j) * obs_cols + car] = ranges[j] / RACER_RANGE;
     if (COUNT) {
         for (int off = 32; off > 0; off >>= 1) iterations += __shfl_xor(iterations, off);
         if (lane == 0) atomicAdd(counter, (unsigned long long)iterations);
@@ -434,21 +576,39 @@ int racer_launch_reset(racer_sim* sim) {
 template <int L>
 static void launch_lidar(racer_sim* sim, float* obs, int64_t obs_cols) {
     int n = sim->n;
-    dim3 grid((unsigned)((n + 3) / 4)), block(256);
+    // A car's beams are split over waves_per_car wavefronts; each wavefront
+    // holds its beam ring and its share of the ranges in LDS.
+    int wpc = sim->waves_per_car;
+    int per = (sim->beams + wpc - 1) / wpc;
+    size_t bytes_per_wave = (size_t)(LIDAR_RING + per) * sizeof(float);
+    int waves = 4 * bytes_per_wave <= 65536 ? 4 : 1;
+    int64_t total = (int64_t)n * wpc;
+    dim3 grid((unsigned)((total + waves - 1) / waves)), block(64 * waves);
+    size_t shmem = waves * bytes_per_wave;
     if (sim->count_iterations)
-        hipLaunchKernelGGL((k_lidar<L, true>), grid, block, 0, sim->dev->stream, sim->map, sim->state->ptr, n, obs,
-                           obs_cols, sim->beams, sim->dirs, sim->counter);
+        hipLaunchKernelGGL((k_lidar<L, true>), grid, block, shmem, sim->dev->stream, sim->map, sim->state->ptr, n, obs,
+                           obs_cols, sim->beams, per, wpc, sim->dirs, sim->counter);
     else
-        hipLaunchKernelGGL((k_lidar<L, false>), grid, block, 0, sim->dev->stream, sim->map, sim->state->ptr, n, obs,
-                           obs_cols, sim->beams, sim->dirs, sim->counter);
+        hipLaunchKernelGGL((k_lidar<L, false>), grid, block, shmem, sim->dev->stream, sim->map, sim->state->ptr, n,
+                           obs, obs_cols, sim->beams, per, wpc, sim->dirs, sim->counter);
+}
+
+static void launch_lidar_any(racer_sim* sim, float* obs, int64_t obs_cols) {
+    if (sim->map.layers == 1)
+        launch_lidar<1>(sim, obs, obs_cols);
+    else if (sim->map.layers == 2)
+        launch_lidar<2>(sim, obs, obs_cols);
+    else
+        launch_lidar<0>(sim, obs, obs_cols);
 }
 
 int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done, float* obs, int64_t obs_cols) {
     int n = sim->n;
     {
         racer_timed timer(sim->dev, "physics");
-        hipLaunchKernelGGL(k_physics, dim3((n + 63) / 64), dim3(64), 0, sim->dev->stream, sim->map, sim->state->ptr,
-                           n, actions, (int64_t)n, reward, done, obs, obs_cols, sim->beams, sim->seed,
+        auto kernel = sim->map.layers == 1 ? k_physics<1> : (sim->map.layers == 2 ? k_physics<2> : k_physics<0>);
+        hipLaunchKernelGGL(kernel, dim3((n + 63) / 64), dim3(64), 0, sim->dev->stream, sim->map, sim->state->ptr, n,
+                           actions, (int64_t)n, reward, done, obs, obs_cols, sim->beams, sim->seed,
                            sim->heading_noise);
     }
     if (racer_launch_check("physics")) return 1;
@@ -460,12 +620,7 @@ int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done
                            sim->map, sim->state->ptr, n, obs, obs_cols, sim->beams);
     } else {
         racer_timed timer(sim->dev, "lidar");
-        if (sim->map.layers == 1)
-            launch_lidar<1>(sim, obs, obs_cols);
-        else if (sim->map.layers == 2)
-            launch_lidar<2>(sim, obs, obs_cols);
-        else
-            launch_lidar<0>(sim, obs, obs_cols);
+        launch_lidar_any(sim, obs, obs_cols);
     }
     if (racer_launch_check("lidar")) return 1;
     if (sim->count_iterations) {
@@ -477,15 +632,21 @@ int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done
     return 0;
 }
 
-int racer_launch_sample(racer_sim* sim, const float* out, int64_t out_cols, const float* log_std, float* actions,
-                        float* logp, float* value, int deterministic, int64_t step) {
-    int n = sim->n;
+int racer_launch_sample_raw(racer_device* dev, int n, const float* out, int64_t out_cols, const float* log_std,
+                            float* actions, int64_t act_cols, float* logp, float* value, int deterministic,
+                            int64_t step, int seed) {
     {
-        racer_timed timer(sim->dev, "sample");
-        hipLaunchKernelGGL(k_sample, dim3((n + 255) / 256), dim3(256), 0, sim->dev->stream, n, out, out_cols,
-                           log_std, actions, (int64_t)n, logp, value, deterministic, step, sim->seed);
+        racer_timed timer(dev, "sample");
+        hipLaunchKernelGGL(k_sample, dim3((n + 255) / 256), dim3(256), 0, dev->stream, n, out, out_cols, log_std,
+                           actions, act_cols, logp, value, deterministic, step, seed);
     }
     return racer_launch_check("sample");
+}
+
+int racer_launch_sample(racer_sim* sim, const float* out, int64_t out_cols, const float* log_std, float* actions,
+                        float* logp, float* value, int deterministic, int64_t step) {
+    return racer_launch_sample_raw(sim->dev, sim->n, out, out_cols, log_std, actions, (int64_t)sim->n, logp, value,
+                                   deterministic, step, sim->seed);
 }
 
 // ---------------------------------------------------------------- Sim C ABI
@@ -500,6 +661,7 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
     racer_config_default(&def);
     if (!cfg) cfg = &def;
     RACER_REQUIRE(cfg->beams >= 2, "at least two lidar beams are needed");
+    RACER_REQUIRE(cfg->beams <= 15000, "at most 15000 lidar beams are supported");
     int nx = (int)track[0], ny = (int)track[1], layers = (int)track[7], wps = (int)track[5];
     RACER_REQUIRE(nx >= 2 && ny >= 2 && layers >= 1 && wps >= 1, "not a track array");
     int64_t plane = (int64_t)nx * ny;
@@ -537,6 +699,7 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
     m.spacing = track[6];
     m.layers = layers;
     m.slope = track[8];
+    m.inv_cell = 1.0f / m.cell;
     m.route = route;
     m.plane = plane;
     m.packed = nullptr;
@@ -624,6 +787,48 @@ extern "C" int racer_sim_count_iterations(racer_sim* sim, int on) {
     return 0;
 }
 extern "C" int64_t racer_sim_last_iterations(racer_sim* sim) { return sim->last_iterations; }
+
+extern "C" int racer_sim_set_waves_per_car(racer_sim* sim, int waves) {
+    racer_clear_error();
+    RACER_REQUIRE(sim && (waves == 1 || waves == 2 || waves == 4), "waves per car must be 1, 2 or 4");
+    sim->waves_per_car = waves;
+    return 0;
+}
+
+extern "C" int racer_sim_waves_per_car(const racer_sim* sim) { return sim->waves_per_car; }
+
+// Times the lidar launch with 1, 2 and 4 wavefronts per car on the current
+// state (the lidar only rewrites the range rows of sim->obs, so repeating it
+// is harmless) and keeps the fastest.
+extern "C" int racer_sim_autotune(racer_sim* sim, int* chosen) {
+    racer_clear_error();
+    hipEvent_t a, b;
+    RACER_TRY(hipEventCreate(&a));
+    RACER_TRY(hipEventCreate(&b));
+    const int options[3] = {1, 2, 4};
+    float best = 1e30f;
+    int pick = sim->waves_per_car;
+    for (int w : options) {
+        sim->waves_per_car = w;
+        launch_lidar_any(sim, sim->obs->ptr, sim->n);  // warm
+        RACER_TRY(hipEventRecord(a, sim->dev->stream));
+        for (int r = 0; r < 3; ++r) launch_lidar_any(sim, sim->obs->ptr, sim->n);
+        RACER_TRY(hipEventRecord(b, sim->dev->stream));
+        if (racer_launch_check("autotune lidar")) return 1;
+        RACER_TRY(hipEventSynchronize(b));
+        float ms = 0.0f;
+        RACER_TRY(hipEventElapsedTime(&ms, a, b));
+        if (ms < best) {
+            best = ms;
+            pick = w;
+        }
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    sim->waves_per_car = pick;
+    if (chosen) *chosen = pick;
+    return 0;
+}
 
 extern "C" int racer_sim_set_schedule(racer_sim* sim, int schedule) {
     racer_clear_error();
