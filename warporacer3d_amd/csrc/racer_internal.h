@@ -113,6 +113,7 @@ struct racer_policy {
     racer_buf *theta = nullptr, *w1t = nullptr, *w2t = nullptr;
     uint16_t* theta_bf16 = nullptr;  // bf16 W0 [H, pad32(IN)] then W1 [H, pad32(HA)], zero padded, for MFMA
     int64_t in_pad = 0, ha_pad = 0;
+    uint16_t *frag0 = nullptr, *frag1 = nullptr;  // W0, W1 in MFMA fragment order (fused forward, H <= 256)
     bool bf16_stale = true;
 };
 
@@ -163,6 +164,10 @@ int racer_pack_bf16(racer_device* dev, const float* src, int64_t ld_src, uint16_
                     int cols);
 inline int64_t racer_pad8(int64_t v) { return (v + 7) / 8 * 8; }
 inline int64_t racer_pad32(int64_t v) { return (v + 31) / 32 * 32; }
+// Fragment-order bf16 weight copies for the fused forward: elements needed for
+// `cols` columns, and the packing of a [rows, cols] float32 matrix (rows <= 256).
+size_t racer_fragment_elems(int cols);
+int racer_pack_fragments(racer_device* dev, const float* src, int64_t ld_src, int rows, int cols, uint16_t* dst);
 // The fused bf16 policy forward (and optionally Sim.sample) over `cols` columns of x.
 int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
                        float* act, float* logp, float* value, int deterministic, int64_t step, int seed);

@@ -221,6 +221,26 @@ RACER_FN float racer_min_quotient(float cap, float num, float den, float inv_den
     return racer_min(cap, num / den);
 }
 
+#if defined(__HIPCC__)
+/* Branch-free global loads through gfx9 buffer descriptors: an offset at or
+ * beyond `bytes` reads as zero in hardware, so guarded loads need no branch
+ * (a branch around a load makes the compiler wait for every load in flight). */
+#define RACER_OOB 0xFFFFFFF0u
+__device__ static inline __amdgpu_buffer_rsrc_t racer_rsrc(const void* base, uint64_t bytes) {
+    uint32_t n = bytes >= 0xFFFFFFF0ull ? 0xFFFFFFF0u : (uint32_t)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)n, 0x00020000);
+}
+__device__ static inline float racer_ld_f32(__amdgpu_buffer_rsrc_t r, uint32_t byte_offset) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_offset, 0, 0));
+}
+__device__ static inline uint4 racer_ld_b128(__amdgpu_buffer_rsrc_t r, uint32_t byte_offset) {
+    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_offset, 0, 0));
+}
+__device__ static inline float2 racer_ld_f32x2(__amdgpu_buffer_rsrc_t r, uint32_t byte_offset) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)byte_offset, 0, 0));
+}
+#endif
+
 /* ---- Philox4x32-10 ---- */
 typedef struct {
     uint32_t v[4];

@@ -3,13 +3,21 @@
 // h1 = tanh(W0 x), h2 = tanh(W1 h1) on MFMA (bf16 operands, float32
 // accumulation), o = W2 h2 in float32, and optionally the Gaussian actions.
 //
-// One workgroup owns FT = 16 samples (columns) and all H rows. Eight
-// wavefronts each own 32 rows (two 16-row MFMA tiles). Weight fragments come
-// straight from the row-padded bf16 weight copies as 16-byte loads (8
-// consecutive k of one row per lane, the layout v_mfma_f32_16x16x32_bf16
-// takes); the input tile is staged in LDS as [column][k] bf16. h1 stays in LDS
-// as the next product's input; h2 goes to LDS in float32 for the output layer,
-// which accumulates fmaf over k in order exactly as the float32 path does.
+// One workgroup owns FT = 16 samples (columns) and all H <= 256 rows, with
+// eight wavefronts. The design keeps many loads in flight per wavefront,
+// because one workgroup per CU has nothing else to hide latency with:
+//  - phase 0: the whole [IN x 16] input slice is loaded at once, rounded to
+//    bf16 and stored in LDS as [column][k] (16-byte stores);
+//  - phase 1 (h1): wavefront w takes rows 64 (w % 4) .. + 63 (four 16-row MFMA
+//    tiles) over half of the k range (w / 4); weight fragments (8 consecutive
+//    k of one row per lane, v_mfma_f32_16x16x32_bf16's operand layout) stream
+//    from a fragment-ordered bf16 copy of the weights (one contiguous 1 KB per
+//    tile and k-step) through a 4-k-step register ring; the two halves are
+//    added in a fixed order through LDS;
+//  - phase 2 (h2): wavefront w takes rows 32 w .. + 31 over all k, its W1
+//    fragments requested before phase 1's reduction;
+//  - phase 3: o = W2 h2 in float32 with fmaf over k in order (as the float32
+//    path), then the sample.
 #include "racer_internal.h"
 
 namespace {
@@ -18,10 +26,10 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int FT = 16;       // columns per workgroup
-constexpr int WAVES = 8;     // 32 rows each: H <= 256
-constexpr int KS = 128;      // k per input stage
-constexpr int LDX = KS + 8;  // LDS row (column-major tile) stride in bf16
+constexpr int WAVES = 8;
+constexpr int RING = 4;      // k-steps of weight fragments in flight per wavefront
 constexpr int W1_STEPS = 9;  // k-steps of the second product held in registers (H + 1 <= 288)
+constexpr int FRAG_TILES = 16;  // 16-row tiles of a fragment-order weight copy (256 rows, zeros past H)
 
 __device__ __forceinline__ uint16_t bf16_of(float f) {
     uint32_t u = racer_f2u(f);
@@ -30,158 +38,179 @@ __device__ __forceinline__ uint16_t bf16_of(float f) {
     return (uint16_t)(u >> 16);
 }
 
+struct FusedLayout {
+    int in32, kx, ha32, ldh, HA;
+    size_t xs, red, h1s, h2f, w2s, outs, total;  // byte offsets
+};
+
+__host__ __device__ inline FusedLayout fused_layout(int H, int IN) {
+    FusedLayout L;
+    L.HA = H + 1;
+    L.in32 = (IN + 31) & ~31;
+    L.kx = L.in32 + 8;
+    L.ha32 = (L.HA + 31) & ~31;
+    L.ldh = L.ha32 + 8;
+    L.xs = 0;
+    L.h1s = L.xs + (size_t)FT * L.kx * 2;
+    L.red = L.h1s + (size_t)FT * L.ldh * 2;
+    size_t red_bytes = (size_t)4 * 4 * 64 * 4 * 4;      // [4 row groups][4 tiles][64 lanes][4]
+    size_t h2_bytes = (size_t)L.HA * FT * 4;
+    L.h2f = L.red;                                       // h2 reuses the reduction buffer
+    L.w2s = L.red + (red_bytes > h2_bytes ? red_bytes : h2_bytes);
+    L.outs = L.w2s + (size_t)3 * L.HA * 4;
+    L.total = L.outs + (size_t)3 * FT * 4;
+    return L;
+}
+
 template <bool SAMPLE>
 __global__ void __launch_bounds__(64 * WAVES) k_policy_fused(
-    const uint16_t* __restrict__ W0, int64_t ld0, const uint16_t* __restrict__ W1, int64_t ld1,
+    const uint16_t* __restrict__ F0, const uint16_t* __restrict__ F1,
     const float* __restrict__ W2, const float* __restrict__ x, int64_t ldx, int H, int IN, int64_t cols,
     float* __restrict__ h1out, float* __restrict__ h2out, float* __restrict__ o, const float* __restrict__ log_std,
     float* act, int64_t act_cols, float* logp, float* value, int deterministic, int64_t step, int seed) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    const int HA = H + 1;
-    const int ha32 = (HA + 31) & ~31;  // k extent of the second product
-    uint16_t* xs = reinterpret_cast<uint16_t*>(smem);                 // [2][FT][LDX]
-    uint16_t* h1s = xs + 2 * FT * LDX;                                // [FT][ha32 + 8]
-    const int ldh = ha32 + 8;
-    float* h2f = reinterpret_cast<float*>(h1s + FT * ldh);            // [HA][FT]
-    float* w2s = h2f + HA * FT;                                       // [3][HA]
-    float* outs = w2s + 3 * HA;                                       // [3][FT]
+    const FusedLayout Ly = fused_layout(H, IN);
+    const int HA = Ly.HA;
+    uint16_t* xs = reinterpret_cast<uint16_t*>(smem + Ly.xs);     // [FT][kx]
+    uint16_t* h1s = reinterpret_cast<uint16_t*>(smem + Ly.h1s);   // [FT][ldh]
+    float* red = reinterpret_cast<float*>(smem + Ly.red);         // [4][4][64][4]
+    float* h2f = reinterpret_cast<float*>(smem + Ly.h2f);         // [HA][FT]
+    float* w2s = reinterpret_cast<float*>(smem + Ly.w2s);         // [3][HA]
+    float* outs = reinterpret_cast<float*>(smem + Ly.outs);       // [3][FT]
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t c0 = (int64_t)blockIdx.x * FT;
     const int r = lane & 15, kg = (lane >> 4) * 8;  // MFMA operand lane layout
-    const int row0 = wave * 32;                      // this wavefront's rows: row0 .. row0 + 31
+    // buffer descriptors: out-of-range offsets (rows >= H, k >= IN, columns >= cols) read zeros
+    const auto xres = racer_rsrc(x, ((uint64_t)(IN - 1) * ldx + cols) * 4);
+    const int steps0 = Ly.in32 / 32, steps1 = Ly.ha32 / 32;
+    const auto w0res = racer_rsrc(F0, (uint64_t)FRAG_TILES * steps0 * 1024);
+    const auto w1res = racer_rsrc(F1, (uint64_t)FRAG_TILES * steps1 * 1024);
 
-    // ---- first product: h1 = tanh(W0 x) ----
-    // input staging: thread t takes k = t / 4 (0..127) and columns 4 (t % 4) .. 4 (t % 4) + 3
-    const int sk = tid >> 2, sc = (tid & 3) * 4;
-    float xr[4];
-    auto load_x = [&](int k0) {
-        const int k = k0 + sk;
+    // ---- phase 0: the input slice, bf16, [column][k] ----
+    {
+        const int chunks = Ly.in32 / 8;  // 8 consecutive k per chunk
+        for (int idx = tid; idx < chunks * FT; idx += blockDim.x) {
+            const int c = idx % FT, kc = (idx / FT) * 8;
+            const int64_t col = c0 + c;
+            float f[8];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int64_t c = c0 + sc + e;
-            xr[e] = (k < IN && c < cols) ? x[(int64_t)k * ldx + c] : 0.0f;
-        }
-    };
-    auto store_x = [&](int buf) {
-        uint16_t* dst = xs + buf * FT * LDX;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) dst[(sc + e) * LDX + sk] = bf16_of(xr[e]);
-    };
-    // weight fragments of this wavefront's two row tiles for one stage (4 k-steps)
-    constexpr int KSTEPS = KS / 32;
-    uint4 wa[2][KSTEPS], wn[2][KSTEPS];
-    auto load_w0 = [&](int k0, uint4 (&w)[2][KSTEPS]) {
-#pragma unroll
-        for (int t = 0; t < 2; ++t)
-#pragma unroll
-            for (int s = 0; s < KSTEPS; ++s) {
-                const int row = row0 + 16 * t + r;
-                const int k = k0 + 32 * s + kg;
-                w[t][s] = (row < H && k < IN) ? *reinterpret_cast<const uint4*>(W0 + (int64_t)row * ld0 + k)
-                                              : make_uint4(0, 0, 0, 0);
+            for (int e = 0; e < 8; ++e) {
+                const int k = kc + e;
+                const uint32_t off = (k < IN && col < cols) ? (uint32_t)(((int64_t)k * ldx + col) * 4) : RACER_OOB;
+                f[e] = racer_ld_f32(xres, off);
             }
-    };
-    f32x4 acc[2];
+            uint16_t v[8];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int idx = tid; idx < 3 * HA; idx += blockDim.x) w2s[idx] = W2[idx];
-
-    load_x(0);
-    load_w0(0, wa);
-    store_x(0);
-    __syncthreads();
-    int buf = 0;
-    for (int k0 = 0; k0 < IN; k0 += KS) {
-        const bool more = k0 + KS < IN;
-        if (more) {
-            load_x(k0 + KS);
-            load_w0(k0 + KS, wn);
+            for (int e = 0; e < 8; ++e) v[e] = bf16_of(f[e]);
+            *reinterpret_cast<uint4*>(xs + c * Ly.kx + kc) = *reinterpret_cast<uint4*>(v);
         }
-        const uint16_t* xb = xs + buf * FT * LDX;
-#pragma unroll
-        for (int s = 0; s < KSTEPS; ++s) {
-            bf16x8 b = *reinterpret_cast<const bf16x8*>(xb + r * LDX + 32 * s + kg);
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                bf16x8 a = *reinterpret_cast<const bf16x8*>(&wa[t][s]);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
-            }
-        }
-        if (more) {
-            store_x(buf ^ 1);
-#pragma unroll
-            for (int t = 0; t < 2; ++t)
-#pragma unroll
-                for (int s = 0; s < KSTEPS; ++s) wa[t][s] = wn[t][s];
-        }
-        __syncthreads();
-        buf ^= 1;
+        for (int idx = tid; idx < 3 * HA; idx += blockDim.x) w2s[idx] = W2[idx];
     }
-    // request the second product's weight fragments now, so their latency
-    // overlaps the first product's epilogue
+    __syncthreads();
+
+    // ---- phase 1: h1 = tanh(W0 x) ----
+    const int mg = wave & 3, kh = wave >> 2;
+    const int steps = Ly.in32 / 32;
+    const int s0 = kh == 0 ? 0 : (steps + 1) / 2, s1 = kh == 0 ? (steps + 1) / 2 : steps;
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    uint4 ring[RING][4];
+    auto load_step = [&](int s, uint4 (&w)[4]) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int tile = 4 * mg + t;
+            const uint32_t off = s < s1 ? (uint32_t)(((tile * steps0 + s) * 64 + lane) * 16) : RACER_OOB;
+            w[t] = racer_ld_b128(w0res, off);
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < RING; ++u) load_step(s0 + u, ring[u]);
+    // No branches inside the ring: a k-step past s1 has all-zero weight
+    // fragments (out-of-range loads) against finite, clamped input columns, so
+    // it adds exactly zero, and the compiler can keep the ring's loads in flight.
+    for (int s = s0; s < s1; s += RING) {
+#pragma unroll
+        for (int u = 0; u < RING; ++u) {
+            const int kk = min(32 * (s + u), Ly.in32 - 32);
+            bf16x8 b = *reinterpret_cast<const bf16x8*>(xs + r * Ly.kx + kk + kg);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&ring[u][t]), b, acc[t],
+                                                                 0, 0, 0);
+            load_step(s + u + RING, ring[u]);
+        }
+    }
+    // request the second product's weight fragments now (rows 32 w .. + 31)
     uint4 w1f[2][W1_STEPS];
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int s = 0; s < W1_STEPS; ++s) {
-            const int row = row0 + 16 * t + r;
-            const int k = 32 * s + kg;
-            w1f[t][s] = (row < H && 32 * s < ha32) ? *reinterpret_cast<const uint4*>(W1 + (int64_t)row * ld1 + k)
-                                                    : make_uint4(0, 0, 0, 0);
+            const int tile = 2 * wave + t;
+            const uint32_t off = s < steps1 ? (uint32_t)(((tile * steps1 + s) * 64 + lane) * 16) : RACER_OOB;
+            w1f[t][s] = racer_ld_b128(w1res, off);
         }
+    // second half of k -> LDS; first half adds it (fixed order)
+    if (kh == 1) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            *reinterpret_cast<f32x4*>(red + ((mg * 4 + t) * 64 + lane) * 4) = acc[t];
+    }
+    __syncthreads();
     // D[i][j] of a 16 x 16 tile: column j = lane % 16, rows 4 (lane / 16) + q
     const int jc = lane & 15, ib = (lane >> 4) * 4;
+    if (kh == 0) {
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+        for (int t = 0; t < 4; ++t) {
+            f32x4 other = *reinterpret_cast<const f32x4*>(red + ((mg * 4 + t) * 64 + lane) * 4);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int row = row0 + 16 * t + ib + q;
-            if (row < H) {
-                float v = racer_tanh(acc[t][q]);
-                h1s[jc * ldh + row] = bf16_of(v);
-                if (c0 + jc < cols) h1out[(int64_t)row * cols + c0 + jc] = v;
+            for (int q = 0; q < 4; ++q) {
+                const int row = 64 * mg + 16 * t + ib + q;
+                if (row < H) {
+                    float v = racer_tanh(acc[t][q] + other[q]);
+                    h1s[jc * Ly.ldh + row] = bf16_of(v);
+                    if (c0 + jc < cols) h1out[(int64_t)row * cols + c0 + jc] = v;
+                }
             }
         }
+    }
     // bias row (the constant 1) and zero padding of the second product's k range
-    for (int idx = tid; idx < FT * (ha32 - H); idx += blockDim.x) {
-        int j = idx / (ha32 - H), k = H + idx % (ha32 - H);
-        h1s[j * ldh + k] = k == H ? bf16_of(1.0f) : (uint16_t)0;
+    for (int idx = tid; idx < FT * (Ly.ha32 - H); idx += blockDim.x) {
+        int j = idx / (Ly.ha32 - H), k = H + idx % (Ly.ha32 - H);
+        h1s[j * Ly.ldh + k] = k == H ? bf16_of(1.0f) : (uint16_t)0;
     }
     __syncthreads();
 
-    // ---- second product: h2 = tanh(W1 h1) ----
-    // W1 fragments were requested before the first product's epilogue (w1f);
-    // up to W1_STEPS k-steps are held in registers, any further ones load here.
-#pragma unroll
-    for (int t = 0; t < 2; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // ---- phase 2: h2 = tanh(W1 h1) ----
+    f32x4 acc2[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
 #pragma unroll
     for (int s = 0; s < W1_STEPS; ++s) {
-        if (32 * s < ha32) {
-            bf16x8 b = *reinterpret_cast<const bf16x8*>(h1s + r * ldh + 32 * s + kg);
+        if (32 * s < Ly.ha32) {
+            bf16x8 b = *reinterpret_cast<const bf16x8*>(h1s + r * Ly.ldh + 32 * s + kg);
 #pragma unroll
             for (int t = 0; t < 2; ++t)
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&w1f[t][s]), b, acc[t], 0,
-                                                                 0, 0);
+                acc2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&w1f[t][s]), b,
+                                                                  acc2[t], 0, 0, 0);
         }
     }
-    for (int k0 = 32 * W1_STEPS; k0 < ha32; k0 += 32) {
-        bf16x8 b = *reinterpret_cast<const bf16x8*>(h1s + r * ldh + k0 + kg);
+    for (int k0 = 32 * W1_STEPS; k0 < Ly.ha32; k0 += 32) {
+        bf16x8 b = *reinterpret_cast<const bf16x8*>(h1s + r * Ly.ldh + k0 + kg);
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            const int row = row0 + 16 * t + r;
-            uint4 w = row < H ? *reinterpret_cast<const uint4*>(W1 + (int64_t)row * ld1 + k0 + kg)
-                              : make_uint4(0, 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&w), b, acc[t], 0, 0, 0);
+            const int tile = 2 * wave + t;
+            uint4 w = racer_ld_b128(w1res, (uint32_t)(((tile * steps1 + k0 / 32) * 64 + lane) * 16));
+            acc2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&w), b, acc2[t], 0, 0, 0);
         }
     }
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int row = row0 + 16 * t + ib + q;
+            const int row = 32 * wave + 16 * t + ib + q;
             if (row < H) {
-                float v = racer_tanh(acc[t][q]);
+                float v = racer_tanh(acc2[t][q]);
                 h2f[row * FT + jc] = v;
                 if (c0 + jc < cols) h2out[(int64_t)row * cols + c0 + jc] = v;
             }
@@ -189,7 +218,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_fused(
     for (int j = tid; j < FT; j += blockDim.x) h2f[H * FT + j] = 1.0f;
     __syncthreads();
 
-    // ---- output layer o = W2 h2 in float32, k in order ----
+    // ---- phase 3: output layer o = W2 h2 in float32, k in order ----
     if (tid < 3 * FT) {
         const int j = tid / FT, c = tid % FT;
         const float* w = w2s + j * HA;
@@ -217,32 +246,64 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_fused(
     }
 }
 
+// Weights in MFMA fragment order: block (tile, step) holds the 64 lanes'
+// operands of rows 16 tile .. + 15 and k 32 step .. + 31, lane l taking row
+// 16 tile + l % 16 and k 32 step + 8 (l / 16) .. + 7, as 1 KB of bf16. A
+// wavefront's operand load for one tile and k-step is then one contiguous 1 KB.
+__global__ void k_pack_fragments(const float* __restrict__ src, int64_t ld_src, int rows, int cols, int steps,
+                                 uint16_t* __restrict__ dst) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)FRAG_TILES * steps * 512) return;
+    const int block = (int)(idx / 512), within = (int)(idx % 512);
+    const int tile = block / steps, step = block % steps;
+    const int lane = within / 8, e = within % 8;
+    const int row = 16 * tile + lane % 16;
+    const int k = 32 * step + (lane / 16) * 8 + e;
+    dst[idx] = (row < rows && k < cols) ? bf16_of(src[(int64_t)row * ld_src + k]) : (uint16_t)0;
+}
+
 }  // namespace
 
-size_t racer_policy_fused_smem(int H) {
-    int HA = H + 1, ha32 = (HA + 31) & ~31;
-    return (size_t)2 * FT * LDX * 2 + (size_t)FT * (ha32 + 8) * 2 + (size_t)HA * FT * 4 + (size_t)3 * HA * 4 +
-           (size_t)3 * FT * 4;
+size_t racer_fragment_elems(int cols) { return (size_t)FRAG_TILES * ((cols + 31) / 32) * 512; }
+
+int racer_pack_fragments(racer_device* dev, const float* src, int64_t ld_src, int rows, int cols, uint16_t* dst) {
+    const int steps = (cols + 31) / 32;
+    const int64_t total = (int64_t)FRAG_TILES * steps * 512;
+    {
+        racer_timed timer(dev, "pack_fragments");
+        hipLaunchKernelGGL(k_pack_fragments, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, dev->stream, src,
+                           ld_src, rows, cols, steps, dst);
+    }
+    return racer_launch_check("pack_fragments");
 }
+
+size_t racer_policy_fused_smem(int H, int IN) { return fused_layout(H, IN).total; }
 
 int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
                        float* act, float* logp, float* value, int deterministic, int64_t step, int seed) {
     racer_device* dev = p->dev;
-    const uint16_t* w0 = p->theta_bf16;
-    const uint16_t* w1 = p->theta_bf16 + (int64_t)p->hidden * p->in_pad;
+    const uint16_t* f0 = p->frag0;
+    const uint16_t* f1 = p->frag1;
     const float* W2 = p->theta->ptr + p->W2;
     const float* log_std = p->theta->ptr + p->LOGSTD;
     dim3 grid((unsigned)((cols + FT - 1) / FT)), block(64 * WAVES);
-    size_t shmem = racer_policy_fused_smem(p->hidden);
+    size_t shmem = racer_policy_fused_smem(p->hidden, (int)p->IN);
+    static bool raised = false;
+    if (!raised) {  // allow more than the default 64 KB of dynamic LDS (gfx950 has 160 KB per CU)
+        RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused<true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused<false>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        raised = true;
+    }
     if (sample) {
         racer_timed timer(dev, "policy_fused_sample");
-        hipLaunchKernelGGL(k_policy_fused<true>, grid, block, shmem, dev->stream, w0, p->in_pad, w1, p->ha_pad, W2, x,
-                           x_cols, p->hidden, (int)p->IN, (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr, log_std,
+        hipLaunchKernelGGL(k_policy_fused<true>, grid, block, shmem, dev->stream, f0, f1, W2, x, x_cols, p->hidden, (int)p->IN, (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr, log_std,
                            act, (int64_t)cols, logp, value, deterministic, step, seed);
     } else {
         racer_timed timer(dev, "policy_fused");
-        hipLaunchKernelGGL(k_policy_fused<false>, grid, block, shmem, dev->stream, w0, p->in_pad, w1, p->ha_pad, W2,
-                           x, x_cols, p->hidden, (int)p->IN, (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr,
+        hipLaunchKernelGGL(k_policy_fused<false>, grid, block, shmem, dev->stream, f0, f1, W2, x,
+                           x_cols, p->hidden, (int)p->IN, (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr,
                            log_std, act, (int64_t)cols, logp, value, deterministic, step, seed);
     }
     return racer_launch_check("policy_fused");

@@ -311,6 +311,10 @@ extern "C" int racer_policy_create(racer_device* dev, int seed, const racer_conf
     p->in_pad = racer_pad32(p->IN);
     p->ha_pad = racer_pad32(p->HA);
     RACER_TRY(hipMalloc(&p->theta_bf16, (size_t)(H * (p->in_pad + p->ha_pad)) * sizeof(uint16_t)));
+    if (H <= 256) {
+        RACER_TRY(hipMalloc(&p->frag0, racer_fragment_elems((int)p->IN) * sizeof(uint16_t)));
+        RACER_TRY(hipMalloc(&p->frag1, racer_fragment_elems((int)p->HA) * sizeof(uint16_t)));
+    }
     hipLaunchKernelGGL(k_policy_init, dim3((unsigned)((p->P + 255) / 256)), dim3(256), 0, dev->stream, p->theta->ptr,
                        p->P, p->IN, p->IN - 1, H, p->HA, p->W1, p->W2, p->LOGSTD, seed);
     if (racer_launch_check("policy init")) return 1;
@@ -326,6 +330,8 @@ extern "C" void racer_policy_destroy(racer_policy* p) {
     racer_free(p->w1t);
     racer_free(p->w2t);
     if (p->theta_bf16) hipFree(p->theta_bf16);
+    if (p->frag0) hipFree(p->frag0);
+    if (p->frag1) hipFree(p->frag1);
     delete p;
 }
 
@@ -363,6 +369,9 @@ static int refresh_bf16(racer_policy* p) {
     uint16_t* w1 = p->theta_bf16 + (int64_t)H * p->in_pad;
     if (racer_pack_bf16(p->dev, th, p->IN, p->theta_bf16, p->in_pad, H, (int)p->IN) ||
         racer_pack_bf16(p->dev, th + p->W1, p->HA, w1, p->ha_pad, H, (int)p->HA))
+        return 1;
+    if (p->frag0 && (racer_pack_fragments(p->dev, th, p->IN, H, (int)p->IN, p->frag0) ||
+                     racer_pack_fragments(p->dev, th + p->W1, p->HA, H, (int)p->HA, p->frag1)))
         return 1;
     p->bf16_stale = false;
     return 0;
