@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -86,13 +87,18 @@ def plain_baseline(R, dev, track, args, steps: int) -> dict:
     for t in range(5):
         run.step(False, t)
     dev.synchronize()
-    dev.profile_reset()
-    dev.profile(True)
     t0 = time.perf_counter()
     for t in range(steps):
         run.step(False, 5 + t)
     dev.synchronize()
     dt = time.perf_counter() - t0
+    # kernel durations in a separate profiled pass
+    dev.profile_only(None)
+    dev.profile_reset()
+    dev.profile(True)
+    for t in range(steps):
+        run.step(False, 5 + steps + t)
+    dev.synchronize()
     dev.profile(False)
     prof = dev.profile_results()
     ms = dt * 1000.0 / steps
@@ -154,29 +160,56 @@ def main() -> int:
     run.sim.count_iterations(False)
     dev.synchronize()
     log("lidar iterations per step:", counts)
-    dev.profile_reset()
-    dev.profile(True)
+    t = args.warmup + 3
+    # Timed region: the Runner steps replayed from a captured HIP graph (the
+    # same kernels and arguments as eager steps, bitwise the same results).
+    chunk = math.gcd(args.steps, 100)
+    graph = run.capture(chunk)
+    graph.launch(t)  # untimed first replay
+    t += chunk
+    dev.synchronize()
     t0 = time.perf_counter()
-    for t in range(args.steps):
-        run.step(False, args.warmup + 3 + t)
+    for k in range(args.steps // chunk):
+        graph.launch(t + k * chunk)
     dev.synchronize()
     dt = time.perf_counter() - t0
-    dev.profile(False)
-    prof = dev.profile_results()
-    # untimed: iteration count on the final state, to bracket the timed region
-    run.sim.count_iterations(True)
-    run.step(False, args.warmup + 3 + args.steps)
-    counts.append(run.sim.last_iterations)
-    run.sim.count_iterations(False)
+    t += args.steps
     ms = dt * 1000.0 / args.steps
     value = args.cars * 1000.0 / ms
-    log(f"{ms:.4f} ms/step, {value:.4g} env-steps/s")
+    log(f"graph replay: {ms:.4f} ms/step, {value:.4g} env-steps/s")
+    assert np.isfinite(run.sim.obs.read()).all(), "non-finite observations"
+    # The same number of eager steps with HIP events around each lidar launch
+    # only (one event pair per step): the lidar's launch duration.
+    dev.profile_reset()
+    dev.profile_only("lidar")
+    dev.profile(True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        run.step(False, t + k)
+    dev.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    dev.profile(False)
+    t += args.steps
+    lidar_n, lidar_ms = dev.profile_results().get("lidar", (0, 0.0))
+    lidar_us = lidar_ms / max(lidar_n, 1) * 1000.0
+    log(f"eager steps with lidar events: {eager_ms:.4f} ms/step; lidar {lidar_us:.2f} us over {lidar_n} launches")
+    # untimed: iteration count on the final state, to bracket the measured steps
+    run.sim.count_iterations(True)
+    run.step(False, t)
+    t += 1
+    counts.append(run.sim.last_iterations)
+    run.sim.count_iterations(False)
+    # every kernel, in a shorter profiled pass (an event pair per launch)
+    dev.profile_only(None)
+    dev.profile_reset()
+    dev.profile(True)
+    for k in range(min(args.steps, 100)):
+        run.step(False, t + k)
+    dev.synchronize()
+    dev.profile(False)
+    prof = dev.profile_results()
     for k, (n, tot) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
         log(f"  {k:20s} launches {n:6d} avg {tot / max(n, 1) * 1000:9.2f} us  total {tot:9.2f} ms")
-    assert np.isfinite(run.sim.obs.read()).all(), "non-finite observations"
-
-    lidar_n, lidar_ms = prof.get("lidar", (0, 0.0))
-    lidar_us = lidar_ms / max(lidar_n, 1) * 1000.0
     iters = float(np.mean(counts))
     lbytes = lidar_bytes(iters, args.cars, args.beams, layers)
     gemm_ms = sum(tot for k, (n, tot) in prof.items() if k.startswith(("gemm", "policy", "head")))
@@ -195,7 +228,7 @@ def main() -> int:
         "avg_launch_us": lidar_us,
     }
     peak = MFMA_BF16_PEAK_TFLOPS if args.precision == "bf16" else FP32_PEAK_TFLOPS
-    gemm_tflops = flops * args.steps / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
+    gemm_tflops = flops * min(args.steps, 100) / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
     policy = {"bound": "mfma" if args.precision == "bf16" else "fp32", "achieved": gemm_tflops, "peak": peak,
               "unit": "TFLOP/s", "frac": gemm_tflops / peak if gemm_tflops else None,
               "flops_per_step": flops}
@@ -221,6 +254,8 @@ def main() -> int:
                    "cars": args.cars, "beams": args.beams, "hidden": args.hidden, "track": args.track,
                    "policy_precision": args.precision, "parallelism": "single GPU",
                    "lidar_waves_per_car": waves_per_car},
+        "eager_ms_per_step": eager_ms, "timing": "graph replays of captured Runner steps; kernel durations from "
+                                                 "HIP events in eager passes of the same steps right after",
         "roofline": roofline, "policy_roofline": policy, "kernels": kernels, "cpu_baseline": cpu,
         "plain_gpu_baseline": plain, "speedup_vs_plain_gpu": value / plain["value"] if plain else None,
     }

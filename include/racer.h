@@ -173,6 +173,29 @@ int racer_runner_step(racer_sim* sim, racer_policy* pol, racer_acts* acts, racer
 int racer_benchmark(racer_sim* sim, racer_policy* pol, racer_acts* acts, racer_buf* scratch, int steps,
                     double* ms_per_step);
 
+/* ---------------- Graphs (no reference counterpart) ---------------- */
+/* The reference enqueues every launch of a step from the host
+ * (main.mojo:183-191, racer/ppo.mojo:190-210). These capture the same launch
+ * sequences once as a HIP graph and replay them; results are bitwise those of
+ * the enqueued calls. Sampling kernels inside a graph take their Philox
+ * offset as (device step base + the step's position), the base being written
+ * before each replay. A graph holds the buffer addresses of the handles it was
+ * captured from: those handles must outlive it. The profiler must be off while
+ * capturing, and the iteration counter of the lidar must be off. */
+typedef struct racer_graph racer_graph;
+/* `steps` Runner steps (forward, sample, physics, lidar); replay i of
+ * racer_graph_launch(g, t0) runs the steps t0 .. t0 + steps - 1. */
+int racer_graph_runner(racer_sim* sim, racer_policy* pol, racer_acts* acts, racer_buf* scratch, int steps,
+                       int deterministic, racer_graph** out);
+int racer_graph_launch(racer_graph* g, int64_t t0);
+/* Trainer.rollout (racer/ppo.mojo:187-232) as one graph: the copy in, ROLLOUT
+ * steps, the copy out, the bootstrap forward, GAE and the reward sum. */
+int racer_graph_rollout(racer_trainer* tr, racer_sim* sim, racer_policy* pol, racer_acts* acts, racer_graph** out);
+/* Replays a rollout graph for `iteration` and writes the mean reward per step
+ * (the value racer_trainer_rollout returns). */
+int racer_trainer_rollout_graph(racer_trainer* tr, racer_graph* g, int iteration, float* mean_reward);
+void racer_graph_destroy(racer_graph* g);
+
 /* ---------------- Instrumentation (no reference counterpart) ---------------- */
 /* Kernel timing with HIP events on the device's stream: when enabled, every
  * launch of a named kernel is bracketed by events; totals are read back here. */
@@ -181,6 +204,9 @@ int racer_profile_enable(racer_device* dev, int on);
 int racer_profile_count(racer_device* dev);
 int racer_profile_get(racer_device* dev, int i, const char** name, int64_t* launches, double* total_ms);
 int racer_profile_reset(racer_device* dev);
+/* Restrict timing to the kernel of this name (NULL or "" times every kernel),
+ * so a timed run carries one event pair per step instead of one per launch. */
+int racer_profile_only(racer_device* dev, const char* name);
 /* Lidar march iterations summed over every beam of the last step (counted by
  * an instrumented variant of the lidar kernel when enabled; 0 otherwise). */
 int racer_sim_count_iterations(racer_sim* sim, int on);

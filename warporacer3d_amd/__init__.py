@@ -142,6 +142,12 @@ def lib() -> C.CDLL:
         "racer_sim_set_waves_per_car": (i, [vp, i]),
         "racer_sim_waves_per_car": (i, [vp]),
         "racer_sim_autotune": (i, [vp, C.POINTER(C.c_int)]),
+        "racer_profile_only": (i, [vp, C.c_char_p]),
+        "racer_graph_runner": (i, [vp, vp, vp, vp, i, i, cp]),
+        "racer_graph_rollout": (i, [vp, vp, vp, vp, cp]),
+        "racer_graph_launch": (i, [vp, i64]),
+        "racer_trainer_rollout_graph": (i, [vp, vp, i, C.POINTER(C.c_float)]),
+        "racer_graph_destroy": (None, [vp]),
     }
     for name in ("obs", "actions", "logp", "value", "reward", "done", "advantage", "grad", "d3", "stats", "acts"):
         sigs[f"racer_trainer_{name}"] = (vp, [vp])
@@ -274,6 +280,10 @@ class Device:
 
     def profile_reset(self) -> None:
         _check(lib().racer_profile_reset(self.handle))
+
+    def profile_only(self, name: str | None) -> None:
+        """Time only the kernel called `name` (None: every kernel)."""
+        _check(lib().racer_profile_only(self.handle, (name or "").encode()))
 
     def profile_results(self) -> dict:
         out = {}
@@ -469,10 +479,42 @@ class Trainer:
     def adam(self, policy: Policy) -> None:
         _check(lib().racer_trainer_adam(self.handle, policy.handle))
 
+    def capture_rollout(self, sim: Sim, policy: Policy, acts: Activations) -> "Graph":
+        """The whole rollout as one HIP graph (replay with rollout_graph)."""
+        h = C.c_void_p()
+        _check(lib().racer_graph_rollout(self.handle, sim.handle, policy.handle, acts.handle, C.byref(h)))
+        return Graph(h.value, (self, sim, policy, acts))
+
+    def rollout_graph(self, graph: "Graph", iteration: int) -> float:
+        """Trainer.rollout by replaying a captured graph; same results."""
+        r = C.c_float()
+        _check(lib().racer_trainer_rollout_graph(self.handle, graph.handle, int(iteration), C.byref(r)))
+        return float(r.value)
+
     def __del__(self):
         try:
             if self.handle:
                 lib().racer_trainer_destroy(self.handle)
+        except Exception:
+            pass
+        self.handle = None
+
+
+class Graph:
+    """A captured HIP graph of Runner steps or of a rollout. Keeps the objects
+    it was captured from alive, since the graph holds their buffers."""
+
+    def __init__(self, handle: int, owners: tuple):
+        self.handle = handle
+        self._owners = owners
+
+    def launch(self, t0: int) -> None:
+        _check(lib().racer_graph_launch(self.handle, int(t0)))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().racer_graph_destroy(self.handle)
         except Exception:
             pass
         self.handle = None
@@ -505,6 +547,14 @@ class Runner:
     def step(self, deterministic: bool, t: int) -> None:
         _check(lib().racer_runner_step(self.sim.handle, self.policy.handle, self.acts.handle, self.scratch.handle,
                                        int(deterministic), int(t)))
+
+    def capture(self, steps: int, deterministic: bool = False) -> Graph:
+        """`steps` Runner steps as one HIP graph; graph.launch(t0) runs steps
+        t0 .. t0 + steps - 1, bitwise as step(deterministic, t) would."""
+        h = C.c_void_p()
+        _check(lib().racer_graph_runner(self.sim.handle, self.policy.handle, self.acts.handle, self.scratch.handle,
+                                        int(steps), int(deterministic), C.byref(h)))
+        return Graph(h.value, (self.sim, self.policy, self.acts, self.scratch))
 
     def benchmark(self, steps: int) -> float:
         """10 warm-up steps then `steps` timed steps; returns ms/step (main.mojo:225-235)."""

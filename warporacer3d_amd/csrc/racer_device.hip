@@ -69,6 +69,7 @@ extern "C" int racer_device_create(int gpu, racer_device** out) {
     if (prop.gcnArchName[0]) dev->name += std::string(" (") + prop.gcnArchName + ")";
     dev->cus = prop.multiProcessorCount;
     RACER_TRY(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking));
+    RACER_TRY(hipMalloc(&dev->sums, RACER_SUM_PARTIALS * sizeof(double)));
     *out = dev;
     return 0;
 }
@@ -82,6 +83,8 @@ extern "C" void racer_device_destroy(racer_device* dev) {
     }
     for (auto e : dev->event_pool) hipEventDestroy(e);
     if (dev->scratch) hipFree(dev->scratch);
+    if (dev->graph_base) hipFree(dev->graph_base);
+    if (dev->sums) hipFree(dev->sums);
     hipStreamDestroy(dev->stream);
     delete dev;
 }
@@ -109,6 +112,18 @@ __global__ void k_fill(float* p, size_t n, float v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_copy4(float4* __restrict__ d, const float4* __restrict__ s, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) d[i] = s[i];
+}
+
+__global__ void k_copy1(float* __restrict__ d, const float* __restrict__ s, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) d[i] = s[i];
 }
 
 int racer_fill(racer_device* dev, float* p, size_t n, float v) {
@@ -185,9 +200,31 @@ extern "C" int racer_copy(racer_buf* dst, size_t dst_offset, racer_buf* src, siz
     RACER_REQUIRE(dst && src && dst_offset + count <= dst->len && src_offset + count <= src->len,
                   "copy exceeds a buffer");
     if (!count) return 0;
-    RACER_TRY(hipMemcpyAsync(dst->ptr + dst_offset, src->ptr + src_offset, count * sizeof(float),
-                             hipMemcpyDeviceToDevice, dst->dev->stream));
-    return 0;
+    float* d = dst->ptr + dst_offset;
+    const float* s = src->ptr + src_offset;
+    if (d < s + count && s < d + count) {  // overlapping ranges of one buffer: the runtime's copy
+        RACER_TRY(hipMemcpyAsync(d, s, count * sizeof(float), hipMemcpyDeviceToDevice, dst->dev->stream));
+        return 0;
+    }
+    return racer_copy_raw(dst->dev, d, s, count);
+}
+
+// Own kernel for disjoint ranges: exactly 4 count bytes read and written (a
+// known amount of traffic against which HBM counters can be calibrated), and
+// a kernel node rather than a copy node when captured into a graph.
+int racer_copy_raw(racer_device* dev, float* d, const float* s, size_t count) {
+    if (!count) return 0;
+    racer_timed t(dev, "copy");
+    bool vec = ((uintptr_t)d % 16 == 0) && ((uintptr_t)s % 16 == 0) && count % 4 == 0;
+    size_t items = vec ? count / 4 : count;
+    size_t blocks = (items + 255) / 256;
+    if (blocks > 32768) blocks = 32768;
+    if (vec)
+        hipLaunchKernelGGL(k_copy4, dim3((unsigned)blocks), dim3(256), 0, dev->stream, (float4*)d,
+                           (const float4*)s, items);
+    else
+        hipLaunchKernelGGL(k_copy1, dim3((unsigned)blocks), dim3(256), 0, dev->stream, d, s, items);
+    return racer_launch_check("copy");
 }
 
 // ---------------------------------------------------------------- profiler
@@ -210,7 +247,8 @@ static int kernel_index(racer_device* dev, const char* name) {
 }
 
 racer_timed::racer_timed(racer_device* d, const char* name) : dev(d) {
-    if (!dev || !dev->profiling) return;
+    if (!dev || !dev->profiling || dev->capturing) return;
+    if (!dev->profile_only.empty() && dev->profile_only != name) return;
     kernel = kernel_index(dev, name);
     start = take_event(dev);
     stop = take_event(dev);
@@ -261,5 +299,11 @@ extern "C" int racer_profile_get(racer_device* dev, int i, const char** name, in
 extern "C" int racer_profile_reset(racer_device* dev) {
     resolve_pending(dev);
     dev->samples.clear();
+    return 0;
+}
+
+extern "C" int racer_profile_only(racer_device* dev, const char* name) {
+    resolve_pending(dev);
+    dev->profile_only = name ? name : "";
     return 0;
 }

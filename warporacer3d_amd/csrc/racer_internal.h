@@ -33,6 +33,8 @@ void racer_clear_error();
     } while (0)
 
 // ---------------------------------------------------------------- device
+constexpr int RACER_SUM_PARTIALS = 4096;
+
 struct racer_profile_sample {
     std::string name;
     int64_t launches = 0;
@@ -53,8 +55,16 @@ struct racer_device {
     std::vector<racer_profile_sample> samples;
     std::vector<racer_pending_event> pending;
     std::vector<hipEvent_t> event_pool;
-    float* scratch = nullptr;  // small device scratch: reductions, counters
+    float* scratch = nullptr;  // split-K partial products (grown on demand)
     size_t scratch_len = 0;
+    double* sums = nullptr;    // RACER_SUM_PARTIALS float64 partial sums of the row reductions
+    std::string profile_only;  // when set, the profiler times only the kernel of this name
+    // Graph capture: while a graph is captured, sampling kernels add the device
+    // scalar *step_base to their step argument, so one captured graph serves
+    // every replay; graph_base is that scalar, written before each replay.
+    bool capturing = false;
+    const int64_t* step_base = nullptr;
+    int64_t* graph_base = nullptr;
 };
 
 struct racer_buf {
@@ -169,7 +179,19 @@ inline int64_t racer_pad32(int64_t v) { return (v + 31) / 32 * 32; }
 size_t racer_fragment_elems(int cols);
 int racer_pack_fragments(racer_device* dev, const float* src, int64_t ld_src, int rows, int cols, uint16_t* dst);
 // The fused bf16 policy forward (and optionally Sim.sample) over `cols` columns of x.
+int racer_policy_fused_setup();
 int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
                        float* act, float* logp, float* value, int deterministic, int64_t step, int seed);
 
 int racer_policy_forward_raw(racer_policy* pol, const float* x, int64_t x_cols, racer_acts* acts, int cols);
+int racer_policy_act(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, float* act,
+                     float* logp, float* value, int deterministic, int64_t step, int seed);
+// Re-derive the bf16 weight copies if an update made them stale (no-op otherwise).
+int racer_policy_refresh(racer_policy* p);
+// Device-to-device copy of count floats on the device's stream (own kernel).
+int racer_copy_raw(racer_device* dev, float* dst, const float* src, size_t count);
+// The work of Trainer.rollout up to the device-side reward sum (no host read).
+int racer_rollout_enqueue(racer_trainer* tr, racer_sim* sim, racer_policy* p, racer_acts* acts, int64_t step0);
+// Reads the reward sum of the last rollout (synchronizes) as a mean per step.
+int racer_trainer_mean_reward(racer_trainer* tr, float* mean_reward);
+int racer_fill(racer_device* dev, float* p, size_t n, float v);

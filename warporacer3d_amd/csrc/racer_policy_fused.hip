@@ -67,7 +67,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_fused(
     const uint16_t* __restrict__ F0, const uint16_t* __restrict__ F1,
     const float* __restrict__ W2, const float* __restrict__ x, int64_t ldx, int H, int IN, int64_t cols,
     float* __restrict__ h1out, float* __restrict__ h2out, float* __restrict__ o, const float* __restrict__ log_std,
-    float* act, int64_t act_cols, float* logp, float* value, int deterministic, int64_t step, int seed) {
+    float* act, int64_t act_cols, float* logp, float* value, int deterministic, int64_t step,
+    const int64_t* step_base, int seed) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const FusedLayout Ly = fused_layout(H, IN);
     const int HA = Ly.HA;
@@ -231,6 +232,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_fused(
         __syncthreads();
         const int64_t col = c0 + tid;
         if (tid < FT && col < cols) {
+            if (step_base) step += *step_base;
             float noise[4];
             racer_normal4(racer_random((uint64_t)(int64_t)seed + 1, (uint64_t)col, (uint64_t)step), noise);
             float probability = 0.0f;
@@ -279,6 +281,16 @@ int racer_pack_fragments(racer_device* dev, const float* src, int64_t ld_src, in
 
 size_t racer_policy_fused_smem(int H, int IN) { return fused_layout(H, IN).total; }
 
+// Allows the fused kernel more than the default 64 KB of dynamic LDS (gfx950
+// has 160 KB per CU); called when a policy that uses it is created.
+int racer_policy_fused_setup() {
+    RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024));
+    RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024));
+    return 0;
+}
+
 int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
                        float* act, float* logp, float* value, int deterministic, int64_t step, int seed) {
     racer_device* dev = p->dev;
@@ -288,23 +300,15 @@ int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_ac
     const float* log_std = p->theta->ptr + p->LOGSTD;
     dim3 grid((unsigned)((cols + FT - 1) / FT)), block(64 * WAVES);
     size_t shmem = racer_policy_fused_smem(p->hidden, (int)p->IN);
-    static bool raised = false;
-    if (!raised) {  // allow more than the default 64 KB of dynamic LDS (gfx950 has 160 KB per CU)
-        RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused<true>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused<false>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        raised = true;
-    }
     if (sample) {
         racer_timed timer(dev, "policy_fused_sample");
         hipLaunchKernelGGL(k_policy_fused<true>, grid, block, shmem, dev->stream, f0, f1, W2, x, x_cols, p->hidden, (int)p->IN, (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr, log_std,
-                           act, (int64_t)cols, logp, value, deterministic, step, seed);
+                           act, (int64_t)cols, logp, value, deterministic, step, dev->step_base, seed);
     } else {
         racer_timed timer(dev, "policy_fused");
         hipLaunchKernelGGL(k_policy_fused<false>, grid, block, shmem, dev->stream, f0, f1, W2, x,
                            x_cols, p->hidden, (int)p->IN, (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr,
-                           log_std, act, (int64_t)cols, logp, value, deterministic, step, seed);
+                           log_std, act, (int64_t)cols, logp, value, deterministic, step, dev->step_base, seed);
     }
     return racer_launch_check("policy_fused");
 }

@@ -10,7 +10,6 @@
 
 #include "racer_internal.h"
 
-int racer_fill(racer_device* dev, float* p, size_t n, float v);
 
 namespace {
 
@@ -65,6 +64,7 @@ __global__ void k_gae(int n, int rollout, const float* reward, const float* done
 // Deterministic two-pass float64 sums of up to four rows (x, and x^2 when
 // `square`): pass 1 writes one partial per block, pass 2 sums them in order.
 constexpr int SUM_BLOCKS = 1024;
+static_assert(4 * SUM_BLOCKS <= RACER_SUM_PARTIALS, "the device's partial-sum buffer holds 4 outputs");
 
 __global__ void __launch_bounds__(256) k_sum_partial(const float* x, int64_t count, int64_t row_stride, int rows,
                                                      int square, double* partial) {
@@ -170,7 +170,8 @@ template <bool SAMPLE>
 __global__ void __launch_bounds__(256) k_head(const float* __restrict__ W2, const float* __restrict__ h2, int HA,
                                               int64_t cols, float* __restrict__ o, const float* log_std,
                                               float* act, int64_t act_cols, float* logp, float* value,
-                                              int deterministic, int64_t step, int seed) {
+                                              int deterministic, int64_t step, const int64_t* step_base,
+                                              int seed) {
     extern __shared__ float tile[];  // [HA][HEAD_COLS], then W2 [3][HA], then out[3][HEAD_COLS]
     float* wl = tile + HA * HEAD_COLS;
     float* out = wl + 3 * HA;
@@ -194,6 +195,7 @@ __global__ void __launch_bounds__(256) k_head(const float* __restrict__ W2, cons
         __syncthreads();
         const int64_t col = c0 + tid;
         if (tid < HEAD_COLS && col < cols) {
+            if (step_base) step += *step_base;
             float noise[4];
             racer_normal4(racer_random((uint64_t)(int64_t)seed + 1, (uint64_t)col, (uint64_t)step), noise);
             float probability = 0.0f;
@@ -218,11 +220,11 @@ int launch_head(racer_policy* p, const float* h2, int64_t cols, float* o, bool s
     if (sample) {
         racer_timed timer(p->dev, "head_sample");
         hipLaunchKernelGGL(k_head<true>, dim3(blocks), dim3(128), shmem, p->dev->stream, W2, h2, (int)p->HA, cols, o,
-                           log_std, act, cols, logp, value, deterministic, step, seed);
+                           log_std, act, cols, logp, value, deterministic, step, p->dev->step_base, seed);
     } else {
         racer_timed timer(p->dev, "head");
         hipLaunchKernelGGL(k_head<false>, dim3(blocks), dim3(128), shmem, p->dev->stream, W2, h2, (int)p->HA, cols,
-                           o, log_std, act, cols, logp, value, deterministic, step, seed);
+                           o, log_std, act, cols, logp, value, deterministic, step, p->dev->step_base, seed);
     }
     return racer_launch_check("head");
 }
@@ -231,8 +233,8 @@ int row_sums(racer_device* dev, const float* x, int64_t count, int64_t row_strid
              double* out64, float* out32, int64_t slot) {
     int outputs = rows * (1 + square);
     int blocks = (int)std::min<int64_t>(SUM_BLOCKS, std::max<int64_t>(1, (count + 255) / 256));
-    if (racer_ensure_scratch(dev, (size_t)outputs * blocks * 2)) return 1;
-    double* partial = reinterpret_cast<double*>(dev->scratch);
+    RACER_REQUIRE(outputs * blocks <= RACER_SUM_PARTIALS, "row sums: too many outputs");
+    double* partial = dev->sums;  // fixed for the device's lifetime, so captured graphs may hold it
     {
         racer_timed timer(dev, "sum");
         hipLaunchKernelGGL(k_sum_partial, dim3(blocks), dim3(256), 0, dev->stream, x, count, row_stride, rows, square,
@@ -314,6 +316,7 @@ extern "C" int racer_policy_create(racer_device* dev, int seed, const racer_conf
     if (H <= 256) {
         RACER_TRY(hipMalloc(&p->frag0, racer_fragment_elems((int)p->IN) * sizeof(uint16_t)));
         RACER_TRY(hipMalloc(&p->frag1, racer_fragment_elems((int)p->HA) * sizeof(uint16_t)));
+        if (p->precision == 1 && racer_policy_fused_setup()) return 1;
     }
     hipLaunchKernelGGL(k_policy_init, dim3((unsigned)((p->P + 255) / 256)), dim3(256), 0, dev->stream, p->theta->ptr,
                        p->P, p->IN, p->IN - 1, H, p->HA, p->W1, p->W2, p->LOGSTD, seed);
@@ -360,7 +363,6 @@ extern "C" int racer_policy_transpose(racer_policy* p) {
     return racer_launch_check("transpose");
 }
 
-// Hidden layers of the forward pass (racer/ppo.mojo:109-110).
 // The bf16 weight copies used by the MFMA products, refreshed after updates.
 static int refresh_bf16(racer_policy* p) {
     if (!p->bf16_stale) return 0;
@@ -376,6 +378,8 @@ static int refresh_bf16(racer_policy* p) {
     p->bf16_stale = false;
     return 0;
 }
+
+int racer_policy_refresh(racer_policy* p) { return p->precision == 1 ? refresh_bf16(p) : 0; }
 
 // Hidden layers of the forward pass (racer/ppo.mojo:109-110) as two products.
 static int forward_hidden(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols) {
@@ -535,28 +539,25 @@ static int check_compatible(racer_trainer* tr, racer_policy* p) {
     return 0;
 }
 
-extern "C" int racer_trainer_rollout(racer_trainer* tr, racer_sim* sim, racer_policy* p, racer_acts* acts,
-                                     int iteration, float* mean_reward) {
-    racer_clear_error();
+// racer/ppo.mojo:190-231: everything up to the reward sum, which stays on the
+// device (float64 in dstats[2]); the sample of step t uses Philox offset step0 + t.
+int racer_rollout_enqueue(racer_trainer* tr, racer_sim* sim, racer_policy* p, racer_acts* acts, int64_t step0) {
     if (check_compatible(tr, p)) return 1;
     RACER_REQUIRE(sim->n == tr->n && sim->beams == tr->beams, "trainer and sim shapes differ");
     RACER_REQUIRE(acts->cols == tr->n && acts->hidden == tr->hidden, "activations must have one column per car");
     racer_device* dev = tr->dev;
     int64_t n = tr->n, IN = tr->IN, R = tr->rollout;
-    // racer/ppo.mojo:190-210
-    RACER_TRY(hipMemcpyAsync(tr->obs->ptr, sim->obs->ptr, IN * n * sizeof(float), hipMemcpyDeviceToDevice,
-                             dev->stream));
+    if (racer_copy_raw(dev, tr->obs->ptr, sim->obs->ptr, (size_t)(IN * n))) return 1;
     for (int64_t t = 0; t < R; ++t) {
         float* obs_t = tr->obs->ptr + t * IN * n;
         if (racer_policy_act(p, obs_t, n, acts, (int)n, tr->actions->ptr + t * 2 * n, tr->logp->ptr + t * n,
-                             tr->value->ptr + t * n, 0, (int64_t)iteration * R + t, sim->seed))
+                             tr->value->ptr + t * n, 0, step0 + t, sim->seed))
             return 1;
         if (racer_launch_step(sim, tr->actions->ptr + t * 2 * n, tr->reward->ptr + t * n, tr->done->ptr + t * n,
                               tr->obs->ptr + (t + 1) * IN * n, n))
             return 1;
     }
-    RACER_TRY(hipMemcpyAsync(sim->obs->ptr, tr->obs->ptr + R * IN * n, IN * n * sizeof(float),
-                             hipMemcpyDeviceToDevice, dev->stream));
+    if (racer_copy_raw(dev, sim->obs->ptr, tr->obs->ptr + R * IN * n, (size_t)(IN * n))) return 1;
     if (racer_policy_forward_raw(p, tr->obs->ptr + R * IN * n, n, acts, (int)n)) return 1;
     {
         racer_timed timer(dev, "gae");
@@ -564,7 +565,19 @@ extern "C" int racer_trainer_rollout(racer_trainer* tr, racer_sim* sim, racer_po
                            tr->reward->ptr, tr->done->ptr, tr->value->ptr, acts->o->ptr + 2 * n, tr->advantage->ptr);
     }
     if (racer_launch_check("gae")) return 1;
-    if (row_sums(dev, tr->reward->ptr, R * n, 0, 1, 0, tr->dstats + 2, tr->stats->ptr, RACER_STAT_REWARD)) return 1;
+    return row_sums(dev, tr->reward->ptr, R * n, 0, 1, 0, tr->dstats + 2, tr->stats->ptr, RACER_STAT_REWARD);
+}
+
+extern "C" int racer_trainer_rollout(racer_trainer* tr, racer_sim* sim, racer_policy* p, racer_acts* acts,
+                                     int iteration, float* mean_reward) {
+    racer_clear_error();
+    if (racer_rollout_enqueue(tr, sim, p, acts, (int64_t)iteration * tr->rollout)) return 1;
+    return racer_trainer_mean_reward(tr, mean_reward);
+}
+
+int racer_trainer_mean_reward(racer_trainer* tr, float* mean_reward) {
+    racer_device* dev = tr->dev;
+    int64_t n = tr->n, R = tr->rollout;
     double total = 0.0;
     RACER_TRY(hipMemcpyAsync(&total, tr->dstats + 2, sizeof(double), hipMemcpyDeviceToHost, dev->stream));
     RACER_TRY(hipStreamSynchronize(dev->stream));

@@ -544,9 +544,11 @@ __global__ void __launch_bounds__(256) k_lidar_flat(racer_dmap m, const float* _
 // Sim.sample's kernel (racer/sim.mojo:337-348).
 __global__ void __launch_bounds__(256) k_sample(int n, const float* out, int64_t out_cols, const float* log_std,
                                                 float* act, int64_t act_cols, float* logp, float* value,
-                                                int deterministic, int64_t step, int seed) {
+                                                int deterministic, int64_t step, const int64_t* step_base,
+                                                int seed) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (step_base) step += *step_base;
     float noise[4];
     racer_normal4(racer_random((uint64_t)(int64_t)seed + 1, (uint64_t)i, (uint64_t)step), noise);
     float probability = 0.0f;
@@ -638,7 +640,7 @@ int racer_launch_sample_raw(racer_device* dev, int n, const float* out, int64_t 
     {
         racer_timed timer(dev, "sample");
         hipLaunchKernelGGL(k_sample, dim3((n + 255) / 256), dim3(256), 0, dev->stream, n, out, out_cols, log_std,
-                           actions, act_cols, logp, value, deterministic, step, seed);
+                           actions, act_cols, logp, value, deterministic, step, dev->step_base, seed);
     }
     return racer_launch_check("sample");
 }
