@@ -6,6 +6,32 @@
  *
  * Layout follows the reference: every buffer is feature-major float32
  * [rows, cols]; a "view" is a base pointer whose rows are `cols` floats apart.
+ *
+ * What pins each part (tests/test_oracle_*.py), without running the reference
+ * (Mojo is not available here):
+ *  - Map.at/probe/surface/progress: the reference test's corridor track and its
+ *    answers (tests/test_mojo.mojo:27-92), the bridge invariants (:100-107),
+ *    brute force on small grids;
+ *  - derivative/advance: the closed forms of tests/test_mojo.mojo:48-55, a
+ *    float64 restatement of the formula, RK4 against a fine float64 integrator,
+ *    the corridor and timeout bookkeeping;
+ *  - scan: the corridor ranges (7.1, 0.73), the closed-form ray-circle range on
+ *    the flat ring, a brute-force fine march that never passes a wall;
+ *  - GAE, the PPO gradient, KL and Adam: the host recurrence, central finite
+ *    differences of a float64 loss (tests/test_mojo.mojo:137-214), the textbook
+ *    Adam step, the init layout and its absolute-index quirk;
+ *  - Philox4x32-10: the Random123 known-answer vectors; uniform and Box-Muller
+ *    normals: their range and moment statistics;
+ *  - sin, cos, exp, log, tanh: ULP bounds against float64 libm and special
+ *    values; the two division shortcuts: exhaustive and sampled checks.
+ * Parity unpinned (no fixture or closed form fixes the bits, and the Mojo
+ * standard library that defines them is absent): the mapping of Mojo's
+ * Random(seed, subsequence, offset) onto the Philox counter and key and its
+ * uint-to-float conversion (this oracle uses counter = (offset, subsequence),
+ * key = seed); the association order of SIMD.reduce_add (this oracle uses
+ * (v0 + v2) + (v1 + v3)); the last-bit behaviour of Mojo's sin, cos, exp,
+ * log and tanh; and the accumulation order of MAX's matmul and sum (this
+ * oracle sums in k order with fmaf, and reduces in float64).
  */
 #ifndef RACER_ORACLE_H
 #define RACER_ORACLE_H
