@@ -77,6 +77,64 @@ def cpu_baseline(track, beams: int, hidden: int, threads: int, seconds: float) -
                       f"({dt:.1f} s) on {threads} host threads"}
 
 
+def ppo_flops(cars: int, beams: int, hidden: int, rollout: int, epochs: int) -> float:
+    """Flops of one PPO iteration: the rollout's (R + 1) forwards per car and the
+    update's epochs x R x n samples of forward plus backward."""
+    IN, HA, H = beams + 5, hidden + 1, hidden
+    fwd = 2.0 * (H * IN + H * HA + 3 * HA)
+    bwd = 2.0 * HA * (3 + H) + fwd
+    return (rollout + 1) * cars * fwd + epochs * rollout * cars * (fwd + bwd)
+
+
+def ppo_leg(R, T, dev, beams: int, iterations: int, cars: int = 16384, precision: int = 1) -> dict:
+    """PPO iterations on the bridge (two 8 m lobes, deck 0.5 m above the
+    crossing, two layers): 16384 cars, 128-step rollouts replayed from a HIP
+    graph, 4 epochs x 8 minibatches, 2 x 256 policy, Adam 3e-4."""
+    arr = T.cached("bridge8")
+    cfg = R.Config(beams=beams, hidden=256, rollout=128, epochs=4, minibatches=8, precision=precision,
+                   heading_noise=0.1)
+    sim = R.Sim(dev, arr, cars, 42, cfg)
+    sim.reset()
+    pol = R.Policy(dev, 42, cfg)
+    acts = R.Activations(dev, cars, cfg)
+    tr = R.Trainer(dev, cars, cfg)
+    sim.autotune()
+    graph = tr.capture_rollout(sim, pol, acts)
+    tr.rollout_graph(graph, 0)  # untimed first iteration
+    tr.update(pol)
+    dev.synchronize()
+    roll, upd, rewards, kls = [], [], [], []
+    for it in range(1, iterations + 1):
+        t0 = time.perf_counter()
+        rewards.append(tr.rollout_graph(graph, it))
+        t1 = time.perf_counter()
+        kls.append(tr.update(pol))
+        t2 = time.perf_counter()
+        roll.append(t1 - t0)
+        upd.append(t2 - t1)
+    # one more iteration, eager and with an event pair per launch: kernel shares
+    dev.profile_only(None)
+    dev.profile_reset()
+    dev.profile(True)
+    tr.rollout(sim, pol, acts, iterations + 1)
+    tr.update(pol)
+    dev.profile(False)
+    prof = dev.profile_results()
+    top = sorted(prof.items(), key=lambda kv: -kv[1][1])[:10]
+    it_s = (sum(roll) + sum(upd)) / iterations
+    flops = ppo_flops(cars, beams, 256, 128, 4)
+    del graph
+    return {"iteration_s": it_s, "rollout_s": sum(roll) / iterations, "update_s": sum(upd) / iterations,
+            "iterations": iterations, "env_steps_per_s_in_rollout": 128 * cars / (sum(roll) / iterations),
+            "flops_per_iteration": flops, "tflops": flops / it_s / 1e12,
+            "mean_reward": rewards, "kl": kls,
+            "config": {"track": "bridge8 (two 8 m lobes, 2 layers)", "cars": cars, "beams": beams, "hidden": 256,
+                       "rollout": 128, "epochs": 4, "minibatches": 8,
+                       "precision": {0: "fp32", 1: "bf16+fp32", 2: "plain fp32"}[precision],
+                       "lidar_waves_per_car": sim.waves_per_car},
+            "kernels_ms_per_iteration": {k: tot for k, (n, tot) in top}}
+
+
 def plain_baseline(R, dev, track, args, steps: int) -> dict:
     """The same Runner step in the reference's own GPU decomposition, on the
     same GPU: one thread per car for physics, one thread per (car, beam) for the
@@ -120,6 +178,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU decomposition baseline")
+    ap.add_argument("--ppo-iterations", type=int, default=3,
+                    help="timed PPO iterations per beam count on the bridge (0 skips the PPO leg)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied once device to device before the run (known-size traffic for counters)")
     args = ap.parse_args()
@@ -237,6 +297,13 @@ def main() -> int:
     if not args.no_plain:
         plain = plain_baseline(R, dev, arr, args, min(args.steps, 50))
         log("plain GPU baseline:", plain)
+    ppo = None
+    if args.ppo_iterations > 0:
+        ppo = {}
+        for beams in (1081, 108):
+            ppo[f"beams{beams}"] = ppo_leg(R, T, dev, beams, args.ppo_iterations)
+            log(f"PPO iteration (bridge8, 16384 cars, {beams} beams):",
+                {k: v for k, v in ppo[f'beams{beams}'].items() if k != "kernels_ms_per_iteration"})
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -258,6 +325,7 @@ def main() -> int:
                                                  "HIP events in eager passes of the same steps right after",
         "roofline": roofline, "policy_roofline": policy, "kernels": kernels, "cpu_baseline": cpu,
         "plain_gpu_baseline": plain, "speedup_vs_plain_gpu": value / plain["value"] if plain else None,
+        "ppo_iteration": ppo,
     }
     print(json.dumps(out), flush=True)
     return 0

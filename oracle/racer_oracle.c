@@ -389,6 +389,21 @@ void o_forward(const float* theta, int beams, int hidden, const float* x, long x
     mm(W2, HA, h2, cols, o, cols, 3, HA, cols, 0);
 }
 
+/* sum_c a[c] b[c] over the batch in the order the weight gradients use (the
+ * order of MAX's matmul is not known, see the header): fmaf in increasing c
+ * within chunks of 4096, then the chunk sums added in order to 0. */
+static float chunked_dot(const float* a, const float* b, long cols) {
+    float total = 0.0f;
+    for (long c0 = 0; c0 < cols || c0 == 0; c0 += 4096) {
+        long c1 = c0 + 4096 < cols ? c0 + 4096 : cols;
+        float acc = 0.0f;
+        for (long c = c0; c < c1; ++c) acc = fmaf(a[c], b[c], acc);
+        total += acc;
+        if (c1 >= cols) break;
+    }
+    return total;
+}
+
 /* ppo.mojo:113-138. d3 holds [3, cols] output gradients; d2, d1 get [HA, cols]
  * (bias rows masked to zero); grad receives the W0, W1, W2 entries. */
 void o_backward(const float* theta, int beams, int hidden, const float* x, long cols, const float* h1, const float* h2,
@@ -418,23 +433,11 @@ void o_backward(const float* theta, int beams, int hidden, const float* x, long 
     float* g2 = g1 + H * HA;
 #pragma omp parallel for schedule(static) num_threads(g_threads)
     for (long j = 0; j < H; ++j) {
-        for (long k = 0; k < IN; ++k) {
-            float acc = 0.0f;
-            for (long c = 0; c < cols; ++c) acc = fmaf(d1[j * cols + c], x[k * cols + c], acc);
-            g0[j * IN + k] = acc;
-        }
-        for (long k = 0; k < HA; ++k) {
-            float acc = 0.0f;
-            for (long c = 0; c < cols; ++c) acc = fmaf(d2[j * cols + c], h1[k * cols + c], acc);
-            g1[j * HA + k] = acc;
-        }
+        for (long k = 0; k < IN; ++k) g0[j * IN + k] = chunked_dot(d1 + j * cols, x + k * cols, cols);
+        for (long k = 0; k < HA; ++k) g1[j * HA + k] = chunked_dot(d2 + j * cols, h1 + k * cols, cols);
     }
     for (long j = 0; j < 3; ++j)
-        for (long k = 0; k < HA; ++k) {
-            float acc = 0.0f;
-            for (long c = 0; c < cols; ++c) acc = fmaf(d3[j * cols + c], h2[k * cols + c], acc);
-            g2[j * HA + k] = acc;
-        }
+        for (long k = 0; k < HA; ++k) g2[j * HA + k] = chunked_dot(d3 + j * cols, h2 + k * cols, cols);
 }
 
 /* ---------------- Trainer: ppo.mojo:187-337 ---------------- */

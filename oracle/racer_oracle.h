@@ -31,7 +31,8 @@
  * key = seed); the association order of SIMD.reduce_add (this oracle uses
  * (v0 + v2) + (v1 + v3)); the last-bit behaviour of Mojo's sin, cos, exp,
  * log and tanh; and the accumulation order of MAX's matmul and sum (this
- * oracle sums in k order with fmaf, and reduces in float64).
+ * oracle sums products in k order with fmaf, weight gradients in 4096-wide
+ * chunks of the batch added in order, and reductions in float64).
  */
 #ifndef RACER_ORACLE_H
 #define RACER_ORACLE_H

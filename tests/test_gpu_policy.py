@@ -157,3 +157,40 @@ def test_fused_bf16_forward_and_sample(dev, beams, hidden, cols):
     assert np.array_equal(sim.actions.read().reshape(2, cols), oa)
     sc = scratch.read()
     assert np.array_equal(sc[:cols], ol) and np.array_equal(sc[cols:], ov)
+
+
+@pytest.mark.parametrize("cols", [9000, 640])
+def test_bf16_weight_gradients_at_batch_scale(dev, cols):
+    """The MFMA weight-gradient products (W0: 256 x 1086, W1: 256 x 257 over K =
+    cols, split into 8192-wide chunks) against a float64 product of the same
+    bf16-rounded operands, within the float32 accumulation bound
+    |G - G64| <= (8192 + chunks) 2^-24 sum_k |a_k b_k|; the float32 output-layer
+    gradient (3 rows, the skinny kernel) bit-identical to the oracle."""
+    print(f"bf16 weight gradients cols={cols}", flush=True)
+    beams, hidden = 1081, 256
+    g, o, cfg = _policy_pair(dev, beams, hidden, precision=1)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, (cfg.IN, cols)).astype(np.float32)
+    x[-1] = 1
+    d3 = rng.normal(0, 0.01, (3, cols)).astype(np.float32)
+    xb, d3b = dev.upload(x), dev.upload(d3)
+    acts = R.Activations(dev, cols, cfg)
+    g.forward(R.mat(xb, cfg.IN, cols), acts)
+    d2, d1, grad = dev.alloc(cfg.HA * cols), dev.alloc(cfg.HA * cols), dev.alloc(cfg.params())
+    g.backward(R.mat(xb, cfg.IN, cols), acts, R.mat(d3b, 3, cols), d2, d1, grad)
+    gg = grad.read()
+    h1 = acts.h1.read().reshape(cfg.HA, cols)
+    h2 = acts.h2.read().reshape(cfg.HA, cols)
+    gd1 = d1.read().reshape(cfg.HA, cols)
+    gd2 = d2.read().reshape(cfg.HA, cols)
+    chunks = -(-cols // 8192)
+    slack = (8192 + chunks) * 2.0 ** -24
+    for a, b, off, rows, width in ((gd1[:hidden], x, 0, hidden, cfg.IN),
+                                   (gd2[:hidden], h1, hidden * cfg.IN, hidden, cfg.HA)):
+        a64, b64 = _bf16(a), _bf16(b)
+        ref = a64 @ b64.T
+        mag = np.abs(a64) @ np.abs(b64).T
+        got = gg[off: off + rows * width].reshape(rows, width).astype(np.float64)
+        assert np.all(np.abs(got - ref) <= slack * mag + 1e-30), np.max(np.abs(got - ref) / (mag + 1e-30))
+    _, _, rgrad = o.backward(x, h1, h2, d3)
+    assert np.array_equal(gg[o.W2: o.LOGSTD], rgrad[o.W2: o.LOGSTD])

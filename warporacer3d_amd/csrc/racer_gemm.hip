@@ -142,6 +142,66 @@ __global__ void __launch_bounds__(256) k_gemm_nt_f32(const float* __restrict__ A
 }
 
 // C[m, n] = sum over chunks c in order of partial[c][m, n].
+// The same chunked product for at most four rows of A (the output layer's
+// gradient, M = 3): thread (j, n) of a 256-thread workgroup accumulates
+// A[j, k] B[n, k] with fmaf over k in order, exactly as k_gemm_nt_f32 does,
+// for 64 rows n. B arrives 64 rows x 64 k per stage through LDS with float4
+// loads (16 threads per 256-byte row); the next stage is loaded while this one
+// is summed.
+constexpr int SK_N = 64, SK_K = 64;
+
+__global__ void __launch_bounds__(256) k_gemm_nt_f32_skinny(const float* __restrict__ A, int64_t lda,
+                                                            const float* __restrict__ B, int64_t ldb,
+                                                            float* __restrict__ out, int64_t out_stride, int64_t ldc,
+                                                            int M, int N, int K, int chunk) {
+    __shared__ float Bs[SK_N][SK_K + 1];
+    __shared__ float As[4][SK_K];
+    const int tid = threadIdx.x;
+    const int n = tid & 63, j = tid >> 6;
+    const int n0 = blockIdx.x * SK_N;
+    const int kbeg = blockIdx.z * chunk;
+    const int kend = min(K, kbeg + chunk);
+    const auto bres = racer_rsrc(B, ((uint64_t)(N - 1) * ldb + K) * 4);
+    const auto ares = racer_rsrc(A, ((uint64_t)(M - 1) * lda + K) * 4);
+    float rb[4][4], ra[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int idx = tid + 256 * i, r = idx >> 4, kq = (idx & 15) * 4;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int k = k0 + kq + e;
+                const uint32_t off =
+                    (n0 + r < N && k < kend) ? (uint32_t)(((int64_t)(n0 + r) * ldb + k) * 4) : RACER_OOB;
+                rb[i][e] = racer_ld_f32(bres, off);
+            }
+        }
+        {
+            const int r = tid >> 6, k = k0 + (tid & 63);
+            const uint32_t off = (r < M && k < kend) ? (uint32_t)(((int64_t)r * lda + k) * 4) : RACER_OOB;
+            ra[0] = racer_ld_f32(ares, off);
+        }
+    };
+    float acc = 0.0f;
+    load(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += SK_K) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int idx = tid + 256 * i, r = idx >> 4, kq = (idx & 15) * 4;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Bs[r][kq + e] = rb[i][e];
+        }
+        As[tid >> 6][tid & 63] = ra[0];
+        __syncthreads();
+        load(k0 + SK_K);
+        const int kv = min(SK_K, kend - k0);
+        if (j < M)
+            for (int q = 0; q < kv; ++q) acc = fmaf(As[j][q], Bs[n][q], acc);
+        __syncthreads();
+    }
+    if (j < M && n0 + n < N) out[(int64_t)blockIdx.z * out_stride + (int64_t)j * ldc + n0 + n] = acc;
+}
+
 __global__ void k_sum_chunks(const float* __restrict__ partial, int64_t stride, int chunks, float* __restrict__ C,
                              int64_t ldc, int M, int N) {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -178,9 +238,15 @@ int racer_gemm_nt_f32(racer_device* dev, const float* A, int64_t lda, const floa
     int chunks = K <= 0 ? 1 : (K + chunk - 1) / chunk;
     int64_t stride = (int64_t)M * ldc;
     if (racer_ensure_scratch(dev, (size_t)(stride * chunks))) return 1;
-    dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, chunks), block(256);
-    {
+    const bool reach = ((uint64_t)(M - 1) * lda + K) * 4 < 0xFFFFFF00ull && ((uint64_t)(N - 1) * ldb + K) * 4 < 0xFFFFFF00ull;
+    if (M <= 4 && reach) {
+        racer_timed timer(dev, "gemm_nt_f32_skinny");
+        dim3 grid((N + SK_N - 1) / SK_N, 1, chunks), block(256);
+        hipLaunchKernelGGL(k_gemm_nt_f32_skinny, grid, block, 0, dev->stream, A, lda, B, ldb, dev->scratch, stride,
+                           ldc, M, N, K, chunk);
+    } else {
         racer_timed timer(dev, "gemm_nt_f32");
+        dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, chunks), block(256);
         hipLaunchKernelGGL(k_gemm_nt_f32, grid, block, 0, dev->stream, A, lda, B, ldb, dev->scratch, stride, ldc, M,
                            N, K, chunk);
     }

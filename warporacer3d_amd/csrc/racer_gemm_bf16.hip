@@ -180,6 +180,104 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
     }
 }
 
+// Weight gradients C[M, N] = A[M, K] B[N, K]^T (racer/ppo.mojo:121-122) with K
+// the long batch dimension and both operands float32 with k contiguous, split
+// over K into chunks whose partial products are summed afterwards in chunk
+// order. One 256-thread workgroup owns a 128 x 128 tile of C; wavefront w
+// owns the 64 x 64 block (w / 2, w % 2) as 2 x 2 v_mfma_f32_32x32x16_bf16
+// accumulators. K advances in stages of 64: each thread fetches eight float4
+// of A and eight of B (16 consecutive threads cover one row's 256 bytes) with
+// buffer loads, rounds them to bf16 (nearest even) into LDS as [row][k], and
+// the next stage's loads are issued before this stage's 16 MFMAs per
+// wavefront. Rows past M or N and k past the chunk read as zeros.
+constexpr int WG_T = 128, WG_K = 64, WG_LD = WG_K + 8;
+
+__global__ void __launch_bounds__(256) k_wgrad_bf16(const float* __restrict__ A, int64_t lda,
+                                                    const float* __restrict__ B, int64_t ldb, float* __restrict__ P,
+                                                    int64_t ldp, int64_t chunk_stride, int M, int N, int K,
+                                                    int kchunk) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[WG_T][WG_LD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[WG_T][WG_LD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.y * WG_T, n0 = blockIdx.x * WG_T;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(K, kbeg + kchunk);
+    // descriptors from the chunk's first k: offsets stay below 4 GB
+    const auto ares = racer_rsrc(A + kbeg, ((uint64_t)(M - 1) * lda + (kend - kbeg)) * 4);
+    const auto bres = racer_rsrc(B + kbeg, ((uint64_t)(N - 1) * ldb + (kend - kbeg)) * 4);
+    uint4 ra[8], rb[8];
+    auto load = [&](int k0) {  // k0 relative to kbeg
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int idx = tid + 256 * i, r = idx >> 4, kq = (idx & 15) * 4;
+            const int k = k0 + kq;
+            const bool kin = kbeg + k < kend;
+            const uint32_t ao = (m0 + r < M && kin) ? (uint32_t)(((int64_t)(m0 + r) * lda + k) * 4) : RACER_OOB;
+            const uint32_t bo = (n0 + r < N && kin) ? (uint32_t)(((int64_t)(n0 + r) * ldb + k) * 4) : RACER_OOB;
+            ra[i] = racer_ld_b128(ares, ao);
+            rb[i] = racer_ld_b128(bres, bo);
+        }
+    };
+    auto stage = [&](int k0) {  // registers -> LDS as bf16, zeroing k at or past the chunk's end
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int idx = tid + 256 * i, r = idx >> 4, kq = (idx & 15) * 4;
+            const int lim = kend - (kbeg + k0 + kq);  // valid elements in this float4
+            const uint32_t* av = reinterpret_cast<const uint32_t*>(&ra[i]);
+            const uint32_t* bv = reinterpret_cast<const uint32_t*>(&rb[i]);
+            uint16_t a4[4], b4[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                a4[e] = e < lim ? to_bf16(__builtin_bit_cast(float, av[e])) : (uint16_t)0;
+                b4[e] = e < lim ? to_bf16(__builtin_bit_cast(float, bv[e])) : (uint16_t)0;
+            }
+            *reinterpret_cast<uint2*>(&As[r][kq]) = *reinterpret_cast<uint2*>(a4);
+            *reinterpret_cast<uint2*>(&Bs[r][kq]) = *reinterpret_cast<uint2*>(b4);
+        }
+    };
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[a][b][q] = 0.0f;
+    const int span = kend - kbeg;
+    load(0);
+    for (int k0 = 0; k0 < span; k0 += WG_K) {
+        stage(k0);
+        __syncthreads();
+        load(k0 + WG_K);  // past the chunk: out-of-range offsets, zeros, never staged
+        const int r = lane & 31, kb = (lane >> 5) * 8;
+#pragma unroll
+        for (int ks = 0; ks < WG_K; ks += 16) {
+            bf16x8 a0 = *reinterpret_cast<const bf16x8*>(&As[wm * 64 + r][ks + kb]);
+            bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&As[wm * 64 + 32 + r][ks + kb]);
+            bf16x8 b0 = *reinterpret_cast<const bf16x8*>(&Bs[wn * 64 + r][ks + kb]);
+            bf16x8 b1 = *reinterpret_cast<const bf16x8*>(&Bs[wn * 64 + 32 + r][ks + kb]);
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D[i][j]: j = lane % 32, i = 8 (q / 4) + 4 (lane / 32) + q % 4
+    float* dst = P + (int64_t)blockIdx.z * chunk_stride;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int j = n0 + wn * 64 + 32 * b + (lane & 31);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int i = m0 + wm * 64 + 32 * a + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                if (i < M && j < N) dst[(int64_t)i * ldp + j] = acc[a][b][q];
+            }
+        }
+}
+
 __global__ void k_pack_bf16(const float* src, int64_t ld_src, uint16_t* dst, int64_t ld_dst, int rows, int cols) {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)rows * ld_dst) return;
@@ -228,6 +326,29 @@ int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const 
 int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
                        int64_t ldc, int M, int N, int K) {
     if (M <= 0 || N <= 0) return 0;
+    // the MFMA tile kernel: 16-byte aligned rows and operands within the reach of 32-bit offsets
+    const bool aligned = lda % 4 == 0 && ldb % 4 == 0 && K % 4 == 0 && (uintptr_t)A % 16 == 0 && (uintptr_t)B % 16 == 0;
+    const bool reach = ((uint64_t)(M - 1) * lda + K) * 4 < 0xFFFFFF00ull && ((uint64_t)(N - 1) * ldb + K) * 4 < 0xFFFFFF00ull;
+    if (aligned && reach && K > 0) {
+        const int chunk = 8192;
+        int chunks = (K + chunk - 1) / chunk;
+        int64_t stride = (int64_t)M * ldc;
+        if (racer_ensure_scratch(dev, (size_t)(stride * chunks))) return 1;
+        dim3 grid((N + WG_T - 1) / WG_T, (M + WG_T - 1) / WG_T, chunks), block(256);
+        {
+            racer_timed timer(dev, "wgrad_bf16");
+            hipLaunchKernelGGL(k_wgrad_bf16, grid, block, 0, dev->stream, A, lda, B, ldb, dev->scratch, ldc, stride,
+                               M, N, K, chunk);
+        }
+        if (racer_launch_check("wgrad_bf16")) return 1;
+        int64_t total = (int64_t)M * N;
+        {
+            racer_timed timer(dev, "sum_chunks");
+            hipLaunchKernelGGL(k_sum_chunks_bf16, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, dev->stream,
+                               dev->scratch, stride, chunks, C, ldc, M, N);
+        }
+        return racer_launch_check("sum_chunks");
+    }
     const int chunk = 8192;
     int chunks = K <= 0 ? 1 : (K + chunk - 1) / chunk;
     int64_t stride = (int64_t)M * ldc;

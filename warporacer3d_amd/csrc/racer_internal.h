@@ -141,6 +141,9 @@ struct racer_trainer {
     racer_buf *obs = nullptr, *actions = nullptr, *logp = nullptr, *value = nullptr, *reward = nullptr,
               *done = nullptr, *advantage = nullptr, *x = nullptr, *d3 = nullptr, *d2 = nullptr, *d1 = nullptr,
               *grad = nullptr, *first = nullptr, *second = nullptr, *stats = nullptr;
+    // The rollout's observations regrouped by minibatch, [MB][IN][batch], built
+    // once per update so that no minibatch gathers strided columns.
+    racer_buf* xall = nullptr;
     racer_acts* acts = nullptr;
     double* dstats = nullptr;  // float64 accumulators: advantage sum and sum of squares, reward sum
 };

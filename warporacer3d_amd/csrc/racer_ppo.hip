@@ -109,6 +109,33 @@ __global__ void k_gather(const float* obs, int64_t n, int64_t IN, int64_t batch,
     for (int64_t r = 0; r < IN; ++r) x[r * batch + b] = src[r * n];
 }
 
+// The gather of every minibatch at once (racer/ppo.mojo:277-279 for all m):
+// xall[m][r][b] = obs[s / n][r][s % n] with s = b MB + m. With n % MB == 0,
+// sample s = t n + i has m = i % MB and b = t (n / MB) + i / MB, so a thread
+// reads MB consecutive cars of one obs row and writes one column of each
+// minibatch: reads and writes are both contiguous across the wavefront.
+template <int MB>
+__global__ void k_permute(const float* __restrict__ obs, int64_t n, int64_t IN, int64_t rows, int64_t batch,
+                          float* __restrict__ xall) {
+    const int64_t per_row = n / MB;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= rows * per_row) return;
+    const int64_t q = g % per_row, row = g / per_row;  // row = t IN + r
+    const int64_t r = row % IN, t = row / IN;
+    const float* src = obs + row * n + q * MB;
+    float v[MB];
+    if constexpr (MB >= 4) {
+#pragma unroll
+        for (int h = 0; h < MB / 4; ++h) *reinterpret_cast<float4*>(v + 4 * h) = reinterpret_cast<const float4*>(src)[h];
+    } else {
+#pragma unroll
+        for (int e = 0; e < MB; ++e) v[e] = src[e];
+    }
+    const int64_t b = t * per_row + q;
+#pragma unroll
+    for (int m = 0; m < MB; ++m) xall[((int64_t)m * IN + r) * batch + b] = v[m];
+}
+
 // minibatch.loss (racer/ppo.mojo:281-304): clipped surrogate and Huber value
 // derivatives, d log std per sample, and the per-sample KL estimate.
 __global__ void k_loss(int64_t batch, int64_t n, int minibatches, int m, const float* actions, const float* o,
@@ -520,6 +547,12 @@ extern "C" int racer_trainer_create(racer_device* dev, int n, const racer_config
         return 1;
     }
     RACER_TRY(hipMalloc(&tr->dstats, 8 * sizeof(double)));
+    const int MB = tr->minibatches;
+    if ((MB == 1 || MB == 2 || MB == 4 || MB == 8) && N % MB == 0 &&
+        racer_alloc(dev, (size_t)(R * tr->IN * N), 0.0f, &tr->xall)) {
+        racer_trainer_destroy(tr);
+        return 1;
+    }
     *out = tr;
     return 0;
 }
@@ -527,7 +560,7 @@ extern "C" int racer_trainer_create(racer_device* dev, int n, const racer_config
 extern "C" void racer_trainer_destroy(racer_trainer* tr) {
     if (!tr) return;
     racer_buf* bufs[] = {tr->obs, tr->actions, tr->logp, tr->value, tr->reward, tr->done, tr->advantage, tr->x,
-                         tr->d3, tr->d2, tr->d1, tr->grad, tr->first, tr->second, tr->stats};
+                         tr->d3, tr->d2, tr->d1, tr->grad, tr->first, tr->second, tr->stats, tr->xall};
     for (auto* b : bufs) racer_free(b);
     racer_acts_destroy(tr->acts);
     if (tr->dstats) hipFree(tr->dstats);
@@ -585,6 +618,8 @@ int racer_trainer_mean_reward(racer_trainer* tr, float* mean_reward) {
     return 0;
 }
 
+static int minibatch_core(racer_trainer* tr, racer_policy* p, int m, float adv_mean, float adv_scale, const float* x);
+
 extern "C" int racer_trainer_minibatch(racer_trainer* tr, racer_policy* p, int m, float adv_mean, float adv_scale) {
     racer_clear_error();
     if (check_compatible(tr, p)) return 1;
@@ -598,7 +633,41 @@ extern "C" int racer_trainer_minibatch(racer_trainer* tr, racer_policy* p, int m
                            tr->minibatches, m, tr->x->ptr);
     }
     if (racer_launch_check("gather")) return 1;
-    if (racer_policy_forward_raw(p, tr->x->ptr, B, tr->acts, (int)B)) return 1;
+    return minibatch_core(tr, p, m, adv_mean, adv_scale, tr->x->ptr);
+}
+
+// Every minibatch's inputs at once, into xall (when the trainer has it).
+static int permute_all(racer_trainer* tr) {
+    racer_device* dev = tr->dev;
+    int64_t rows = (int64_t)tr->rollout * tr->IN, per_row = tr->n / tr->minibatches;
+    unsigned blocks = (unsigned)((rows * per_row + 255) / 256);
+    racer_timed timer(dev, "permute");
+    switch (tr->minibatches) {
+        case 1:
+            hipLaunchKernelGGL(k_permute<1>, dim3(blocks), dim3(256), 0, dev->stream, tr->obs->ptr, (int64_t)tr->n,
+                               tr->IN, rows, tr->batch, tr->xall->ptr);
+            break;
+        case 2:
+            hipLaunchKernelGGL(k_permute<2>, dim3(blocks), dim3(256), 0, dev->stream, tr->obs->ptr, (int64_t)tr->n,
+                               tr->IN, rows, tr->batch, tr->xall->ptr);
+            break;
+        case 4:
+            hipLaunchKernelGGL(k_permute<4>, dim3(blocks), dim3(256), 0, dev->stream, tr->obs->ptr, (int64_t)tr->n,
+                               tr->IN, rows, tr->batch, tr->xall->ptr);
+            break;
+        default:
+            hipLaunchKernelGGL(k_permute<8>, dim3(blocks), dim3(256), 0, dev->stream, tr->obs->ptr, (int64_t)tr->n,
+                               tr->IN, rows, tr->batch, tr->xall->ptr);
+            break;
+    }
+    return racer_launch_check("permute");
+}
+
+static int minibatch_core(racer_trainer* tr, racer_policy* p, int m, float adv_mean, float adv_scale, const float* x) {
+    racer_device* dev = tr->dev;
+    int64_t B = tr->batch, n = tr->n;
+    unsigned blocks = (unsigned)((B + 255) / 256);
+    if (racer_policy_forward_raw(p, x, B, tr->acts, (int)B)) return 1;
     {
         racer_timed timer(dev, "loss");
         hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, dev->stream, B, n, tr->minibatches, m,
@@ -606,7 +675,7 @@ extern "C" int racer_trainer_minibatch(racer_trainer* tr, racer_policy* p, int m
                            p->theta->ptr + p->LOGSTD, adv_mean, adv_scale, tr->d3->ptr);
     }
     if (racer_launch_check("loss")) return 1;
-    if (backward_raw(p, tr->x->ptr, tr->acts, tr->d3->ptr, tr->d2->ptr, tr->d1->ptr, tr->grad->ptr, B)) return 1;
+    if (backward_raw(p, x, tr->acts, tr->d3->ptr, tr->d2->ptr, tr->d1->ptr, tr->grad->ptr, B)) return 1;
     // log-std gradient: row sums of d3[3..4] (racer/ppo.mojo:313)
     if (row_sums(dev, tr->d3->ptr + 3 * B, B, B, 2, 0, nullptr, tr->grad->ptr, p->LOGSTD)) return 1;
     // squared gradient norm (racer/ppo.mojo:314-315)
@@ -645,9 +714,16 @@ extern "C" int racer_trainer_update(racer_trainer* tr, racer_policy* p, float* k
     double var = mom[1] / (double)samples - mean * mean;
     float adv_mean = (float)mean;
     float adv_scale = (float)(1.0 / std::sqrt(std::max(var, 1e-8)));
+    if (tr->xall && permute_all(tr)) return 1;
     for (int epoch = 0; epoch < tr->epochs; ++epoch)
         for (int m = 0; m < tr->minibatches; ++m) {
-            if (racer_trainer_minibatch(tr, p, (m + epoch) % tr->minibatches, adv_mean, adv_scale)) return 1;
+            const int mm = (m + epoch) % tr->minibatches;
+            if (tr->xall) {
+                if (minibatch_core(tr, p, mm, adv_mean, adv_scale, tr->xall->ptr + (int64_t)mm * tr->IN * tr->batch))
+                    return 1;
+            } else if (racer_trainer_minibatch(tr, p, mm, adv_mean, adv_scale)) {
+                return 1;
+            }
             tr->step += 1;
             if (racer_trainer_adam(tr, p)) return 1;
         }
