@@ -211,17 +211,22 @@ int racer_profile_only(racer_device* dev, const char* name);
  * an instrumented variant of the lidar kernel when enabled; 0 otherwise). */
 int racer_sim_count_iterations(racer_sim* sim, int on);
 int64_t racer_sim_last_iterations(racer_sim* sim);
+/* Wavefront march-loop trips of the same counted launch (schedule 0): lane
+ * utilisation = iterations / (64 trips). */
+int64_t racer_sim_last_trips(racer_sim* sim);
 /* Lidar launch schedule: 0 (default) = one wavefront per car with beam refill;
  * 1 = one thread per (car, beam), beams fastest, the launch shape of the
  * reference's 2-D Device.run (racer/device.mojo:67-79), for timing the plain
  * baseline. Both give identical results. */
 int racer_sim_set_schedule(racer_sim* sim, int schedule);
+int racer_sim_schedule(const racer_sim* sim);
 /* Wavefronts that share one car's beams in the default schedule (1, 2 or 4;
  * results are identical for every choice). */
 int racer_sim_set_waves_per_car(racer_sim* sim, int waves);
 int racer_sim_waves_per_car(const racer_sim* sim);
 /* Autotuner: times the lidar with 1, 2 and 4 wavefronts per car on the current
- * state and keeps the fastest; writes the choice. Call after reset or a step. */
+ * state and keeps the fastest; writes the choice. Call after reset or a step,
+ * not while capturing. */
 int racer_sim_autotune(racer_sim* sim, int* chosen);
 
 #ifdef __cplusplus

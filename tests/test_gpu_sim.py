@@ -182,5 +182,8 @@ def test_autotune_keeps_results(dev):
     g.reset()
     o.reset()
     chosen = g.autotune()
-    assert chosen in (1, 2, 4) and g.waves_per_car == chosen
+    assert g.schedule == 0 and chosen in (1, 2, 4) and g.waves_per_car == chosen
     _assert_same(g, o, "after autotune")
+    g.step()
+    o.step()
+    _assert_same(g, o, "step after autotune")

@@ -143,6 +143,8 @@ def lib() -> C.CDLL:
         "racer_sim_waves_per_car": (i, [vp]),
         "racer_sim_autotune": (i, [vp, C.POINTER(C.c_int)]),
         "racer_profile_only": (i, [vp, C.c_char_p]),
+        "racer_sim_schedule": (i, [vp]),
+        "racer_sim_last_trips": (i64, [vp]),
         "racer_graph_runner": (i, [vp, vp, vp, vp, i, i, cp]),
         "racer_graph_rollout": (i, [vp, vp, vp, vp, cp]),
         "racer_graph_launch": (i, [vp, i64]),
@@ -347,8 +349,13 @@ class Sim:
         _check(lib().racer_sim_count_iterations(self.handle, int(on)))
 
     def set_schedule(self, schedule: int) -> None:
-        """0: wave per car with beam refill (default); 1: thread per (car, beam) as the reference launches it."""
+        """0: wavefront per car with beam refill (default); 1: thread per (car, beam) as the reference
+        launches it."""
         _check(lib().racer_sim_set_schedule(self.handle, int(schedule)))
+
+    @property
+    def schedule(self) -> int:
+        return int(lib().racer_sim_schedule(self.handle))
 
     def set_waves_per_car(self, waves: int) -> None:
         _check(lib().racer_sim_set_waves_per_car(self.handle, int(waves)))
@@ -366,6 +373,11 @@ class Sim:
     @property
     def last_iterations(self) -> int:
         return int(lib().racer_sim_last_iterations(self.handle))
+
+    @property
+    def last_trips(self) -> int:
+        """Wavefront loop trips of the last counted lidar launch (schedule 0)."""
+        return int(lib().racer_sim_last_trips(self.handle))
 
     def __del__(self):
         try:

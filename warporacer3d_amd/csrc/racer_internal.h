@@ -108,11 +108,12 @@ struct racer_sim {
     float* packed = nullptr;
     float2* dirs = nullptr;  // (cos, sin) of every beam's azimuth
     racer_dmap map{};
-    int schedule = 0;  // lidar launch: 0 wave per car with beam refill, 1 thread per (car, beam)
-    int waves_per_car = 2;  // wavefronts sharing one car's beams (1, 2 or 4); see racer_sim_autotune
+    int schedule = 0;       // lidar launch: 0 wavefront per car with beam refill, 1 thread per (car, beam)
+    int waves_per_car = 2;  // schedule 0: wavefronts sharing one car's beams (1, 2 or 4)
     bool count_iterations = false;
     unsigned long long* counter = nullptr;
     int64_t last_iterations = 0;
+    int64_t last_trips = 0;  // wavefront loop trips of the counted launch (schedule 0)
 };
 
 // ---------------------------------------------------------------- policy

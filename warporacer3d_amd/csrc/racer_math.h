@@ -206,7 +206,8 @@ RACER_FN float racer_nearest_index(float a, float b, float inv_b) {
     float s0 = q0 + 0.5f;
     float f0 = floorf(s0);
     float margin = (racer_abs(q0) + 1.0f) * 0x1p-20f;
-    if (s0 - f0 > margin && (f0 + 1.0f) - s0 > margin) return f0;
+    /* both distances exceed the margin (a NaN distance fails the test) */
+    if (racer_min(s0 - f0, (f0 + 1.0f) - s0) > margin) return f0;
     return floorf(a / b + 0.5f);
 }
 

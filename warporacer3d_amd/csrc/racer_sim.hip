@@ -346,6 +346,40 @@ __device__ __forceinline__ void probe_packed(const racer_dmap& m, __amdgpu_buffe
     }
 }
 
+// Map.probe's (clearance, height) from the packed records with the load
+// always issued: off the map the offset is out of range, the hardware returns
+// zeros, and clearance 0 is not drivable while height 0 - 1 is the grid's -1.
+template <int L>
+__device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_rsrc_t rec, float x, float y, float z,
+                                           float* clearance, float* height) {
+    float fx = racer_nearest_index(x - m.x0, m.cell, m.inv_cell);
+    float fy = racer_nearest_index(y - m.y0, m.cell, m.inv_cell);
+    bool on = fx >= 0.0f && fy >= 0.0f && fx < (float)m.nx && fy < (float)m.ny;
+    // cell = fy nx + fx, exact in float (both below 2^12, the sum below 2^24)
+    uint32_t off = on ? (uint32_t)fmaf(fy, (float)m.nx, fx) * (uint32_t)(8 * L) : RACER_OOB;
+    float off_map = on ? 0.0f : 1.0f;
+    if constexpr (L == 1) {
+        float2 r = racer_ld_f32x2(rec, off);
+        bool take = r.x > 0.0f && fabsf(r.y - z) < 1e30f;
+        *clearance = take ? r.x : -1.0f;
+        *height = r.y - off_map;
+    } else {
+        float4 r = __builtin_bit_cast(float4, racer_ld_b128(rec, off));
+        float cl = -1.0f, h = r.y, gap = 1e30f;
+        if (r.x > 0.0f && fabsf(r.y - z) < gap) {
+            cl = r.x;
+            h = r.y;
+            gap = fabsf(r.y - z);
+        }
+        if (r.z > 0.0f && fabsf(r.w - z) < gap) {
+            cl = r.z;
+            h = r.w;
+        }
+        *clearance = cl;
+        *height = h - off_map;
+    }
+}
+
 constexpr int LIDAR_RECORD = 5;               // dx, dy, dz, descent, 1/descent
 constexpr int LIDAR_RING = 2 * LIDAR_RECORD * 64;  // two chunks of 64 beam records
 
@@ -378,6 +412,23 @@ __global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __rest
     const int lo = (global_wave % waves_per_car) * per_wave;
     const int beams = min(total_beams, lo + per_wave) - lo;
     if (beams <= 0) return;
+    // Beams are marched starting nearest the middle of the fan (straight
+    // ahead, the longest ranges) and ending at its edges (the walls beside and
+    // behind, the shortest), so the lanes left idle at the end of the share
+    // wait on short beams. Virtual index v maps to beam lo + order(v).
+    const int centre = (total_beams - 1) / 2 - lo;  // relative position of the middle beam
+    const int mode = centre >= beams - 1 ? 1 : (centre <= 0 ? 0 : 2);  // 1 descending, 0 ascending, 2 outward
+    const int pairs = min(centre, beams - 1 - centre);
+    auto order = [&](int v) -> int {
+        if (mode == 0) return v;
+        if (mode == 1) return beams - 1 - v;
+        if (v <= 2 * pairs) {
+            const int k = (v + 1) >> 1;
+            return (v & 1) ? centre + k : centre - k;
+        }
+        const int extra = v - 2 * pairs;
+        return centre - pairs > 0 ? centre - pairs - extra : centre + pairs + extra;
+    };
     float* ring = lds + wave * (LIDAR_RING + per_wave);
     float* ranges = ring + LIDAR_RING;
     __amdgpu_buffer_rsrc_t rec = __builtin_amdgcn_make_buffer_rsrc(
@@ -393,11 +444,11 @@ __global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __rest
     const float slope15 = 1.5f * m.slope;
     const float half_cell = 0.5f * m.cell;
 
-    // Set-up of beams 64 c .. 64 c + 63, one per lane, into ring slot c % 2.
+    // Set-up of virtual beams 64 c .. 64 c + 63, one per lane, into ring slot c % 2.
     auto setup_chunk = [&](int c) {
         int b = c * 64 + lane;
         if (b < beams) {
-            float2 cs = dirs[lo + b];
+            float2 cs = dirs[lo + order(b)];
             float d0 = cs.x * ch + cs.y * (-sh);
             float d1 = cs.x * sh + cs.y * ch;
             float d2 = cs.x * along + cs.y * across;
@@ -415,9 +466,11 @@ __global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __rest
     };
     float dx = 0.0f, dy = 0.0f, dz = 0.0f, descent = 0.0f, inv_descent = 0.0f;
     float t = 0.0f, last = 0.0f, last_floor = 0.0f;
+    int rel = 0;  // the lane's beam, relative to lo
     auto take = [&](int b) {
         const float* slot = ring + ((b >> 6) & 1) * (LIDAR_RECORD * 64);
         int j = b & 63;
+        rel = order(b);
         dx = slot[0 * 64 + j];
         dy = slot[1 * 64 + j];
         dz = slot[2 * 64 + j];
@@ -436,14 +489,17 @@ __global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __rest
     int b = lane;
     if (b < beams) take(b);
     int next = 64;
-    uint64_t iterations = 0;
+    uint64_t iterations = 0, trips = 0;
     while (__ballot(b < beams) != 0ull) {
         bool finished = false;
         if (b < beams) {
             if (COUNT) ++iterations;
             float px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
             float clearance, height;
-            probe_packed<L>(m, rec, px, py, pz, &clearance, &height);
+            if constexpr (L == 1 || L == 2)
+                probe_flat<L>(m, rec, px, py, pz, &clearance, &height);
+            else
+                probe_packed<L>(m, rec, px, py, pz, &clearance, &height);
             float fl = (pz - height) - 0.03f;
             float range = 0.0f;
             if (fl < 0.0f) {
@@ -464,8 +520,9 @@ __global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __rest
                     }
                 }
             }
-            if (finished) ranges[b] = range;
+            if (finished) ranges[rel] = range;
         }
+        if (COUNT && lane == 0) ++trips;
         const uint64_t done_mask = __ballot(finished);
         if (done_mask != 0ull) {
             const int count = (int)__popcll(done_mask);
@@ -491,7 +548,10 @@ __global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __rest
     for (int j = lane; j < beams; j += 64) obs[(int64_t)(4 + lo + j) * obs_cols + car] = ranges[j] / RACER_RANGE;
     if (COUNT) {
         for (int off = 32; off > 0; off >>= 1) iterations += __shfl_xor(iterations, off);
-        if (lane == 0) atomicAdd(counter, (unsigned long long)iterations);
+        if (lane == 0) {
+            atomicAdd(counter, (unsigned long long)iterations);
+            atomicAdd(counter + 1, (unsigned long long)trips);
+        }
     }
 }
 
@@ -614,7 +674,8 @@ int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done
                            sim->heading_noise);
     }
     if (racer_launch_check("physics")) return 1;
-    if (sim->count_iterations) RACER_TRY(hipMemsetAsync(sim->counter, 0, sizeof(unsigned long long), sim->dev->stream));
+    if (sim->count_iterations)
+        RACER_TRY(hipMemsetAsync(sim->counter, 0, 2 * sizeof(unsigned long long), sim->dev->stream));
     if (sim->schedule == 1) {
         int64_t threads = (int64_t)n * sim->beams;
         racer_timed timer(sim->dev, "lidar_flat");
@@ -626,10 +687,11 @@ int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done
     }
     if (racer_launch_check("lidar")) return 1;
     if (sim->count_iterations) {
-        unsigned long long count = 0;
-        RACER_TRY(hipMemcpyAsync(&count, sim->counter, sizeof(count), hipMemcpyDeviceToHost, sim->dev->stream));
+        unsigned long long count[2] = {0, 0};
+        RACER_TRY(hipMemcpyAsync(count, sim->counter, sizeof(count), hipMemcpyDeviceToHost, sim->dev->stream));
         RACER_TRY(hipStreamSynchronize(sim->dev->stream));
-        sim->last_iterations = (int64_t)count;
+        sim->last_iterations = (int64_t)count[0];
+        sim->last_trips = (int64_t)count[1];
     }
     return 0;
 }
@@ -667,6 +729,7 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
     int nx = (int)track[0], ny = (int)track[1], layers = (int)track[7], wps = (int)track[5];
     RACER_REQUIRE(nx >= 2 && ny >= 2 && layers >= 1 && wps >= 1, "not a track array");
     int64_t plane = (int64_t)nx * ny;
+    RACER_REQUIRE(plane <= (1 << 24), "track grids of more than 2^24 cells are not supported");
     int64_t route = RACER_HEADER + 3 * (int64_t)layers * plane;
     RACER_REQUIRE((size_t)(route + 4 * (int64_t)wps) <= len, "not a track array");
     RACER_REQUIRE(track[4] > 0.0f, "not a track array: the cell size must be positive");
@@ -721,7 +784,7 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
     RACER_TRY(hipMalloc(&sim->dirs, dirs.size() * sizeof(float2)));
     RACER_TRY(hipMemcpyAsync(sim->dirs, dirs.data(), dirs.size() * sizeof(float2), hipMemcpyHostToDevice,
                              dev->stream));
-    RACER_TRY(hipMalloc(&sim->counter, sizeof(unsigned long long)));
+    RACER_TRY(hipMalloc(&sim->counter, 2 * sizeof(unsigned long long)));
     RACER_TRY(hipStreamSynchronize(dev->stream));
     *out = sim;
     return 0;
@@ -789,6 +852,7 @@ extern "C" int racer_sim_count_iterations(racer_sim* sim, int on) {
     return 0;
 }
 extern "C" int64_t racer_sim_last_iterations(racer_sim* sim) { return sim->last_iterations; }
+extern "C" int64_t racer_sim_last_trips(racer_sim* sim) { return sim->last_trips; }
 
 extern "C" int racer_sim_set_waves_per_car(racer_sim* sim, int waves) {
     racer_clear_error();
@@ -799,18 +863,20 @@ extern "C" int racer_sim_set_waves_per_car(racer_sim* sim, int waves) {
 
 extern "C" int racer_sim_waves_per_car(const racer_sim* sim) { return sim->waves_per_car; }
 
-// Times the lidar launch with 1, 2 and 4 wavefronts per car on the current
-// state (the lidar only rewrites the range rows of sim->obs, so repeating it
-// is harmless) and keeps the fastest.
+// Times the lidar on the current state (it only rewrites the range rows of
+// sim->obs, so repeating it is harmless) with 1, 2 and 4 wavefronts per car
+// and keeps the fastest; writes the chosen count.
 extern "C" int racer_sim_autotune(racer_sim* sim, int* chosen) {
     racer_clear_error();
+    RACER_REQUIRE(!sim->dev->capturing, "cannot autotune while capturing a graph");
     hipEvent_t a, b;
     RACER_TRY(hipEventCreate(&a));
     RACER_TRY(hipEventCreate(&b));
-    const int options[3] = {1, 2, 4};
+    const bool counting = sim->count_iterations;
+    sim->count_iterations = false;
     float best = 1e30f;
     int pick = sim->waves_per_car;
-    for (int w : options) {
+    for (int w : {1, 2, 4}) {
         sim->waves_per_car = w;
         launch_lidar_any(sim, sim->obs->ptr, sim->n);  // warm
         RACER_TRY(hipEventRecord(a, sim->dev->stream));
@@ -827,6 +893,7 @@ extern "C" int racer_sim_autotune(racer_sim* sim, int* chosen) {
     }
     hipEventDestroy(a);
     hipEventDestroy(b);
+    sim->count_iterations = counting;
     sim->waves_per_car = pick;
     if (chosen) *chosen = pick;
     return 0;
@@ -834,7 +901,12 @@ extern "C" int racer_sim_autotune(racer_sim* sim, int* chosen) {
 
 extern "C" int racer_sim_set_schedule(racer_sim* sim, int schedule) {
     racer_clear_error();
-    RACER_REQUIRE(sim && (schedule == 0 || schedule == 1), "schedule must be 0 (wave per car) or 1 (thread per beam)");
+    RACER_REQUIRE(sim && (schedule == 0 || schedule == 1),
+                  "schedule must be 0 (wavefront per car) or 1 (thread per beam)");
     sim->schedule = schedule;
     return 0;
 }
+
+extern "C" int racer_sim_schedule(const racer_sim* sim) { return sim->schedule; }
+
+
