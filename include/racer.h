@@ -173,6 +173,34 @@ int racer_runner_step(racer_sim* sim, racer_policy* pol, racer_acts* acts, racer
 int racer_benchmark(racer_sim* sim, racer_policy* pol, racer_acts* acts, racer_buf* scratch, int steps,
                     double* ms_per_step);
 
+/* ---------------- Batches (no reference counterpart) ---------------- */
+/* The reference steps one Sim per launch (racer/sim.mojo:314-321). A batch
+ * groups 1 .. 16 existing sims (any tracks and car counts, one beam count) and
+ * steps all of them with one launch per stage; every sim keeps its own
+ * buffers and seed, so its results are bitwise those of stepping it alone.
+ * The sims must outlive the batch. `cfg` gives the hidden width of the
+ * activations the batch keeps for the shared policy. */
+typedef struct racer_batch racer_batch;
+int racer_batch_create(racer_device* dev, racer_sim** sims, int count, const racer_config* cfg, racer_batch** out);
+void racer_batch_destroy(racer_batch* b);
+int racer_batch_count(const racer_batch* b);
+int64_t racer_batch_cars(const racer_batch* b);
+/* Sim.step (racer/sim.mojo:314-321) of every sim, from its own actions buffer. */
+int racer_batch_step(racer_batch* b);
+/* Runner.step (main.mojo:183-191) of every sim with one shared policy: forward,
+ * sample (each sim's seed, Philox offset t), physics, lidar. Log probabilities
+ * and values land in the batch scratch: sim k's 2 n_k floats (log
+ * probabilities, then values) start at 2 (n_0 + .. + n_(k-1)). */
+int racer_batch_runner_step(racer_batch* b, racer_policy* pol, int deterministic, int64_t t);
+racer_buf* racer_batch_scratch(racer_batch* b);
+/* The batched lidar's wavefronts per car (1, 2, 4), its autotuner, and its
+ * march-iteration counter (as the Sim ones). */
+int racer_batch_set_waves_per_car(racer_batch* b, int waves);
+int racer_batch_waves_per_car(const racer_batch* b);
+int racer_batch_autotune(racer_batch* b, int* chosen);
+int racer_batch_count_iterations(racer_batch* b, int on);
+int64_t racer_batch_last_iterations(const racer_batch* b);
+
 /* ---------------- Graphs (no reference counterpart) ---------------- */
 /* The reference enqueues every launch of a step from the host
  * (main.mojo:183-191, racer/ppo.mojo:190-210). These capture the same launch
@@ -188,6 +216,8 @@ typedef struct racer_graph racer_graph;
 int racer_graph_runner(racer_sim* sim, racer_policy* pol, racer_acts* acts, racer_buf* scratch, int steps,
                        int deterministic, racer_graph** out);
 int racer_graph_launch(racer_graph* g, int64_t t0);
+/* `steps` batched Runner steps (racer_batch_runner_step) as one graph. */
+int racer_graph_batch(racer_batch* b, racer_policy* pol, int steps, int deterministic, racer_graph** out);
 /* Trainer.rollout (racer/ppo.mojo:187-232) as one graph: the copy in, ROLLOUT
  * steps, the copy out, the bootstrap forward, GAE and the reward sum. */
 int racer_graph_rollout(racer_trainer* tr, racer_sim* sim, racer_policy* pol, racer_acts* acts, racer_graph** out);

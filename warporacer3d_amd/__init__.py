@@ -150,6 +150,19 @@ def lib() -> C.CDLL:
         "racer_graph_launch": (i, [vp, i64]),
         "racer_trainer_rollout_graph": (i, [vp, vp, i, C.POINTER(C.c_float)]),
         "racer_graph_destroy": (None, [vp]),
+        "racer_graph_batch": (i, [vp, vp, i, i, cp]),
+        "racer_batch_create": (i, [vp, C.POINTER(C.c_void_p), i, C.POINTER(Config), cp]),
+        "racer_batch_destroy": (None, [vp]),
+        "racer_batch_count": (i, [vp]),
+        "racer_batch_cars": (i64, [vp]),
+        "racer_batch_step": (i, [vp]),
+        "racer_batch_runner_step": (i, [vp, vp, i, i64]),
+        "racer_batch_scratch": (vp, [vp]),
+        "racer_batch_set_waves_per_car": (i, [vp, i]),
+        "racer_batch_waves_per_car": (i, [vp]),
+        "racer_batch_autotune": (i, [vp, C.POINTER(C.c_int)]),
+        "racer_batch_count_iterations": (i, [vp, i]),
+        "racer_batch_last_iterations": (i64, [vp]),
     }
     for name in ("obs", "actions", "logp", "value", "reward", "done", "advantage", "grad", "d3", "stats", "acts"):
         sigs[f"racer_trainer_{name}"] = (vp, [vp])
@@ -507,6 +520,76 @@ class Trainer:
         try:
             if self.handle:
                 lib().racer_trainer_destroy(self.handle)
+        except Exception:
+            pass
+        self.handle = None
+
+
+def runner_step(sim: "Sim", policy: "Policy", acts: "Activations", scratch: Buffer, deterministic: bool,
+                t: int) -> None:
+    """Runner.step (main.mojo:183-191) on explicit objects: forward, sample into
+    sim.actions and scratch (log probabilities, then values), physics, lidar."""
+    _check(lib().racer_runner_step(sim.handle, policy.handle, acts.handle, scratch.handle, int(deterministic),
+                                   int(t)))
+
+
+class Batch:
+    """Sims stepped together, one launch per stage (bitwise as stepping each alone)."""
+
+    def __init__(self, device: "Device", sims: list, config: Config | None = None):
+        self.device = device
+        self.config = config or Config()
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.handle for s in self.sims])
+        h = C.c_void_p()
+        _check(lib().racer_batch_create(device.handle, arr, len(self.sims), C.byref(self.config), C.byref(h)))
+        self.handle = h.value
+        self.cars = int(lib().racer_batch_cars(self.handle))
+        self.scratch = _wrap(device, lib().racer_batch_scratch(self.handle))
+        self.starts = [0]
+        for s in self.sims:
+            self.starts.append(self.starts[-1] + s.n)
+
+    def step(self) -> None:
+        _check(lib().racer_batch_step(self.handle))
+
+    def runner_step(self, policy: "Policy", deterministic: bool, t: int) -> None:
+        _check(lib().racer_batch_runner_step(self.handle, policy.handle, int(deterministic), int(t)))
+
+    def logp_value(self, k: int):
+        """Log probabilities and values of sim k from the last runner step."""
+        a = self.scratch.read()
+        n, o = self.sims[k].n, 2 * self.starts[k]
+        return a[o: o + n], a[o + n: o + 2 * n]
+
+    def set_waves_per_car(self, waves: int) -> None:
+        _check(lib().racer_batch_set_waves_per_car(self.handle, int(waves)))
+
+    @property
+    def waves_per_car(self) -> int:
+        return int(lib().racer_batch_waves_per_car(self.handle))
+
+    def autotune(self) -> int:
+        w = C.c_int()
+        _check(lib().racer_batch_autotune(self.handle, C.byref(w)))
+        return int(w.value)
+
+    def count_iterations(self, on: bool = True) -> None:
+        _check(lib().racer_batch_count_iterations(self.handle, int(on)))
+
+    @property
+    def last_iterations(self) -> int:
+        return int(lib().racer_batch_last_iterations(self.handle))
+
+    def capture(self, policy: "Policy", steps: int, deterministic: bool = False) -> "Graph":
+        h = C.c_void_p()
+        _check(lib().racer_graph_batch(self.handle, policy.handle, int(steps), int(deterministic), C.byref(h)))
+        return Graph(h.value, (self, policy))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().racer_batch_destroy(self.handle)
         except Exception:
             pass
         self.handle = None

@@ -87,6 +87,29 @@ extern "C" int racer_graph_runner(racer_sim* sim, racer_policy* pol, racer_acts*
     return 0;
 }
 
+extern "C" int racer_graph_batch(racer_batch* b, racer_policy* pol, int steps, int deterministic,
+                                 racer_graph** out) {
+    racer_clear_error();
+    *out = nullptr;
+    RACER_REQUIRE(b && pol, "null argument");
+    RACER_REQUIRE(steps >= 1, "a graph needs at least one step");
+    RACER_REQUIRE(!b->count_iterations, "turn the lidar iteration counter off before capturing");
+    auto* g = new racer_graph();
+    g->dev = b->dev;
+    g->policy = pol;
+    int rc = capture(b->dev, pol, g, [&]() -> int {
+        for (int t = 0; t < steps; ++t)
+            if (racer_batch_runner_enqueue(b, pol, deterministic, t)) return 1;
+        return 0;
+    });
+    if (rc) {
+        racer_graph_destroy(g);
+        return 1;
+    }
+    *out = g;
+    return 0;
+}
+
 extern "C" int racer_graph_rollout(racer_trainer* tr, racer_sim* sim, racer_policy* pol, racer_acts* acts,
                                    racer_graph** out) {
     racer_clear_error();

@@ -149,6 +149,60 @@ struct racer_trainer {
     double* dstats = nullptr;  // float64 accumulators: advantage sum and sum of squares, reward sum
 };
 
+// ---------------------------------------------------------------- batched launches
+// Up to RACER_BATCH_MAX sims stepped by one launch per stage: each kernel
+// finds its instance from the prefix sums `start` of its work items.
+constexpr int RACER_BATCH_MAX = 16;
+
+struct racer_sim_slot {
+    racer_dmap map;
+    float *state, *obs, *actions, *reward, *done;
+    const float2* dirs;
+    int n, seed, beams;
+    float heading_noise;
+};
+
+struct racer_sim_table {
+    int count;
+    int64_t start[RACER_BATCH_MAX + 1];
+    racer_sim_slot slot[RACER_BATCH_MAX];
+};
+
+struct racer_policy_slot {
+    const float* x;
+    int64_t ldx, act_cols;
+    int64_t cols;
+    float *h1, *h2, *o, *act, *logp, *value;
+    int seed;
+};
+
+struct racer_policy_table {
+    int count;
+    int64_t start[RACER_BATCH_MAX + 1];  // in 16-column tiles
+    racer_policy_slot slot[RACER_BATCH_MAX];
+};
+
+struct racer_batch {
+    racer_device* dev = nullptr;
+    std::vector<racer_sim*> sims;
+    int beams = 0, hidden = 0;
+    int64_t cars = 0;
+    int waves_per_car = 2;
+    std::vector<racer_acts*> acts;  // per instance, written by the batched forward
+    racer_buf* scratch = nullptr;   // log probabilities then values: instance k's 2 n_k floats at 2 start_k
+    std::vector<int64_t> car_start;
+    bool count_iterations = false;
+    unsigned long long* counter = nullptr;
+    int64_t last_iterations = 0;
+};
+
+// Physics, then the lidar, of every instance of the batch (each reads its own
+// sim's actions and writes its own sim's reward, done, state and obs).
+int racer_launch_batch_step(racer_batch* b);
+int racer_launch_batch_lidar(racer_batch* b);
+int racer_policy_fused_batch(racer_policy* p, const racer_policy_table& tab, int deterministic, int64_t step);
+int racer_batch_runner_enqueue(racer_batch* b, racer_policy* p, int deterministic, int64_t t);
+
 // ---------------------------------------------------------------- kernels (launchers)
 int racer_launch_reset(racer_sim* sim);
 int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done, float* obs, int64_t obs_cols);

@@ -62,13 +62,14 @@ __host__ __device__ inline FusedLayout fused_layout(int H, int IN) {
     return L;
 }
 
+// One workgroup's 16 columns c0 .. c0 + 15 of the forward (and sample).
 template <bool SAMPLE>
-__global__ void __launch_bounds__(64 * WAVES) k_policy_fused(
+__device__ __forceinline__ void policy_tile(
     const uint16_t* __restrict__ F0, const uint16_t* __restrict__ F1,
     const float* __restrict__ W2, const float* __restrict__ x, int64_t ldx, int H, int IN, int64_t cols,
     float* __restrict__ h1out, float* __restrict__ h2out, float* __restrict__ o, const float* __restrict__ log_std,
     float* act, int64_t act_cols, float* logp, float* value, int deterministic, int64_t step,
-    const int64_t* step_base, int seed) {
+    const int64_t* step_base, int seed, const int64_t c0) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const FusedLayout Ly = fused_layout(H, IN);
     const int HA = Ly.HA;
@@ -80,7 +81,6 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_fused(
     float* outs = reinterpret_cast<float*>(smem + Ly.outs);       // [3][FT]
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t c0 = (int64_t)blockIdx.x * FT;
     const int r = lane & 15, kg = (lane >> 4) * 8;  // MFMA operand lane layout
     // buffer descriptors: out-of-range offsets (rows >= H, k >= IN, columns >= cols) read zeros
     const auto xres = racer_rsrc(x, ((uint64_t)(IN - 1) * ldx + cols) * 4);
@@ -248,6 +248,32 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_fused(
     }
 }
 
+template <bool SAMPLE>
+__global__ void __launch_bounds__(64 * WAVES) k_policy_fused(
+    const uint16_t* __restrict__ F0, const uint16_t* __restrict__ F1,
+    const float* __restrict__ W2, const float* __restrict__ x, int64_t ldx, int H, int IN, int64_t cols,
+    float* __restrict__ h1out, float* __restrict__ h2out, float* __restrict__ o, const float* __restrict__ log_std,
+    float* act, int64_t act_cols, float* logp, float* value, int deterministic, int64_t step,
+    const int64_t* step_base, int seed) {
+    policy_tile<SAMPLE>(F0, F1, W2, x, ldx, H, IN, cols, h1out, h2out, o, log_std, act, act_cols, logp, value,
+                        deterministic, step, step_base, seed, (int64_t)blockIdx.x * FT);
+}
+
+// The forward and sample of several sims' observations in one launch:
+// workgroup b serves the instance whose tile range [start[k], start[k + 1])
+// holds b, with that instance's buffers and Philox seed.
+__global__ void __launch_bounds__(64 * WAVES) k_policy_fused_batch(
+    const uint16_t* __restrict__ F0, const uint16_t* __restrict__ F1, const float* __restrict__ W2, int H, int IN,
+    const float* __restrict__ log_std, racer_policy_table tab, int deterministic, int64_t step,
+    const int64_t* step_base) {
+    const int64_t b = blockIdx.x;
+    int k = 0;
+    while (k + 1 < tab.count && b >= tab.start[k + 1]) ++k;
+    const racer_policy_slot& q = tab.slot[k];
+    policy_tile<true>(F0, F1, W2, q.x, q.ldx, H, IN, q.cols, q.h1, q.h2, q.o, log_std, q.act, q.act_cols, q.logp,
+                      q.value, deterministic, step, step_base, q.seed, (b - tab.start[k]) * FT);
+}
+
 // Weights in MFMA fragment order: block (tile, step) holds the 64 lanes'
 // operands of rows 16 tile .. + 15 and k 32 step .. + 31, lane l taking row
 // 16 tile + l % 16 and k 32 step + 8 (l / 16) .. + 7, as 1 KB of bf16. A
@@ -288,7 +314,23 @@ int racer_policy_fused_setup() {
                                   160 * 1024));
     RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024));
+    RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024));
     return 0;
+}
+
+int racer_policy_fused_batch(racer_policy* p, const racer_policy_table& tab, int deterministic, int64_t step) {
+    racer_device* dev = p->dev;
+    if (tab.count <= 0 || tab.start[tab.count] <= 0) return 0;
+    dim3 grid((unsigned)tab.start[tab.count]), block(64 * WAVES);
+    size_t shmem = racer_policy_fused_smem(p->hidden, (int)p->IN);
+    {
+        racer_timed timer(dev, "policy_fused_sample");
+        hipLaunchKernelGGL(k_policy_fused_batch, grid, block, shmem, dev->stream, p->frag0, p->frag1,
+                           p->theta->ptr + p->W2, p->hidden, (int)p->IN, p->theta->ptr + p->LOGSTD, tab, deterministic,
+                           step, dev->step_base);
+    }
+    return racer_launch_check("policy_fused_batch");
 }
 
 int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,

@@ -392,20 +392,17 @@ constexpr int LIDAR_RING = 2 * LIDAR_RECORD * 64;  // two chunks of 64 beam reco
 // wavefront into a two-chunk LDS ring, so a refill is five LDS reads rather
 // than a divergent pass of divisions and square roots. Ranges collect in LDS
 // and are scaled and stored in one full-lane pass at the end.
+//
+// lidar_wave marches wavefront `global_wave`'s share (car global_wave /
+// waves_per_car); the kernels below map workgroups to wavefronts.
 template <int L, bool COUNT>
-__global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __restrict__ s, int n,
-                                               float* __restrict__ obs, int64_t obs_cols, int total_beams,
-                                               int per_wave, int waves_per_car, const float2* __restrict__ dirs,
-                                               unsigned long long* counter) {
+__device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __restrict__ s, int n,
+                                           float* __restrict__ obs, int64_t obs_cols, int total_beams, int per_wave,
+                                           int waves_per_car, const float2* __restrict__ dirs,
+                                           unsigned long long* counter, int global_wave) {
     extern __shared__ float lds[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    // Workgroups go round-robin over the 8 XCDs; give each XCD a contiguous run
-    // of cars so the range stores of neighbouring cars (adjacent floats of one
-    // obs row) meet in one L2 and leave it as whole lines.
-    int block = blockIdx.x;
-    if ((gridDim.x & 7) == 0) block = (block & 7) * (gridDim.x >> 3) + (block >> 3);
-    const int global_wave = (int)(((int64_t)block * blockDim.x + threadIdx.x) >> 6);
     const int car = global_wave / waves_per_car;
     if (car >= n) return;
     // This wavefront's share of the car's beams: [lo, lo + beams).
@@ -555,6 +552,62 @@ __global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __rest
     }
 }
 
+// Workgroups go round-robin over the 8 XCDs; remapping the workgroup index
+// gives each XCD a contiguous run of cars, so the range stores of
+// neighbouring cars (adjacent floats of one obs row) meet in one L2.
+__device__ __forceinline__ int xcd_block() {
+    int block = blockIdx.x;
+    if ((gridDim.x & 7) == 0) block = (block & 7) * (gridDim.x >> 3) + (block >> 3);
+    return block;
+}
+
+template <int L, bool COUNT>
+__global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __restrict__ s, int n,
+                                               float* __restrict__ obs, int64_t obs_cols, int total_beams,
+                                               int per_wave, int waves_per_car, const float2* __restrict__ dirs,
+                                               unsigned long long* counter) {
+    const int global_wave = (int)(((int64_t)xcd_block() * blockDim.x + threadIdx.x) >> 6);
+    lidar_wave<L, COUNT>(m, s, n, obs, obs_cols, total_beams, per_wave, waves_per_car, dirs, counter, global_wave);
+}
+
+// The lidar of several sims in one launch: wavefront g belongs to the
+// instance whose range [start[k], start[k + 1]) of wavefronts holds g.
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_lidar_batch(racer_sim_table tab, int per_wave, int waves_per_car,
+                                                     unsigned long long* counter) {
+    const int64_t g = ((int64_t)xcd_block() * blockDim.x + threadIdx.x) >> 6;
+    int k = 0;
+    while (k + 1 < tab.count && g >= tab.start[k + 1]) ++k;
+    if (g >= tab.start[tab.count]) return;
+    const racer_sim_slot& q = tab.slot[k];
+    const int w = (int)(g - tab.start[k]);
+    if (q.map.layers == 1)
+        lidar_wave<1, COUNT>(q.map, q.state, q.n, q.obs, q.n, q.beams, per_wave, waves_per_car, q.dirs, counter, w);
+    else if (q.map.layers == 2)
+        lidar_wave<2, COUNT>(q.map, q.state, q.n, q.obs, q.n, q.beams, per_wave, waves_per_car, q.dirs, counter, w);
+    else
+        lidar_wave<0, COUNT>(q.map, q.state, q.n, q.obs, q.n, q.beams, per_wave, waves_per_car, q.dirs, counter, w);
+}
+
+// Physics of several sims in one launch (one thread per car of any instance).
+__global__ void __launch_bounds__(64) k_physics_batch(racer_sim_table tab) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int k = 0;
+    while (k + 1 < tab.count && g >= tab.start[k + 1]) ++k;
+    if (g >= tab.start[tab.count]) return;
+    const racer_sim_slot& q = tab.slot[k];
+    const int i = (int)(g - tab.start[k]);
+    if (q.map.layers == 1)
+        advance<1>(q.map, q.state, q.n, q.actions, q.n, q.reward, q.done, q.obs, q.n, q.beams, i, q.seed,
+                   q.heading_noise);
+    else if (q.map.layers == 2)
+        advance<2>(q.map, q.state, q.n, q.actions, q.n, q.reward, q.done, q.obs, q.n, q.beams, i, q.seed,
+                   q.heading_noise);
+    else
+        advance<0>(q.map, q.state, q.n, q.actions, q.n, q.reward, q.done, q.obs, q.n, q.beams, i, q.seed,
+                   q.heading_noise);
+}
+
 // The reference's own lidar launch (racer/sim.mojo:317-321 through the 2-D
 // Device.run of racer/device.mojo:67-79): one thread per (car, beam), beams
 // fastest, every thread recomputing its car's frame and its beam's azimuth.
@@ -692,6 +745,71 @@ int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done
         RACER_TRY(hipStreamSynchronize(sim->dev->stream));
         sim->last_iterations = (int64_t)count[0];
         sim->last_trips = (int64_t)count[1];
+    }
+    return 0;
+}
+
+// The instance table of a batch, with work-item prefix sums of `per_car` items per car.
+static racer_sim_table sim_table(const racer_batch* b, int per_car) {
+    racer_sim_table tab{};
+    tab.count = (int)b->sims.size();
+    tab.start[0] = 0;
+    for (int k = 0; k < tab.count; ++k) {
+        const racer_sim* sim = b->sims[k];
+        racer_sim_slot& q = tab.slot[k];
+        q.map = sim->map;
+        q.state = sim->state->ptr;
+        q.obs = sim->obs->ptr;
+        q.actions = sim->actions->ptr;
+        q.reward = sim->reward->ptr;
+        q.done = sim->done->ptr;
+        q.dirs = sim->dirs;
+        q.n = sim->n;
+        q.seed = sim->seed;
+        q.beams = sim->beams;
+        q.heading_noise = sim->heading_noise;
+        tab.start[k + 1] = tab.start[k] + (int64_t)sim->n * per_car;
+    }
+    return tab;
+}
+
+int racer_launch_batch_step(racer_batch* b) {
+    racer_device* dev = b->dev;
+    {
+        const racer_sim_table tab = sim_table(b, 1);
+        racer_timed timer(dev, "physics");
+        hipLaunchKernelGGL(k_physics_batch, dim3((unsigned)((tab.start[tab.count] + 63) / 64)), dim3(64), 0,
+                           dev->stream, tab);
+    }
+    if (racer_launch_check("physics batch")) return 1;
+    return racer_launch_batch_lidar(b);
+}
+
+int racer_launch_batch_lidar(racer_batch* b) {
+    racer_device* dev = b->dev;
+    if (b->count_iterations) RACER_TRY(hipMemsetAsync(b->counter, 0, 2 * sizeof(unsigned long long), dev->stream));
+    {
+        const int wpc = b->waves_per_car;
+        const int per = (b->beams + wpc - 1) / wpc;
+        const racer_sim_table tab = sim_table(b, wpc);
+        const size_t bytes_per_wave = (size_t)(LIDAR_RING + per) * sizeof(float);
+        const int waves = 4 * bytes_per_wave <= 65536 ? 4 : 1;
+        const int64_t total = tab.start[tab.count];
+        dim3 grid((unsigned)((total + waves - 1) / waves)), block(64 * waves);
+        racer_timed timer(dev, "lidar");
+        if (b->count_iterations)
+            hipLaunchKernelGGL(k_lidar_batch<true>, grid, block, waves * bytes_per_wave, dev->stream, tab, per, wpc,
+                               b->counter);
+        else
+            hipLaunchKernelGGL(k_lidar_batch<false>, grid, block, waves * bytes_per_wave, dev->stream, tab, per, wpc,
+                               b->counter);
+    }
+    if (racer_launch_check("lidar batch")) return 1;
+    if (b->count_iterations) {
+        unsigned long long count[2] = {0, 0};
+        RACER_TRY(hipMemcpyAsync(count, b->counter, sizeof(count), hipMemcpyDeviceToHost, dev->stream));
+        RACER_TRY(hipStreamSynchronize(dev->stream));
+        b->last_iterations = (int64_t)count[0];
     }
     return 0;
 }
