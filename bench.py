@@ -135,6 +135,70 @@ def ppo_leg(R, T, dev, beams: int, iterations: int, cars: int = 16384, precision
             "kernels_ms_per_iteration": {k: tot for k, (n, tot) in top}}
 
 
+BATCH_TRACKS = ("ring10", "ramp10", "bank10", "bridge8", "random0", "random1", "random2", "random3")
+BATCH_CARS = (256, 512, 1024, 2048, 4096, 4096, 8192, 8192)
+
+
+def batch_leg(R, T, dev, steps: int) -> dict:
+    """Eight uneven instances (flat, ramp, bank, bridge and four random routes;
+    256 .. 8192 cars, 1081 beams, one 2 x 256 bf16 policy): Runner steps of all
+    of them from one captured graph of batched launches, against the same
+    instances replayed from one graph each, one after another."""
+    cfg = R.Config(beams=1081, hidden=256, precision=1, heading_noise=0.1)
+    pol = R.Policy(dev, 42, cfg)
+
+    def instances():
+        sims = []
+        for k, (name, n) in enumerate(zip(BATCH_TRACKS, BATCH_CARS)):
+            s = R.Sim(dev, T.cached(name), n, 100 + k, cfg)
+            s.reset()
+            sims.append(s)
+        return sims
+
+    total = sum(BATCH_CARS)
+    chunk = math.gcd(steps, 50)
+    # batched: one launch per stage for all eight instances
+    sims = instances()
+    batch = R.Batch(dev, sims, cfg)
+    for t in range(5):
+        batch.runner_step(pol, False, t)
+    wpc = batch.autotune()
+    graph = batch.capture(pol, chunk)
+    graph.launch(5)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps // chunk):
+        graph.launch(5 + chunk * (k + 1))
+    dev.synchronize()
+    batched_ms = (time.perf_counter() - t0) * 1000.0 / steps
+    del graph, batch, sims
+    # separate: each instance's own graph of Runner steps, replayed in turn
+    sims = instances()
+    own = []
+    for s in sims:
+        acts = R.Activations(dev, s.n, cfg)
+        scratch = dev.alloc(2 * s.n)
+        for t in range(5):
+            R.runner_step(s, pol, acts, scratch, False, t)
+        s.autotune()
+        own.append(R.capture_runner(s, pol, acts, scratch, chunk))
+    for g in own:
+        g.launch(5)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps // chunk):
+        for g in own:
+            g.launch(5 + chunk * (k + 1))
+    dev.synchronize()
+    separate_ms = (time.perf_counter() - t0) * 1000.0 / steps
+    del own, sims
+    return {"instances": [{"track": t, "cars": n} for t, n in zip(BATCH_TRACKS, BATCH_CARS)], "beams": 1081,
+            "cars": total, "steps": steps, "lidar_waves_per_car": wpc,
+            "batched_graph": {"ms_per_step": batched_ms, "env_steps_per_s": total * 1000.0 / batched_ms},
+            "separate_graphs": {"ms_per_step": separate_ms, "env_steps_per_s": total * 1000.0 / separate_ms},
+            "speedup": separate_ms / batched_ms}
+
+
 def plain_baseline(R, dev, track, args, steps: int) -> dict:
     """The same Runner step in the reference's own GPU decomposition, on the
     same GPU: one thread per car for physics, one thread per (car, beam) for the
@@ -178,6 +242,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU decomposition baseline")
+    ap.add_argument("--batch-steps", type=int, default=100,
+                    help="timed steps of the eight-instance batched launch (0 skips it)")
     ap.add_argument("--ppo-iterations", type=int, default=3,
                     help="timed PPO iterations per beam count on the bridge (0 skips the PPO leg)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -297,6 +363,10 @@ def main() -> int:
     if not args.no_plain:
         plain = plain_baseline(R, dev, arr, args, min(args.steps, 50))
         log("plain GPU baseline:", plain)
+    batched = None
+    if args.batch_steps > 0:
+        batched = batch_leg(R, T, dev, args.batch_steps)
+        log("eight-instance batch:", batched)
     ppo = None
     if args.ppo_iterations > 0:
         ppo = {}
@@ -325,7 +395,7 @@ def main() -> int:
                                                  "HIP events in eager passes of the same steps right after",
         "roofline": roofline, "policy_roofline": policy, "kernels": kernels, "cpu_baseline": cpu,
         "plain_gpu_baseline": plain, "speedup_vs_plain_gpu": value / plain["value"] if plain else None,
-        "ppo_iteration": ppo,
+        "ppo_iteration": ppo, "batch": batched,
     }
     print(json.dumps(out), flush=True)
     return 0

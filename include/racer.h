@@ -201,6 +201,65 @@ int racer_batch_autotune(racer_batch* b, int* chosen);
 int racer_batch_count_iterations(racer_batch* b, int on);
 int64_t racer_batch_last_iterations(const racer_batch* b);
 
+/* ---------------- Learned dynamics surrogate (no reference counterpart) ----------------
+ * A residual MLP (11 inputs -> 3 x 128 tanh -> 5) that predicts one step's
+ * pose change of the RK4 integration (racer/sim.mojo:184-219): inputs x, y, z,
+ * yaw, speed, yaw rate, steer, pitch and roll slopes, throttle, steer command;
+ * outputs the displacement in the car's frame (forward, left) and the heading,
+ * speed and steer changes. Trained on transitions the sim records, by the
+ * library's own products (precision 1: bf16 MFMA hidden layers, float32
+ * accumulation; precision 0: float32 throughout). `cols` is the largest batch
+ * (and car count) the working buffers hold. */
+typedef struct racer_surrogate racer_surrogate;
+typedef struct racer_dataset racer_dataset;
+int racer_surrogate_create(racer_device* dev, int seed, int precision, int64_t cols, racer_surrogate** out);
+void racer_surrogate_destroy(racer_surrogate* s);
+int64_t racer_surrogate_params(const racer_surrogate* s);
+/* Weights: layer l's [w_(l+1), w_l + 1] (bias last) concatenated, l = 0 .. 3.
+ * Call racer_surrogate_mark_updated after writing them. */
+racer_buf* racer_surrogate_theta(racer_surrogate* s);
+int racer_surrogate_mark_updated(racer_surrogate* s);
+racer_buf* racer_surrogate_grad(racer_surrogate* s);
+/* Normalisation: in_mean[11], in_inv_std[11], out_mean[5], out_std[5], out_inv_std[5]. */
+racer_buf* racer_surrogate_stats(racer_surrogate* s);
+/* Loss and gradient (into the grad buffer) of one batch: x [12, B] normalised
+ * inputs with a last row of ones, y [5, B] normalised targets, mask [1, B];
+ * loss = sum_b mask_b sum_o (out - y)^2 / (5 B). Outputs land in
+ * racer_surrogate_outputs ([5, B] in its first 5 B floats). */
+int racer_surrogate_batch_gradient(racer_surrogate* s, const racer_view* x, const racer_view* y,
+                                   const racer_view* mask);
+racer_buf* racer_surrogate_outputs(racer_surrogate* s);
+/* Transitions: inputs [12, capacity], targets [5, capacity], mask [capacity]
+ * (0 for steps that ended in a crash or timeout). */
+int racer_dataset_create(racer_device* dev, int64_t capacity, racer_dataset** out);
+void racer_dataset_destroy(racer_dataset* d);
+int64_t racer_dataset_count(const racer_dataset* d);
+racer_buf* racer_dataset_features(racer_dataset* d);
+racer_buf* racer_dataset_targets(racer_dataset* d);
+racer_buf* racer_dataset_mask(racer_dataset* d);
+/* `steps` Runner steps (main.mojo:183-191) with transitions recorded (Philox offsets t0 ..). */
+int racer_dataset_record(racer_dataset* d, racer_sim* sim, racer_policy* pol, racer_acts* acts, racer_buf* scratch,
+                         int steps, int64_t t0);
+int racer_surrogate_fit_normalization(racer_surrogate* s, racer_dataset* d);
+/* Adam (lr) over `epochs` passes of interleaved minibatches of `batch`. */
+int racer_surrogate_train(racer_surrogate* s, racer_dataset* d, int epochs, int64_t batch, float lr);
+/* Mean squared normalised error per usable transition of the dataset. */
+int racer_surrogate_loss(racer_surrogate* s, racer_dataset* d, double* loss);
+/* Sim.step (racer/sim.mojo:314-321) with the surrogate's pose change in place
+ * of the RK4 integration; cars whose prediction lands below min_clearance from
+ * the walls or on another layer take the exact step (fallback[i] = 1). The
+ * crash test, progress, reward, respawn and lidar are the exact ones. */
+int racer_surrogate_step(racer_surrogate* s, racer_sim* sim, float min_clearance, racer_buf* fallback);
+/* One step of the kinematics (racer/sim.mojo:184-219) in float64 on the GPU
+ * (the surrogate's judge): st holds [6, n] doubles (x, y, z, heading, speed,
+ * steer) in 12 n floats and is updated in place; friction comes from the sim's
+ * state; no crash, progress or respawn. */
+int racer_sim_step_f64(racer_sim* sim, racer_buf* st, const racer_view* actions);
+/* Runner.step (main.mojo:183-191) with racer_surrogate_step. */
+int racer_surrogate_runner_step(racer_surrogate* s, racer_sim* sim, racer_policy* pol, racer_acts* acts,
+                                racer_buf* scratch, int deterministic, int64_t t, float min_clearance,
+                                racer_buf* fallback);
+
 /* ---------------- Graphs (no reference counterpart) ---------------- */
 /* The reference enqueues every launch of a step from the host
  * (main.mojo:183-191, racer/ppo.mojo:190-210). These capture the same launch

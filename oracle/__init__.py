@@ -60,6 +60,11 @@ def lib():
             "o_scan_one": (F, [P, C.c_size_t, P, L, P, L, L, I, I, P]),
             "o_check_nearest_index": (L, [F, F, F]),
             "o_check_min_quotient": (L, [L, C.c_uint64]),
+            "o_surrogate_features": (None, [P, C.c_size_t, P, L, P, L, P, L]),
+            "o_surrogate_targets": (None, [P, P, P, L, P, L, P]),
+            "o_mlp_forward": (None, [P, P, I, P, L, P]),
+            "o_mlp_gradient": (C.c_double, [P, P, I, P, P, P, L, P]),
+            "o_step_f64": (None, [P, C.c_size_t, L, P, P, P, L]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(_lib, name)
@@ -299,3 +304,60 @@ class Trainer:
     def minibatch(self, m, adv_mean, adv_scale):
         return minibatch(self.policy, self.n, self.R, self.MB, self.obs, self.actions, self.logp, self.value,
                          self.advantage, m, adv_mean, adv_scale)
+
+
+# ---------------- learned dynamics surrogate (no reference counterpart)
+SURROGATE_WIDTHS = (11, 128, 128, 128, 5)
+
+
+def surrogate_features(track, state, actions) -> np.ndarray:
+    """The surrogate's 11 inputs and constant row for every car, [12, n]."""
+    t = f32(track)
+    n = state.shape[-1]
+    feat = np.zeros((12, n), np.float32)
+    lib().o_surrogate_features(ptr(t), t.size, ptr(f32(state)), n, ptr(f32(actions)), actions.shape[-1], ptr(feat), n)
+    return feat
+
+
+def surrogate_targets(prev, state, done):
+    n = state.shape[-1]
+    tgt = np.zeros((5, n), np.float32)
+    mask = np.zeros(n, np.float32)
+    lib().o_surrogate_targets(ptr(f32(prev)), ptr(f32(state)), ptr(f32(done)), n, ptr(tgt), n, ptr(mask))
+    return tgt, mask
+
+
+def surrogate_params(widths=SURROGATE_WIDTHS) -> int:
+    return sum(widths[l + 1] * (widths[l] + 1) for l in range(len(widths) - 1))
+
+
+def mlp_forward(theta, x, widths=SURROGATE_WIDTHS) -> list:
+    """Per-layer outputs (hidden ones with their constant row) of the float32 MLP."""
+    w = np.array(widths, np.int32)
+    cols = x.shape[1]
+    layers = len(widths) - 1
+    sizes = [(widths[l + 1] + (1 if l < layers - 1 else 0)) for l in range(layers)]
+    acts = np.zeros(sum(sizes) * cols, np.float32)
+    lib().o_mlp_forward(ptr(f32(theta)), ptr(w), layers, ptr(f32(x)), cols, ptr(acts))
+    out, o = [], 0
+    for r in sizes:
+        out.append(acts[o: o + r * cols].reshape(r, cols))
+        o += r * cols
+    return out
+
+
+def mlp_gradient(theta, x, y, mask, widths=SURROGATE_WIDTHS):
+    """(loss, gradient) of the masked mean squared error, float32 orders of the HIP surrogate."""
+    w = np.array(widths, np.int32)
+    grad = np.zeros(surrogate_params(widths), np.float32)
+    loss = lib().o_mlp_gradient(ptr(f32(theta)), ptr(w), len(widths) - 1, ptr(f32(x)), ptr(f32(y)), ptr(f32(mask)),
+                                x.shape[1], ptr(grad))
+    return float(loss), grad
+
+
+def step_f64(track, st: np.ndarray, friction, actions) -> None:
+    """One float64 RK4 environment step of st = [x, y, z, heading, speed, steer] rows (in place)."""
+    t = f32(track)
+    assert st.dtype == np.float64 and st.flags["C_CONTIGUOUS"]
+    n = st.shape[1]
+    lib().o_step_f64(ptr(t), t.size, n, ptr(st), ptr(f32(friction)), ptr(f32(actions)), actions.shape[-1])

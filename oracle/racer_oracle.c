@@ -635,3 +635,227 @@ long o_check_min_quotient(long count, uint64_t seed) {
     }
     return bad;
 }
+
+/* ---------------- Learned dynamics surrogate (no reference counterpart) ---------------- */
+
+/* The surrogate's 11 inputs of car i and its constant-1 row (the HIP feature
+ * kernel's definition): x, y, z, yaw, speed, yaw rate, steer, along and
+ * across slopes, clamped throttle and steer command. */
+void o_surrogate_features(const float* d, size_t len, const float* s, long n, const float* act, long act_cols,
+                          float* feat, long ldf) {
+    omap m;
+    if (om_init(&m, d, len)) return;
+    for (long i = 0; i < n; ++i) {
+        float x = s[RACER_X * n + i], y = s[RACER_Y * n + i], z = s[RACER_Z * n + i];
+        float heading = s[RACER_HEADING * n + i], speed = s[RACER_SPEED * n + i], steer = s[RACER_STEER * n + i];
+        ospot spot = om_probe(&m, d, x, y, z);
+        osurface g = om_surface(&m, d, spot.layer, x, y);
+        float along, across;
+        o_slopes(g, heading, &along, &across);
+        float f[11] = {x, y, z, heading, speed, ((speed * racer_sin(steer)) / racer_cos(steer)) / RACER_WHEELBASE,
+                       steer, along, across, racer_clamp(act[act_cols + i], -1.0f, 1.0f),
+                       racer_clamp(act[i], -1.0f, 1.0f)};
+        for (int r = 0; r < 11; ++r) feat[r * ldf + i] = f[r];
+        feat[11 * ldf + i] = 1.0f;
+    }
+}
+
+/* The one-step targets: displacement in the starting car frame, wrapped
+ * heading change, speed and steer changes; mask 0 after a crash or timeout. */
+void o_surrogate_targets(const float* prev, const float* s, const float* done, long n, float* tgt, long ldt,
+                         float* mask) {
+    for (long i = 0; i < n; ++i) {
+        float h = prev[RACER_HEADING * n + i];
+        float ch = racer_cos(h), sh = racer_sin(h);
+        float dx = s[RACER_X * n + i] - prev[RACER_X * n + i];
+        float dy = s[RACER_Y * n + i] - prev[RACER_Y * n + i];
+        float dh = s[RACER_HEADING * n + i] - h;
+        if (dh > RACER_PI) dh -= RACER_2PI;
+        if (dh < -RACER_PI) dh += RACER_2PI;
+        tgt[0 * ldt + i] = ch * dx + sh * dy;
+        tgt[1 * ldt + i] = ch * dy - sh * dx;
+        tgt[2 * ldt + i] = dh;
+        tgt[3 * ldt + i] = s[RACER_SPEED * n + i] - prev[RACER_SPEED * n + i];
+        tgt[4 * ldt + i] = s[RACER_STEER * n + i] - prev[RACER_STEER * n + i];
+        mask[i] = done[i] == 0.0f ? 1.0f : 0.0f;
+    }
+}
+
+/* The surrogate MLP in float32: widths[0 .. layers]; layer l's weights
+ * [widths[l+1], widths[l] + 1] (bias last) at consecutive offsets; x holds
+ * [widths[0] + 1, cols] with a last row of ones. acts receives each layer's
+ * output in turn: hidden [widths[l] + 1, cols] (tanh, last row 1), then the
+ * output [widths[layers], cols]. Products sum k in order with fmaf. */
+void o_mlp_forward(const float* theta, const int* widths, int layers, const float* x, long cols, float* acts) {
+    const float* in = x;
+    const float* w = theta;
+    float* out = acts;
+    for (int l = 0; l < layers; ++l) {
+        int rows = widths[l + 1], k = widths[l] + 1;
+        int hidden = l < layers - 1;
+        mm(w, k, in, cols, out, cols, rows, k, cols, hidden);
+        if (hidden)
+            for (long c = 0; c < cols; ++c) out[(long)rows * cols + c] = 1.0f;
+        w += (long)rows * k;
+        in = out;
+        out += (long)(rows + hidden) * cols;
+    }
+}
+
+/* Loss sum_b mask_b sum_o (out - y)^2 / (O cols) and its gradient (the HIP
+ * surrogate's orders: output gradient ((e + e) mask) / (O cols), products
+ * through the layers in k order times (1 - h^2), weight gradients by
+ * chunked_dot). Returns the loss in float64. */
+double o_mlp_gradient(const float* theta, const int* widths, int layers, const float* x, const float* y,
+                      const float* mask, long cols, float* grad) {
+    long total = 0, P = 0;
+    for (int l = 0; l < layers; ++l) {
+        total += (long)(widths[l + 1] + (l < layers - 1)) * cols;
+        P += (long)widths[l + 1] * (widths[l] + 1);
+    }
+    float* acts = (float*)malloc(sizeof(float) * total);
+    o_mlp_forward(theta, widths, layers, x, cols, acts);
+    const float* a[16];
+    const float* wl[16];
+    a[0] = x;
+    long ao = 0, wo = 0;
+    for (int l = 0; l < layers; ++l) {
+        a[l + 1] = acts + ao;
+        wl[l] = theta + wo;
+        ao += (long)(widths[l + 1] + (l < layers - 1)) * cols;
+        wo += (long)widths[l + 1] * (widths[l] + 1);
+    }
+    int O = widths[layers];
+    float* delta = (float*)malloc(sizeof(float) * (size_t)O * cols);
+    double loss = 0.0;
+    float denom = (float)((long)O * cols);
+    for (long c = 0; c < cols; ++c)
+        for (int o = 0; o < O; ++o) {
+            float e = a[layers][o * cols + c] - y[o * cols + c];
+            delta[o * cols + c] = ((e + e) * mask[c]) / denom;
+            loss += (double)mask[c] * (double)e * (double)e;
+        }
+    loss /= (double)denom;
+    long go = P;
+    for (int l = layers - 1; l >= 0; --l) {
+        int rows = widths[l + 1], k = widths[l] + 1;
+        go -= (long)rows * k;
+        for (int j = 0; j < rows; ++j)
+            for (int q = 0; q < k; ++q) grad[go + (long)j * k + q] = chunked_dot(delta + (long)j * cols, a[l] + (long)q * cols, cols);
+        if (l == 0) break;
+        int below = widths[l];
+        float* next = (float*)malloc(sizeof(float) * (size_t)below * cols);
+        for (long c = 0; c < cols; ++c)
+            for (int q = 0; q < below; ++q) {
+                float acc = 0.0f;
+                for (int j = 0; j < rows; ++j) acc = fmaf(wl[l][(long)j * k + q], delta[(long)j * cols + c], acc);
+                float h = a[l][(long)q * cols + c];
+                next[(long)q * cols + c] = acc * (1.0f - h * h);
+            }
+        free(delta);
+        delta = next;
+    }
+    free(delta);
+    free(acts);
+    return loss;
+}
+
+/* ---------------- FP64 RK4 (the judge of the surrogate) ---------------- */
+
+static void f64_probe(const omap* m, const float* d, double x, double y, double z, int* layer, double* clearance,
+                      double* height) {
+    double fx = floor((x - m->x0) / m->cell + 0.5), fy = floor((y - m->y0) / m->cell + 0.5);
+    int on = fx >= 0.0 && fy >= 0.0 && fx < (double)m->nx && fy < (double)m->ny;
+    long c = on ? (long)fy * m->nx + (long)fx : 0;
+    *layer = 0;
+    *clearance = -1.0;
+    *height = on ? (double)d[om_grid(m, 0, RACER_HEIGHT) + c] : -1.0;
+    double gap = 1e300;
+    for (int l = 0; l < m->layers && on; ++l) {
+        double cl = d[om_grid(m, l, RACER_CLEARANCE) + c], h = d[om_grid(m, l, RACER_HEIGHT) + c];
+        if (cl > 0.0 && fabs(h - z) < gap) {
+            *layer = l;
+            *clearance = cl;
+            *height = h;
+            gap = fabs(h - z);
+        }
+    }
+}
+
+static void f64_surface(const omap* m, const float* d, int layer, double x, double y, double* sx, double* sy) {
+    double fx = (x - m->x0) / m->cell, fy = (y - m->y0) / m->cell;
+    double cx = floor(fx), cy = floor(fy);
+    long ix = !(cx >= 0.0) ? 0 : (cx > (double)(m->nx - 2) ? m->nx - 2 : (long)cx);
+    long iy = !(cy >= 0.0) ? 0 : (cy > (double)(m->ny - 2) ? m->ny - 2 : (long)cy);
+    double tx = fmin(fmax(fx - (double)ix, 0.0), 1.0), ty = fmin(fmax(fy - (double)iy, 0.0), 1.0);
+    long k = om_grid(m, layer, RACER_HEIGHT) + iy * m->nx + ix;
+    double h0 = d[k], h1 = d[k + 1], h2 = d[k + m->nx], h3 = d[k + m->nx + 1];
+    *sx = ((h1 - h0) * (1.0 - ty) + (h3 - h2) * ty) / m->cell;
+    *sy = ((h2 - h0) * (1.0 - tx) + (h3 - h1) * tx) / m->cell;
+}
+
+static void f64_derivative(const double p[4], double steer, double accel, double grip, double along, double across,
+                           double out[4]) {
+    const double G = RACER_G, WB = RACER_WHEELBASE;
+    double nz = 1.0 / sqrt(1.0 + along * along + across * across);
+    double load = grip * G * nz;
+    double v = fmax(fabs(p[3]), 0.5) * (p[3] >= 0.0 ? 1.0 : -1.0);
+    double slide = G * across * nz;
+    double lateral = fmin(fmax(v * p[3] * tan(steer) / WB + slide, -load), load);
+    double yaw = (lateral - slide) / v;
+    double longitudinal = sqrt(fmax(load * load - lateral * lateral, 0.0));
+    out[0] = p[3] * cos(p[2]);
+    out[1] = p[3] * sin(p[2]);
+    out[2] = yaw;
+    out[3] = fmin(fmax(accel, -longitudinal), longitudinal) - G * along * nz;
+}
+
+/* One environment step of the kinematics (sim.mojo:184-219) in float64 for n
+ * cars: st rows x, y, z, heading, speed, steer (updated in place); friction
+ * and actions as the sim's. No crash or respawn: the judge is for windows
+ * without them. */
+void o_step_f64(const float* d, size_t len, long n, double* st, const float* friction, const float* act,
+                long act_cols) {
+    omap m;
+    if (om_init(&m, d, len)) return;
+    const double DT = 1.0 / 240.0;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (long i = 0; i < n; ++i) {
+        double p[4] = {st[0 * n + i], st[1 * n + i], st[3 * n + i], st[4 * n + i]};
+        double steer = st[5 * n + i];
+        int layer;
+        double clearance, height, sx, sy;
+        f64_probe(&m, d, p[0], p[1], st[2 * n + i], &layer, &clearance, &height);
+        f64_surface(&m, d, layer, p[0], p[1], &sx, &sy);
+        double rate = fmin(fmax((double)act[i], -1.0), 1.0) * RACER_STEER_RATE;
+        double accel = fmin(fmax((double)act[act_cols + i], -1.0), 1.0) * RACER_ACCEL_MAX;
+        double grip = RACER_GRIP * (double)friction[i];
+        for (int sub = 0; sub < RACER_SUBSTEPS; ++sub) {
+            double c = cos(p[2]), s = sin(p[2]);
+            double along = sx * c + sy * s, across = -sx * s + sy * c;
+            double half = steer + rate * DT / 2.0, full = steer + rate * DT;
+            double k1[4], k2[4], k3[4], k4[4], q[4];
+            f64_derivative(p, steer, accel, grip, along, across, k1);
+            for (int j = 0; j < 4; ++j) q[j] = p[j] + k1[j] * DT / 2.0;
+            f64_derivative(q, half, accel, grip, along, across, k2);
+            for (int j = 0; j < 4; ++j) q[j] = p[j] + k2[j] * DT / 2.0;
+            f64_derivative(q, half, accel, grip, along, across, k3);
+            for (int j = 0; j < 4; ++j) q[j] = p[j] + k3[j] * DT;
+            f64_derivative(q, full, accel, grip, along, across, k4);
+            for (int j = 0; j < 4; ++j) p[j] += (k1[j] + 2.0 * (k2[j] + k3[j]) + k4[j]) * DT / 6.0;
+            p[3] = fmin(fmax(p[3], -(double)RACER_SPEED_MAX), (double)RACER_SPEED_MAX);
+            steer = fmin(fmax(full, -(double)RACER_STEER_MAX), (double)RACER_STEER_MAX);
+            f64_probe(&m, d, p[0], p[1], height, &layer, &clearance, &height);
+            f64_surface(&m, d, layer, p[0], p[1], &sx, &sy);
+        }
+        const double PI = 3.14159265358979323846;
+        if (p[2] > PI) p[2] -= 2.0 * PI;
+        if (p[2] < -PI) p[2] += 2.0 * PI;
+        st[0 * n + i] = p[0];
+        st[1 * n + i] = p[1];
+        st[2 * n + i] = height;
+        st[3 * n + i] = p[2];
+        st[4 * n + i] = p[3];
+        st[5 * n + i] = steer;
+    }
+}

@@ -111,6 +111,18 @@ void o_adam(float* theta, long P, long logstd_at, const float* grad, float* firs
             int step);
 float o_bias_correction(float beta, int step);
 
+/* Learned dynamics surrogate: inputs, targets, the float32 MLP and the
+ * float64 RK4 step that judges it (see the .c file). */
+void o_surrogate_features(const float* d, size_t len, const float* s, long n, const float* act, long act_cols,
+                          float* feat, long ldf);
+void o_surrogate_targets(const float* prev, const float* s, const float* done, long n, float* tgt, long ldt,
+                         float* mask);
+void o_mlp_forward(const float* theta, const int* widths, int layers, const float* x, long cols, float* acts);
+double o_mlp_gradient(const float* theta, const int* widths, int layers, const float* x, const float* y,
+                      const float* mask, long cols, float* grad);
+void o_step_f64(const float* d, size_t len, long n, double* st, const float* friction, const float* act,
+                long act_cols);
+
 /* Flat wrappers of the Map lookups and of one lidar beam, for the tests. */
 int o_probe(const float* d, size_t len, float x, float y, float z, float out[3]);
 int o_surface(const float* d, size_t len, int layer, float x, float y, float out[3]);

@@ -151,6 +151,28 @@ def lib() -> C.CDLL:
         "racer_trainer_rollout_graph": (i, [vp, vp, i, C.POINTER(C.c_float)]),
         "racer_graph_destroy": (None, [vp]),
         "racer_graph_batch": (i, [vp, vp, i, i, cp]),
+        "racer_surrogate_create": (i, [vp, i, i, i64, cp]),
+        "racer_surrogate_destroy": (None, [vp]),
+        "racer_surrogate_params": (i64, [vp]),
+        "racer_surrogate_theta": (vp, [vp]),
+        "racer_surrogate_mark_updated": (i, [vp]),
+        "racer_surrogate_grad": (vp, [vp]),
+        "racer_surrogate_stats": (vp, [vp]),
+        "racer_surrogate_batch_gradient": (i, [vp, VP, VP, VP]),
+        "racer_surrogate_outputs": (vp, [vp]),
+        "racer_dataset_create": (i, [vp, i64, cp]),
+        "racer_dataset_destroy": (None, [vp]),
+        "racer_dataset_count": (i64, [vp]),
+        "racer_dataset_features": (vp, [vp]),
+        "racer_dataset_targets": (vp, [vp]),
+        "racer_dataset_mask": (vp, [vp]),
+        "racer_dataset_record": (i, [vp, vp, vp, vp, vp, i, i64]),
+        "racer_surrogate_fit_normalization": (i, [vp, vp]),
+        "racer_surrogate_train": (i, [vp, vp, i, i64, f]),
+        "racer_surrogate_loss": (i, [vp, vp, C.POINTER(C.c_double)]),
+        "racer_surrogate_step": (i, [vp, vp, f, vp]),
+        "racer_sim_step_f64": (i, [vp, vp, VP]),
+        "racer_surrogate_runner_step": (i, [vp, vp, vp, vp, vp, i, i64, f, vp]),
         "racer_batch_create": (i, [vp, C.POINTER(C.c_void_p), i, C.POINTER(Config), cp]),
         "racer_batch_destroy": (None, [vp]),
         "racer_batch_count": (i, [vp]),
@@ -387,6 +409,11 @@ class Sim:
     def last_iterations(self) -> int:
         return int(lib().racer_sim_last_iterations(self.handle))
 
+    def step_f64(self, st: Buffer, actions: View) -> None:
+        """One float64 RK4 step of the kinematics: st holds [6, n] doubles (x, y, z, heading, speed,
+        steer) as 12 n floats; friction from this sim's state."""
+        _check(lib().racer_sim_step_f64(self.handle, st.handle, actions.c))
+
     @property
     def last_trips(self) -> int:
         """Wavefront loop trips of the last counted lidar launch (schedule 0)."""
@@ -533,6 +560,15 @@ def runner_step(sim: "Sim", policy: "Policy", acts: "Activations", scratch: Buff
                                    int(t)))
 
 
+def capture_runner(sim: "Sim", policy: "Policy", acts: "Activations", scratch: Buffer, steps: int,
+                   deterministic: bool = False) -> "Graph":
+    """`steps` runner_step calls on explicit objects as one HIP graph."""
+    h = C.c_void_p()
+    _check(lib().racer_graph_runner(sim.handle, policy.handle, acts.handle, scratch.handle, int(steps),
+                                    int(deterministic), C.byref(h)))
+    return Graph(h.value, (sim, policy, acts, scratch))
+
+
 class Batch:
     """Sims stepped together, one launch per stage (bitwise as stepping each alone)."""
 
@@ -590,6 +626,91 @@ class Batch:
         try:
             if self.handle:
                 lib().racer_batch_destroy(self.handle)
+        except Exception:
+            pass
+        self.handle = None
+
+
+class Dataset:
+    """Recorded transitions of a sim: inputs [12, capacity], targets [5, capacity], mask."""
+
+    def __init__(self, device: "Device", capacity: int):
+        self.device = device
+        h = C.c_void_p()
+        _check(lib().racer_dataset_create(device.handle, int(capacity), C.byref(h)))
+        self.handle = h.value
+        self.capacity = int(capacity)
+        L = lib()
+        self.features = _wrap(device, L.racer_dataset_features(self.handle))
+        self.targets = _wrap(device, L.racer_dataset_targets(self.handle))
+        self.mask = _wrap(device, L.racer_dataset_mask(self.handle))
+
+    @property
+    def count(self) -> int:
+        return int(lib().racer_dataset_count(self.handle))
+
+    def record(self, sim: "Sim", policy: "Policy", acts: "Activations", scratch: Buffer, steps: int, t0: int = 0):
+        _check(lib().racer_dataset_record(self.handle, sim.handle, policy.handle, acts.handle, scratch.handle,
+                                          int(steps), int(t0)))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().racer_dataset_destroy(self.handle)
+        except Exception:
+            pass
+        self.handle = None
+
+
+class Surrogate:
+    """The learned dynamics surrogate (11 -> 3 x 128 tanh -> 5 residual MLP)."""
+
+    INPUTS, HIDDEN, OUTPUTS = 11, 128, 5
+
+    def __init__(self, device: "Device", seed: int = 1, precision: int = 1, cols: int = 16384):
+        self.device = device
+        h = C.c_void_p()
+        _check(lib().racer_surrogate_create(device.handle, int(seed), int(precision), int(cols), C.byref(h)))
+        self.handle = h.value
+        self.cols = int(cols)
+        L = lib()
+        self.P = int(L.racer_surrogate_params(self.handle))
+        self.theta = _wrap(device, L.racer_surrogate_theta(self.handle))
+        self.grad = _wrap(device, L.racer_surrogate_grad(self.handle))
+        self.stats = _wrap(device, L.racer_surrogate_stats(self.handle))
+        self.outputs = _wrap(device, L.racer_surrogate_outputs(self.handle))
+
+    def mark_updated(self) -> None:
+        _check(lib().racer_surrogate_mark_updated(self.handle))
+
+    def batch_gradient(self, x: View, y: View, mask: View) -> None:
+        _check(lib().racer_surrogate_batch_gradient(self.handle, x.c, y.c, mask.c))
+
+    def fit_normalization(self, data: Dataset) -> None:
+        _check(lib().racer_surrogate_fit_normalization(self.handle, data.handle))
+
+    def train(self, data: Dataset, epochs: int, batch: int, lr: float = 1e-3) -> None:
+        _check(lib().racer_surrogate_train(self.handle, data.handle, int(epochs), int(batch), float(lr)))
+
+    def loss(self, data: Dataset) -> float:
+        v = C.c_double()
+        _check(lib().racer_surrogate_loss(self.handle, data.handle, C.byref(v)))
+        return float(v.value)
+
+    def step(self, sim: "Sim", min_clearance: float = 0.1, fallback: Buffer | None = None) -> None:
+        _check(lib().racer_surrogate_step(self.handle, sim.handle, float(min_clearance),
+                                          fallback.handle if fallback is not None else None))
+
+    def runner_step(self, sim: "Sim", policy: "Policy", acts: "Activations", scratch: Buffer, deterministic: bool,
+                    t: int, min_clearance: float = 0.1, fallback: Buffer | None = None) -> None:
+        _check(lib().racer_surrogate_runner_step(self.handle, sim.handle, policy.handle, acts.handle, scratch.handle,
+                                                 int(deterministic), int(t), float(min_clearance),
+                                                 fallback.handle if fallback is not None else None))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().racer_surrogate_destroy(self.handle)
         except Exception:
             pass
         self.handle = None

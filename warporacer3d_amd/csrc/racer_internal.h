@@ -203,6 +203,17 @@ int racer_launch_batch_lidar(racer_batch* b);
 int racer_policy_fused_batch(racer_policy* p, const racer_policy_table& tab, int deterministic, int64_t step);
 int racer_batch_runner_enqueue(racer_batch* b, racer_policy* p, int deterministic, int64_t t);
 
+// ---------------------------------------------------------------- surrogate (sim side)
+// Surrogate inputs (11 rows + a constant-1 row) of a sim's cars for the given
+// actions, normalised when mean / inv_std are given.
+int racer_launch_surrogate_features(racer_sim* sim, const float* act, int64_t act_cols, float* feat, int64_t ldf,
+                                    const float* mean, const float* inv_std);
+// One-step targets (5 rows) and the no-crash mask from the state before a step.
+int racer_launch_surrogate_targets(racer_sim* sim, const float* prev, float* tgt, int64_t ldt, float* mask);
+// Sim.step with the surrogate's predicted pose change (RK4 fallback per car), then the lidar.
+int racer_launch_surrogate_step(racer_sim* sim, const float* pred, int64_t ldp, const float* out_mean,
+                                const float* out_std, float min_clearance, float* fallback);
+
 // ---------------------------------------------------------------- kernels (launchers)
 int racer_launch_reset(racer_sim* sim);
 int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done, float* obs, int64_t obs_cols);

@@ -205,6 +205,11 @@ __device__ __forceinline__ void probe_surface(const racer_dmap& m, float x, floa
     }
 }
 
+template <int L>
+__device__ void finish(const racer_dmap& m, float* s, int n, float* reward, float* done, float* obs,
+                       int64_t obs_cols, int beams, int i, int seed, float heading_noise, float p[4], float steer,
+                       const Spot& spot, Surf ground, bool crashed);
+
 // advance (racer/sim.mojo:184-235): one 60 Hz step of car i.
 template <int L>
 __device__ void advance(const racer_dmap& m, float* s, int n, const float* act, int64_t act_cols, float* reward,
@@ -243,6 +248,16 @@ __device__ void advance(const racer_dmap& m, float* s, int n, const float* act, 
         crashed = crashed || spot.clearance < RACER_HALF_DIAG ||
                   ground.sx * ground.sx + ground.sy * ground.sy > 1.0f;
     }
+    finish<L>(m, s, n, reward, done, obs, obs_cols, beams, i, seed, heading_noise, p, steer, spot, ground, crashed);
+}
+
+// The end of advance (racer/sim.mojo:219-235) from the integrated pose p =
+// (x, y, heading, speed), steer, the last probe and the crash flag: heading
+// wrap, progress, reward, done code, state and return, respawn, obs rows.
+template <int L>
+__device__ void finish(const racer_dmap& m, float* s, int n, float* reward, float* done, float* obs,
+                       int64_t obs_cols, int beams, int i, int seed, float heading_noise, float p[4], float steer,
+                       const Spot& spot, Surf ground, bool crashed) {
     if (p[2] > RACER_PI) p[2] -= RACER_2PI;
     if (p[2] < -RACER_PI) p[2] += RACER_2PI;
     float progress = map_progress(m, spot.layer, p[0], p[1]);
@@ -284,6 +299,210 @@ __global__ void __launch_bounds__(64) k_physics(racer_dmap m, float* s, int n, c
                                                 int seed, float heading_noise) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) advance<L>(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, i, seed, heading_noise);
+}
+
+// ---- learned dynamics surrogate (no reference counterpart) ----
+// Inputs of the surrogate for car i (rows of feat, ldf apart): x, y, z, yaw,
+// speed, yaw rate v tan(steer) / wheelbase, steer, pitch and roll (the road's
+// rise per metre ahead and to the left under the car), throttle and steer
+// command (the clamped actions); row 11 is the constant 1 that carries the
+// first layer's bias. With mean / inv_std the 11 inputs are normalised.
+template <int L>
+__global__ void __launch_bounds__(256) k_surrogate_features(racer_dmap m, const float* __restrict__ s, int n,
+                                                            const float* __restrict__ act, int64_t act_cols,
+                                                            float* __restrict__ feat, int64_t ldf,
+                                                            const float* __restrict__ mean,
+                                                            const float* __restrict__ inv_std) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float x = s[RACER_X * n + i], y = s[RACER_Y * n + i], z = s[RACER_Z * n + i];
+    const float heading = s[RACER_HEADING * n + i], speed = s[RACER_SPEED * n + i], steer = s[RACER_STEER * n + i];
+    Spot spot;
+    Surf ground;
+    probe_surface<L>(m, x, y, z, &spot, &ground);
+    float along, across;
+    slopes(ground, heading, &along, &across);
+    float f[11] = {x,
+                   y,
+                   z,
+                   heading,
+                   speed,
+                   ((speed * racer_sin(steer)) / racer_cos(steer)) / RACER_WHEELBASE,
+                   steer,
+                   along,
+                   across,
+                   racer_clamp(act[act_cols + i], -1.0f, 1.0f),
+                   racer_clamp(act[i], -1.0f, 1.0f)};
+#pragma unroll
+    for (int r = 0; r < 11; ++r) {
+        const float v = mean ? (f[r] - mean[r]) * inv_std[r] : f[r];
+        feat[(int64_t)r * ldf + i] = v;
+    }
+    feat[(int64_t)11 * ldf + i] = 1.0f;
+}
+
+// One-step targets from the state before (prev) and after (s) a step: the
+// displacement in the car's frame at the start (forward, left), the heading
+// change wrapped to [-pi, pi], the speed and steer changes; mask 1 unless the
+// step ended in a crash or timeout (then the car respawned).
+__global__ void __launch_bounds__(256) k_surrogate_targets(const float* __restrict__ prev, const float* __restrict__ s,
+                                                           const float* __restrict__ done, int n,
+                                                           float* __restrict__ tgt, int64_t ldt,
+                                                           float* __restrict__ mask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float h = prev[RACER_HEADING * n + i];
+    const float ch = racer_cos(h), sh = racer_sin(h);
+    const float dx = s[RACER_X * n + i] - prev[RACER_X * n + i];
+    const float dy = s[RACER_Y * n + i] - prev[RACER_Y * n + i];
+    float dh = s[RACER_HEADING * n + i] - h;
+    if (dh > RACER_PI) dh -= RACER_2PI;
+    if (dh < -RACER_PI) dh += RACER_2PI;
+    tgt[0 * ldt + i] = ch * dx + sh * dy;
+    tgt[1 * ldt + i] = ch * dy - sh * dx;
+    tgt[2 * ldt + i] = dh;
+    tgt[3 * ldt + i] = s[RACER_SPEED * n + i] - prev[RACER_SPEED * n + i];
+    tgt[4 * ldt + i] = s[RACER_STEER * n + i] - prev[RACER_STEER * n + i];
+    mask[i] = done[i] == 0.0f ? 1.0f : 0.0f;
+}
+
+// advance with the pose change predicted by the surrogate (pred rows:
+// normalised forward, left, heading, speed, steer changes; out = pred std +
+// mean) in place of the RK4 integration. The rest of the step is the exact
+// one (finish): crash from the probe at the predicted position, progress,
+// reward, respawn, obs rows. A car whose prediction lands below
+// min_clearance of the walls or on another layer takes the exact RK4 step
+// instead (fallback[i] = 1 when given).
+template <int L>
+__global__ void __launch_bounds__(64) k_physics_surrogate(racer_dmap m, float* s, int n, const float* act,
+                                                          int64_t act_cols, float* reward, float* done, float* obs,
+                                                          int64_t obs_cols, int beams, int seed, float heading_noise,
+                                                          const float* __restrict__ pred, int64_t ldp,
+                                                          const float* __restrict__ out_mean,
+                                                          const float* __restrict__ out_std, float min_clearance,
+                                                          float* fallback) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float x = s[RACER_X * n + i], y = s[RACER_Y * n + i], z = s[RACER_Z * n + i];
+    const float h = s[RACER_HEADING * n + i];
+    Spot now;
+    Surf ground;
+    probe_surface<L>(m, x, y, z, &now, &ground);
+    float d[5];
+#pragma unroll
+    for (int o = 0; o < 5; ++o) d[o] = pred[o * ldp + i] * out_std[o] + out_mean[o];
+    const float ch = racer_cos(h), sh = racer_sin(h);
+    float p[4] = {x + (ch * d[0] - sh * d[1]), y + (sh * d[0] + ch * d[1]), h + d[2],
+                  racer_clamp(s[RACER_SPEED * n + i] + d[3], -RACER_SPEED_MAX, RACER_SPEED_MAX)};
+    const float steer = racer_clamp(s[RACER_STEER * n + i] + d[4], -RACER_STEER_MAX, RACER_STEER_MAX);
+    Spot spot;
+    probe_surface<L>(m, p[0], p[1], now.height, &spot, &ground);
+    const bool back = !(spot.clearance >= min_clearance) || spot.layer != now.layer;
+    if (fallback) fallback[i] = back ? 1.0f : 0.0f;
+    if (back) {
+        advance<L>(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, i, seed, heading_noise);
+        return;
+    }
+    const bool crashed = spot.clearance < RACER_HALF_DIAG || ground.sx * ground.sx + ground.sy * ground.sy > 1.0f;
+    finish<L>(m, s, n, reward, done, obs, obs_cols, beams, i, seed, heading_noise, p, steer, spot, ground, crashed);
+}
+
+// ---- float64 RK4 (the surrogate's judge) ----
+// The kinematics of advance (racer/sim.mojo:184-219) in float64 with the
+// true step 1/240 s: probe and bilinear surface on the float grids read as
+// doubles, libm-accurate double sin, cos, tan and sqrt. No crash, progress or
+// respawn: the judge is applied to windows without them.
+__device__ void f64_probe(const racer_dmap& m, double x, double y, double z, int* layer, double* clearance,
+                          double* height) {
+    const double fx = floor((x - (double)m.x0) / (double)m.cell + 0.5);
+    const double fy = floor((y - (double)m.y0) / (double)m.cell + 0.5);
+    const bool on = fx >= 0.0 && fy >= 0.0 && fx < (double)m.nx && fy < (double)m.ny;
+    const int64_t c = on ? (int64_t)fy * m.nx + (int64_t)fx : 0;
+    *layer = 0;
+    *clearance = -1.0;
+    *height = on ? (double)grid_at(m, 0, RACER_HEIGHT, c) : -1.0;
+    double gap = 1e300;
+    for (int l = 0; l < m.layers && on; ++l) {
+        const double cl = grid_at(m, l, RACER_CLEARANCE, c), h = grid_at(m, l, RACER_HEIGHT, c);
+        if (cl > 0.0 && fabs(h - z) < gap) {
+            *layer = l;
+            *clearance = cl;
+            *height = h;
+            gap = fabs(h - z);
+        }
+    }
+}
+
+__device__ void f64_surface(const racer_dmap& m, int layer, double x, double y, double* sx, double* sy) {
+    const double fx = (x - (double)m.x0) / (double)m.cell, fy = (y - (double)m.y0) / (double)m.cell;
+    const double cx = floor(fx), cy = floor(fy);
+    const int64_t ix = !(cx >= 0.0) ? 0 : (cx > (double)(m.nx - 2) ? m.nx - 2 : (int64_t)cx);
+    const int64_t iy = !(cy >= 0.0) ? 0 : (cy > (double)(m.ny - 2) ? m.ny - 2 : (int64_t)cy);
+    const double tx = fmin(fmax(fx - (double)ix, 0.0), 1.0), ty = fmin(fmax(fy - (double)iy, 0.0), 1.0);
+    const int64_t k = iy * m.nx + ix;
+    const double h0 = grid_at(m, layer, RACER_HEIGHT, k), h1 = grid_at(m, layer, RACER_HEIGHT, k + 1);
+    const double h2 = grid_at(m, layer, RACER_HEIGHT, k + m.nx), h3 = grid_at(m, layer, RACER_HEIGHT, k + m.nx + 1);
+    *sx = ((h1 - h0) * (1.0 - ty) + (h3 - h2) * ty) / (double)m.cell;
+    *sy = ((h2 - h0) * (1.0 - tx) + (h3 - h1) * tx) / (double)m.cell;
+}
+
+__device__ void f64_derivative(const double p[4], double steer, double accel, double grip, double along,
+                               double across, double out[4]) {
+    const double G = RACER_G, WB = RACER_WHEELBASE;
+    const double nz = 1.0 / sqrt(1.0 + along * along + across * across);
+    const double load = grip * G * nz;
+    const double v = fmax(fabs(p[3]), 0.5) * (p[3] >= 0.0 ? 1.0 : -1.0);
+    const double slide = G * across * nz;
+    const double lateral = fmin(fmax(v * p[3] * tan(steer) / WB + slide, -load), load);
+    const double longitudinal = sqrt(fmax(load * load - lateral * lateral, 0.0));
+    out[0] = p[3] * cos(p[2]);
+    out[1] = p[3] * sin(p[2]);
+    out[2] = (lateral - slide) / v;
+    out[3] = fmin(fmax(accel, -longitudinal), longitudinal) - G * along * nz;
+}
+
+// st rows (doubles): x, y, z, heading, speed, steer, updated in place.
+__global__ void __launch_bounds__(64) k_step_f64(racer_dmap m, const float* s, int n, double* st, const float* act,
+                                                 int64_t act_cols) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double DT = 1.0 / 240.0;
+    double p[4] = {st[0 * n + i], st[1 * n + i], st[3 * n + i], st[4 * n + i]};
+    double steer = st[5 * n + i];
+    int layer;
+    double clearance, height, sx, sy;
+    f64_probe(m, p[0], p[1], st[2 * n + i], &layer, &clearance, &height);
+    f64_surface(m, layer, p[0], p[1], &sx, &sy);
+    const double rate = fmin(fmax((double)act[i], -1.0), 1.0) * RACER_STEER_RATE;
+    const double accel = fmin(fmax((double)act[act_cols + i], -1.0), 1.0) * RACER_ACCEL_MAX;
+    const double grip = RACER_GRIP * (double)s[RACER_FRICTION * n + i];
+    for (int sub = 0; sub < RACER_SUBSTEPS; ++sub) {
+        const double c = cos(p[2]), sn = sin(p[2]);
+        const double along = sx * c + sy * sn, across = -sx * sn + sy * c;
+        const double half = steer + rate * DT / 2.0, full = steer + rate * DT;
+        double k1[4], k2[4], k3[4], k4[4], q[4];
+        f64_derivative(p, steer, accel, grip, along, across, k1);
+        for (int j = 0; j < 4; ++j) q[j] = p[j] + k1[j] * DT / 2.0;
+        f64_derivative(q, half, accel, grip, along, across, k2);
+        for (int j = 0; j < 4; ++j) q[j] = p[j] + k2[j] * DT / 2.0;
+        f64_derivative(q, half, accel, grip, along, across, k3);
+        for (int j = 0; j < 4; ++j) q[j] = p[j] + k3[j] * DT;
+        f64_derivative(q, full, accel, grip, along, across, k4);
+        for (int j = 0; j < 4; ++j) p[j] += (k1[j] + 2.0 * (k2[j] + k3[j]) + k4[j]) * DT / 6.0;
+        p[3] = fmin(fmax(p[3], -(double)RACER_SPEED_MAX), (double)RACER_SPEED_MAX);
+        steer = fmin(fmax(full, -(double)RACER_STEER_MAX), (double)RACER_STEER_MAX);
+        f64_probe(m, p[0], p[1], height, &layer, &clearance, &height);
+        f64_surface(m, layer, p[0], p[1], &sx, &sy);
+    }
+    const double PI = 3.14159265358979323846;
+    if (p[2] > PI) p[2] -= 2.0 * PI;
+    if (p[2] < -PI) p[2] += 2.0 * PI;
+    st[0 * n + i] = p[0];
+    st[1 * n + i] = p[1];
+    st[2 * n + i] = height;
+    st[3 * n + i] = p[2];
+    st[4 * n + i] = p[3];
+    st[5 * n + i] = steer;
 }
 
 // Sim.reset's spawn kernel (racer/sim.mojo:299-304).
@@ -812,6 +1031,65 @@ int racer_launch_batch_lidar(racer_batch* b) {
         b->last_iterations = (int64_t)count[0];
     }
     return 0;
+}
+
+int racer_launch_surrogate_features(racer_sim* sim, const float* act, int64_t act_cols, float* feat, int64_t ldf,
+                                    const float* mean, const float* inv_std) {
+    const int n = sim->n;
+    auto kernel = sim->map.layers == 1 ? k_surrogate_features<1>
+                                       : (sim->map.layers == 2 ? k_surrogate_features<2> : k_surrogate_features<0>);
+    {
+        racer_timed timer(sim->dev, "surrogate_features");
+        hipLaunchKernelGGL(kernel, dim3((n + 255) / 256), dim3(256), 0, sim->dev->stream, sim->map, sim->state->ptr,
+                           n, act, act_cols, feat, ldf, mean, inv_std);
+    }
+    return racer_launch_check("surrogate features");
+}
+
+int racer_launch_surrogate_targets(racer_sim* sim, const float* prev, float* tgt, int64_t ldt, float* mask) {
+    const int n = sim->n;
+    {
+        racer_timed timer(sim->dev, "surrogate_targets");
+        hipLaunchKernelGGL(k_surrogate_targets, dim3((n + 255) / 256), dim3(256), 0, sim->dev->stream, prev,
+                           sim->state->ptr, sim->done->ptr, n, tgt, ldt, mask);
+    }
+    return racer_launch_check("surrogate targets");
+}
+
+int racer_launch_surrogate_step(racer_sim* sim, const float* pred, int64_t ldp, const float* out_mean,
+                                const float* out_std, float min_clearance, float* fallback) {
+    const int n = sim->n;
+    auto kernel = sim->map.layers == 1 ? k_physics_surrogate<1>
+                                       : (sim->map.layers == 2 ? k_physics_surrogate<2> : k_physics_surrogate<0>);
+    {
+        racer_timed timer(sim->dev, "physics_surrogate");
+        hipLaunchKernelGGL(kernel, dim3((n + 63) / 64), dim3(64), 0, sim->dev->stream, sim->map, sim->state->ptr, n,
+                           sim->actions->ptr, (int64_t)n, sim->reward->ptr, sim->done->ptr, sim->obs->ptr,
+                           (int64_t)n, sim->beams, sim->seed, sim->heading_noise, pred, ldp, out_mean, out_std,
+                           min_clearance, fallback);
+    }
+    if (racer_launch_check("physics surrogate")) return 1;
+    {
+        racer_timed timer(sim->dev, "lidar");
+        launch_lidar_any(sim, sim->obs->ptr, (int64_t)n);
+    }
+    return racer_launch_check("lidar");
+}
+
+extern "C" int racer_sim_step_f64(racer_sim* sim, racer_buf* st, const racer_view* actions_p) {
+    racer_clear_error();
+    RACER_REQUIRE(sim && st && actions_p, "null argument");
+    const int n = sim->n;
+    RACER_REQUIRE(st->len >= (size_t)(12 * n), "the float64 state needs 12 floats (6 doubles) per car");
+    RACER_REQUIRE((uintptr_t)st->ptr % 8 == 0, "the float64 state must be 8-byte aligned");
+    if (racer_check_view(*actions_p, 2, n, "actions")) return 1;
+    {
+        racer_timed timer(sim->dev, "step_f64");
+        hipLaunchKernelGGL(k_step_f64, dim3((n + 63) / 64), dim3(64), 0, sim->dev->stream, sim->map,
+                           sim->state->ptr, n, reinterpret_cast<double*>(st->ptr), view_ptr(*actions_p),
+                           actions_p->cols);
+    }
+    return racer_launch_check("step f64");
 }
 
 int racer_launch_sample_raw(racer_device* dev, int n, const float* out, int64_t out_cols, const float* log_std,
