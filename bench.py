@@ -199,6 +199,104 @@ def batch_leg(R, T, dev, steps: int) -> dict:
             "speedup": separate_ms / batched_ms}
 
 
+def surrogate_leg(R, T, dev, transitions: int, epochs: int, cars: int = 16384) -> dict:
+    """The learned dynamics surrogate on the bank track (0.1 m rise per metre
+    outward): transitions from Runner steps of 16384 cars x 1081 beams with a
+    random-init 2 x 256 bf16 policy, bf16 training of the 3 x 128 residual MLP,
+    then 8-step imagined rollouts of 16384 cars (exact lidar, RK4 fallback below
+    0.1 m clearance or on a layer change) judged by float64 RK4 on the GPU."""
+    import numpy as np
+
+    arr = T.cached("bank10")
+    cfg = R.Config(beams=1081, hidden=256, precision=1, heading_noise=0.1)
+    pol = R.Policy(dev, 42, cfg)
+    sim = R.Sim(dev, arr, cars, 5, cfg)
+    sim.reset()
+    acts = R.Activations(dev, cars, cfg)
+    scratch = dev.alloc(2 * cars)
+    steps = max(1, transitions // cars)
+    data = R.Dataset(dev, steps * cars)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    data.record(sim, pol, acts, scratch, steps, 0)
+    dev.synchronize()
+    record_s = time.perf_counter() - t0
+    sur = R.Surrogate(dev, seed=1, precision=1, cols=cars)
+    sur.fit_normalization(data)
+    loss0 = sur.loss(data)
+    t0 = time.perf_counter()
+    sur.train(data, epochs=epochs, batch=cars, lr=2e-3)
+    dev.synchronize()
+    train_s = time.perf_counter() - t0
+    loss1 = sur.loss(data)
+    minibatches = epochs * (data.count // cars)
+
+    # 8-step windows from the recorded sim's current state, the same actions for every method
+    rng = np.random.default_rng(17)
+    actions = rng.uniform(-1, 1, (8, 2, cars)).astype(np.float32)
+    twins = []
+    for _ in range(2):
+        s = R.Sim(dev, arr, cars, 5, cfg)
+        dev.copy(s.state, 0, sim.state, 0, 11 * cars)
+        dev.copy(s.obs, 0, sim.obs, 0, (cfg.beams + 5) * cars)
+        twins.append(s)
+    imagined, exact = twins
+    start = sim.state.read().reshape(11, cars)
+    st = np.ascontiguousarray(start[[0, 1, 2, 3, 4, 5]].astype(np.float64))
+    judge = dev.upload(st.view(np.float32).ravel())
+    fallback = dev.alloc(cars)
+    fell = np.zeros(cars, bool)
+    respawned = np.zeros(cars, bool)
+    errors = {}
+    for k in range(8):
+        ab = dev.upload(actions[k])
+        imagined.actions.write(actions[k])
+        exact.actions.write(actions[k])
+        sur.step(imagined, 0.1, fallback)
+        exact.step()
+        sim.step_f64(judge, R.mat(ab, 2, cars))
+        fell |= fallback.read() > 0
+        respawned |= (imagined.done.read() != 0) | (exact.done.read() != 0)
+        if k in (0, 7):
+            j = judge.read().view(np.float64).reshape(6, cars)
+            a = imagined.state.read().reshape(11, cars)
+            b = exact.state.read().reshape(11, cars)
+            ok = ~respawned
+            def rms(v):
+                return float(np.sqrt(np.mean(v[ok] ** 2)))
+            errors[f"{k + 1}_step"] = {
+                "cars": int(ok.sum()),
+                "surrogate": {"position_m": rms(np.hypot(a[0] - j[0], a[1] - j[1])),
+                              "heading_rad": rms(np.angle(np.exp(1j * (a[3] - j[3])))),
+                              "speed_mps": rms(a[4] - j[4]), "steer_rad": rms(a[5] - j[5])},
+                "float32_rk4": {"position_m": rms(np.hypot(b[0] - j[0], b[1] - j[1])),
+                                "heading_rad": rms(np.angle(np.exp(1j * (b[3] - j[3])))),
+                                "speed_mps": rms(b[4] - j[4]), "steer_rad": rms(b[5] - j[5])}}
+    # physics time per step at 16384 cars: RK4 kernel against inputs + MLP + surrogate step kernel
+    dev.profile_only(None)
+    dev.profile_reset()
+    dev.profile(True)
+    for k in range(20):
+        exact.actions.write(actions[k % 8])
+        exact.step()
+        imagined.actions.write(actions[k % 8])
+        sur.step(imagined, 0.1, None)
+    dev.synchronize()
+    dev.profile(False)
+    prof = dev.profile_results()
+    def per(names):
+        return sum(prof[k][1] for k in names if k in prof) * 1000.0 / 20
+    rk4_us = per(["physics"])
+    sur_us = per(["surrogate_features", "gemm_bf16_tanh", "gemm_f32", "physics_surrogate"])
+    return {"track": "bank10", "cars": cars, "beams": 1081, "transitions": int(data.count),
+            "record_s": record_s, "train_s": train_s, "epochs": epochs, "minibatch": cars, "adam_steps": minibatches,
+            "network": "11 -> 3 x 128 tanh -> 5, residual (pose change), bf16 MFMA hidden layers",
+            "normalised_loss": {"before": loss0, "after": loss1},
+            "fallback_fraction_8_steps": float(fell.mean()), "error_vs_float64_rk4": errors,
+            "physics_us_per_step": {"rk4_float32": rk4_us, "surrogate": sur_us,
+                                    "speedup": rk4_us / sur_us if sur_us else None}}
+
+
 def plain_baseline(R, dev, track, args, steps: int) -> dict:
     """The same Runner step in the reference's own GPU decomposition, on the
     same GPU: one thread per car for physics, one thread per (car, beam) for the
@@ -242,6 +340,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU decomposition baseline")
+    ap.add_argument("--surrogate-transitions", type=int, default=10_000_000,
+                    help="transitions recorded for the learned dynamics surrogate (0 skips it)")
+    ap.add_argument("--surrogate-epochs", type=int, default=2)
     ap.add_argument("--batch-steps", type=int, default=100,
                     help="timed steps of the eight-instance batched launch (0 skips it)")
     ap.add_argument("--ppo-iterations", type=int, default=3,
@@ -363,6 +464,10 @@ def main() -> int:
     if not args.no_plain:
         plain = plain_baseline(R, dev, arr, args, min(args.steps, 50))
         log("plain GPU baseline:", plain)
+    surrogate = None
+    if args.surrogate_transitions > 0:
+        surrogate = surrogate_leg(R, T, dev, args.surrogate_transitions, args.surrogate_epochs)
+        log("surrogate:", surrogate)
     batched = None
     if args.batch_steps > 0:
         batched = batch_leg(R, T, dev, args.batch_steps)
@@ -395,7 +500,7 @@ def main() -> int:
                                                  "HIP events in eager passes of the same steps right after",
         "roofline": roofline, "policy_roofline": policy, "kernels": kernels, "cpu_baseline": cpu,
         "plain_gpu_baseline": plain, "speedup_vs_plain_gpu": value / plain["value"] if plain else None,
-        "ppo_iteration": ppo, "batch": batched,
+        "ppo_iteration": ppo, "batch": batched, "surrogate": surrogate,
     }
     print(json.dumps(out), flush=True)
     return 0
