@@ -628,23 +628,6 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     const int lo = (global_wave % waves_per_car) * per_wave;
     const int beams = min(total_beams, lo + per_wave) - lo;
     if (beams <= 0) return;
-    // Beams are marched starting nearest the middle of the fan (straight
-    // ahead, the longest ranges) and ending at its edges (the walls beside and
-    // behind, the shortest), so the lanes left idle at the end of the share
-    // wait on short beams. Virtual index v maps to beam lo + order(v).
-    const int centre = (total_beams - 1) / 2 - lo;  // relative position of the middle beam
-    const int mode = centre >= beams - 1 ? 1 : (centre <= 0 ? 0 : 2);  // 1 descending, 0 ascending, 2 outward
-    const int pairs = min(centre, beams - 1 - centre);
-    auto order = [&](int v) -> int {
-        if (mode == 0) return v;
-        if (mode == 1) return beams - 1 - v;
-        if (v <= 2 * pairs) {
-            const int k = (v + 1) >> 1;
-            return (v & 1) ? centre + k : centre - k;
-        }
-        const int extra = v - 2 * pairs;
-        return centre - pairs > 0 ? centre - pairs - extra : centre + pairs + extra;
-    };
     float* ring = lds + wave * (LIDAR_RING + per_wave);
     float* ranges = ring + LIDAR_RING;
     __amdgpu_buffer_rsrc_t rec = __builtin_amdgcn_make_buffer_rsrc(
@@ -660,11 +643,11 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     const float slope15 = 1.5f * m.slope;
     const float half_cell = 0.5f * m.cell;
 
-    // Set-up of virtual beams 64 c .. 64 c + 63, one per lane, into ring slot c % 2.
+    // Set-up of beams 64 c .. 64 c + 63, one per lane, into ring slot c % 2.
     auto setup_chunk = [&](int c) {
         int b = c * 64 + lane;
         if (b < beams) {
-            float2 cs = dirs[lo + order(b)];
+            float2 cs = dirs[lo + b];
             float d0 = cs.x * ch + cs.y * (-sh);
             float d1 = cs.x * sh + cs.y * ch;
             float d2 = cs.x * along + cs.y * across;
@@ -682,11 +665,9 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     };
     float dx = 0.0f, dy = 0.0f, dz = 0.0f, descent = 0.0f, inv_descent = 0.0f;
     float t = 0.0f, last = 0.0f, last_floor = 0.0f;
-    int rel = 0;  // the lane's beam, relative to lo
     auto take = [&](int b) {
         const float* slot = ring + ((b >> 6) & 1) * (LIDAR_RECORD * 64);
         int j = b & 63;
-        rel = order(b);
         dx = slot[0 * 64 + j];
         dy = slot[1 * 64 + j];
         dz = slot[2 * 64 + j];
@@ -736,7 +717,7 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
                     }
                 }
             }
-            if (finished) ranges[rel] = range;
+            if (finished) ranges[b] = range;
         }
         if (COUNT && lane == 0) ++trips;
         const uint64_t done_mask = __ballot(finished);
