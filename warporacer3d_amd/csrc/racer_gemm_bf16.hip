@@ -17,12 +17,10 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;  // LDS row stride in bf16 (80 bytes)
 
-__device__ __forceinline__ uint16_t to_bf16(float f) {
-    uint32_t u = racer_f2u(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// Round to nearest even with gfx950's v_cvt_pk_bf16_f32 (checked against the
+// integer rounding (u + 0x7fff + lsb) >> 16 on all 2^32 patterns: equal for
+// every finite value; a NaN comes out quiet).
+__device__ __forceinline__ uint16_t to_bf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
 // B operand modes: NN reads B[K][N] (k-major rows), NT reads B[N][K].
 template <bool A_BF16, bool B_NT, int EPI>
@@ -118,7 +116,8 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const void* __restrict__ Ap, 
 template <int EPI>
 __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict__ A, int64_t lda,
                                                       const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
-                                                      int64_t ldc, int M, int N, int K) {
+                                                      int64_t ldc, int M, int N, int K, const float* __restrict__ H,
+                                                      int64_t ldh) {
     constexpr int SK = 64, LD = SK + 8;
     __shared__ __attribute__((aligned(16))) uint16_t As[TM][LD];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][LD];
@@ -130,19 +129,21 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
     const int bcol = tid & 63, bkc = (tid >> 6) * 16;  // B: 16 consecutive k of one column
     uint4 ra[2];
     float rb[16];
+    // buffer loads: out-of-range offsets (rows past M, k past K, columns past N) read zeros
+    const auto ares = racer_rsrc(A, (uint64_t)M * lda * 2);
+    const auto bres = racer_rsrc(B, ((uint64_t)(K - 1) * ldb + N) * 4);
     auto load = [&](int k0) {
         const int m = m0 + arow;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int k = k0 + akc + 8 * h;
-            ra[h] = (m < M && k < K) ? *reinterpret_cast<const uint4*>(A + (int64_t)m * lda + k)
-                                     : make_uint4(0, 0, 0, 0);
+            const int k = k0 + akc + 8 * h;  // rows are zero-padded to lda, a multiple of 8
+            ra[h] = racer_ld_b128(ares, (m < M && k < K) ? (uint32_t)(((int64_t)m * lda + k) * 2) : RACER_OOB);
         }
         const int n = n0 + bcol;
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             const int k = k0 + bkc + e;
-            rb[e] = (n < N && k < K) ? B[(int64_t)k * ldb + n] : 0.0f;
+            rb[e] = racer_ld_f32(bres, (n < N && k < K) ? (uint32_t)(((int64_t)k * ldb + n) * 4) : RACER_OOB);
         }
     };
     f32x16 acc;
@@ -175,6 +176,10 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
         if (i < M && j < N) {
             float v = acc[q];
             if constexpr (EPI == RACER_EPI_TANH) v = racer_tanh(v);
+            if constexpr (EPI == RACER_EPI_MASK) {
+                const float h = H[(int64_t)i * ldh + j];
+                v = v * (1.0f - h * h);
+            }
             C[(int64_t)i * ldc + j] = v;
         }
     }
@@ -310,16 +315,22 @@ int racer_pack_bf16(racer_device* dev, const float* src, int64_t ld_src, uint16_
 }
 
 int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const float* B, int64_t ldb, float* C,
-                       int64_t ldc, int M, int N, int K, int epi) {
+                       int64_t ldc, int M, int N, int K, int epi, const float* H, int64_t ldh) {
     if (M <= 0 || N <= 0) return 0;
+    RACER_REQUIRE(lda % 8 == 0 && (uint64_t)M * lda * 2 < 0xFFFFFF00ull &&
+                      ((uint64_t)(K - 1) * ldb + N) * 4 < 0xFFFFFF00ull,
+                  "bf16 product operands exceed the 32-bit offsets of the buffer loads");
     dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, 1), block(256);
-    racer_timed timer(dev, epi == RACER_EPI_TANH ? "gemm_bf16_tanh" : "gemm_bf16");
+    racer_timed timer(dev, epi == RACER_EPI_TANH ? "gemm_bf16_tanh" : (epi == RACER_EPI_MASK ? "gemm_bf16_mask" : "gemm_bf16"));
     if (epi == RACER_EPI_TANH)
         hipLaunchKernelGGL(k_gemm_bf16_nn<RACER_EPI_TANH>, grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc, M, N,
-                           K);
+                           K, H, ldh);
+    else if (epi == RACER_EPI_MASK)
+        hipLaunchKernelGGL(k_gemm_bf16_nn<RACER_EPI_MASK>, grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc, M, N,
+                           K, H, ldh);
     else
         hipLaunchKernelGGL(k_gemm_bf16_nn<RACER_EPI_NONE>, grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc, M, N,
-                           K);
+                           K, H, ldh);
     return racer_launch_check("gemm_nn_bf16");
 }
 

@@ -125,6 +125,8 @@ struct racer_policy {
     uint16_t* theta_bf16 = nullptr;  // bf16 W0 [H, pad32(IN)] then W1 [H, pad32(HA)], zero padded, for MFMA
     int64_t in_pad = 0, ha_pad = 0;
     uint16_t *frag0 = nullptr, *frag1 = nullptr;  // W0, W1 in MFMA fragment order (fused forward, H <= 256)
+    uint16_t* w1t_bf16 = nullptr;  // bf16 w1t [HA, pad8(H)] for the backward's through product
+    int64_t h_pad = 0;
     bool bf16_stale = true;
 };
 
@@ -235,7 +237,7 @@ int racer_gemm_nt_f32(racer_device* dev, const float* A, int64_t lda, const floa
 // bf16 MFMA variants (operands rounded to bf16, float32 accumulation)
 // A is a bf16 copy with rows padded to a multiple of 8 values (lda % 8 == 0, zeros beyond K).
 int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const float* B, int64_t ldb, float* C,
-                       int64_t ldc, int M, int N, int K, int epi);
+                       int64_t ldc, int M, int N, int K, int epi, const float* H = nullptr, int64_t ldh = 0);
 int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
                        int64_t ldc, int M, int N, int K);
 // dst[r, c] = bf16(src[r, c]) for c < cols, 0 for cols <= c < ld_dst.

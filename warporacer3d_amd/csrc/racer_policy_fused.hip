@@ -31,12 +31,9 @@ constexpr int RING = 4;      // k-steps of weight fragments in flight per wavefr
 constexpr int W1_STEPS = 9;  // k-steps of the second product held in registers (H + 1 <= 288)
 constexpr int FRAG_TILES = 16;  // 16-row tiles of a fragment-order weight copy (256 rows, zeros past H)
 
-__device__ __forceinline__ uint16_t bf16_of(float f) {
-    uint32_t u = racer_f2u(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// Round to nearest even with gfx950's v_cvt_pk_bf16_f32 (equal to the integer
+// rounding (u + 0x7fff + lsb) >> 16 for every finite float; a NaN comes out quiet).
+__device__ __forceinline__ uint16_t bf16_of(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
 struct FusedLayout {
     int in32, kx, ha32, ldh, HA;

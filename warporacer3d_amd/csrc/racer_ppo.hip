@@ -340,6 +340,8 @@ extern "C" int racer_policy_create(racer_device* dev, int seed, const racer_conf
     p->in_pad = racer_pad32(p->IN);
     p->ha_pad = racer_pad32(p->HA);
     RACER_TRY(hipMalloc(&p->theta_bf16, (size_t)(H * (p->in_pad + p->ha_pad)) * sizeof(uint16_t)));
+    p->h_pad = racer_pad8(H);
+    RACER_TRY(hipMalloc(&p->w1t_bf16, (size_t)(p->HA * p->h_pad) * sizeof(uint16_t)));
     if (H <= 256) {
         RACER_TRY(hipMalloc(&p->frag0, racer_fragment_elems((int)p->IN) * sizeof(uint16_t)));
         RACER_TRY(hipMalloc(&p->frag1, racer_fragment_elems((int)p->HA) * sizeof(uint16_t)));
@@ -360,6 +362,7 @@ extern "C" void racer_policy_destroy(racer_policy* p) {
     racer_free(p->w1t);
     racer_free(p->w2t);
     if (p->theta_bf16) hipFree(p->theta_bf16);
+    if (p->w1t_bf16) hipFree(p->w1t_bf16);
     if (p->frag0) hipFree(p->frag0);
     if (p->frag1) hipFree(p->frag1);
     delete p;
@@ -402,6 +405,7 @@ static int refresh_bf16(racer_policy* p) {
     if (p->frag0 && (racer_pack_fragments(p->dev, th, p->IN, H, (int)p->IN, p->frag0) ||
                      racer_pack_fragments(p->dev, th + p->W1, p->HA, H, (int)p->HA, p->frag1)))
         return 1;
+    if (racer_pack_bf16(p->dev, p->w1t->ptr, H, p->w1t_bf16, p->h_pad, (int)p->HA, H)) return 1;
     p->bf16_stale = false;
     return 0;
 }
@@ -480,11 +484,20 @@ static int backward_raw(racer_policy* p, const float* x, racer_acts* a, const fl
     int64_t HA = p->HA, IN = p->IN;
     const float* h1 = a->h1->ptr;
     const float* h2 = a->h2->ptr;
-    // through (racer/ppo.mojo:125-138): d2 = (1 - h2^2) (w2t d3); d1 = (1 - h1^2) (w1t d2[:H])
+    // through (racer/ppo.mojo:125-138): d2 = (1 - h2^2) (w2t d3); d1 = (1 - h1^2) (w1t d2[:H]),
+    // the second on MFMA in bf16 precision
     if (racer_gemm_nn_f32(dev, p->w2t->ptr, RACER_OUT, d3, cols, d2, cols, (int)HA, (int)cols, RACER_OUT,
-                          RACER_EPI_MASK, h2, cols) ||
-        racer_gemm_nn_f32(dev, p->w1t->ptr, H, d2, cols, d1, cols, (int)HA, (int)cols, H, RACER_EPI_MASK, h1, cols))
+                          RACER_EPI_MASK, h2, cols))
         return 1;
+    if (p->precision == 1) {
+        if (refresh_bf16(p) ||
+            racer_gemm_nn_bf16(dev, p->w1t_bf16, p->h_pad, d2, cols, d1, cols, (int)HA, (int)cols, H, RACER_EPI_MASK,
+                               h1, cols))
+            return 1;
+    } else if (racer_gemm_nn_f32(dev, p->w1t->ptr, H, d2, cols, d1, cols, (int)HA, (int)cols, H, RACER_EPI_MASK, h1,
+                                 cols)) {
+        return 1;
+    }
     // weight gradients (racer/ppo.mojo:121-123)
     if (p->precision == 1) {
         if (racer_gemm_nt_bf16(dev, d1, cols, x, cols, grad, IN, H, (int)IN, (int)cols) ||
