@@ -272,28 +272,35 @@ def surrogate_leg(R, T, dev, transitions: int, epochs: int, cars: int = 16384) -
                 "float32_rk4": {"position_m": rms(np.hypot(b[0] - j[0], b[1] - j[1])),
                                 "heading_rad": rms(np.angle(np.exp(1j * (b[3] - j[3])))),
                                 "speed_mps": rms(b[4] - j[4]), "steer_rad": rms(b[5] - j[5])}}
-    # physics time per step at 16384 cars: RK4 kernel against inputs + MLP + surrogate step kernel
-    dev.profile_only(None)
-    dev.profile_reset()
-    dev.profile(True)
-    for k in range(20):
-        exact.actions.write(actions[k % 8])
-        exact.step()
-        imagined.actions.write(actions[k % 8])
-        sur.step(imagined, 0.1, None)
-    dev.synchronize()
-    dev.profile(False)
-    prof = dev.profile_results()
-    def per(names):
+    # physics time per step at 16384 cars: the RK4 kernel against the fused surrogate step
+    # (inputs, MLP and pose update in one launch) and against its separate kernels
+    def physics_us(fused: bool) -> dict:
+        sur.set_fused(fused)
+        dev.profile_only(None)
+        dev.profile_reset()
+        dev.profile(True)
+        for k in range(20):
+            exact.actions.write(actions[k % 8])
+            exact.step()
+            imagined.actions.write(actions[k % 8])
+            sur.step(imagined, 0.1, None)
+        dev.synchronize()
+        dev.profile(False)
+        return dev.profile_results()
+    def per(prof, names):
         return sum(prof[k][1] for k in names if k in prof) * 1000.0 / 20
-    rk4_us = per(["physics"])
-    sur_us = per(["surrogate_features", "gemm_bf16_tanh", "gemm_f32", "physics_surrogate"])
+    separate = physics_us(False)
+    fused = physics_us(True)
+    rk4_us = per(fused, ["physics"])
+    sur_us = per(fused, ["surrogate_fused"])
+    sep_us = per(separate, ["surrogate_features", "gemm_bf16_tanh", "gemm_f32", "physics_surrogate"])
     return {"track": "bank10", "cars": cars, "beams": 1081, "transitions": int(data.count),
             "record_s": record_s, "train_s": train_s, "epochs": epochs, "minibatch": cars, "adam_steps": minibatches,
             "network": "11 -> 3 x 128 tanh -> 5, residual (pose change), bf16 MFMA hidden layers",
             "normalised_loss": {"before": loss0, "after": loss1},
             "fallback_fraction_8_steps": float(fell.mean()), "error_vs_float64_rk4": errors,
-            "physics_us_per_step": {"rk4_float32": rk4_us, "surrogate": sur_us,
+            "physics_us_per_step": {"rk4_float32": rk4_us, "surrogate_fused": sur_us,
+                                    "surrogate_separate_kernels": sep_us,
                                     "speedup": rk4_us / sur_us if sur_us else None}}
 
 

@@ -250,6 +250,9 @@ int racer_surrogate_loss(racer_surrogate* s, racer_dataset* d, double* loss);
  * the walls or on another layer take the exact step (fallback[i] = 1). The
  * crash test, progress, reward, respawn and lidar are the exact ones. */
 int racer_surrogate_step(racer_surrogate* s, racer_sim* sim, float min_clearance, racer_buf* fallback);
+/* bf16 surrogates: imagined steps as one fused launch (1, the default) or as
+ * separate input, product and step kernels (0); the results are the same. */
+int racer_surrogate_set_fused(racer_surrogate* s, int fused);
 /* One step of the kinematics (racer/sim.mojo:184-219) in float64 on the GPU
  * (the surrogate's judge): st holds [6, n] doubles (x, y, z, heading, speed,
  * steer) in 12 n floats and is updated in place; friction comes from the sim's

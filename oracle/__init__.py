@@ -49,6 +49,7 @@ def lib():
             "o_exp": (F, [F]),
             "o_log": (F, [F]),
             "o_tanh": (F, [F]),
+            "o_tanh_mlp": (F, [F]),
             "o_philox": (None, [P, P, P]),
             "o_normal4": (None, [C.c_uint64, C.c_uint64, C.c_uint64, P]),
             "o_uniform4": (None, [C.c_uint64, C.c_uint64, C.c_uint64, P]),

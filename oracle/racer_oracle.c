@@ -548,6 +548,7 @@ float o_cos(float x) { return racer_cos(x); }
 float o_exp(float x) { return racer_exp(x); }
 float o_log(float x) { return racer_log(x); }
 float o_tanh(float x) { return racer_tanh(x); }
+float o_tanh_mlp(float x) { return racer_tanh_mlp(x); }
 void o_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
     racer_u32x4 r = racer_philox(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
     for (int j = 0; j < 4; ++j) out[j] = r.v[j];

@@ -144,6 +144,7 @@ float o_cos(float x);
 float o_exp(float x);
 float o_log(float x);
 float o_tanh(float x);
+float o_tanh_mlp(float x); /* the bf16 surrogate's tanh with libm exp2f and a division */
 void o_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void o_normal4(uint64_t seed, uint64_t subsequence, uint64_t offset, float out[4]);
 void o_uniform4(uint64_t seed, uint64_t subsequence, uint64_t offset, float out[4]);

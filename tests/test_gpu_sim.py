@@ -187,3 +187,25 @@ def test_autotune_keeps_results(dev):
     g.step()
     o.step()
     _assert_same(g, o, "step after autotune")
+
+
+def test_objects_outlive_the_device_handle():
+    """Destroying the device handle first (a garbage collector may finalise a
+    cycle in any order) leaves the objects made on it working; the device goes
+    with the last of them."""
+    print("device handle destroyed before its objects", flush=True)
+    d = R.Device(True)
+    cfg = R.Config(beams=32, hidden=16, heading_noise=0.1)
+    sim = R.Sim(d, track("ramp"), 64, 3, cfg)
+    sim.reset()
+    sur = R.Surrogate(d, precision=1, cols=64)
+    buf = d.alloc(64)
+    R.lib().racer_device_destroy(d.handle)
+    d.handle = None
+    sim.step()
+    sur.step(sim, 0.1, buf)
+    state = sim.state.read()
+    assert np.all(np.isfinite(state))
+    del sur, buf
+    sim.step()
+    del sim

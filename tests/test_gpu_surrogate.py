@@ -107,6 +107,40 @@ def test_all_fallback_is_the_exact_step(dev):
         assert np.all(fallback.read() == 1.0)
 
 
+@pytest.mark.parametrize("name,n", [("bank10", 1000), ("bridge", 777)])
+def test_fused_step_matches_separate_kernels(dev, name, n):
+    """The one-launch bf16 step against the separate inputs, bf16 products,
+    float32 output product and step kernels: the same bits, over ragged car
+    counts, both map kinds, cars served by the network and cars that fall back."""
+    print(f"surrogate fused vs separate {name} n={n}", flush=True)
+    _, sim, pol, acts, scratch = _setup(dev, n=n, name=name)
+    data = R.Dataset(dev, 24 * n)
+    data.record(sim, pol, acts, scratch, 24, 0)
+    sur = R.Surrogate(dev, seed=2, precision=1, cols=n)
+    sur.fit_normalization(data)
+    sur.train(data, epochs=1, batch=n, lr=2e-3)
+    _, a, _, _, _ = _setup(dev, n=n, name=name, seed=8)
+    _, b, _, _, _ = _setup(dev, n=n, name=name, seed=8)
+    fa, fb = dev.alloc(n), dev.alloc(n)
+    rng = np.random.default_rng(5)
+    served = []
+    for t, clearance in enumerate((0.1, 1.0, 2.0, 4.0)):  # from mostly served to mostly fallen back
+        act = rng.uniform(-1.2, 1.2, (2, n)).astype(np.float32)
+        a.actions.write(act)
+        b.actions.write(act)
+        sur.set_fused(True)
+        sur.step(a, min_clearance=clearance, fallback=fa)
+        sur.set_fused(False)
+        sur.step(b, min_clearance=clearance, fallback=fb)
+        for buf in ("state", "obs", "reward", "done"):
+            assert np.array_equal(getattr(a, buf).read(), getattr(b, buf).read()), (t, buf)
+        assert np.array_equal(fa.read(), fb.read()), t
+        served.append(float((fa.read() == 0).mean()))
+    sur.set_fused(True)
+    print(f"served by the network per step: {served}", flush=True)
+    assert max(served) > 0.5 and min(served) < 0.9, served
+
+
 def _one_step_error(dev, sur, sim, n, rng):
     """Position error (m) of one surrogate step against the float64 judge over
     cars that neither fall back nor crash."""

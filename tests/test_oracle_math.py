@@ -44,6 +44,18 @@ def test_elementary_accuracy(name, ref, xs, bound):
     assert _max_ulps(name, ref, xs) <= bound
 
 
+def test_surrogate_tanh_accuracy():
+    """The bf16 surrogate's branch-free tanh (host form: libm exp2f, a true
+    division; the GPU form swaps in its approximate exp2 and reciprocal, about
+    an ulp each): within a few ulp of tanh, the right limits and NaN."""
+    xs = np.concatenate([RNG.uniform(-10, 10, 20000), RNG.uniform(-0.6, 0.6, 20000),
+                         RNG.uniform(0.5, 0.6, 5000), 10.0 ** RNG.uniform(-30, -1, 2000)]).astype(np.float32)
+    assert _max_ulps("tanh_mlp", np.tanh, xs) <= 4.0
+    f = O.elementary("tanh_mlp")
+    assert f(60.0) == 1.0 and f(-60.0) == -1.0 and f(float("inf")) == 1.0 and f(0.0) == 0.0
+    assert np.isnan(f(float("nan")))
+
+
 def test_special_values():
     assert O.elementary("sin")(0.0) == 0.0 and O.elementary("cos")(0.0) == 1.0
     assert O.elementary("exp")(0.0) == 1.0 and O.elementary("log")(1.0) == 0.0

@@ -171,6 +171,7 @@ def lib() -> C.CDLL:
         "racer_surrogate_train": (i, [vp, vp, i, i64, f]),
         "racer_surrogate_loss": (i, [vp, vp, C.POINTER(C.c_double)]),
         "racer_surrogate_step": (i, [vp, vp, f, vp]),
+        "racer_surrogate_set_fused": (i, [vp, i]),
         "racer_sim_step_f64": (i, [vp, vp, VP]),
         "racer_surrogate_runner_step": (i, [vp, vp, vp, vp, vp, i, i64, f, vp]),
         "racer_batch_create": (i, [vp, C.POINTER(C.c_void_p), i, C.POINTER(Config), cp]),
@@ -696,6 +697,10 @@ class Surrogate:
         v = C.c_double()
         _check(lib().racer_surrogate_loss(self.handle, data.handle, C.byref(v)))
         return float(v.value)
+
+    def set_fused(self, fused: bool) -> None:
+        """bf16 imagined steps as one fused launch (default) or as separate kernels (same results)."""
+        _check(lib().racer_surrogate_set_fused(self.handle, int(fused)))
 
     def step(self, sim: "Sim", min_clearance: float = 0.1, fallback: Buffer | None = None) -> None:
         _check(lib().racer_surrogate_step(self.handle, sim.handle, float(min_clearance),

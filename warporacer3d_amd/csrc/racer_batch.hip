@@ -26,6 +26,7 @@ extern "C" int racer_batch_create(racer_device* dev, racer_sim** sims, int count
     }
     auto* b = new racer_batch();
     b->dev = dev;
+    racer_device_retain(dev);
     b->beams = sims[0]->beams;
     b->hidden = cfg->hidden;
     b->car_start.push_back(0);
@@ -59,7 +60,9 @@ extern "C" void racer_batch_destroy(racer_batch* b) {
     for (auto* a : b->acts) racer_acts_destroy(a);
     racer_free(b->scratch);
     if (b->counter) hipFree(b->counter);
+    racer_device* dev = b->dev;
     delete b;
+    racer_device_release(dev);
 }
 
 extern "C" int racer_batch_count(const racer_batch* b) { return (int)b->sims.size(); }

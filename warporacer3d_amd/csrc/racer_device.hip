@@ -74,8 +74,12 @@ extern "C" int racer_device_create(int gpu, racer_device** out) {
     return 0;
 }
 
-extern "C" void racer_device_destroy(racer_device* dev) {
-    if (!dev) return;
+void racer_device_retain(racer_device* dev) {
+    if (dev) ++dev->refs;
+}
+
+void racer_device_release(racer_device* dev) {
+    if (!dev || --dev->refs > 0) return;
     hipStreamSynchronize(dev->stream);
     for (auto& p : dev->pending) {
         hipEventDestroy(p.start);
@@ -88,6 +92,8 @@ extern "C" void racer_device_destroy(racer_device* dev) {
     hipStreamDestroy(dev->stream);
     delete dev;
 }
+
+extern "C" void racer_device_destroy(racer_device* dev) { racer_device_release(dev); }
 
 extern "C" const char* racer_device_name(racer_device* dev) { return dev ? dev->name.c_str() : ""; }
 
@@ -140,11 +146,13 @@ extern "C" int racer_alloc(racer_device* dev, size_t count, float fill, racer_bu
     RACER_REQUIRE(dev, "null device");
     auto* b = new racer_buf();
     b->dev = dev;
+    racer_device_retain(dev);
     b->len = count;
     if (count) {
         hipError_t e = hipMalloc(&b->ptr, count * sizeof(float));
         if (e != hipSuccess) {
             delete b;
+            racer_device_release(dev);
             racer_set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
             return 1;
         }
@@ -161,7 +169,9 @@ extern "C" void racer_free(racer_buf* buf) {
         hipStreamSynchronize(buf->dev->stream);
         hipFree(buf->ptr);
     }
+    racer_device* dev = buf->dev;
     delete buf;
+    racer_device_release(dev);
 }
 
 extern "C" size_t racer_buf_len(const racer_buf* buf) { return buf ? buf->len : 0; }

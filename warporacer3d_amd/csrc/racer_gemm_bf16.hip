@@ -176,6 +176,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
         if (i < M && j < N) {
             float v = acc[q];
             if constexpr (EPI == RACER_EPI_TANH) v = racer_tanh(v);
+            if constexpr (EPI == RACER_EPI_TANH_MLP) v = racer_tanh_mlp(v);
             if constexpr (EPI == RACER_EPI_MASK) {
                 const float h = H[(int64_t)i * ldh + j];
                 v = v * (1.0f - h * h);
@@ -321,10 +322,15 @@ int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const 
                       ((uint64_t)(K - 1) * ldb + N) * 4 < 0xFFFFFF00ull,
                   "bf16 product operands exceed the 32-bit offsets of the buffer loads");
     dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, 1), block(256);
-    racer_timed timer(dev, epi == RACER_EPI_TANH ? "gemm_bf16_tanh" : (epi == RACER_EPI_MASK ? "gemm_bf16_mask" : "gemm_bf16"));
+    racer_timed timer(dev, epi == RACER_EPI_TANH || epi == RACER_EPI_TANH_MLP
+                               ? "gemm_bf16_tanh"
+                               : (epi == RACER_EPI_MASK ? "gemm_bf16_mask" : "gemm_bf16"));
     if (epi == RACER_EPI_TANH)
         hipLaunchKernelGGL(k_gemm_bf16_nn<RACER_EPI_TANH>, grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc, M, N,
                            K, H, ldh);
+    else if (epi == RACER_EPI_TANH_MLP)
+        hipLaunchKernelGGL(k_gemm_bf16_nn<RACER_EPI_TANH_MLP>, grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc, M,
+                           N, K, H, ldh);
     else if (epi == RACER_EPI_MASK)
         hipLaunchKernelGGL(k_gemm_bf16_nn<RACER_EPI_MASK>, grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc, M, N,
                            K, H, ldh);

@@ -73,6 +73,7 @@ extern "C" int racer_graph_runner(racer_sim* sim, racer_policy* pol, racer_acts*
     RACER_REQUIRE(!sim->count_iterations, "turn the lidar iteration counter off before capturing");
     auto* g = new racer_graph();
     g->dev = sim->dev;
+    racer_device_retain(g->dev);
     g->policy = pol;
     int rc = capture(sim->dev, pol, g, [&]() -> int {
         for (int t = 0; t < steps; ++t)
@@ -96,6 +97,7 @@ extern "C" int racer_graph_batch(racer_batch* b, racer_policy* pol, int steps, i
     RACER_REQUIRE(!b->count_iterations, "turn the lidar iteration counter off before capturing");
     auto* g = new racer_graph();
     g->dev = b->dev;
+    racer_device_retain(g->dev);
     g->policy = pol;
     int rc = capture(b->dev, pol, g, [&]() -> int {
         for (int t = 0; t < steps; ++t)
@@ -118,6 +120,7 @@ extern "C" int racer_graph_rollout(racer_trainer* tr, racer_sim* sim, racer_poli
     RACER_REQUIRE(!sim->count_iterations, "turn the lidar iteration counter off before capturing");
     auto* g = new racer_graph();
     g->dev = tr->dev;
+    racer_device_retain(g->dev);
     g->policy = pol;
     g->trainer = tr;
     g->rollout = tr->rollout;
@@ -150,5 +153,7 @@ extern "C" void racer_graph_destroy(racer_graph* g) {
     if (!g) return;
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
+    racer_device* dev = g->dev;
     delete g;
+    racer_device_release(dev);
 }

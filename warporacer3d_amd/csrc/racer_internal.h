@@ -65,7 +65,14 @@ struct racer_device {
     bool capturing = false;
     const int64_t* step_base = nullptr;
     int64_t* graph_base = nullptr;
+    // Owners: the handle's creator plus every object made on the device. The
+    // device is freed when the last of them lets go, so objects may be
+    // destroyed in any order (a garbage collector finalises cycles in none).
+    int refs = 1;
 };
+
+void racer_device_retain(racer_device* dev);
+void racer_device_release(racer_device* dev);
 
 struct racer_buf {
     racer_device* dev = nullptr;
@@ -215,6 +222,16 @@ int racer_launch_surrogate_targets(racer_sim* sim, const float* prev, float* tgt
 // Sim.step with the surrogate's predicted pose change (RK4 fallback per car), then the lidar.
 int racer_launch_surrogate_step(racer_sim* sim, const float* pred, int64_t ldp, const float* out_mean,
                                 const float* out_std, float min_clearance, float* fallback);
+// The bf16 surrogate's hidden layers packed for the fused step: 76 blocks of
+// 512 bf16 (one v_mfma_f32_32x32x16_bf16 A operand each), layer 0 as 4 row
+// tiles x 1 k-step (K = 12 -> 16), layers 1 and 2 as 4 x 9 (K = 129 -> 144).
+constexpr int RACER_SF_STEPS0 = 1, RACER_SF_STEPS = 9;
+constexpr int RACER_SF_BASE1 = 4, RACER_SF_BASE2 = 40, RACER_SF_BLOCKS = 76;
+// One fused launch of the bf16 surrogate step (inputs, 3 MFMA layers, the
+// float32 output layer, the pose update with RK4 fallback), then the lidar.
+// w3: the output layer [5, 129]; stats: the normalisation (37 floats).
+int racer_launch_surrogate_fused(racer_sim* sim, const uint16_t* frags, const float* w3, const float* stats,
+                                 float min_clearance, float* fallback);
 
 // ---------------------------------------------------------------- kernels (launchers)
 int racer_launch_reset(racer_sim* sim);
@@ -226,7 +243,8 @@ int racer_launch_sample_raw(racer_device* dev, int n, const float* out, int64_t 
                             int64_t step, int seed);
 
 // GEMM epilogues
-enum { RACER_EPI_NONE = 0, RACER_EPI_TANH = 1, RACER_EPI_MASK = 2 };
+// TANH_MLP: racer_tanh_mlp (the bf16 surrogate's hidden layers), bf16 NN products only
+enum { RACER_EPI_NONE = 0, RACER_EPI_TANH = 1, RACER_EPI_MASK = 2, RACER_EPI_TANH_MLP = 3 };
 // C[M, N] = epi(A[M, K] B[K, N]); row-major with leading dimensions; mask epilogue
 // multiplies by (1 - h^2) with h = H[m, n].
 int racer_gemm_nn_f32(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,

@@ -286,6 +286,7 @@ extern "C" int racer_acts_create(racer_device* dev, int cols, const racer_config
     RACER_REQUIRE(cols > 0, "activations need at least one column");
     auto* a = new racer_acts();
     a->dev = dev;
+    racer_device_retain(dev);
     a->cols = cols;
     a->hidden = cfg->hidden;
     size_t HA = (size_t)cfg->hidden + 1;
@@ -303,7 +304,9 @@ extern "C" void racer_acts_destroy(racer_acts* a) {
     racer_free(a->h1);
     racer_free(a->h2);
     racer_free(a->o);
+    racer_device* dev = a->dev;
     delete a;
+    racer_device_release(dev);
 }
 
 extern "C" racer_buf* racer_acts_h1(racer_acts* a) { return a->h1; }
@@ -322,6 +325,7 @@ extern "C" int racer_policy_create(racer_device* dev, int seed, const racer_conf
     RACER_REQUIRE(cfg->precision >= 0 && cfg->precision <= 2, "precision must be 0 (float32), 1 (bf16) or 2 (plain)");
     auto* p = new racer_policy();
     p->dev = dev;
+    racer_device_retain(dev);
     p->beams = cfg->beams;
     p->hidden = cfg->hidden;
     p->precision = cfg->precision;
@@ -365,7 +369,9 @@ extern "C" void racer_policy_destroy(racer_policy* p) {
     if (p->w1t_bf16) hipFree(p->w1t_bf16);
     if (p->frag0) hipFree(p->frag0);
     if (p->frag1) hipFree(p->frag1);
+    racer_device* dev = p->dev;
     delete p;
+    racer_device_release(dev);
 }
 
 extern "C" racer_buf* racer_policy_theta(racer_policy* p) { return p->theta; }
@@ -536,6 +542,7 @@ extern "C" int racer_trainer_create(racer_device* dev, int n, const racer_config
                   "cars x rollout must be a multiple of the minibatch count");
     auto* tr = new racer_trainer();
     tr->dev = dev;
+    racer_device_retain(dev);
     tr->n = n;
     tr->beams = cfg->beams;
     tr->hidden = cfg->hidden;
@@ -577,7 +584,9 @@ extern "C" void racer_trainer_destroy(racer_trainer* tr) {
     for (auto* b : bufs) racer_free(b);
     racer_acts_destroy(tr->acts);
     if (tr->dstats) hipFree(tr->dstats);
+    racer_device* dev = tr->dev;
     delete tr;
+    racer_device_release(dev);
 }
 
 static int check_compatible(racer_trainer* tr, racer_policy* p) {

@@ -302,11 +302,46 @@ __global__ void __launch_bounds__(64) k_physics(racer_dmap m, float* s, int n, c
 }
 
 // ---- learned dynamics surrogate (no reference counterpart) ----
-// Inputs of the surrogate for car i (rows of feat, ldf apart): x, y, z, yaw,
-// speed, yaw rate v tan(steer) / wheelbase, steer, pitch and roll (the road's
-// rise per metre ahead and to the left under the car), throttle and steer
-// command (the clamped actions); row 11 is the constant 1 that carries the
-// first layer's bias. With mean / inv_std the 11 inputs are normalised.
+// The state a surrogate step starts from: x, y, z, heading, speed, steer and
+// the probe at the car (layer, clearance, height).
+struct SurStart {
+    float x, y, z, h, speed, steer;
+    Spot now;
+};
+
+// Inputs of the surrogate for car i: x, y, z, yaw, speed, yaw rate
+// v tan(steer) / wheelbase, steer, pitch and roll (the road's rise per metre
+// ahead and to the left under the car), throttle and steer command (the
+// clamped actions). Also returns the start state for surrogate_apply.
+template <int L>
+__device__ __forceinline__ void surrogate_inputs(const racer_dmap& m, const float* __restrict__ s, int n,
+                                                 const float* __restrict__ act, int64_t act_cols, int i,
+                                                 float f[11], SurStart* c) {
+    c->x = s[RACER_X * n + i];
+    c->y = s[RACER_Y * n + i];
+    c->z = s[RACER_Z * n + i];
+    c->h = s[RACER_HEADING * n + i];
+    c->speed = s[RACER_SPEED * n + i];
+    c->steer = s[RACER_STEER * n + i];
+    Surf ground;
+    probe_surface<L>(m, c->x, c->y, c->z, &c->now, &ground);
+    float along, across;
+    slopes(ground, c->h, &along, &across);
+    f[0] = c->x;
+    f[1] = c->y;
+    f[2] = c->z;
+    f[3] = c->h;
+    f[4] = c->speed;
+    f[5] = ((c->speed * racer_sin(c->steer)) / racer_cos(c->steer)) / RACER_WHEELBASE;
+    f[6] = c->steer;
+    f[7] = along;
+    f[8] = across;
+    f[9] = racer_clamp(act[act_cols + i], -1.0f, 1.0f);
+    f[10] = racer_clamp(act[i], -1.0f, 1.0f);
+}
+
+// The inputs as rows of feat (ldf apart), normalised with mean / inv_std when
+// given; row 11 is the constant 1 that carries the first layer's bias.
 template <int L>
 __global__ void __launch_bounds__(256) k_surrogate_features(racer_dmap m, const float* __restrict__ s, int n,
                                                             const float* __restrict__ act, int64_t act_cols,
@@ -315,24 +350,9 @@ __global__ void __launch_bounds__(256) k_surrogate_features(racer_dmap m, const 
                                                             const float* __restrict__ inv_std) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float x = s[RACER_X * n + i], y = s[RACER_Y * n + i], z = s[RACER_Z * n + i];
-    const float heading = s[RACER_HEADING * n + i], speed = s[RACER_SPEED * n + i], steer = s[RACER_STEER * n + i];
-    Spot spot;
-    Surf ground;
-    probe_surface<L>(m, x, y, z, &spot, &ground);
-    float along, across;
-    slopes(ground, heading, &along, &across);
-    float f[11] = {x,
-                   y,
-                   z,
-                   heading,
-                   speed,
-                   ((speed * racer_sin(steer)) / racer_cos(steer)) / RACER_WHEELBASE,
-                   steer,
-                   along,
-                   across,
-                   racer_clamp(act[act_cols + i], -1.0f, 1.0f),
-                   racer_clamp(act[i], -1.0f, 1.0f)};
+    float f[11];
+    SurStart c;
+    surrogate_inputs<L>(m, s, n, act, act_cols, i, f, &c);
 #pragma unroll
     for (int r = 0; r < 11; ++r) {
         const float v = mean ? (f[r] - mean[r]) * inv_std[r] : f[r];
@@ -366,13 +386,42 @@ __global__ void __launch_bounds__(256) k_surrogate_targets(const float* __restri
     mask[i] = done[i] == 0.0f ? 1.0f : 0.0f;
 }
 
-// advance with the pose change predicted by the surrogate (pred rows:
+// advance from start c with the pose change predicted by the surrogate (pred:
 // normalised forward, left, heading, speed, steer changes; out = pred std +
 // mean) in place of the RK4 integration. The rest of the step is the exact
 // one (finish): crash from the probe at the predicted position, progress,
 // reward, respawn, obs rows. A car whose prediction lands below
 // min_clearance of the walls or on another layer takes the exact RK4 step
 // instead (fallback[i] = 1 when given).
+template <int L>
+__device__ __forceinline__ void surrogate_apply(const racer_dmap& m, float* s, int n, const float* act,
+                                                int64_t act_cols, float* reward, float* done, float* obs,
+                                                int64_t obs_cols, int beams, int seed, float heading_noise,
+                                                const float pred[5], const SurStart& c,
+                                                const float* __restrict__ out_mean,
+                                                const float* __restrict__ out_std, float min_clearance,
+                                                float* fallback, int i) {
+    float d[5];
+#pragma unroll
+    for (int o = 0; o < 5; ++o) d[o] = pred[o] * out_std[o] + out_mean[o];
+    const float ch = racer_cos(c.h), sh = racer_sin(c.h);
+    float p[4] = {c.x + (ch * d[0] - sh * d[1]), c.y + (sh * d[0] + ch * d[1]), c.h + d[2],
+                  racer_clamp(c.speed + d[3], -RACER_SPEED_MAX, RACER_SPEED_MAX)};
+    const float steer = racer_clamp(c.steer + d[4], -RACER_STEER_MAX, RACER_STEER_MAX);
+    Spot spot;
+    Surf ground;
+    probe_surface<L>(m, p[0], p[1], c.now.height, &spot, &ground);
+    const bool back = !(spot.clearance >= min_clearance) || spot.layer != c.now.layer;
+    if (fallback) fallback[i] = back ? 1.0f : 0.0f;
+    if (back) {
+        advance<L>(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, i, seed, heading_noise);
+        return;
+    }
+    const bool crashed = spot.clearance < RACER_HALF_DIAG || ground.sx * ground.sx + ground.sy * ground.sy > 1.0f;
+    finish<L>(m, s, n, reward, done, obs, obs_cols, beams, i, seed, heading_noise, p, steer, spot, ground, crashed);
+}
+
+// The surrogate step with the predictions of a separate forward (pred rows ldp apart).
 template <int L>
 __global__ void __launch_bounds__(64) k_physics_surrogate(racer_dmap m, float* s, int n, const float* act,
                                                           int64_t act_cols, float* reward, float* done, float* obs,
@@ -383,28 +432,128 @@ __global__ void __launch_bounds__(64) k_physics_surrogate(racer_dmap m, float* s
                                                           float* fallback) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float x = s[RACER_X * n + i], y = s[RACER_Y * n + i], z = s[RACER_Z * n + i];
-    const float h = s[RACER_HEADING * n + i];
-    Spot now;
-    Surf ground;
-    probe_surface<L>(m, x, y, z, &now, &ground);
-    float d[5];
+    float q[5];
 #pragma unroll
-    for (int o = 0; o < 5; ++o) d[o] = pred[o * ldp + i] * out_std[o] + out_mean[o];
-    const float ch = racer_cos(h), sh = racer_sin(h);
-    float p[4] = {x + (ch * d[0] - sh * d[1]), y + (sh * d[0] + ch * d[1]), h + d[2],
-                  racer_clamp(s[RACER_SPEED * n + i] + d[3], -RACER_SPEED_MAX, RACER_SPEED_MAX)};
-    const float steer = racer_clamp(s[RACER_STEER * n + i] + d[4], -RACER_STEER_MAX, RACER_STEER_MAX);
-    Spot spot;
-    probe_surface<L>(m, p[0], p[1], now.height, &spot, &ground);
-    const bool back = !(spot.clearance >= min_clearance) || spot.layer != now.layer;
-    if (fallback) fallback[i] = back ? 1.0f : 0.0f;
-    if (back) {
-        advance<L>(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, i, seed, heading_noise);
-        return;
+    for (int o = 0; o < 5; ++o) q[o] = pred[o * ldp + i];
+    SurStart c{s[RACER_X * n + i], s[RACER_Y * n + i], s[RACER_Z * n + i], s[RACER_HEADING * n + i],
+               s[RACER_SPEED * n + i], s[RACER_STEER * n + i], Spot{}};
+    Surf ground;
+    probe_surface<L>(m, c.x, c.y, c.z, &c.now, &ground);
+    surrogate_apply<L>(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, seed, heading_noise, q, c,
+                       out_mean, out_std, min_clearance, fallback, i);
+}
+
+// The whole bf16 surrogate step for 32 cars per workgroup in one launch:
+// inputs (wavefront 0, lanes 0 .. 31, one car each, which keep the start
+// state in registers for the pose update), the three 128-wide tanh layers on
+// v_mfma_f32_32x32x16_bf16 with wavefront w owning rows 32 w .. + 31 of each
+// layer (weights from the fragment-ordered copy, loaded ahead under the
+// previous phase; activations in LDS as bf16 [car][k]), the float32 output
+// layer with k in order (output o on wavefront o % 4, lanes 32 o / 4 ..),
+// then the pose update (wavefront 0). The operands, instruction and k order
+// of each product are those of the separate bf16 and float32 products, so the
+// step is the same as the unfused one: the only difference is the unfused
+// products' trailing all-zero k-steps, which change an accumulator only when
+// it is exactly -0.
+constexpr int SF_CARS = 32, SF_LDX = 152;  // bf16 row stride: 144 k + 8 (conflict-free 16-byte reads)
+
+template <int L>
+__global__ void __launch_bounds__(256) k_surrogate_fused(racer_dmap m, float* s, int n, const float* act,
+                                                         int64_t act_cols, float* reward, float* done, float* obs,
+                                                         int64_t obs_cols, int beams, int seed, float heading_noise,
+                                                         const uint16_t* __restrict__ frags,
+                                                         const float* __restrict__ w3,
+                                                         const float* __restrict__ st, float min_clearance,
+                                                         float* fallback) {
+    typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+    typedef float f32x16 __attribute__((ext_vector_type(16)));
+    __shared__ __attribute__((aligned(16))) uint16_t xa[SF_CARS][SF_LDX];
+    __shared__ __attribute__((aligned(16))) uint16_t xb[SF_CARS][SF_LDX];
+    __shared__ float h3[129][SF_CARS];  // float32 input of the output layer, [k][car]
+    __shared__ float w3s[5][129];
+    __shared__ float pred[5][SF_CARS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int car0 = (int)blockIdx.x * SF_CARS;
+    const int j = lane & 31, half = lane >> 5;
+    // this wavefront's row tile of layers 0 and 1, loading under the inputs phase
+    const bf16x8* fw = reinterpret_cast<const bf16x8*>(frags) + lane;
+    const bf16x8 f0 = fw[(size_t)wave * 64];
+    bf16x8 f1[RACER_SF_STEPS], f2[RACER_SF_STEPS];
+#pragma unroll
+    for (int k = 0; k < RACER_SF_STEPS; ++k) f1[k] = fw[(size_t)(RACER_SF_BASE1 + wave * RACER_SF_STEPS + k) * 64];
+    const uint16_t one = 0x3f80;  // bf16 1.0
+    // constant rows: xa k = 11 (layer 0's bias) and k = 128, xb k = 128; zeros past them
+    for (int idx = tid; idx < SF_CARS * SF_LDX; idx += 256) {
+        const int r = idx / SF_LDX, k = idx % SF_LDX;
+        if (k >= 11) xa[r][k] = (k == 11 || k == 128) ? one : (uint16_t)0;
+        if (k >= 128) xb[r][k] = k == 128 ? one : (uint16_t)0;
     }
-    const bool crashed = spot.clearance < RACER_HALF_DIAG || ground.sx * ground.sx + ground.sy * ground.sy > 1.0f;
-    finish<L>(m, s, n, reward, done, obs, obs_cols, beams, i, seed, heading_noise, p, steer, spot, ground, crashed);
+    for (int idx = tid; idx < 5 * 129; idx += 256) w3s[idx / 129][idx % 129] = w3[idx];
+    SurStart c;
+    const int i = car0 + lane;
+    const bool mine = wave == 0 && lane < SF_CARS && i < n;
+    if (wave == 0 && lane < SF_CARS) {
+        float f[11];
+        if (i < n) {
+            surrogate_inputs<L>(m, s, n, act, act_cols, i, f, &c);
+        } else {
+#pragma unroll
+            for (int r = 0; r < 11; ++r) f[r] = 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < 11; ++r)
+            xa[lane][r] = __builtin_bit_cast(uint16_t, (__bf16)((f[r] - st[r]) * st[11 + r]));
+        h3[128][lane] = 1.0f;
+    }
+    __syncthreads();
+    auto tile = [&](const uint16_t (*x)[SF_LDX], const bf16x8* a, int steps) {
+        f32x16 acc;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[q] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < RACER_SF_STEPS; ++k) {
+            if (k < steps) {
+                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&x[j][16 * k + 8 * half]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k], bv, acc, 0, 0, 0);
+            }
+        }
+        return acc;
+    };
+    auto row = [&](int q) { return 32 * wave + 8 * (q >> 2) + 4 * half + (q & 3); };
+    {
+        const f32x16 acc = tile(xa, &f0, RACER_SF_STEPS0);
+#pragma unroll
+        for (int k = 0; k < RACER_SF_STEPS; ++k) f2[k] = fw[(size_t)(RACER_SF_BASE2 + wave * RACER_SF_STEPS + k) * 64];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) xb[j][row(q)] = __builtin_bit_cast(uint16_t, (__bf16)racer_tanh_mlp(acc[q]));
+    }
+    __syncthreads();
+    {
+        const f32x16 acc = tile(xb, f1, RACER_SF_STEPS);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) xa[j][row(q)] = __builtin_bit_cast(uint16_t, (__bf16)racer_tanh_mlp(acc[q]));
+    }
+    __syncthreads();
+    {
+        const f32x16 acc = tile(xa, f2, RACER_SF_STEPS);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) h3[row(q)][j] = racer_tanh_mlp(acc[q]);
+    }
+    __syncthreads();
+    const int o = wave + 4 * half;
+    if (o < 5) {
+        float acc = 0.0f;
+        for (int k = 0; k < 129; ++k) acc = fmaf(w3s[o][k], h3[k][j], acc);
+        pred[o][j] = acc;
+    }
+    __syncthreads();
+    if (mine) {
+        float q[5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) q[r] = pred[r][lane];
+        surrogate_apply<L>(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, seed, heading_noise, q, c,
+                           st + 22, st + 27, min_clearance, fallback, i);
+    }
 }
 
 // ---- float64 RK4 (the surrogate's judge) ----
@@ -1057,6 +1206,26 @@ int racer_launch_surrogate_step(racer_sim* sim, const float* pred, int64_t ldp, 
     return racer_launch_check("lidar");
 }
 
+int racer_launch_surrogate_fused(racer_sim* sim, const uint16_t* frags, const float* w3, const float* stats,
+                                 float min_clearance, float* fallback) {
+    const int n = sim->n;
+    auto kernel = sim->map.layers == 1 ? k_surrogate_fused<1>
+                                       : (sim->map.layers == 2 ? k_surrogate_fused<2> : k_surrogate_fused<0>);
+    {
+        racer_timed timer(sim->dev, "surrogate_fused");
+        hipLaunchKernelGGL(kernel, dim3((n + SF_CARS - 1) / SF_CARS), dim3(256), 0, sim->dev->stream, sim->map,
+                           sim->state->ptr, n, sim->actions->ptr, (int64_t)n, sim->reward->ptr, sim->done->ptr,
+                           sim->obs->ptr, (int64_t)n, sim->beams, sim->seed, sim->heading_noise, frags, w3, stats,
+                           min_clearance, fallback);
+    }
+    if (racer_launch_check("surrogate fused")) return 1;
+    {
+        racer_timed timer(sim->dev, "lidar");
+        launch_lidar_any(sim, sim->obs->ptr, (int64_t)n);
+    }
+    return racer_launch_check("lidar");
+}
+
 extern "C" int racer_sim_step_f64(racer_sim* sim, racer_buf* st, const racer_view* actions_p) {
     racer_clear_error();
     RACER_REQUIRE(sim && st && actions_p, "null argument");
@@ -1119,6 +1288,7 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
     }
     auto* sim = new racer_sim();
     sim->dev = dev;
+    racer_device_retain(dev);
     sim->n = n;
     sim->seed = seed;
     sim->beams = cfg->beams;
@@ -1179,7 +1349,9 @@ extern "C" void racer_sim_destroy(racer_sim* sim) {
     if (sim->packed) hipFree(sim->packed);
     if (sim->dirs) hipFree(sim->dirs);
     if (sim->counter) hipFree(sim->counter);
+    racer_device* dev = sim->dev;
     delete sim;
+    racer_device_release(dev);
 }
 
 extern "C" int racer_sim_reset(racer_sim* sim) {

@@ -7,9 +7,10 @@
 // Layer l's weights W_l are [w_(l+1), w_l + 1] with the bias as the last
 // column, fed by a constant-1 row under each activation (the layout the
 // policy uses, racer/ppo.mojo:40-44), so every product is one GEMM of the
-// project's own kernels: bf16 MFMA for the hidden layers in precision 1,
-// float32 with k in order in precision 0 (bitwise the oracle's), the output
-// layer in float32 always. Targets are the displacement in the car's frame
+// project's own kernels: bf16 MFMA for the hidden layers in precision 1
+// with the branch-free racer_tanh_mlp (training and imagined steps alike),
+// float32 with k in order and racer_tanh in precision 0 (bitwise the
+// oracle's), the output layer in float32 always. Targets are the displacement in the car's frame
 // (forward, left), the heading, speed and steer changes, normalised by their
 // dataset mean and deviation; the loss is their masked mean squared error.
 #include "racer_internal.h"
@@ -152,6 +153,24 @@ __global__ void k_sur_moments_partial(const float* __restrict__ a, int64_t ld, i
     }
 }
 
+// The hidden layers' weights in v_mfma_f32_32x32x16_bf16 A-operand order for
+// the fused step: block (layer l, tile t, k-step q) holds lane l's 8 values
+// of row 32 t + l % 32, k = 16 q + 8 (l / 32) .. + 7 of W_l (the bias column
+// at k = w_l), zeros past it; the same bf16 rounding as the packed copy.
+__global__ void k_sur_pack_frags(const float* __restrict__ theta, const int64_t* off, uint16_t* __restrict__ frags) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)RACER_SF_BLOCKS * 512) return;
+    const int block = (int)(idx / 512), within = (int)(idx % 512);
+    const int l = block >= RACER_SF_BASE2 ? 2 : (block >= RACER_SF_BASE1 ? 1 : 0);
+    const int steps = l == 0 ? RACER_SF_STEPS0 : RACER_SF_STEPS;
+    const int base = l == 0 ? 0 : (l == 1 ? RACER_SF_BASE1 : RACER_SF_BASE2);
+    const int local = block - base, tile = local / steps, step = local % steps;
+    const int lane = within / 8, e = within % 8;
+    const int row = 32 * tile + lane % 32, k = 16 * step + (lane / 32) * 8 + e;
+    const int cols = S_WIDTH[l] + 1;
+    frags[idx] = k < cols ? __builtin_bit_cast(uint16_t, (__bf16)theta[off[l] + (int64_t)row * cols + k]) : (uint16_t)0;
+}
+
 // Adam (Kingma and Ba) without clipping; c1, c2 are the bias corrections.
 __global__ void k_sur_adam(float* theta, int64_t P, const float* grad, float* first, float* second, float lr, float c1,
                            float c2) {
@@ -176,7 +195,9 @@ struct racer_surrogate {
     racer_buf *theta = nullptr, *grad = nullptr, *first = nullptr, *second = nullptr, *wt = nullptr,
               *stats = nullptr;
     uint16_t* theta_bf16 = nullptr;
+    uint16_t* frags = nullptr;  // hidden layers in MFMA fragment order (the fused step)
     bool stale = true;
+    bool fused = true;  // bf16 imagined steps in one launch
     int64_t cols = 0;  // capacity in samples of the working buffers
     int64_t ones_for = -1;  // the sample count whose constant rows are in place
     racer_buf* act[S_LAYERS + 1] = {};
@@ -203,10 +224,13 @@ extern "C" void racer_surrogate_destroy(racer_surrogate* s) {
         racer_free(s->delta[l]);
     }
     if (s->theta_bf16) hipFree(s->theta_bf16);
+    if (s->frags) hipFree(s->frags);
     if (s->doff) hipFree(s->doff);
     if (s->partial) hipFree(s->partial);
     if (s->acc) hipFree(s->acc);
+    racer_device* dev = s->dev;
     delete s;
+    racer_device_release(dev);
 }
 
 static int sur_refresh(racer_surrogate* s) {
@@ -218,6 +242,8 @@ static int sur_refresh(racer_surrogate* s) {
                             rows, cols + 1))
             return 1;
     }
+    hipLaunchKernelGGL(k_sur_pack_frags, dim3((RACER_SF_BLOCKS * 512 + 255) / 256), dim3(256), 0, s->dev->stream,
+                       s->theta->ptr, s->doff, s->frags);
     {
         racer_timed timer(s->dev, "surrogate_transpose");
         hipLaunchKernelGGL(k_sur_transpose, dim3((unsigned)((S_H * S_H + 255) / 256), S_LAYERS), dim3(256), 0,
@@ -236,6 +262,7 @@ extern "C" int racer_surrogate_create(racer_device* dev, int seed, int precision
     RACER_REQUIRE(cols >= 1, "the surrogate needs room for at least one sample");
     auto* s = new racer_surrogate();
     s->dev = dev;
+    racer_device_retain(dev);
     s->precision = precision;
     s->seed = seed;
     s->cols = cols;
@@ -262,6 +289,7 @@ extern "C" int racer_surrogate_create(racer_device* dev, int seed, int precision
               racer_alloc(dev, (size_t)(S_WIDTH[l] * cols), 0.0f, &s->delta[l]);
     }
     if (bad || hipMalloc(&s->theta_bf16, (size_t)p16 * sizeof(uint16_t)) != hipSuccess ||
+        hipMalloc(&s->frags, (size_t)RACER_SF_BLOCKS * 512 * sizeof(uint16_t)) != hipSuccess ||
         hipMalloc(&s->doff, (size_t)(2 * S_LAYERS + 1) * sizeof(int64_t)) != hipSuccess ||
         hipMalloc(&s->partial, (size_t)3 * 16 * SUM_BLOCKS_S * sizeof(double)) != hipSuccess ||
         hipMalloc(&s->acc, 2 * sizeof(double)) != hipSuccess) {
@@ -312,7 +340,7 @@ static int sur_forward(racer_surrogate* s, int64_t B) {
         int rc;
         if (s->precision == 1 && l < S_LAYERS - 1)
             rc = racer_gemm_nn_bf16(s->dev, s->theta_bf16 + s->off16[l], s->ld16[l], s->act[l]->ptr, B,
-                                    s->act[l + 1]->ptr, B, M, (int)B, K, epi);
+                                    s->act[l + 1]->ptr, B, M, (int)B, K, RACER_EPI_TANH_MLP);
         else
             rc = racer_gemm_nn_f32(s->dev, s->theta->ptr + s->off[l], K, s->act[l]->ptr, B, s->act[l + 1]->ptr, B, M,
                                    (int)B, K, epi, nullptr, 0);
@@ -388,7 +416,9 @@ extern "C" void racer_dataset_destroy(racer_dataset* d) {
     if (!d) return;
     racer_buf* bufs[] = {d->feat, d->tgt, d->mask, d->prev};
     for (auto* b : bufs) racer_free(b);
+    racer_device* dev = d->dev;
     delete d;
+    racer_device_release(dev);
 }
 
 extern "C" int racer_dataset_create(racer_device* dev, int64_t capacity, racer_dataset** out) {
@@ -397,6 +427,7 @@ extern "C" int racer_dataset_create(racer_device* dev, int64_t capacity, racer_d
     RACER_REQUIRE(dev && capacity >= 1, "bad dataset size");
     auto* d = new racer_dataset();
     d->dev = dev;
+    racer_device_retain(dev);
     d->cap = capacity;
     if (racer_alloc(dev, (size_t)((S_IN + 1) * capacity), 0.0f, &d->feat) ||
         racer_alloc(dev, (size_t)(S_OUT * capacity), 0.0f, &d->tgt) ||
@@ -540,12 +571,26 @@ int racer_surrogate_step_raw(racer_surrogate* s, racer_sim* sim, float min_clear
     RACER_REQUIRE(n <= s->cols, "more cars than the surrogate's working buffers");
     RACER_REQUIRE(s->dev == sim->dev, "the surrogate lives on another device");
     const float* st = s->stats->ptr;
+    if (s->precision == 1 && s->fused) {
+        if (sur_refresh(s)) return 1;
+        return racer_launch_surrogate_fused(sim, s->frags, s->theta->ptr + s->off[S_LAYERS - 1], st, min_clearance,
+                                            fallback);
+    }
     if (racer_launch_surrogate_features(sim, sim->actions->ptr, n, s->act[0]->ptr, n, st + ST_IN_MEAN,
                                         st + ST_IN_INV) ||
         sur_forward(s, n))
         return 1;
     return racer_launch_surrogate_step(sim, s->act[S_LAYERS]->ptr, n, st + ST_OUT_MEAN, st + ST_OUT_STD,
                                        min_clearance, fallback);
+}
+
+// Whether bf16 imagined steps run as one fused launch (the default) or as
+// the separate inputs, products and step kernels; both give the same result.
+extern "C" int racer_surrogate_set_fused(racer_surrogate* s, int fused) {
+    racer_clear_error();
+    RACER_REQUIRE(s, "null surrogate");
+    s->fused = fused != 0;
+    return 0;
 }
 
 extern "C" int racer_surrogate_step(racer_surrogate* s, racer_sim* sim, float min_clearance, racer_buf* fallback) {
