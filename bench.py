@@ -387,13 +387,14 @@ def main() -> int:
     log("autotuned lidar wavefronts per car:", waves_per_car)
     # march iterations of the lidar, counted by the instrumented variant on the warm state (untimed)
     run.sim.count_iterations(True)
-    counts = []
+    counts, trips = [], []
     for t in range(3):
         run.step(False, args.warmup + t)
         counts.append(run.sim.last_iterations)
+        trips.append(run.sim.last_trips)
     run.sim.count_iterations(False)
     dev.synchronize()
-    log("lidar iterations per step:", counts)
+    log("lidar iterations per step:", counts, "wavefront loop trips per step:", trips)
     t = args.warmup + 3
     # Timed region: the Runner steps replayed from a captured HIP graph (the
     # same kernels and arguments as eager steps, bitwise the same results).

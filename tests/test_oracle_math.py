@@ -77,11 +77,15 @@ def test_uniform_and_normal_statistics():
     assert not np.array_equal(O.normal4(1, 0, 0), O.normal4(1, 0, 1))
 
 
-@pytest.mark.parametrize("cell", [0.025, 0.05])
+@pytest.mark.parametrize("cell", [0.025, 0.05, 0.1, 0.0625])
 def test_nearest_index_shortcut_exhaustive(cell):
-    """racer_nearest_index equals floorf(a / cell + 0.5f) for every float a in [-40, 40]."""
+    """racer_nearest_index equals floorf(a / cell + 0.5f) for every float a in
+    [-40, 40], and for every float a with 40 <= |a| <= 4096 (grids up to 160 k
+    cells wide)."""
     O.set_threads(8)
     assert O.lib().o_check_nearest_index(cell, -40.0, 40.0) == 0
+    assert O.lib().o_check_nearest_index(cell, 40.0, 4096.0) == 0
+    assert O.lib().o_check_nearest_index(cell, -4096.0, -40.0) == 0
 
 
 def test_min_quotient_shortcut_sampled():

@@ -231,15 +231,19 @@ RACER_FN float racer_tanh_mlp(float x) {
  * q = RN(a / b) by at most 3 ulp(q) (relative errors of 2^-24 from inv_b, 2^-24
  * from the product, 2^-24 from q itself), and s0 = RN(q0 + 0.5) from
  * RN(q + 0.5) by at most 3 ulp(q) + ulp(q + 0.5) <= 5 ulp(max(|q0|, 1)). So
- * when s0 is more than (|q0| + 1) 2^-20 (> 8 such ulps) away from both
- * neighbouring integers, both floors agree; otherwise the true division runs. */
+ * when s0 is more than (|q0| + 1) 2^-20 - 2^-25 (> 7.75 such ulps) away from
+ * both neighbouring integers, both floors agree; otherwise the true division
+ * runs. The test: s0 - f0 is exact, lim = RN(0.5 - (|q0| + 1) 2^-20) is one
+ * fused rounding (error <= 2^-26 while lim > 0), |(s0 - f0) - 0.5| is off by
+ * at most 2^-26, so passing it puts s0 - f0 strictly inside
+ * ((|q0| + 1) 2^-20 - 2^-25, 1 - that). A NaN fails it; |q0| >= 2^19 - 1
+ * (lim <= 0) always takes the division. */
 RACER_FN float racer_nearest_index(float a, float b, float inv_b) {
     float q0 = a * inv_b;
     float s0 = q0 + 0.5f;
     float f0 = floorf(s0);
-    float margin = (racer_abs(q0) + 1.0f) * 0x1p-20f;
-    /* both distances exceed the margin (a NaN distance fails the test) */
-    if (racer_min(s0 - f0, (f0 + 1.0f) - s0) > margin) return f0;
+    float lim = fmaf(racer_abs(q0), -0x1p-20f, 0.5f - 0x1p-20f);
+    if (racer_abs((s0 - f0) - 0.5f) < lim) return f0;
     return floorf(a / b + 0.5f);
 }
 

@@ -722,7 +722,8 @@ __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_
                                            float* clearance, float* height) {
     float fx = racer_nearest_index(x - m.x0, m.cell, m.inv_cell);
     float fy = racer_nearest_index(y - m.y0, m.cell, m.inv_cell);
-    bool on = fx >= 0.0f && fy >= 0.0f && fx < (float)m.nx && fy < (float)m.ny;
+    // non-short-circuit: the four compares combine without branches
+    bool on = (fx >= 0.0f) & (fy >= 0.0f) & (fx < (float)m.nx) & (fy < (float)m.ny);
     // cell = fy nx + fx, exact in float (both below 2^12, the sum below 2^24)
     uint32_t off = on ? (uint32_t)fmaf(fy, (float)m.nx, fx) * (uint32_t)(8 * L) : RACER_OOB;
     float off_map = on ? 0.0f : 1.0f;
@@ -770,7 +771,10 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
                                            unsigned long long* counter, int global_wave) {
     extern __shared__ float lds[];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // wavefront-uniform (the compiler cannot see it through threadIdx.x): the
+    // car, its beam share and the refill bookkeeping then live in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    global_wave = __builtin_amdgcn_readfirstlane(global_wave);
     const int car = global_wave / waves_per_car;
     if (car >= n) return;
     // This wavefront's share of the car's beams: [lo, lo + beams).
@@ -792,7 +796,8 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     const float slope15 = 1.5f * m.slope;
     const float half_cell = 0.5f * m.cell;
 
-    // Set-up of beams 64 c .. 64 c + 63, one per lane, into ring slot c % 2.
+    // Set-up of beams 64 c .. 64 c + 63, one per lane, into ring half c % 2:
+    // field f of beam b lives at ring[128 f + b % 128].
     auto setup_chunk = [&](int c) {
         int b = c * 64 + lane;
         if (b < beams) {
@@ -804,24 +809,23 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
             float dx = d0 / norm, dy = d1 / norm, dz = d2 / norm;
             float horizontal = sqrtf(dx * dx + dy * dy);
             float descent = slope15 * horizontal + racer_max(-dz, 0.0f);
-            float* slot = ring + (c & 1) * (LIDAR_RECORD * 64);
-            slot[0 * 64 + lane] = dx;
-            slot[1 * 64 + lane] = dy;
-            slot[2 * 64 + lane] = dz;
-            slot[3 * 64 + lane] = descent;
-            slot[4 * 64 + lane] = 1.0f / descent;
+            float* slot = ring + (c & 1) * 64 + lane;
+            slot[0 * 128] = dx;
+            slot[1 * 128] = dy;
+            slot[2 * 128] = dz;
+            slot[3 * 128] = descent;
+            slot[4 * 128] = 1.0f / descent;
         }
     };
     float dx = 0.0f, dy = 0.0f, dz = 0.0f, descent = 0.0f, inv_descent = 0.0f;
     float t = 0.0f, last = 0.0f, last_floor = 0.0f;
     auto take = [&](int b) {
-        const float* slot = ring + ((b >> 6) & 1) * (LIDAR_RECORD * 64);
-        int j = b & 63;
-        dx = slot[0 * 64 + j];
-        dy = slot[1 * 64 + j];
-        dz = slot[2 * 64 + j];
-        descent = slot[3 * 64 + j];
-        inv_descent = slot[4 * 64 + j];
+        const float* slot = ring + (b & 127);
+        dx = slot[0 * 128];
+        dy = slot[1 * 128];
+        dz = slot[2 * 128];
+        descent = slot[3 * 128];
+        inv_descent = slot[4 * 128];
         t = 0.0f;
         last = 0.0f;
         last_floor = 0.0f;
@@ -882,7 +886,9 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
             if (finished) {
-                b = next + (int)__popcll(done_mask & ((1ull << lane) - 1ull));
+                // finished lanes below this one
+                b = next + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(done_mask >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)done_mask, 0u));
                 if (b < beams) take(b);
             }
             next += count;
@@ -924,7 +930,8 @@ __global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __rest
 template <bool COUNT>
 __global__ void __launch_bounds__(256) k_lidar_batch(racer_sim_table tab, int per_wave, int waves_per_car,
                                                      unsigned long long* counter) {
-    const int64_t g = ((int64_t)xcd_block() * blockDim.x + threadIdx.x) >> 6;
+    // wavefront-uniform, so the instance lookup below is scalar
+    const int64_t g = __builtin_amdgcn_readfirstlane((int)(((int64_t)xcd_block() * blockDim.x + threadIdx.x) >> 6));
     int k = 0;
     while (k + 1 < tab.count && g >= tab.start[k + 1]) ++k;
     if (g >= tab.start[tab.count]) return;
