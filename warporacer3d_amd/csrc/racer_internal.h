@@ -71,6 +71,17 @@ struct racer_device {
     int refs = 1;
 };
 
+// Workgroups go round-robin over the 8 XCDs; remapping the workgroup index
+// gives each XCD a contiguous run of cars (when the grid is a multiple of 8),
+// so neighbouring cars' stores (adjacent floats of one obs row) meet in one
+// L2, and the physics, lidar and forward kernels of a step keep each car on
+// the same XCD.
+__device__ __forceinline__ int xcd_block() {
+    int block = blockIdx.x;
+    if ((gridDim.x & 7) == 0) block = (block & 7) * (gridDim.x >> 3) + (block >> 3);
+    return block;
+}
+
 void racer_device_retain(racer_device* dev);
 void racer_device_release(racer_device* dev);
 

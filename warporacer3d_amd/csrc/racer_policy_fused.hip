@@ -60,7 +60,10 @@ __host__ __device__ inline FusedLayout fused_layout(int H, int IN) {
 }
 
 // One workgroup's 16 columns c0 .. c0 + 15 of the forward (and sample).
-template <bool SAMPLE>
+// HS > 0: each half of the first product's k range is exactly HS k-steps and
+// the fragment ring is unrolled over them (straight-line code, so each k-step
+// waits only for its own fragments); HS = 0: any k range, as a loop.
+template <bool SAMPLE, int HS>
 __device__ __forceinline__ void policy_tile(
     const uint16_t* __restrict__ F0, const uint16_t* __restrict__ F1,
     const float* __restrict__ W2, const float* __restrict__ x, int64_t ldx, int H, int IN, int64_t cols,
@@ -128,16 +131,39 @@ __device__ __forceinline__ void policy_tile(
     // No branches inside the ring: a k-step past s1 has all-zero weight
     // fragments (out-of-range loads) against finite, clamped input columns, so
     // it adds exactly zero, and the compiler can keep the ring's loads in flight.
-    for (int s = s0; s < s1; s += RING) {
+    if constexpr (HS > 0) {
+        // Unrolled: k-step j consumes ring slot j % RING and refills it with
+        // step j + RING, so RING - 1 later steps' fragments stay in flight
+        // behind each step's MFMAs. (As a loop, the wait at the loop head
+        // covers every outstanding load, and the ring degenerates into
+        // load-all, wait-all, compute-all.)
 #pragma unroll
-        for (int u = 0; u < RING; ++u) {
-            const int kk = min(32 * (s + u), Ly.in32 - 32);
+        for (int j = 0; j < HS; ++j) {
+            const int u = j % RING;
+            const int kk = 32 * (s0 + j);
             bf16x8 b = *reinterpret_cast<const bf16x8*>(xs + r * Ly.kx + kk + kg);
 #pragma unroll
             for (int t = 0; t < 4; ++t)
                 acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&ring[u][t]), b, acc[t],
                                                                  0, 0, 0);
-            load_step(s + u + RING, ring[u]);
+            // pinned here, so RING - 1 k-steps of fragments stay in flight
+            // (the scheduler otherwise sinks each refill to just before its use)
+            __builtin_amdgcn_sched_barrier(0);
+            if (j + RING < HS) load_step(s0 + j + RING, ring[u]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else {
+        for (int s = s0; s < s1; s += RING) {
+#pragma unroll
+            for (int u = 0; u < RING; ++u) {
+                const int kk = min(32 * (s + u), Ly.in32 - 32);
+                bf16x8 b = *reinterpret_cast<const bf16x8*>(xs + r * Ly.kx + kk + kg);
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&ring[u][t]), b,
+                                                                     acc[t], 0, 0, 0);
+                load_step(s + u + RING, ring[u]);
+            }
         }
     }
     // request the second product's weight fragments now (rows 32 w .. + 31)
@@ -245,20 +271,21 @@ __device__ __forceinline__ void policy_tile(
     }
 }
 
-template <bool SAMPLE>
+template <bool SAMPLE, int HS>
 __global__ void __launch_bounds__(64 * WAVES) k_policy_fused(
     const uint16_t* __restrict__ F0, const uint16_t* __restrict__ F1,
     const float* __restrict__ W2, const float* __restrict__ x, int64_t ldx, int H, int IN, int64_t cols,
     float* __restrict__ h1out, float* __restrict__ h2out, float* __restrict__ o, const float* __restrict__ log_std,
     float* act, int64_t act_cols, float* logp, float* value, int deterministic, int64_t step,
     const int64_t* step_base, int seed) {
-    policy_tile<SAMPLE>(F0, F1, W2, x, ldx, H, IN, cols, h1out, h2out, o, log_std, act, act_cols, logp, value,
-                        deterministic, step, step_base, seed, (int64_t)blockIdx.x * FT);
+    policy_tile<SAMPLE, HS>(F0, F1, W2, x, ldx, H, IN, cols, h1out, h2out, o, log_std, act, act_cols, logp, value,
+                            deterministic, step, step_base, seed, (int64_t)xcd_block() * FT);
 }
 
 // The forward and sample of several sims' observations in one launch:
 // workgroup b serves the instance whose tile range [start[k], start[k + 1])
 // holds b, with that instance's buffers and Philox seed.
+template <int HS>
 __global__ void __launch_bounds__(64 * WAVES) k_policy_fused_batch(
     const uint16_t* __restrict__ F0, const uint16_t* __restrict__ F1, const float* __restrict__ W2, int H, int IN,
     const float* __restrict__ log_std, racer_policy_table tab, int deterministic, int64_t step,
@@ -267,7 +294,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_fused_batch(
     int k = 0;
     while (k + 1 < tab.count && b >= tab.start[k + 1]) ++k;
     const racer_policy_slot& q = tab.slot[k];
-    policy_tile<true>(F0, F1, W2, q.x, q.ldx, H, IN, q.cols, q.h1, q.h2, q.o, log_std, q.act, q.act_cols, q.logp,
+    policy_tile<true, HS>(F0, F1, W2, q.x, q.ldx, H, IN, q.cols, q.h1, q.h2, q.o, log_std, q.act, q.act_cols, q.logp,
                       q.value, deterministic, step, step_base, q.seed, (b - tab.start[k]) * FT);
 }
 
@@ -306,14 +333,25 @@ size_t racer_policy_fused_smem(int H, int IN) { return fused_layout(H, IN).total
 
 // Allows the fused kernel more than the default 64 KB of dynamic LDS (gfx950
 // has 160 KB per CU); called when a policy that uses it is created.
-int racer_policy_fused_setup() {
-    RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+// The unrolled ring's instantiations: half k ranges of 17 k-steps (1086
+// inputs: 1081 beams) and 2 (113 inputs: 108 beams); other shapes loop.
+template <int HS>
+static int fused_setup_one() {
+    RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused<true, HS>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024));
-    RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused<false, HS>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024));
-    RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+    RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused_batch<HS>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024));
     return 0;
+}
+
+int racer_policy_fused_setup() { return fused_setup_one<0>() || fused_setup_one<2>() || fused_setup_one<17>(); }
+
+// HS for a first product of IN inputs: its k-steps split into two equal halves of HS.
+static int unrolled_half(int IN) {
+    const int steps = (IN + 31) / 32;
+    return steps == 34 ? 17 : (steps == 4 ? 2 : 0);
 }
 
 int racer_policy_fused_batch(racer_policy* p, const racer_policy_table& tab, int deterministic, int64_t step) {
@@ -323,9 +361,11 @@ int racer_policy_fused_batch(racer_policy* p, const racer_policy_table& tab, int
     size_t shmem = racer_policy_fused_smem(p->hidden, (int)p->IN);
     {
         racer_timed timer(dev, "policy_fused_sample");
-        hipLaunchKernelGGL(k_policy_fused_batch, grid, block, shmem, dev->stream, p->frag0, p->frag1,
-                           p->theta->ptr + p->W2, p->hidden, (int)p->IN, p->theta->ptr + p->LOGSTD, tab, deterministic,
-                           step, dev->step_base);
+        const int hs = unrolled_half((int)p->IN);
+        auto kernel = hs == 17 ? k_policy_fused_batch<17> : (hs == 2 ? k_policy_fused_batch<2> : k_policy_fused_batch<0>);
+        hipLaunchKernelGGL(kernel, grid, block, shmem, dev->stream, p->frag0, p->frag1, p->theta->ptr + p->W2,
+                           p->hidden, (int)p->IN, p->theta->ptr + p->LOGSTD, tab, deterministic, step,
+                           dev->step_base);
     }
     return racer_launch_check("policy_fused_batch");
 }
@@ -339,15 +379,20 @@ int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_ac
     const float* log_std = p->theta->ptr + p->LOGSTD;
     dim3 grid((unsigned)((cols + FT - 1) / FT)), block(64 * WAVES);
     size_t shmem = racer_policy_fused_smem(p->hidden, (int)p->IN);
+    const int hs = unrolled_half((int)p->IN);
     if (sample) {
+        auto kernel = hs == 17 ? k_policy_fused<true, 17> : (hs == 2 ? k_policy_fused<true, 2> : k_policy_fused<true, 0>);
         racer_timed timer(dev, "policy_fused_sample");
-        hipLaunchKernelGGL(k_policy_fused<true>, grid, block, shmem, dev->stream, f0, f1, W2, x, x_cols, p->hidden, (int)p->IN, (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr, log_std,
-                           act, (int64_t)cols, logp, value, deterministic, step, dev->step_base, seed);
+        hipLaunchKernelGGL(kernel, grid, block, shmem, dev->stream, f0, f1, W2, x, x_cols, p->hidden, (int)p->IN,
+                           (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr, log_std, act, (int64_t)cols, logp, value,
+                           deterministic, step, dev->step_base, seed);
     } else {
+        auto kernel =
+            hs == 17 ? k_policy_fused<false, 17> : (hs == 2 ? k_policy_fused<false, 2> : k_policy_fused<false, 0>);
         racer_timed timer(dev, "policy_fused");
-        hipLaunchKernelGGL(k_policy_fused<false>, grid, block, shmem, dev->stream, f0, f1, W2, x,
-                           x_cols, p->hidden, (int)p->IN, (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr,
-                           log_std, act, (int64_t)cols, logp, value, deterministic, step, dev->step_base, seed);
+        hipLaunchKernelGGL(kernel, grid, block, shmem, dev->stream, f0, f1, W2, x, x_cols, p->hidden, (int)p->IN,
+                           (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr, log_std, act, (int64_t)cols, logp, value,
+                           deterministic, step, dev->step_base, seed);
     }
     return racer_launch_check("policy_fused");
 }

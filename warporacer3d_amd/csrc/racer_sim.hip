@@ -297,7 +297,7 @@ template <int L>
 __global__ void __launch_bounds__(64) k_physics(racer_dmap m, float* s, int n, const float* act, int64_t act_cols,
                                                 float* reward, float* done, float* obs, int64_t obs_cols, int beams,
                                                 int seed, float heading_noise) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i < n) advance<L>(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, i, seed, heading_noise);
 }
 
@@ -905,15 +905,6 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
             atomicAdd(counter + 1, (unsigned long long)trips);
         }
     }
-}
-
-// Workgroups go round-robin over the 8 XCDs; remapping the workgroup index
-// gives each XCD a contiguous run of cars, so the range stores of
-// neighbouring cars (adjacent floats of one obs row) meet in one L2.
-__device__ __forceinline__ int xcd_block() {
-    int block = blockIdx.x;
-    if ((gridDim.x & 7) == 0) block = (block & 7) * (gridDim.x >> 3) + (block >> 3);
-    return block;
 }
 
 template <int L, bool COUNT>
