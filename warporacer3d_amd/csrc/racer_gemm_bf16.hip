@@ -196,18 +196,34 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
 // buffer loads, rounds them to bf16 (nearest even) into LDS as [row][k], and
 // the next stage's loads are issued before this stage's 16 MFMAs per
 // wavefront. Rows past M or N and k past the chunk read as zeros.
+//
+// The grid is one-dimensional: workgroup g runs on XCD g % 8 (dispatch is
+// round-robin over the XCDs), and when the chunk count is a multiple of 8
+// every tile of chunk z is placed on XCD z % 8, so the tiles that share a
+// chunk's A and B rows stream them through one L2 (one HBM read of each k
+// stage instead of one per tile); otherwise the tiles go in plain order.
 constexpr int WG_T = 128, WG_K = 64, WG_LD = WG_K + 8;
 
 __global__ void __launch_bounds__(256) k_wgrad_bf16(const float* __restrict__ A, int64_t lda,
                                                     const float* __restrict__ B, int64_t ldb, float* __restrict__ P,
                                                     int64_t ldp, int64_t chunk_stride, int M, int N, int K,
-                                                    int kchunk) {
+                                                    int kchunk, int tiles_x, int tiles_y, int chunks) {
     __shared__ __attribute__((aligned(16))) uint16_t As[WG_T][WG_LD];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[WG_T][WG_LD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int m0 = blockIdx.y * WG_T, n0 = blockIdx.x * WG_T;
-    const int kbeg = blockIdx.z * kchunk;
+    const int tiles = tiles_x * tiles_y, g = blockIdx.x;
+    int z, tile;
+    if ((chunks & 7) == 0) {
+        const int slot = g >> 3;
+        z = (slot / tiles) * 8 + (g & 7);
+        tile = slot % tiles;
+    } else {
+        z = g / tiles;
+        tile = g % tiles;
+    }
+    const int m0 = (tile / tiles_x) * WG_T, n0 = (tile % tiles_x) * WG_T;
+    const int kbeg = z * kchunk;
     const int kend = min(K, kbeg + kchunk);
     // descriptors from the chunk's first k: offsets stay below 4 GB
     const auto ares = racer_rsrc(A + kbeg, ((uint64_t)(M - 1) * lda + (kend - kbeg)) * 4);
@@ -270,7 +286,7 @@ __global__ void __launch_bounds__(256) k_wgrad_bf16(const float* __restrict__ A,
         __syncthreads();
     }
     // D[i][j]: j = lane % 32, i = 8 (q / 4) + 4 (lane / 32) + q % 4
-    float* dst = P + (int64_t)blockIdx.z * chunk_stride;
+    float* dst = P + (int64_t)z * chunk_stride;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -351,11 +367,12 @@ int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const flo
         int chunks = (K + chunk - 1) / chunk;
         int64_t stride = (int64_t)M * ldc;
         if (racer_ensure_scratch(dev, (size_t)(stride * chunks))) return 1;
-        dim3 grid((N + WG_T - 1) / WG_T, (M + WG_T - 1) / WG_T, chunks), block(256);
+        const int tx = (N + WG_T - 1) / WG_T, ty = (M + WG_T - 1) / WG_T;
+        dim3 grid((unsigned)(tx * ty * chunks)), block(256);
         {
             racer_timed timer(dev, "wgrad_bf16");
             hipLaunchKernelGGL(k_wgrad_bf16, grid, block, 0, dev->stream, A, lda, B, ldb, dev->scratch, ldc, stride,
-                               M, N, K, chunk);
+                               M, N, K, chunk, tx, ty, chunks);
         }
         if (racer_launch_check("wgrad_bf16")) return 1;
         int64_t total = (int64_t)M * N;
