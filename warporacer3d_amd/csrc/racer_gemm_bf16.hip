@@ -124,7 +124,10 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    // one-dimensional grid, tile t = column tile x row tiles + row tile: the
+    // row tiles that share a column block of B are neighbours on one XCD
+    const int tiles_m = (M + TM - 1) / TM, t = xcd_block();
+    const int m0 = (t % tiles_m) * TM, n0 = (t / tiles_m) * TN;
     const int arow = tid >> 2, akc = (tid & 3) * 16;  // A: 16 consecutive k of one row
     const int bcol = tid & 63, bkc = (tid >> 6) * 16;  // B: 16 consecutive k of one column
     uint4 ra[2];
@@ -337,7 +340,7 @@ int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const 
     RACER_REQUIRE(lda % 8 == 0 && (uint64_t)M * lda * 2 < 0xFFFFFF00ull &&
                       ((uint64_t)(K - 1) * ldb + N) * 4 < 0xFFFFFF00ull,
                   "bf16 product operands exceed the 32-bit offsets of the buffer loads");
-    dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, 1), block(256);
+    dim3 grid((unsigned)(((N + TN - 1) / TN) * ((M + TM - 1) / TM))), block(256);
     racer_timed timer(dev, epi == RACER_EPI_TANH || epi == RACER_EPI_TANH_MLP
                                ? "gemm_bf16_tanh"
                                : (epi == RACER_EPI_MASK ? "gemm_bf16_mask" : "gemm_bf16"));
