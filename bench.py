@@ -86,6 +86,45 @@ def ppo_flops(cars: int, beams: int, hidden: int, rollout: int, epochs: int) -> 
     return (rollout + 1) * cars * fwd + epochs * rollout * cars * (fwd + bwd)
 
 
+def forward_precision(R, dev, pol, tr, cfg, cars: int, t: int = 64) -> dict:
+    """Error of the policy outputs (steer mean, throttle mean, value) on the
+    observations of rollout step t (all cars): the bf16 forward and a float32
+    forward of the same weights, each against a float64 restatement of the
+    three layers (numpy, in this function)."""
+    import numpy as np
+
+    IN, H, HA = cfg.IN, cfg.hidden, cfg.hidden + 1
+    x = R.mat(tr.obs, IN, cars, t * IN * cars)
+    acts_b = R.Activations(dev, cars, cfg)
+    pol.forward(x, acts_b)
+    cfg32 = R.Config(beams=cfg.beams, hidden=cfg.hidden, precision=0, heading_noise=0.1)
+    pol32 = R.Policy(dev, 42, cfg32)
+    pol32.theta.write(pol.theta.read())
+    acts_f = R.Activations(dev, cars, cfg32)
+    pol32.forward(x, acts_f)
+    dev.synchronize()
+    o_b = acts_b.o.read()[:3 * cars].reshape(3, cars).astype(np.float64)
+    o_f = acts_f.o.read()[:3 * cars].reshape(3, cars).astype(np.float64)
+    th = pol.theta.read().astype(np.float64)
+    W0 = th[:H * IN].reshape(H, IN)
+    W1 = th[H * IN:H * IN + H * HA].reshape(H, HA)
+    W2 = th[H * IN + H * HA:H * IN + H * HA + 3 * HA].reshape(3, HA)
+    X = x.read().astype(np.float64).reshape(IN, cars)
+    ones = np.ones((1, cars))
+    h1 = np.tanh(W0 @ X)
+    h2 = np.tanh(W1 @ np.vstack([h1, ones]))
+    o = W2 @ np.vstack([h2, ones])
+    out = {}
+    for r, name in enumerate(("steer_mean", "throttle_mean", "value")):
+        scale = float(np.sqrt(np.mean(o[r] ** 2)))
+        out[name] = {"rms_float64": scale,
+                     "bf16": {"rms_error": float(np.sqrt(np.mean((o_b[r] - o[r]) ** 2))),
+                              "max_error": float(np.max(np.abs(o_b[r] - o[r])))},
+                     "float32": {"rms_error": float(np.sqrt(np.mean((o_f[r] - o[r]) ** 2))),
+                                 "max_error": float(np.max(np.abs(o_f[r] - o[r])))}}
+    return {"columns": cars, "rollout_step": t, "outputs": out}
+
+
 def ppo_leg(R, T, dev, beams: int, iterations: int, cars: int = 16384, precision: int = 1) -> dict:
     """PPO iterations on the bridge (two 8 m lobes, deck 0.5 m above the
     crossing, two layers): 16384 cars, 128-step rollouts replayed from a HIP
@@ -121,6 +160,7 @@ def ppo_leg(R, T, dev, beams: int, iterations: int, cars: int = 16384, precision
     dev.profile(False)
     prof = dev.profile_results()
     top = sorted(prof.items(), key=lambda kv: -kv[1][1])[:10]
+    precision_check = forward_precision(R, dev, pol, tr, cfg, cars) if precision == 1 else None
     it_s = (sum(roll) + sum(upd)) / iterations
     flops = ppo_flops(cars, beams, 256, 128, 4)
     del graph
@@ -132,7 +172,8 @@ def ppo_leg(R, T, dev, beams: int, iterations: int, cars: int = 16384, precision
                        "rollout": 128, "epochs": 4, "minibatches": 8,
                        "precision": {0: "fp32", 1: "bf16+fp32", 2: "plain fp32"}[precision],
                        "lidar_waves_per_car": sim.waves_per_car},
-            "kernels_ms_per_iteration": {k: tot for k, (n, tot) in top}}
+            "kernels_ms_per_iteration": {k: tot for k, (n, tot) in top},
+            "forward_error_vs_float64": precision_check}
 
 
 BATCH_TRACKS = ("ring10", "ramp10", "bank10", "bridge8", "random0", "random1", "random2", "random3")
