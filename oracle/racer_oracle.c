@@ -616,6 +616,26 @@ long o_check_nearest_index(float b, float lo, float hi) {
     return bad;
 }
 
+/* The same with the caller-supplied limit of a run of calls, Q = max(|lo|,
+ * |hi|) / b (with 0.1 % slack) bounding |a / b| over the range. */
+long o_check_nearest_index_lim(float b, float lo, float hi) {
+    float inv = 1.0f / b;
+    float lim = racer_nearest_limit(racer_max(racer_abs(lo), racer_abs(hi)) * inv * 1.001f);
+    uint32_t ulo = racer_f2u(lo), uhi = racer_f2u(hi);
+    int64_t olo = (ulo & 0x80000000u) ? -(int64_t)(ulo & 0x7fffffffu) : (int64_t)ulo;
+    int64_t ohi = (uhi & 0x80000000u) ? -(int64_t)(uhi & 0x7fffffffu) : (int64_t)uhi;
+    long bad = 0;
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(+ : bad)
+    for (int64_t o = olo; o <= ohi; ++o) {
+        uint32_t u = o < 0 ? (uint32_t)(-o) | 0x80000000u : (uint32_t)o;
+        float a = racer_u2f(u);
+        float want = floorf(a / b + 0.5f);
+        float got = racer_nearest_index_lim(a, b, inv, lim);
+        if (racer_f2u(want) != racer_f2u(got)) ++bad;
+    }
+    return bad;
+}
+
 /* `count` seeded random (cap, num, den) triples: mismatches of
  * racer_min_quotient against racer_min(cap, num / den). */
 long o_check_min_quotient(long count, uint64_t seed) {

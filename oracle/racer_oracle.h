@@ -132,6 +132,7 @@ float o_scan_one(const float* d, size_t len, const float* s, long n, const float
 
 /* Exhaustive / sampled checks of the shared header's division shortcuts. */
 long o_check_nearest_index(float b, float lo, float hi);
+long o_check_nearest_index_lim(float b, float lo, float hi);
 long o_check_min_quotient(long count, uint64_t seed);
 
 /* Threads used by the parallel loops (1 = serial). */

@@ -88,6 +88,15 @@ def test_nearest_index_shortcut_exhaustive(cell):
     assert O.lib().o_check_nearest_index(cell, -4096.0, -40.0) == 0
 
 
+@pytest.mark.parametrize("cell", [0.025, 0.05])
+def test_nearest_index_with_run_limit_exhaustive(cell):
+    """The lidar's form, one limit for a run of calls whose |a / cell| is
+    bounded: equal to floorf(a / cell + 0.5f) for every float of each run."""
+    O.set_threads(8)
+    for lo, hi in ((-40.0, 40.0), (-12.0, 3.0), (100.0, 130.0), (-2000.0, -1900.0)):
+        assert O.lib().o_check_nearest_index_lim(cell, lo, hi) == 0, (lo, hi)
+
+
 def test_min_quotient_shortcut_sampled():
     O.set_threads(8)
     assert O.lib().o_check_min_quotient(4_000_000, 11) == 0

@@ -247,6 +247,19 @@ RACER_FN float racer_nearest_index(float a, float b, float inv_b) {
     return floorf(a / b + 0.5f);
 }
 
+/* The same with the limit supplied by the caller for a run of calls:
+ * racer_nearest_limit(Q) = RN(0.5 - (Q + 1) 2^-20) is at most the per-call
+ * limit of every q0 with |q0| <= Q (rounding is monotone), so for such calls
+ * the test is stricter and the result the same. */
+RACER_FN float racer_nearest_limit(float Q) { return fmaf(Q, -0x1p-20f, 0.5f - 0x1p-20f); }
+RACER_FN float racer_nearest_index_lim(float a, float b, float inv_b, float lim) {
+    float q0 = a * inv_b;
+    float s0 = q0 + 0.5f;
+    float f0 = floorf(s0);
+    if (racer_abs((s0 - f0) - 0.5f) < lim) return f0;
+    return floorf(a / b + 0.5f);
+}
+
 /* racer_min(cap, num / den) with num / den correctly rounded, skipping the
  * division when the reciprocal estimate is clearly above cap (then so is the
  * exact quotient: |q0 - q| <= 2^-22 |q|). den >= 0; inv_den = RN(1/den),
@@ -254,7 +267,9 @@ RACER_FN float racer_nearest_index(float a, float b, float inv_b) {
  * racer_min returns cap either way when the estimate is +inf). */
 RACER_FN float racer_min_quotient(float cap, float num, float den, float inv_den) {
     float q0 = num * inv_den;
-    if (q0 > cap + racer_abs(cap) * 0x1p-20f + 0x1p-126f) return cap;
+    /* |cap| 2^-20 is exact above the subnormal range, so the fused form equals
+     * cap + |cap| 2^-20 there; below it the 2^-126 term dominates either way */
+    if (q0 > fmaf(racer_abs(cap), 0x1p-20f, cap) + 0x1p-126f) return cap;
     return racer_min(cap, num / den);
 }
 

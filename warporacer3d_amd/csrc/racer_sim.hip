@@ -719,9 +719,10 @@ __device__ __forceinline__ void probe_packed(const racer_dmap& m, __amdgpu_buffe
 // zeros, and clearance 0 is not drivable while height 0 - 1 is the grid's -1.
 template <int L>
 __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_rsrc_t rec, float x, float y, float z,
-                                           float* clearance, float* height) {
-    float fx = racer_nearest_index(x - m.x0, m.cell, m.inv_cell);
-    float fy = racer_nearest_index(y - m.y0, m.cell, m.inv_cell);
+                                           float limx, float limy, float* clearance, float* height) {
+    // limx, limy: racer_nearest_limit of bounds on |x - x0| / cell and |y - y0| / cell
+    float fx = racer_nearest_index_lim(x - m.x0, m.cell, m.inv_cell, limx);
+    float fy = racer_nearest_index_lim(y - m.y0, m.cell, m.inv_cell, limy);
     // non-short-circuit: the four compares combine without branches
     bool on = (fx >= 0.0f) & (fy >= 0.0f) & (fx < (float)m.nx) & (fy < (float)m.ny);
     // cell = fy nx + fx, exact in float (both below 2^12, the sum below 2^24)
@@ -795,6 +796,12 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     const float oz = (s[RACER_Z * n + car] + RACER_MOUNT_FORWARD * along) + RACER_MOUNT_UP;
     const float slope15 = 1.5f * m.slope;
     const float half_cell = 0.5f * m.cell;
+    // Every probe of the march lies within RACER_RANGE of the origin (t < RANGE,
+    // |direction| <= 1), so |x - x0| / cell stays below Qx (and likewise y):
+    // one nearest-cell limit per axis for the whole wavefront.
+    const float qx = (fabsf(ox - m.x0) + (RACER_RANGE + 1.0f)) * m.inv_cell * 1.001f;
+    const float qy = (fabsf(oy - m.y0) + (RACER_RANGE + 1.0f)) * m.inv_cell * 1.001f;
+    const float limx = racer_nearest_limit(qx), limy = racer_nearest_limit(qy);
 
     // Set-up of beams 64 c .. 64 c + 63, one per lane, into ring half c % 2:
     // field f of beam b lives at ring[128 f + b % 128].
@@ -847,7 +854,7 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
             float px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
             float clearance, height;
             if constexpr (L == 1 || L == 2)
-                probe_flat<L>(m, rec, px, py, pz, &clearance, &height);
+                probe_flat<L>(m, rec, px, py, pz, limx, limy, &clearance, &height);
             else
                 probe_packed<L>(m, rec, px, py, pz, &clearance, &height);
             float fl = (pz - height) - 0.03f;
