@@ -752,6 +752,8 @@ __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_
 
 constexpr int LIDAR_RECORD = 5;               // dx, dy, dz, descent, 1/descent
 constexpr int LIDAR_RING = 2 * LIDAR_RECORD * 64;  // two chunks of 64 beam records
+constexpr int LIDAR_COOP_WAVES = 16;                // wavefronts per workgroup with cooperative stores
+constexpr size_t LIDAR_COOP_MAX_SHMEM = 80 * 1024;  // two such workgroups per CU (160 KB of LDS)
 
 // scan (racer/sim.mojo:238-266) for every beam of one car per wavefront.
 //
@@ -765,11 +767,17 @@ constexpr int LIDAR_RING = 2 * LIDAR_RECORD * 64;  // two chunks of 64 beam reco
 //
 // lidar_wave marches wavefront `global_wave`'s share (car global_wave /
 // waves_per_car); the kernels below map workgroups to wavefronts.
-template <int L, bool COUNT>
+//
+// COOP: the workgroup's cars (consecutive, from wg_car0) keep all their
+// ranges in LDS ([car][beam] after the wavefronts' rings) and the workgroup
+// stores them together after one barrier, each obs row as a run of
+// consecutive cars: partial 128-byte lines of 32 bytes (8 cars) instead of one
+// float per line from each wavefront. Otherwise each wavefront stores its own.
+template <int L, bool COUNT, bool COOP>
 __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __restrict__ s, int n,
                                            float* __restrict__ obs, int64_t obs_cols, int total_beams, int per_wave,
                                            int waves_per_car, const float2* __restrict__ dirs,
-                                           unsigned long long* counter, int global_wave) {
+                                           unsigned long long* counter, int global_wave, int wg_car0) {
     extern __shared__ float lds[];
     const int lane = threadIdx.x & 63;
     // wavefront-uniform (the compiler cannot see it through threadIdx.x): the
@@ -777,13 +785,17 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     global_wave = __builtin_amdgcn_readfirstlane(global_wave);
     const int car = global_wave / waves_per_car;
-    if (car >= n) return;
     // This wavefront's share of the car's beams: [lo, lo + beams).
     const int lo = (global_wave % waves_per_car) * per_wave;
     const int beams = min(total_beams, lo + per_wave) - lo;
-    if (beams <= 0) return;
-    float* ring = lds + wave * (LIDAR_RING + per_wave);
-    float* ranges = ring + LIDAR_RING;
+    const bool active = car < n && beams > 0;
+    if constexpr (!COOP) {
+        if (!active) return;
+    }
+    const int wg_waves = (int)(blockDim.x >> 6);
+    float* ring = COOP ? lds + wave * LIDAR_RING : lds + wave * (LIDAR_RING + per_wave);
+    float* ranges = COOP ? lds + wg_waves * LIDAR_RING + (int64_t)(car - wg_car0) * total_beams + lo : ring + LIDAR_RING;
+    if (active) {
     __amdgpu_buffer_rsrc_t rec = __builtin_amdgcn_make_buffer_rsrc(
         (void*)m.packed, (short)0, (int)(m.plane * (L > 0 ? L : 1) * 8), 0x00020000);
 
@@ -901,10 +913,12 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
             next += count;
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int j = lane; j < beams; j += 64) obs[(int64_t)(4 + lo + j) * obs_cols + car] = ranges[j] / RACER_RANGE;
+    if constexpr (!COOP) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int j = lane; j < beams; j += 64) obs[(int64_t)(4 + lo + j) * obs_cols + car] = ranges[j] / RACER_RANGE;
+    }
     if (COUNT) {
         for (int off = 32; off > 0; off >>= 1) iterations += __shfl_xor(iterations, off);
         if (lane == 0) {
@@ -912,15 +926,29 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
             atomicAdd(counter + 1, (unsigned long long)trips);
         }
     }
+    }  // active
+    if constexpr (COOP) {
+        __syncthreads();
+        const int cars = wg_waves / waves_per_car;
+        const float* all = lds + wg_waves * LIDAR_RING;
+        for (int idx = (int)threadIdx.x; idx < cars * total_beams; idx += (int)blockDim.x) {
+            const int c = idx % cars, j = idx / cars;
+            if (wg_car0 + c < n) obs[(int64_t)(4 + j) * obs_cols + wg_car0 + c] = all[c * total_beams + j] / RACER_RANGE;
+        }
+    }
 }
 
-template <int L, bool COUNT>
-__global__ void __launch_bounds__(256) k_lidar(racer_dmap m, const float* __restrict__ s, int n,
-                                               float* __restrict__ obs, int64_t obs_cols, int total_beams,
-                                               int per_wave, int waves_per_car, const float2* __restrict__ dirs,
-                                               unsigned long long* counter) {
-    const int global_wave = (int)(((int64_t)xcd_block() * blockDim.x + threadIdx.x) >> 6);
-    lidar_wave<L, COUNT>(m, s, n, obs, obs_cols, total_beams, per_wave, waves_per_car, dirs, counter, global_wave);
+template <int L, bool COUNT, bool COOP>
+__global__ void __launch_bounds__(COOP ? 1024 : 256) k_lidar(racer_dmap m, const float* __restrict__ s, int n,
+                                                             float* __restrict__ obs, int64_t obs_cols,
+                                                             int total_beams, int per_wave, int waves_per_car,
+                                                             const float2* __restrict__ dirs,
+                                                             unsigned long long* counter) {
+    const int block = xcd_block();
+    const int global_wave = (int)(((int64_t)block * blockDim.x + threadIdx.x) >> 6);
+    const int wg_car0 = (int)((int64_t)block * (blockDim.x >> 6) / waves_per_car);
+    lidar_wave<L, COUNT, COOP>(m, s, n, obs, obs_cols, total_beams, per_wave, waves_per_car, dirs, counter,
+                               global_wave, wg_car0);
 }
 
 // The lidar of several sims in one launch: wavefront g belongs to the
@@ -936,11 +964,14 @@ __global__ void __launch_bounds__(256) k_lidar_batch(racer_sim_table tab, int pe
     const racer_sim_slot& q = tab.slot[k];
     const int w = (int)(g - tab.start[k]);
     if (q.map.layers == 1)
-        lidar_wave<1, COUNT>(q.map, q.state, q.n, q.obs, q.n, q.beams, per_wave, waves_per_car, q.dirs, counter, w);
+        lidar_wave<1, COUNT, false>(q.map, q.state, q.n, q.obs, q.n, q.beams, per_wave, waves_per_car, q.dirs, counter,
+                                     w, 0);
     else if (q.map.layers == 2)
-        lidar_wave<2, COUNT>(q.map, q.state, q.n, q.obs, q.n, q.beams, per_wave, waves_per_car, q.dirs, counter, w);
+        lidar_wave<2, COUNT, false>(q.map, q.state, q.n, q.obs, q.n, q.beams, per_wave, waves_per_car, q.dirs, counter,
+                                     w, 0);
     else
-        lidar_wave<0, COUNT>(q.map, q.state, q.n, q.obs, q.n, q.beams, per_wave, waves_per_car, q.dirs, counter, w);
+        lidar_wave<0, COUNT, false>(q.map, q.state, q.n, q.obs, q.n, q.beams, per_wave, waves_per_car, q.dirs, counter,
+                                     w, 0);
 }
 
 // Physics of several sims in one launch (one thread per car of any instance).
@@ -1049,17 +1080,42 @@ static void launch_lidar(racer_sim* sim, float* obs, int64_t obs_cols) {
     // holds its beam ring and its share of the ranges in LDS.
     int wpc = sim->waves_per_car;
     int per = (sim->beams + wpc - 1) / wpc;
+    int64_t total = (int64_t)n * wpc;
+    // cooperative stores when a workgroup holds at least 8 cars (runs of 32
+    // bytes; 4 cars measured slower than per-wavefront stores) and 16
+    // wavefronts' rings plus their cars' ranges leave room for two workgroups per CU
+    const size_t coop_shmem = ((size_t)LIDAR_COOP_WAVES * LIDAR_RING + (size_t)(LIDAR_COOP_WAVES / wpc) * sim->beams) *
+                              sizeof(float);
+    if (LIDAR_COOP_WAVES / wpc >= 8 && coop_shmem <= LIDAR_COOP_MAX_SHMEM) {
+        dim3 grid((unsigned)((total + LIDAR_COOP_WAVES - 1) / LIDAR_COOP_WAVES)), block(64 * LIDAR_COOP_WAVES);
+        if (sim->count_iterations)
+            hipLaunchKernelGGL((k_lidar<L, true, true>), grid, block, coop_shmem, sim->dev->stream, sim->map,
+                               sim->state->ptr, n, obs, obs_cols, sim->beams, per, wpc, sim->dirs, sim->counter);
+        else
+            hipLaunchKernelGGL((k_lidar<L, false, true>), grid, block, coop_shmem, sim->dev->stream, sim->map,
+                               sim->state->ptr, n, obs, obs_cols, sim->beams, per, wpc, sim->dirs, sim->counter);
+        return;
+    }
     size_t bytes_per_wave = (size_t)(LIDAR_RING + per) * sizeof(float);
     int waves = 4 * bytes_per_wave <= 65536 ? 4 : 1;
-    int64_t total = (int64_t)n * wpc;
     dim3 grid((unsigned)((total + waves - 1) / waves)), block(64 * waves);
     size_t shmem = waves * bytes_per_wave;
     if (sim->count_iterations)
-        hipLaunchKernelGGL((k_lidar<L, true>), grid, block, shmem, sim->dev->stream, sim->map, sim->state->ptr, n, obs,
-                           obs_cols, sim->beams, per, wpc, sim->dirs, sim->counter);
+        hipLaunchKernelGGL((k_lidar<L, true, false>), grid, block, shmem, sim->dev->stream, sim->map, sim->state->ptr,
+                           n, obs, obs_cols, sim->beams, per, wpc, sim->dirs, sim->counter);
     else
-        hipLaunchKernelGGL((k_lidar<L, false>), grid, block, shmem, sim->dev->stream, sim->map, sim->state->ptr, n,
-                           obs, obs_cols, sim->beams, per, wpc, sim->dirs, sim->counter);
+        hipLaunchKernelGGL((k_lidar<L, false, false>), grid, block, shmem, sim->dev->stream, sim->map,
+                           sim->state->ptr, n, obs, obs_cols, sim->beams, per, wpc, sim->dirs, sim->counter);
+}
+
+// The cooperative kernels may use more than the default 64 KB of dynamic LDS.
+static int lidar_coop_setup() {
+    const void* kernels[] = {(const void*)k_lidar<0, false, true>, (const void*)k_lidar<0, true, true>,
+                             (const void*)k_lidar<1, false, true>, (const void*)k_lidar<1, true, true>,
+                             (const void*)k_lidar<2, false, true>, (const void*)k_lidar<2, true, true>};
+    for (const void* k : kernels)
+        RACER_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LIDAR_COOP_MAX_SHMEM));
+    return 0;
 }
 
 static void launch_lidar_any(racer_sim* sim, float* obs, int64_t obs_cols) {
@@ -1338,6 +1394,10 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
                              dev->stream));
     RACER_TRY(hipMalloc(&sim->counter, 2 * sizeof(unsigned long long)));
     RACER_TRY(hipStreamSynchronize(dev->stream));
+    if (lidar_coop_setup()) {
+        racer_sim_destroy(sim);
+        return 1;
+    }
     *out = sim;
     return 0;
 }
