@@ -137,7 +137,7 @@ def ppo_leg(R, T, dev, beams: int, iterations: int, cars: int = 16384, precision
     pol = R.Policy(dev, 42, cfg)
     acts = R.Activations(dev, cars, cfg)
     tr = R.Trainer(dev, cars, cfg)
-    sim.autotune()
+    waves_by_replay(dev, sim.set_waves_per_car, lambda k: tr.capture_rollout(sim, pol, acts), 0)
     graph = tr.capture_rollout(sim, pol, acts)
     tr.rollout_graph(graph, 0)  # untimed first iteration
     tr.update(pol)
@@ -180,6 +180,30 @@ BATCH_TRACKS = ("ring10", "ramp10", "bank10", "bridge8", "random0", "random1", "
 BATCH_CARS = (256, 512, 1024, 2048, 4096, 4096, 8192, 8192)
 
 
+def waves_by_replay(dev, set_waves, capture, t: int, steps: int = 10) -> tuple[int, int]:
+    """The lidar's wavefronts per car (1, 2 or 4) with the fastest replay of a
+    short captured graph of the real steps (the lidar-only autotuner times
+    back-to-back lidar launches, whose cache state differs from a step's).
+    Returns the choice and the next free step index."""
+    best, pick = None, 2
+    for w in (1, 2, 4):
+        set_waves(w)
+        g = capture(steps)
+        g.launch(t)
+        t += steps
+        dev.synchronize()
+        t0 = time.perf_counter()
+        g.launch(t)
+        t += steps
+        dev.synchronize()
+        dt = time.perf_counter() - t0
+        if best is None or dt < best:
+            best, pick = dt, w
+        del g
+    set_waves(pick)
+    return pick, t
+
+
 def batch_leg(R, T, dev, steps: int) -> dict:
     """Eight uneven instances (flat, ramp, bank, bridge and four random routes;
     256 .. 8192 cars, 1081 beams, one 2 x 256 bf16 policy): Runner steps of all
@@ -203,33 +227,35 @@ def batch_leg(R, T, dev, steps: int) -> dict:
     batch = R.Batch(dev, sims, cfg)
     for t in range(5):
         batch.runner_step(pol, False, t)
-    wpc = batch.autotune()
+    wpc, t = waves_by_replay(dev, batch.set_waves_per_car, lambda k: batch.capture(pol, k), 5)
     graph = batch.capture(pol, chunk)
-    graph.launch(5)
+    graph.launch(t)
     dev.synchronize()
     t0 = time.perf_counter()
     for k in range(steps // chunk):
-        graph.launch(5 + chunk * (k + 1))
+        graph.launch(t + chunk * (k + 1))
     dev.synchronize()
     batched_ms = (time.perf_counter() - t0) * 1000.0 / steps
     del graph, batch, sims
     # separate: each instance's own graph of Runner steps, replayed in turn
     sims = instances()
-    own = []
+    own, t_own = [], 5
     for s in sims:
         acts = R.Activations(dev, s.n, cfg)
         scratch = dev.alloc(2 * s.n)
         for t in range(5):
             R.runner_step(s, pol, acts, scratch, False, t)
-        s.autotune()
+        _, t_s = waves_by_replay(dev, s.set_waves_per_car,
+                                 lambda k, s=s, acts=acts, scratch=scratch: R.capture_runner(s, pol, acts, scratch, k), 5)
+        t_own = max(t_own, t_s)
         own.append(R.capture_runner(s, pol, acts, scratch, chunk))
     for g in own:
-        g.launch(5)
+        g.launch(t_own)
     dev.synchronize()
     t0 = time.perf_counter()
     for k in range(steps // chunk):
         for g in own:
-            g.launch(5 + chunk * (k + 1))
+            g.launch(t_own + chunk * (k + 1))
     dev.synchronize()
     separate_ms = (time.perf_counter() - t0) * 1000.0 / steps
     del own, sims
@@ -424,19 +450,19 @@ def main() -> int:
     for t in range(args.warmup):
         run.step(False, t)
     dev.synchronize()
-    waves_per_car = run.sim.autotune()
-    log("autotuned lidar wavefronts per car:", waves_per_car)
+    waves_per_car, t = waves_by_replay(dev, run.sim.set_waves_per_car, run.capture, args.warmup)
+    log("lidar wavefronts per car (fastest graph replay of 1, 2, 4):", waves_per_car)
     # march iterations of the lidar, counted by the instrumented variant on the warm state (untimed)
     run.sim.count_iterations(True)
     counts, trips = [], []
-    for t in range(3):
-        run.step(False, args.warmup + t)
+    for k in range(3):
+        run.step(False, t + k)
         counts.append(run.sim.last_iterations)
         trips.append(run.sim.last_trips)
     run.sim.count_iterations(False)
     dev.synchronize()
     log("lidar iterations per step:", counts, "wavefront loop trips per step:", trips)
-    t = args.warmup + 3
+    t += 3
     # Timed region: the Runner steps replayed from a captured HIP graph (the
     # same kernels and arguments as eager steps, bitwise the same results).
     chunk = math.gcd(args.steps, 100)
