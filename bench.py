@@ -271,7 +271,7 @@ def surrogate_leg(R, T, dev, transitions: int, epochs: int, cars: int = 16384) -
     outward): transitions from Runner steps of 16384 cars x 1081 beams with a
     random-init 2 x 256 bf16 policy, bf16 training of the 3 x 128 residual MLP,
     then 8-step imagined rollouts of 16384 cars (exact lidar, RK4 fallback below
-    0.1 m clearance or on a layer change) judged by float64 RK4 on the GPU."""
+    0.1 m clearance or on a layer change) measured against float64 RK4 on the GPU."""
     import numpy as np
 
     arr = T.cached("bank10")
@@ -310,7 +310,7 @@ def surrogate_leg(R, T, dev, transitions: int, epochs: int, cars: int = 16384) -
     imagined, exact = twins
     start = sim.state.read().reshape(11, cars)
     st = np.ascontiguousarray(start[[0, 1, 2, 3, 4, 5]].astype(np.float64))
-    judge = dev.upload(st.view(np.float32).ravel())
+    ref64 = dev.upload(st.view(np.float32).ravel())
     fallback = dev.alloc(cars)
     fell = np.zeros(cars, bool)
     respawned = np.zeros(cars, bool)
@@ -321,11 +321,11 @@ def surrogate_leg(R, T, dev, transitions: int, epochs: int, cars: int = 16384) -
         exact.actions.write(actions[k])
         sur.step(imagined, 0.1, fallback)
         exact.step()
-        sim.step_f64(judge, R.mat(ab, 2, cars))
+        sim.step_f64(ref64, R.mat(ab, 2, cars))
         fell |= fallback.read() > 0
         respawned |= (imagined.done.read() != 0) | (exact.done.read() != 0)
         if k in (0, 7):
-            j = judge.read().view(np.float64).reshape(6, cars)
+            j = ref64.read().view(np.float64).reshape(6, cars)
             a = imagined.state.read().reshape(11, cars)
             b = exact.state.read().reshape(11, cars)
             ok = ~respawned

@@ -254,7 +254,7 @@ int racer_surrogate_step(racer_surrogate* s, racer_sim* sim, float min_clearance
  * separate input, product and step kernels (0); the results are the same. */
 int racer_surrogate_set_fused(racer_surrogate* s, int fused);
 /* One step of the kinematics (racer/sim.mojo:184-219) in float64 on the GPU
- * (the surrogate's judge): st holds [6, n] doubles (x, y, z, heading, speed,
+ * (the surrogate's reference): st holds [6, n] doubles (x, y, z, heading, speed,
  * steer) in 12 n floats and is updated in place; friction comes from the sim's
  * state; no crash, progress or respawn. */
 int racer_sim_step_f64(racer_sim* sim, racer_buf* st, const racer_view* actions);

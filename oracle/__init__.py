@@ -1,6 +1,6 @@
 """TEST INFRASTRUCTURE ONLY: ctypes binding of the CPU oracle (racer_oracle.c).
 
-Imported by tests/, __graft_entry__.smoke() (as the checker) and bench.py's
+Imported by tests/, __graft_entry__.smoke() (to compare against) and bench.py's
 cpu_baseline leg. Arrays are numpy float32, feature-major [rows, cols]; every
 function here restates one reference function and works on host memory only.
 """

@@ -2,7 +2,7 @@
  * TEST INFRASTRUCTURE ONLY: the CPU oracle. A plain-C restatement of the
  * reference's per-step hot path, one function per reference function, each
  * citing the file:line it follows. Only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg load it, as the checker or the CPU baseline.
+ * bench.py's cpu_baseline leg load it, to compare against or as the CPU baseline.
  *
  * It shares exactly two headers with the product: the constant/row layout
  * (warporacer3d_amd/csrc/racer_contract.h, the reference's own constants) and
@@ -781,7 +781,7 @@ double o_mlp_gradient(const float* theta, const int* widths, int layers, const f
     return loss;
 }
 
-/* ---------------- FP64 RK4 (the judge of the surrogate) ---------------- */
+/* ---------------- FP64 RK4 (the surrogate's float64 reference) ---------------- */
 
 static void f64_probe(const omap* m, const float* d, double x, double y, double z, int* layer, double* clearance,
                       double* height) {
@@ -833,7 +833,7 @@ static void f64_derivative(const double p[4], double steer, double accel, double
 
 /* One environment step of the kinematics (sim.mojo:184-219) in float64 for n
  * cars: st rows x, y, z, heading, speed, steer (updated in place); friction
- * and actions as the sim's. No crash or respawn: the judge is for windows
+ * and actions as the sim's. No crash or respawn: the reference is for windows
  * without them. */
 void o_step_f64(const float* d, size_t len, long n, double* st, const float* friction, const float* act,
                 long act_cols) {

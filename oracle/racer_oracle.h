@@ -1,7 +1,7 @@
 /*
  * TEST INFRASTRUCTURE ONLY. The CPU oracle: a plain-C restatement of the
  * reference's simulation and PPO (racer/sim.mojo, racer/ppo.mojo), used by the
- * tests, by smoke() as the checker, and by bench.py's cpu_baseline leg. No
+ * tests, by smoke() to compare against, and by bench.py's cpu_baseline leg. No
  * product code links or calls it.
  *
  * Layout follows the reference: every buffer is feature-major float32
@@ -112,7 +112,7 @@ void o_adam(float* theta, long P, long logstd_at, const float* grad, float* firs
 float o_bias_correction(float beta, int step);
 
 /* Learned dynamics surrogate: inputs, targets, the float32 MLP and the
- * float64 RK4 step that judges it (see the .c file). */
+ * float64 RK4 step it is measured against (see the .c file). */
 void o_surrogate_features(const float* d, size_t len, const float* s, long n, const float* act, long act_cols,
                           float* feat, long ldf);
 void o_surrogate_targets(const float* prev, const float* s, const float* done, long n, float* tgt, long ldt,

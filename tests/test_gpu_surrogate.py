@@ -1,6 +1,6 @@
 """The learned dynamics surrogate on the GPU: its inputs and targets and its
 float32 gradient bit for bit against the oracle, the bf16 gradient within its
-rounding bound, the float64 judge against the oracle's, the fallback to the
+rounding bound, the float64 reference step against the oracle's, the fallback to the
 exact step, and what training gains: error against the float64 RK4 far below
 the untrained network's while the network serves most car-steps."""
 import numpy as np
@@ -76,8 +76,8 @@ def test_float32_gradient_bit_identical_and_bf16_bounded(dev, cols):
             assert np.dot(got, ref) / (np.linalg.norm(got) * np.linalg.norm(ref)) > 0.999
 
 
-def test_f64_judge_matches_oracle(dev):
-    print("f64 judge", flush=True)
+def test_f64_reference_matches_oracle(dev):
+    print("f64 reference step", flush=True)
     _, sim, pol, acts, scratch = _setup(dev, n=300, name="bridge8")
     for t in range(4):
         R.runner_step(sim, pol, acts, scratch, False, t)
@@ -142,7 +142,7 @@ def test_fused_step_matches_separate_kernels(dev, name, n):
 
 
 def _one_step_error(dev, sur, sim, n, rng):
-    """Position error (m) of one surrogate step against the float64 judge over
+    """Position error (m) of one surrogate step against the float64 reference over
     cars that neither fall back nor crash."""
     s = sim.state.read().reshape(11, n)
     st = np.ascontiguousarray(s[[0, 1, 2, 3, 4, 5]].astype(np.float64))
@@ -150,12 +150,12 @@ def _one_step_error(dev, sur, sim, n, rng):
     sim.actions.write(act)
     buf, ab = _f64_buffer(dev, st), dev.upload(act)
     sim.step_f64(buf, R.mat(ab, 2, n))
-    judge = _read_f64(buf, n)
+    ref = _read_f64(buf, n)
     fallback = dev.alloc(n)
     sur.step(sim, min_clearance=0.1, fallback=fallback)
     s1 = sim.state.read().reshape(11, n)
     ok = (fallback.read() == 0) & (sim.done.read() == 0)
-    err = np.hypot(s1[0] - judge[0], s1[1] - judge[1])[ok]
+    err = np.hypot(s1[0] - ref[0], s1[1] - ref[1])[ok]
     return float(np.sqrt(np.mean(err ** 2))), float(ok.mean())
 
 

@@ -1,6 +1,6 @@
 """Pins of the oracle's surrogate pieces: the float32 MLP and its gradient
 against a float64 numpy MLP and central finite differences; the float64 RK4
-step (the surrogate's judge) against a closed form and against the float32
+step (the surrogate's float64 reference) against a closed form and against the float32
 step of the reference restatement."""
 import numpy as np
 import pytest

@@ -73,7 +73,7 @@ def lib() -> C.CDLL:
     if _lib is not None:
         return _lib
     if not LIB_PATH.exists():
-        raise RacerError(f"{LIB_PATH} is missing: run warporacer3d_amd/build.py (or __graft_entry__.build())")
+        raise RacerError(f"{LIB_PATH} is missing: run warporacer3d_amd/build.py")
     L = C.CDLL(str(LIB_PATH))
     vp, i, i64, f, sz = C.c_void_p, C.c_int, C.c_int64, C.c_float, C.c_size_t
     cp = C.POINTER(C.c_void_p)

@@ -556,11 +556,11 @@ __global__ void __launch_bounds__(256) k_surrogate_fused(racer_dmap m, float* s,
     }
 }
 
-// ---- float64 RK4 (the surrogate's judge) ----
+// ---- float64 RK4 (the reference the surrogate is measured against) ----
 // The kinematics of advance (racer/sim.mojo:184-219) in float64 with the
 // true step 1/240 s: probe and bilinear surface on the float grids read as
 // doubles, libm-accurate double sin, cos, tan and sqrt. No crash, progress or
-// respawn: the judge is applied to windows without them.
+// respawn: it is applied to windows without them.
 __device__ void f64_probe(const racer_dmap& m, double x, double y, double z, int* layer, double* clearance,
                           double* height) {
     const double fx = floor((x - (double)m.x0) / (double)m.cell + 0.5);
