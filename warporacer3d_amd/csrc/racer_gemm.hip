@@ -148,13 +148,13 @@ __global__ void __launch_bounds__(256) k_gemm_nt_f32(const float* __restrict__ A
 // for 64 rows n. B arrives 64 rows x 64 k per stage through LDS with float4
 // loads (16 threads per 256-byte row); the next stage is loaded while this one
 // is summed.
-constexpr int SK_N = 64, SK_K = 64;
+constexpr int SK_N = 64, SK_K = 64, SK_STAGES = 4;
 
 __global__ void __launch_bounds__(256) k_gemm_nt_f32_skinny(const float* __restrict__ A, int64_t lda,
                                                             const float* __restrict__ B, int64_t ldb,
                                                             float* __restrict__ out, int64_t out_stride, int64_t ldc,
                                                             int M, int N, int K, int chunk) {
-    __shared__ float Bs[SK_N][SK_K + 1];
+    __shared__ float Bs[SK_K][SK_N + 1];  // [k][column]: the fmaf loop's reads are conflict-free
     __shared__ float As[4][SK_K];
     const int tid = threadIdx.x;
     const int n = tid & 63, j = tid >> 6;
@@ -163,41 +163,59 @@ __global__ void __launch_bounds__(256) k_gemm_nt_f32_skinny(const float* __restr
     const int kend = min(K, kbeg + chunk);
     const auto bres = racer_rsrc(B, ((uint64_t)(N - 1) * ldb + K) * 4);
     const auto ares = racer_rsrc(A, ((uint64_t)(M - 1) * lda + K) * 4);
-    float rb[4][4], ra[4];
-    auto load = [&](int k0) {
+    // SK_STAGES stages of SK_K in flight in registers: the chunk streams once
+    // from HBM, and one stage at a time leaves the wait for each stage's
+    // loads exposed
+    // B: 16 consecutive threads cover one column's 64 k as float4 loads (the
+    // host sends only 16-byte aligned rows and chunk bounds, so a float4 is
+    // wholly inside or wholly past the chunk)
+    float4 rb[SK_STAGES][4];
+    float ra[SK_STAGES];
+    auto load = [&](int k0, int u) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int idx = tid + 256 * i, r = idx >> 4, kq = (idx & 15) * 4;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int k = k0 + kq + e;
-                const uint32_t off =
-                    (n0 + r < N && k < kend) ? (uint32_t)(((int64_t)(n0 + r) * ldb + k) * 4) : RACER_OOB;
-                rb[i][e] = racer_ld_f32(bres, off);
-            }
+            const int k = k0 + kq;
+            const uint32_t off = (n0 + r < N && k < kend) ? (uint32_t)(((int64_t)(n0 + r) * ldb + k) * 4) : RACER_OOB;
+            rb[u][i] = __builtin_bit_cast(float4, racer_ld_b128(bres, off));
         }
-        {
-            const int r = tid >> 6, k = k0 + (tid & 63);
-            const uint32_t off = (r < M && k < kend) ? (uint32_t)(((int64_t)r * lda + k) * 4) : RACER_OOB;
-            ra[0] = racer_ld_f32(ares, off);
-        }
+        const int r = tid >> 6, k = k0 + (tid & 63);
+        const uint32_t off = (r < M && k < kend) ? (uint32_t)(((int64_t)r * lda + k) * 4) : RACER_OOB;
+        ra[u] = racer_ld_f32(ares, off);
     };
     float acc = 0.0f;
-    load(kbeg);
-    for (int k0 = kbeg; k0 < kend; k0 += SK_K) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int idx = tid + 256 * i, r = idx >> 4, kq = (idx & 15) * 4;
+    for (int u = 0; u < SK_STAGES; ++u) load(kbeg + u * SK_K, u);
+    for (int k0 = kbeg; k0 < kend; k0 += SK_STAGES * SK_K) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) Bs[r][kq + e] = rb[i][e];
+        for (int u = 0; u < SK_STAGES; ++u) {
+            const int ks = k0 + u * SK_K;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int idx = tid + 256 * i, r = idx >> 4, kq = (idx & 15) * 4;
+                Bs[kq + 0][r] = rb[u][i].x;
+                Bs[kq + 1][r] = rb[u][i].y;
+                Bs[kq + 2][r] = rb[u][i].z;
+                Bs[kq + 3][r] = rb[u][i].w;
+            }
+            As[tid >> 6][tid & 63] = ra[u];
+            __syncthreads();
+            // the slot is free again: this stage's successor SK_STAGES stages on
+            // (past the chunk the offsets are out of range and read zeros)
+            load(ks + SK_STAGES * SK_K, u);
+            const int kv = min(SK_K, kend - ks);
+            if (j < M) {
+                if (kv == SK_K) {
+                    // unrolled, so the LDS reads go out ahead of the fmaf chain
+                    // (k in order, as the loop below)
+#pragma unroll
+                    for (int q = 0; q < SK_K; ++q) acc = fmaf(As[j][q], Bs[q][n], acc);
+                } else {
+                    for (int q = 0; q < kv; ++q) acc = fmaf(As[j][q], Bs[q][n], acc);
+                }
+            }
+            __syncthreads();
         }
-        As[tid >> 6][tid & 63] = ra[0];
-        __syncthreads();
-        load(k0 + SK_K);
-        const int kv = min(SK_K, kend - k0);
-        if (j < M)
-            for (int q = 0; q < kv; ++q) acc = fmaf(As[j][q], Bs[n][q], acc);
-        __syncthreads();
     }
     if (j < M && n0 + n < N) out[(int64_t)blockIdx.z * out_stride + (int64_t)j * ldc + n0 + n] = acc;
 }
@@ -239,7 +257,8 @@ int racer_gemm_nt_f32(racer_device* dev, const float* A, int64_t lda, const floa
     int64_t stride = (int64_t)M * ldc;
     if (racer_ensure_scratch(dev, (size_t)(stride * chunks))) return 1;
     const bool reach = ((uint64_t)(M - 1) * lda + K) * 4 < 0xFFFFFF00ull && ((uint64_t)(N - 1) * ldb + K) * 4 < 0xFFFFFF00ull;
-    if (M <= 4 && reach) {
+    const bool aligned4 = ldb % 4 == 0 && K % 4 == 0 && (uintptr_t)B % 16 == 0;  // float4 loads of B
+    if (M <= 4 && reach && aligned4) {
         racer_timed timer(dev, "gemm_nt_f32_skinny");
         dim3 grid((N + SK_N - 1) / SK_N, 1, chunks), block(256);
         hipLaunchKernelGGL(k_gemm_nt_f32_skinny, grid, block, 0, dev->stream, A, lda, B, ldb, dev->scratch, stride,

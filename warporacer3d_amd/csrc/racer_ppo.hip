@@ -90,14 +90,26 @@ __global__ void __launch_bounds__(256) k_sum_partial(const float* x, int64_t cou
     }
 }
 
-__global__ void k_sum_final(const double* partial, int blocks, int outputs, double* out, float* out_f32,
-                            int64_t f32_slot) {
-    int o = threadIdx.x;
-    if (o >= outputs) return;
-    double acc = 0.0;
-    for (int b = 0; b < blocks; ++b) acc += partial[(int64_t)o * blocks + b];
-    if (out) out[o] = acc;
-    if (out_f32) out_f32[f32_slot + o] = (float)acc;
+// Output o = the sum of its `blocks` partials in a fixed order: thread t adds
+// partials t, t + 256, ... in order, then a fixed tree over the 256 threads.
+__global__ void __launch_bounds__(256) k_sum_final(const double* partial, int blocks, int outputs, double* out,
+                                                   float* out_f32, int64_t f32_slot) {
+    __shared__ double red[256];
+    for (int o = 0; o < outputs; ++o) {
+        double acc = 0.0;
+        for (int b = (int)threadIdx.x; b < blocks; b += 256) acc += partial[(int64_t)o * blocks + b];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            if (out) out[o] = red[0];
+            if (out_f32) out_f32[f32_slot + o] = (float)red[0];
+        }
+        __syncthreads();
+    }
 }
 
 // minibatch.gather (racer/ppo.mojo:276-279): x[r, b] = obs of sample s = b MB + m.
@@ -266,7 +278,7 @@ int row_sums(racer_device* dev, const float* x, int64_t count, int64_t row_strid
         racer_timed timer(dev, "sum");
         hipLaunchKernelGGL(k_sum_partial, dim3(blocks), dim3(256), 0, dev->stream, x, count, row_stride, rows, square,
                            partial);
-        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, dev->stream, partial, blocks, outputs, out64, out32,
+        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, dev->stream, partial, blocks, outputs, out64, out32,
                            slot);
     }
     return racer_launch_check("row sums");
