@@ -63,7 +63,7 @@ __host__ __device__ inline FusedLayout fused_layout(int H, int IN) {
 // HS > 0: each half of the first product's k range is exactly HS k-steps and
 // the fragment ring is unrolled over them (straight-line code, so each k-step
 // waits only for its own fragments); HS = 0: any k range, as a loop.
-template <bool SAMPLE, int HS>
+template <bool SAMPLE, int HS, bool VEC>
 __device__ __forceinline__ void policy_tile(
     const uint16_t* __restrict__ F0, const uint16_t* __restrict__ F1,
     const float* __restrict__ W2, const float* __restrict__ x, int64_t ldx, int H, int IN, int64_t cols,
@@ -89,7 +89,35 @@ __device__ __forceinline__ void policy_tile(
     const auto w1res = racer_rsrc(F1, (uint64_t)FRAG_TILES * steps1 * 1024);
 
     // ---- phase 0: the input slice, bf16, [column][k] ----
-    {
+    if constexpr (VEC) {
+        // 16-byte loads of 4 adjacent columns (x 16-byte aligned, ldx and cols
+        // multiples of 4): a thread takes 8 consecutive k of 4 columns with 8
+        // loads and stores 4 columns' 8 k as 16-byte LDS writes
+        const int chunks = Ly.in32 / 8;
+        for (int idx = tid; idx < chunks * (FT / 4); idx += blockDim.x) {
+            const int cq = (idx % (FT / 4)) * 4, kc = (idx / (FT / 4)) * 8;
+            const int64_t col = c0 + cq;
+            float4 f[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = kc + e;
+                const uint32_t off = (k < IN && col < cols) ? (uint32_t)(((int64_t)k * ldx + col) * 4) : RACER_OOB;
+                f[e] = __builtin_bit_cast(float4, racer_ld_b128(xres, off));
+            }
+            uint16_t v[4][8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                v[0][e] = bf16_of(f[e].x);
+                v[1][e] = bf16_of(f[e].y);
+                v[2][e] = bf16_of(f[e].z);
+                v[3][e] = bf16_of(f[e].w);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                *reinterpret_cast<uint4*>(xs + (cq + q) * Ly.kx + kc) = *reinterpret_cast<uint4*>(v[q]);
+        }
+        for (int idx = tid; idx < 3 * HA; idx += blockDim.x) w2s[idx] = W2[idx];
+    } else {
         const int chunks = Ly.in32 / 8;  // 8 consecutive k per chunk
         for (int idx = tid; idx < chunks * FT; idx += blockDim.x) {
             const int c = idx % FT, kc = (idx / FT) * 8;
@@ -271,15 +299,15 @@ __device__ __forceinline__ void policy_tile(
     }
 }
 
-template <bool SAMPLE, int HS>
+template <bool SAMPLE, int HS, bool VEC>
 __global__ void __launch_bounds__(64 * WAVES) k_policy_fused(
     const uint16_t* __restrict__ F0, const uint16_t* __restrict__ F1,
     const float* __restrict__ W2, const float* __restrict__ x, int64_t ldx, int H, int IN, int64_t cols,
     float* __restrict__ h1out, float* __restrict__ h2out, float* __restrict__ o, const float* __restrict__ log_std,
     float* act, int64_t act_cols, float* logp, float* value, int deterministic, int64_t step,
     const int64_t* step_base, int seed) {
-    policy_tile<SAMPLE, HS>(F0, F1, W2, x, ldx, H, IN, cols, h1out, h2out, o, log_std, act, act_cols, logp, value,
-                            deterministic, step, step_base, seed, (int64_t)xcd_block() * FT);
+    policy_tile<SAMPLE, HS, VEC>(F0, F1, W2, x, ldx, H, IN, cols, h1out, h2out, o, log_std, act, act_cols, logp,
+                                 value, deterministic, step, step_base, seed, (int64_t)xcd_block() * FT);
 }
 
 // The forward and sample of several sims' observations in one launch:
@@ -294,7 +322,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_fused_batch(
     int k = 0;
     while (k + 1 < tab.count && b >= tab.start[k + 1]) ++k;
     const racer_policy_slot& q = tab.slot[k];
-    policy_tile<true, HS>(F0, F1, W2, q.x, q.ldx, H, IN, q.cols, q.h1, q.h2, q.o, log_std, q.act, q.act_cols, q.logp,
+    policy_tile<true, HS, false>(F0, F1, W2, q.x, q.ldx, H, IN, q.cols, q.h1, q.h2, q.o, log_std, q.act, q.act_cols, q.logp,
                       q.value, deterministic, step, step_base, q.seed, (b - tab.start[k]) * FT);
 }
 
@@ -337,10 +365,10 @@ size_t racer_policy_fused_smem(int H, int IN) { return fused_layout(H, IN).total
 // inputs: 1081 beams) and 2 (113 inputs: 108 beams); other shapes loop.
 template <int HS>
 static int fused_setup_one() {
-    RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused<true, HS>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024));
-    RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused<false, HS>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024));
+    const void* kernels[] = {(const void*)k_policy_fused<true, HS, false>, (const void*)k_policy_fused<false, HS, false>,
+                             (const void*)k_policy_fused<true, HS, true>, (const void*)k_policy_fused<false, HS, true>};
+    for (const void* k : kernels)
+        RACER_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused_batch<HS>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024));
     return 0;
@@ -380,15 +408,21 @@ int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_ac
     dim3 grid((unsigned)((cols + FT - 1) / FT)), block(64 * WAVES);
     size_t shmem = racer_policy_fused_smem(p->hidden, (int)p->IN);
     const int hs = unrolled_half((int)p->IN);
+    const bool vec = x_cols % 4 == 0 && cols % 4 == 0 && (uintptr_t)x % 16 == 0;
     if (sample) {
-        auto kernel = hs == 17 ? k_policy_fused<true, 17> : (hs == 2 ? k_policy_fused<true, 2> : k_policy_fused<true, 0>);
+        auto kernel = vec ? (hs == 17 ? k_policy_fused<true, 17, true>
+                                      : (hs == 2 ? k_policy_fused<true, 2, true> : k_policy_fused<true, 0, true>))
+                          : (hs == 17 ? k_policy_fused<true, 17, false>
+                                      : (hs == 2 ? k_policy_fused<true, 2, false> : k_policy_fused<true, 0, false>));
         racer_timed timer(dev, "policy_fused_sample");
         hipLaunchKernelGGL(kernel, grid, block, shmem, dev->stream, f0, f1, W2, x, x_cols, p->hidden, (int)p->IN,
                            (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr, log_std, act, (int64_t)cols, logp, value,
                            deterministic, step, dev->step_base, seed);
     } else {
-        auto kernel =
-            hs == 17 ? k_policy_fused<false, 17> : (hs == 2 ? k_policy_fused<false, 2> : k_policy_fused<false, 0>);
+        auto kernel = vec ? (hs == 17 ? k_policy_fused<false, 17, true>
+                                      : (hs == 2 ? k_policy_fused<false, 2, true> : k_policy_fused<false, 0, true>))
+                          : (hs == 17 ? k_policy_fused<false, 17, false>
+                                      : (hs == 2 ? k_policy_fused<false, 2, false> : k_policy_fused<false, 0, false>));
         racer_timed timer(dev, "policy_fused");
         hipLaunchKernelGGL(kernel, grid, block, shmem, dev->stream, f0, f1, W2, x, x_cols, p->hidden, (int)p->IN,
                            (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr, log_std, act, (int64_t)cols, logp, value,
