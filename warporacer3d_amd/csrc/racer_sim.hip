@@ -666,12 +666,22 @@ __global__ void __launch_bounds__(256) k_reset(racer_dmap m, float* s, float* ac
 
 // Per-cell records for the lidar: (clearance, height) of every layer side by
 // side, so one probe is one 8-byte (one layer) or 16-byte (two layers) load.
+// A clearance that is not positive (NaN included) is stored as -1: Map.probe
+// only takes a layer whose clearance is positive and reports -1 otherwise, so
+// every reader's test gives the same answer and a single-layer probe can use
+// the stored value as is. Record `plane` (after the grid) is all -1, the
+// reference's off-map answer, for probes that load it instead of a cell.
 __global__ void k_pack(const float* d, int64_t plane, int layers, float* packed) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= plane) return;
+    if (c > plane) return;
     for (int l = 0; l < layers; ++l) {
-        packed[(c * layers + l) * 2 + 0] = d[RACER_HEADER + (int64_t)(3 * l + RACER_CLEARANCE) * plane + c];
-        packed[(c * layers + l) * 2 + 1] = d[RACER_HEADER + (int64_t)(3 * l + RACER_HEIGHT) * plane + c];
+        float cl = -1.0f, h = -1.0f;
+        if (c < plane) {
+            cl = d[RACER_HEADER + (int64_t)(3 * l + RACER_CLEARANCE) * plane + c];
+            h = d[RACER_HEADER + (int64_t)(3 * l + RACER_HEIGHT) * plane + c];
+        }
+        packed[(c * layers + l) * 2 + 0] = cl > 0.0f ? cl : -1.0f;
+        packed[(c * layers + l) * 2 + 1] = h;
     }
 }
 
@@ -715,24 +725,37 @@ __device__ __forceinline__ void probe_packed(const racer_dmap& m, __amdgpu_buffe
 }
 
 // Map.probe's (clearance, height) from the packed records with the load
-// always issued: off the map the offset is out of range, the hardware returns
-// zeros, and clearance 0 is not drivable while height 0 - 1 is the grid's -1.
+// always issued: off the map it reads the all -1 record after the grid.
+//
+// The nearest cell is racer_nearest_index_lim on both axes with one branch
+// for the rare division of either (a lane whose shortcut holds gets the same
+// value from the division, which is the formula the shortcut reproduces);
+// `live` = false lanes (their results unused) never take it.
 template <int L>
 __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_rsrc_t rec, float x, float y, float z,
-                                           float limx, float limy, float* clearance, float* height) {
+                                           float limx, float limy, bool live, float* clearance, float* height) {
     // limx, limy: racer_nearest_limit of bounds on |x - x0| / cell and |y - y0| / cell
-    float fx = racer_nearest_index_lim(x - m.x0, m.cell, m.inv_cell, limx);
-    float fy = racer_nearest_index_lim(y - m.y0, m.cell, m.inv_cell, limy);
-    // non-short-circuit: the four compares combine without branches
-    bool on = (fx >= 0.0f) & (fy >= 0.0f) & (fx < (float)m.nx) & (fy < (float)m.ny);
+    const float ax = x - m.x0, ay = y - m.y0;
+    const float sx = ax * m.inv_cell + 0.5f, sy = ay * m.inv_cell + 0.5f;
+    float fx = floorf(sx), fy = floorf(sy);
+    // non-short-circuit & and |: one exec mask, not a nest of them
+    const bool slow_x = !(racer_abs((sx - fx) - 0.5f) < limx), slow_y = !(racer_abs((sy - fy) - 0.5f) < limy);
+    if (live & (slow_x | slow_y)) {
+        fx = floorf(ax / m.cell + 0.5f);
+        fy = floorf(ay / m.cell + 0.5f);
+    }
+    // 0 <= f < n as one unsigned compare of the bit patterns: f is a floor
+    // of a sum with 0.5, never -0.0, so a set sign bit means negative, and
+    // NaN and +inf patterns lie above every finite n
+    const bool on = (__float_as_uint(fx) < __float_as_uint((float)m.nx)) &
+                    (__float_as_uint(fy) < __float_as_uint((float)m.ny));
     // cell = fy nx + fx, exact in float (both below 2^12, the sum below 2^24)
-    uint32_t off = on ? (uint32_t)fmaf(fy, (float)m.nx, fx) * (uint32_t)(8 * L) : RACER_OOB;
-    float off_map = on ? 0.0f : 1.0f;
+    const uint32_t off = on ? (uint32_t)fmaf(fy, (float)m.nx, fx) * (uint32_t)(8 * L) : (uint32_t)(m.plane * 8 * L);
     if constexpr (L == 1) {
+        // the stored clearance is -1 unless positive
         float2 r = racer_ld_f32x2(rec, off);
-        bool take = r.x > 0.0f && fabsf(r.y - z) < 1e30f;
-        *clearance = take ? r.x : -1.0f;
-        *height = r.y - off_map;
+        *clearance = fabsf(r.y - z) < 1e30f ? r.x : -1.0f;
+        *height = r.y;
     } else {
         float4 r = __builtin_bit_cast(float4, racer_ld_b128(rec, off));
         float cl = -1.0f, h = r.y, gap = 1e30f;
@@ -746,11 +769,11 @@ __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_
             h = r.w;
         }
         *clearance = cl;
-        *height = h - off_map;
+        *height = h;
     }
 }
 
-constexpr int LIDAR_RECORD = 5;               // dx, dy, dz, descent, 1/descent
+constexpr int LIDAR_RECORD = 4;               // dx, dy, dz, descent
 constexpr int LIDAR_RING = 2 * LIDAR_RECORD * 64;  // two chunks of 64 beam records
 constexpr int LIDAR_COOP_WAVES = 16;                // wavefronts per workgroup with cooperative stores
 constexpr size_t LIDAR_COOP_MAX_SHMEM = 80 * 1024;  // two such workgroups per CU (160 KB of LDS)
@@ -760,8 +783,8 @@ constexpr size_t LIDAR_COOP_MAX_SHMEM = 80 * 1024;  // two such workgroups per C
 // Each lane marches one beam at a time; a lane whose beam ends takes the next
 // unclaimed beam of the car (ballot + popcount), so the wavefront stays full
 // until the car's last beams. Beam set-up (direction, normalisation, floor
-// descent rate and its reciprocal) is done 64 beams at a time by the whole
-// wavefront into a two-chunk LDS ring, so a refill is five LDS reads rather
+// descent rate) is done 64 beams at a time by the whole wavefront into a
+// two-chunk LDS ring, so a refill is four LDS reads rather
 // than a divergent pass of divisions and square roots. Ranges collect in LDS
 // and are scaled and stored in one full-lane pass at the end.
 //
@@ -797,7 +820,7 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     float* ranges = COOP ? lds + wg_waves * LIDAR_RING + (int64_t)(car - wg_car0) * total_beams + lo : ring + LIDAR_RING;
     if (active) {
     __amdgpu_buffer_rsrc_t rec = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)m.packed, (short)0, (int)(m.plane * (L > 0 ? L : 1) * 8), 0x00020000);
+        (void*)m.packed, (short)0, (int)((m.plane + 1) * (L > 0 ? L : 1) * 8), 0x00020000);
 
     const float heading = s[RACER_HEADING * n + car];
     const float along = obs[2 * obs_cols + car];
@@ -833,10 +856,9 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
             slot[1 * 128] = dy;
             slot[2 * 128] = dz;
             slot[3 * 128] = descent;
-            slot[4 * 128] = 1.0f / descent;
         }
     };
-    float dx = 0.0f, dy = 0.0f, dz = 0.0f, descent = 0.0f, inv_descent = 0.0f;
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f, descent = 0.0f;
     float t = 0.0f, last = 0.0f, last_floor = 0.0f;
     auto take = [&](int b) {
         const float* slot = ring + (b & 127);
@@ -844,7 +866,6 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
         dy = slot[1 * 128];
         dz = slot[2 * 128];
         descent = slot[3 * 128];
-        inv_descent = slot[4 * 128];
         t = 0.0f;
         last = 0.0f;
         last_floor = 0.0f;
@@ -859,38 +880,36 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     if (b < beams) take(b);
     int next = 64;
     uint64_t iterations = 0, trips = 0;
+    // The march step below runs on every lane with selects in place of
+    // branches (lanes past the share's end compute on their last beam's
+    // registers and finish nothing): each divergent branch costs the
+    // wavefront exec-mask bookkeeping, as much issue time as a VALU
+    // instruction, on every trip. The step's quotient is always divided: some
+    // live lane needs the exact division on almost every trip (the floor
+    // bound is the smaller one on about one iteration in nine), so a
+    // shortcut around it only adds its test.
     while (__ballot(b < beams) != 0ull) {
-        bool finished = false;
-        if (b < beams) {
-            if (COUNT) ++iterations;
-            float px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
-            float clearance, height;
-            if constexpr (L == 1 || L == 2)
-                probe_flat<L>(m, rec, px, py, pz, limx, limy, &clearance, &height);
-            else
-                probe_packed<L>(m, rec, px, py, pz, &clearance, &height);
-            float fl = (pz - height) - 0.03f;
-            float range = 0.0f;
-            if (fl < 0.0f) {
-                range = racer_min(last + ((t - last) * last_floor) / (last_floor - fl), RACER_RANGE);
-                finished = true;
-            } else {
-                float step = racer_min_quotient(clearance - m.cell, fl, descent, inv_descent);
-                if (step < half_cell) {
-                    range = racer_min(t, RACER_RANGE);
-                    finished = true;
-                } else {
-                    last = t;
-                    last_floor = fl;
-                    t += step;
-                    if (!(t < RACER_RANGE)) {
-                        range = racer_min(t, RACER_RANGE);
-                        finished = true;
-                    }
-                }
-            }
-            if (finished) ranges[b] = range;
-        }
+        const bool live = b < beams;
+        if (COUNT) iterations += live ? 1 : 0;
+        const float px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
+        float clearance, height;
+        if constexpr (L == 1 || L == 2)
+            probe_flat<L>(m, rec, px, py, pz, limx, limy, live, &clearance, &height);
+        else
+            probe_packed<L>(m, rec, px, py, pz, &clearance, &height);
+        const float fl = (pz - height) - 0.03f;
+        const float step = racer_min(clearance - m.cell, fl / descent);
+        // fl < 0: the floor is hit between last and t; else step < cell / 2:
+        // stop at t; else advance, leaving the range at t + step >= RANGE
+        const bool hit = fl < 0.0f;
+        const bool advance = !hit & !(step < half_cell);
+        const float next_t = t + step;
+        last = advance ? t : last;
+        last_floor = advance ? fl : last_floor;
+        t = advance ? next_t : t;
+        const bool finished = live & (!advance | !(t < RACER_RANGE));
+        float range = racer_min(t, RACER_RANGE);
+        if (live & hit) range = racer_min(last + ((t - last) * last_floor) / (last_floor - fl), RACER_RANGE);
         if (COUNT && lane == 0) ++trips;
         const uint64_t done_mask = __ballot(finished);
         if (done_mask != 0ull) {
@@ -904,11 +923,16 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
+            // finished lanes below this one
+            const int claimed = next + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(done_mask >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)done_mask, 0u));
+            // one exec region: store the range, then take the next beam (a
+            // lane whose claim is past the share reads a stale ring slot and
+            // stays idle, since b >= beams)
             if (finished) {
-                // finished lanes below this one
-                b = next + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(done_mask >> 32),
-                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)done_mask, 0u));
-                if (b < beams) take(b);
+                ranges[b] = range;
+                b = claimed;
+                take(claimed);
             }
             next += count;
         }
@@ -1377,8 +1401,8 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
     m.plane = plane;
     m.packed = nullptr;
     if (layers <= 2) {
-        RACER_TRY(hipMalloc(&sim->packed, (size_t)plane * layers * 2 * sizeof(float)));
-        hipLaunchKernelGGL(k_pack, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, dev->stream, m.d, plane,
+        RACER_TRY(hipMalloc(&sim->packed, (size_t)(plane + 1) * layers * 2 * sizeof(float)));
+        hipLaunchKernelGGL(k_pack, dim3((unsigned)((plane + 1 + 255) / 256)), dim3(256), 0, dev->stream, m.d, plane,
                            layers, sim->packed);
         if (racer_launch_check("pack")) return 1;
         m.packed = sim->packed;
