@@ -204,27 +204,32 @@ __device__ __forceinline__ void policy_tile(
             const uint32_t off = s < steps1 ? (uint32_t)(((tile * steps1 + s) * 64 + lane) * 16) : RACER_OOB;
             w1f[t][s] = racer_ld_b128(w1res, off);
         }
-    // second half of k -> LDS; first half adds it (fixed order)
-    if (kh == 1) {
+    // The two k halves finish two tiles each: half 0 tiles 0 and 1, half 1
+    // tiles 2 and 3, each handing the other its partial sums of the other two
+    // through LDS. Every element is (first half) + (second half), whichever
+    // wavefront adds them (addition commutes), and the epilogue's tanh work is
+    // spread over all eight wavefronts instead of four.
+    const bool first = kh == 0;
+    const f32x4 give[2] = {first ? acc[2] : acc[0], first ? acc[3] : acc[1]};
+    const f32x4 keep[2] = {first ? acc[0] : acc[2], first ? acc[1] : acc[3]};
+    const int give_t = first ? 2 : 0, keep_t = first ? 0 : 2;
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
-            *reinterpret_cast<f32x4*>(red + ((mg * 4 + t) * 64 + lane) * 4) = acc[t];
-    }
+    for (int t = 0; t < 2; ++t)
+        *reinterpret_cast<f32x4*>(red + ((mg * 4 + give_t + t) * 64 + lane) * 4) = give[t];
     __syncthreads();
     // D[i][j] of a 16 x 16 tile: column j = lane % 16, rows 4 (lane / 16) + q
     const int jc = lane & 15, ib = (lane >> 4) * 4;
-    if (kh == 0) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            f32x4 other = *reinterpret_cast<const f32x4*>(red + ((mg * 4 + t) * 64 + lane) * 4);
+    for (int t = 0; t < 2; ++t) {
+        const int tile = keep_t + t;
+        const f32x4 other = *reinterpret_cast<const f32x4*>(red + ((mg * 4 + tile) * 64 + lane) * 4);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int row = 64 * mg + 16 * t + ib + q;
-                if (row < H) {
-                    float v = racer_tanh(acc[t][q] + other[q]);
-                    h1s[jc * Ly.ldh + row] = bf16_of(v);
-                    if (c0 + jc < cols) h1out[(int64_t)row * cols + c0 + jc] = v;
-                }
+        for (int q = 0; q < 4; ++q) {
+            const int row = 64 * mg + 16 * tile + ib + q;
+            if (row < H) {
+                float v = racer_tanh(keep[t][q] + other[q]);
+                h1s[jc * Ly.ldh + row] = bf16_of(v);
+                if (c0 + jc < cols) h1out[(int64_t)row * cols + c0 + jc] = v;
             }
         }
     }
