@@ -2,6 +2,9 @@
 // (racer/ppo.mojo:108-111, then Sim.sample, racer/sim.mojo:337-348):
 // h1 = tanh(W0 x), h2 = tanh(W1 h1) on MFMA (bf16 operands, float32
 // accumulation), o = W2 h2 in float32, and optionally the Gaussian actions.
+// The hidden layers' tanh is the branch-free racer_tanh_mlp (a few ulp from
+// tanh; the bf16 products' own rounding is far larger), as in the bf16 GEMM
+// forward for wider policies, so every bf16 forward computes the same values.
 //
 // One workgroup owns FT = 16 samples (columns) and all H <= 256 rows, with
 // eight wavefronts. The design keeps many loads in flight per wavefront,
@@ -227,7 +230,7 @@ __device__ __forceinline__ void policy_tile(
         for (int q = 0; q < 4; ++q) {
             const int row = 64 * mg + 16 * tile + ib + q;
             if (row < H) {
-                float v = racer_tanh(keep[t][q] + other[q]);
+                float v = racer_tanh_mlp(keep[t][q] + other[q]);
                 h1s[jc * Ly.ldh + row] = bf16_of(v);
                 if (c0 + jc < cols) h1out[(int64_t)row * cols + c0 + jc] = v;
             }
@@ -267,7 +270,7 @@ __device__ __forceinline__ void policy_tile(
         for (int q = 0; q < 4; ++q) {
             const int row = 32 * wave + 16 * t + ib + q;
             if (row < H) {
-                float v = racer_tanh(acc2[t][q]);
+                float v = racer_tanh_mlp(acc2[t][q]);
                 h2f[row * FT + jc] = v;
                 if (c0 + jc < cols) h2out[(int64_t)row * cols + c0 + jc] = v;
             }

@@ -439,9 +439,9 @@ static int forward_hidden(racer_policy* p, const float* x, int64_t x_cols, racer
         uint16_t* w0 = p->theta_bf16;
         uint16_t* w1 = p->theta_bf16 + (int64_t)H * p->in_pad;
         if (racer_gemm_nn_bf16(p->dev, w0, p->in_pad, x, x_cols, a->h1->ptr, cols, H, cols, (int)p->IN,
-                               RACER_EPI_TANH) ||
+                               RACER_EPI_TANH_MLP) ||
             racer_gemm_nn_bf16(p->dev, w1, p->ha_pad, a->h1->ptr, cols, a->h2->ptr, cols, H, cols, (int)p->HA,
-                               RACER_EPI_TANH))
+                               RACER_EPI_TANH_MLP))
             return 1;
     } else {
         if (racer_gemm_nn_f32(p->dev, th, p->IN, x, x_cols, a->h1->ptr, cols, H, cols, (int)p->IN, RACER_EPI_TANH,
