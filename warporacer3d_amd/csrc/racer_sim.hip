@@ -775,7 +775,7 @@ __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_
 
 constexpr int LIDAR_RECORD = 4;               // dx, dy, dz, descent
 constexpr int LIDAR_RING = 2 * LIDAR_RECORD * 64;  // two chunks of 64 beam records
-constexpr int LIDAR_COOP_WAVES = 16;                // wavefronts per workgroup with cooperative stores
+constexpr int LIDAR_COOP_CARS = 8;                  // cars per workgroup with cooperative stores
 constexpr size_t LIDAR_COOP_MAX_SHMEM = 80 * 1024;  // two such workgroups per CU (160 KB of LDS)
 
 // scan (racer/sim.mojo:238-266) for every beam of one car per wavefront.
@@ -1105,13 +1105,15 @@ static void launch_lidar(racer_sim* sim, float* obs, int64_t obs_cols) {
     int wpc = sim->waves_per_car;
     int per = (sim->beams + wpc - 1) / wpc;
     int64_t total = (int64_t)n * wpc;
-    // cooperative stores when a workgroup holds at least 8 cars (runs of 32
-    // bytes; 4 cars measured slower than per-wavefront stores) and 16
-    // wavefronts' rings plus their cars' ranges leave room for two workgroups per CU
-    const size_t coop_shmem = ((size_t)LIDAR_COOP_WAVES * LIDAR_RING + (size_t)(LIDAR_COOP_WAVES / wpc) * sim->beams) *
-                              sizeof(float);
-    if (LIDAR_COOP_WAVES / wpc >= 8 && coop_shmem <= LIDAR_COOP_MAX_SHMEM) {
-        dim3 grid((unsigned)((total + LIDAR_COOP_WAVES - 1) / LIDAR_COOP_WAVES)), block(64 * LIDAR_COOP_WAVES);
+    // cooperative stores from workgroups of 8 cars (runs of 32 bytes; 4 cars
+    // measured slower than per-wavefront stores) when their wavefronts fit one
+    // workgroup and their rings plus the cars' ranges leave room for two
+    // workgroups per CU
+    const int coop_waves = LIDAR_COOP_CARS * wpc;
+    const size_t coop_shmem =
+        ((size_t)coop_waves * LIDAR_RING + (size_t)LIDAR_COOP_CARS * sim->beams) * sizeof(float);
+    if (coop_waves <= 16 && coop_shmem <= LIDAR_COOP_MAX_SHMEM) {
+        dim3 grid((unsigned)((total + coop_waves - 1) / coop_waves)), block(64 * coop_waves);
         if (sim->count_iterations)
             hipLaunchKernelGGL((k_lidar<L, true, true>), grid, block, coop_shmem, sim->dev->stream, sim->map,
                                sim->state->ptr, n, obs, obs_cols, sim->beams, per, wpc, sim->dirs, sim->counter);
