@@ -176,6 +176,57 @@ def ppo_leg(R, T, dev, beams: int, iterations: int, cars: int = 16384, precision
             "forward_error_vs_float64": precision_check}
 
 
+def train_parity_leg(R, T, dev, iterations: int, seeds: int, beams: int = 108, cars: int = 16384,
+                     last: int = 20) -> dict:
+    """The PPO training curve in bf16 (bf16 MFMA products, float32 accumulation)
+    against float32 products, on the bridge with the cfg-3 settings (16384
+    cars, 128-step rollouts, 4 epochs x 8 minibatches, 2 x 256, Adam 3e-4):
+    the reward per step and the KL that the reference's training loop reports
+    each iteration (main.mojo:60-68), from the same seeds. The reference
+    computes no loss value; the statistic is the mean reward per step over
+    the last `last` iterations, whose bf16 - fp32 difference is set beside the
+    spread between seeds within one precision."""
+    arr = T.cached("bridge8")
+    runs = {}
+    for precision in (1, 0):
+        name = "bf16" if precision == 1 else "fp32"
+        for seed in range(42, 42 + seeds):
+            cfg = R.Config(beams=beams, hidden=256, rollout=128, epochs=4, minibatches=8, precision=precision,
+                           heading_noise=0.1)
+            sim = R.Sim(dev, arr, cars, seed, cfg)
+            sim.reset()
+            pol = R.Policy(dev, seed, cfg)
+            acts = R.Activations(dev, cars, cfg)
+            tr = R.Trainer(dev, cars, cfg)
+            graph = tr.capture_rollout(sim, pol, acts)
+            rewards, kls = [], []
+            t0 = time.perf_counter()
+            for it in range(iterations):
+                rewards.append(tr.rollout_graph(graph, it))
+                kls.append(tr.update(pol))
+                if (it + 1) % 50 == 0:
+                    log(f"train parity {name} seed {seed}: iteration {it + 1}, reward/step {rewards[-1]:.5f}, "
+                        f"KL {kls[-1]:.5f}")
+            dev.synchronize()
+            seconds = time.perf_counter() - t0
+            del graph
+            tail = rewards[-last:]
+            runs[f"{name}_seed{seed}"] = {
+                "precision": name, "seed": seed, "seconds": seconds,
+                "mean_reward_last": sum(tail) / len(tail), "mean_kl_last": sum(kls[-last:]) / len(tail),
+                "reward_per_step": rewards[::10] + [rewards[-1]], "finite": all(k == k for k in kls)}
+    bf = [r["mean_reward_last"] for r in runs.values() if r["precision"] == "bf16"]
+    fp = [r["mean_reward_last"] for r in runs.values() if r["precision"] == "fp32"]
+    per_seed = [abs(b - f) / max(abs(f), 1e-12) for b, f in zip(bf, fp)]
+    mean_bf, mean_fp = sum(bf) / len(bf), sum(fp) / len(fp)
+    spread = (max(fp) - min(fp)) / max(abs(mean_fp), 1e-12) if len(fp) > 1 else None
+    return {"iterations": iterations, "seeds": seeds, "statistic": f"mean reward per step over the last {last} "
+            "iterations", "beams": beams, "cars": cars, "track": "bridge8 (two 8 m lobes, 2 layers)",
+            "bf16_mean": mean_bf, "fp32_mean": mean_fp,
+            "relative_difference": abs(mean_bf - mean_fp) / max(abs(mean_fp), 1e-12),
+            "relative_difference_per_seed": per_seed, "fp32_seed_spread": spread, "runs": runs}
+
+
 BATCH_TRACKS = ("ring10", "ramp10", "bank10", "bridge8", "random0", "random1", "random2", "random3")
 BATCH_CARS = (256, 512, 1024, 2048, 4096, 4096, 8192, 8192)
 
@@ -421,6 +472,9 @@ def main() -> int:
                     help="timed steps of the eight-instance batched launch (0 skips it)")
     ap.add_argument("--ppo-iterations", type=int, default=3,
                     help="timed PPO iterations per beam count on the bridge (0 skips the PPO leg)")
+    ap.add_argument("--train-parity", type=int, default=0,
+                    help="PPO iterations of the bf16 / fp32 training-curve comparison (0: skip; minutes per 300)")
+    ap.add_argument("--train-parity-seeds", type=int, default=2)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied once device to device before the run (known-size traffic for counters)")
     args = ap.parse_args()
@@ -554,6 +608,10 @@ def main() -> int:
             ppo[f"beams{beams}"] = ppo_leg(R, T, dev, beams, args.ppo_iterations)
             log(f"PPO iteration (bridge8, 16384 cars, {beams} beams):",
                 {k: v for k, v in ppo[f'beams{beams}'].items() if k != "kernels_ms_per_iteration"})
+    parity = None
+    if args.train_parity > 0:
+        parity = train_parity_leg(R, T, dev, args.train_parity, args.train_parity_seeds)
+        log("training parity:", {k: v for k, v in parity.items() if k != "runs"})
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -577,6 +635,8 @@ def main() -> int:
         "plain_gpu_baseline": plain, "speedup_vs_plain_gpu": value / plain["value"] if plain else None,
         "ppo_iteration": ppo, "batch": batched, "surrogate": surrogate,
     }
+    if parity is not None:
+        out["train_parity"] = parity
     print(json.dumps(out), flush=True)
     return 0
 
