@@ -43,20 +43,31 @@ struct FusedLayout {
     size_t xs, red, h1s, h2f, w2s, outs, total;  // byte offsets
 };
 
-__host__ __device__ inline FusedLayout fused_layout(int H, int IN) {
+// wide: the reduction buffer and h2 also reuse the input slice's bytes, which
+// the first product's MFMA loop is the last to read (a barrier separates them),
+// so three workgroups fit one CU's LDS.
+__host__ __device__ inline FusedLayout fused_layout(int H, int IN, bool wide = false) {
     FusedLayout L;
     L.HA = H + 1;
     L.in32 = (IN + 31) & ~31;
     L.kx = L.in32 + 8;
     L.ha32 = (L.HA + 31) & ~31;
     L.ldh = L.ha32 + 8;
+    const size_t xs_bytes = (size_t)FT * L.kx * 2;
+    const size_t red_bytes = (size_t)4 * 4 * 64 * 4 * 4;  // [4 row groups][4 tiles][64 lanes][4]
+    const size_t h2_bytes = (size_t)L.HA * FT * 4;
+    const size_t rh_bytes = red_bytes > h2_bytes ? red_bytes : h2_bytes;
     L.xs = 0;
-    L.h1s = L.xs + (size_t)FT * L.kx * 2;
-    L.red = L.h1s + (size_t)FT * L.ldh * 2;
-    size_t red_bytes = (size_t)4 * 4 * 64 * 4 * 4;      // [4 row groups][4 tiles][64 lanes][4]
-    size_t h2_bytes = (size_t)L.HA * FT * 4;
-    L.h2f = L.red;                                       // h2 reuses the reduction buffer
-    L.w2s = L.red + (red_bytes > h2_bytes ? red_bytes : h2_bytes);
+    if (wide) {
+        L.red = 0;
+        L.h1s = xs_bytes > rh_bytes ? xs_bytes : rh_bytes;
+        L.w2s = L.h1s + (size_t)FT * L.ldh * 2;
+    } else {
+        L.h1s = xs_bytes;
+        L.red = L.h1s + (size_t)FT * L.ldh * 2;
+        L.w2s = L.red + rh_bytes;
+    }
+    L.h2f = L.red;  // h2 reuses the reduction buffer
     L.outs = L.w2s + (size_t)3 * L.HA * 4;
     L.total = L.outs + (size_t)3 * FT * 4;
     return L;
@@ -66,7 +77,12 @@ __host__ __device__ inline FusedLayout fused_layout(int H, int IN) {
 // HS > 0: each half of the first product's k range is exactly HS k-steps and
 // the fragment ring is unrolled over them (straight-line code, so each k-step
 // waits only for its own fragments); HS = 0: any k range, as a loop.
-template <bool SAMPLE, int HS, bool VEC>
+// WIDE, for launches of many workgroups per CU: a two-step ring, the second
+// product's fragments loaded as it goes rather than held from phase 1, and
+// the aliased layout, so three workgroups share a CU and one's loads overlap
+// another's MFMA and tanh work (with one workgroup per CU the deep ring and the
+// early W1 requests hide the latency instead). Same arithmetic, same results.
+template <bool SAMPLE, int HS, bool VEC, bool WIDE>
 __device__ __forceinline__ void policy_tile(
     const uint16_t* __restrict__ F0, const uint16_t* __restrict__ F1,
     const float* __restrict__ W2, const float* __restrict__ x, int64_t ldx, int H, int IN, int64_t cols,
@@ -74,8 +90,9 @@ __device__ __forceinline__ void policy_tile(
     float* act, int64_t act_cols, float* logp, float* value, int deterministic, int64_t step,
     const int64_t* step_base, int seed, const int64_t c0) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    const FusedLayout Ly = fused_layout(H, IN);
+    const FusedLayout Ly = fused_layout(H, IN, WIDE);
     const int HA = Ly.HA;
+    constexpr int R = WIDE ? 2 : RING;  // k-steps of weight fragments in flight
     uint16_t* xs = reinterpret_cast<uint16_t*>(smem + Ly.xs);     // [FT][kx]
     uint16_t* h1s = reinterpret_cast<uint16_t*>(smem + Ly.h1s);   // [FT][ldh]
     float* red = reinterpret_cast<float*>(smem + Ly.red);         // [4][4][64][4]
@@ -148,7 +165,7 @@ __device__ __forceinline__ void policy_tile(
     f32x4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    uint4 ring[RING][4];
+    uint4 ring[R][4];
     auto load_step = [&](int s, uint4 (&w)[4]) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
@@ -158,55 +175,61 @@ __device__ __forceinline__ void policy_tile(
         }
     };
 #pragma unroll
-    for (int u = 0; u < RING; ++u) load_step(s0 + u, ring[u]);
+    for (int u = 0; u < R; ++u) load_step(s0 + u, ring[u]);
     // No branches inside the ring: a k-step past s1 has all-zero weight
     // fragments (out-of-range loads) against finite, clamped input columns, so
     // it adds exactly zero, and the compiler can keep the ring's loads in flight.
     if constexpr (HS > 0) {
-        // Unrolled: k-step j consumes ring slot j % RING and refills it with
-        // step j + RING, so RING - 1 later steps' fragments stay in flight
+        // Unrolled: k-step j consumes ring slot j % R and refills it with
+        // step j + R, so R - 1 later steps' fragments stay in flight
         // behind each step's MFMAs. (As a loop, the wait at the loop head
         // covers every outstanding load, and the ring degenerates into
         // load-all, wait-all, compute-all.)
 #pragma unroll
         for (int j = 0; j < HS; ++j) {
-            const int u = j % RING;
+            const int u = j % R;
             const int kk = 32 * (s0 + j);
             bf16x8 b = *reinterpret_cast<const bf16x8*>(xs + r * Ly.kx + kk + kg);
 #pragma unroll
             for (int t = 0; t < 4; ++t)
                 acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&ring[u][t]), b, acc[t],
                                                                  0, 0, 0);
-            // pinned here, so RING - 1 k-steps of fragments stay in flight
+            // pinned here, so R - 1 k-steps of fragments stay in flight
             // (the scheduler otherwise sinks each refill to just before its use)
             __builtin_amdgcn_sched_barrier(0);
-            if (j + RING < HS) load_step(s0 + j + RING, ring[u]);
+            if (j + R < HS) load_step(s0 + j + R, ring[u]);
             __builtin_amdgcn_sched_barrier(0);
         }
     } else {
-        for (int s = s0; s < s1; s += RING) {
+        for (int s = s0; s < s1; s += R) {
 #pragma unroll
-            for (int u = 0; u < RING; ++u) {
+            for (int u = 0; u < R; ++u) {
                 const int kk = min(32 * (s + u), Ly.in32 - 32);
                 bf16x8 b = *reinterpret_cast<const bf16x8*>(xs + r * Ly.kx + kk + kg);
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
                     acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&ring[u][t]), b,
                                                                      acc[t], 0, 0, 0);
-                load_step(s + u + RING, ring[u]);
+                load_step(s + u + R, ring[u]);
             }
         }
     }
     // request the second product's weight fragments now (rows 32 w .. + 31)
-    uint4 w1f[2][W1_STEPS];
+    constexpr int W1_HELD = WIDE ? 0 : W1_STEPS;
+    uint4 w1f[2][W1_HELD > 0 ? W1_HELD : 1];
+    if constexpr (W1_HELD > 0) {
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+        for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int s = 0; s < W1_STEPS; ++s) {
-            const int tile = 2 * wave + t;
-            const uint32_t off = s < steps1 ? (uint32_t)(((tile * steps1 + s) * 64 + lane) * 16) : RACER_OOB;
-            w1f[t][s] = racer_ld_b128(w1res, off);
-        }
+            for (int s = 0; s < W1_HELD; ++s) {
+                const int tile = 2 * wave + t;
+                const uint32_t off = s < steps1 ? (uint32_t)(((tile * steps1 + s) * 64 + lane) * 16) : RACER_OOB;
+                w1f[t][s] = racer_ld_b128(w1res, off);
+            }
+    }
+    // the reduction buffer overlays the input slice: every wavefront's last
+    // read of it (its final k-step) comes first
+    if constexpr (WIDE) __syncthreads();
     // The two k halves finish two tiles each: half 0 tiles 0 and 1, half 1
     // tiles 2 and 3, each handing the other its partial sums of the other two
     // through LDS. Every element is (first half) + (second half), whichever
@@ -245,17 +268,19 @@ __device__ __forceinline__ void policy_tile(
 
     // ---- phase 2: h2 = tanh(W1 h1) ----
     f32x4 acc2[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
+    if constexpr (W1_HELD > 0) {
 #pragma unroll
-    for (int s = 0; s < W1_STEPS; ++s) {
-        if (32 * s < Ly.ha32) {
-            bf16x8 b = *reinterpret_cast<const bf16x8*>(h1s + r * Ly.ldh + 32 * s + kg);
+        for (int s = 0; s < W1_HELD; ++s) {
+            if (32 * s < Ly.ha32) {
+                bf16x8 b = *reinterpret_cast<const bf16x8*>(h1s + r * Ly.ldh + 32 * s + kg);
 #pragma unroll
-            for (int t = 0; t < 2; ++t)
-                acc2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&w1f[t][s]), b,
-                                                                  acc2[t], 0, 0, 0);
+                for (int t = 0; t < 2; ++t)
+                    acc2[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&w1f[t][s]), b,
+                                                                      acc2[t], 0, 0, 0);
+            }
         }
     }
-    for (int k0 = 32 * W1_STEPS; k0 < Ly.ha32; k0 += 32) {
+    for (int k0 = 32 * W1_HELD; k0 < Ly.ha32; k0 += 32) {
         bf16x8 b = *reinterpret_cast<const bf16x8*>(h1s + r * Ly.ldh + k0 + kg);
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
@@ -307,15 +332,16 @@ __device__ __forceinline__ void policy_tile(
     }
 }
 
-template <bool SAMPLE, int HS, bool VEC>
-__global__ void __launch_bounds__(64 * WAVES) k_policy_fused(
+template <bool SAMPLE, int HS, bool VEC, bool WIDE>
+__global__ void __launch_bounds__(64 * WAVES, WIDE ? 6 : 2) k_policy_fused(
     const uint16_t* __restrict__ F0, const uint16_t* __restrict__ F1,
     const float* __restrict__ W2, const float* __restrict__ x, int64_t ldx, int H, int IN, int64_t cols,
     float* __restrict__ h1out, float* __restrict__ h2out, float* __restrict__ o, const float* __restrict__ log_std,
     float* act, int64_t act_cols, float* logp, float* value, int deterministic, int64_t step,
     const int64_t* step_base, int seed) {
-    policy_tile<SAMPLE, HS, VEC>(F0, F1, W2, x, ldx, H, IN, cols, h1out, h2out, o, log_std, act, act_cols, logp,
-                                 value, deterministic, step, step_base, seed, (int64_t)xcd_block() * FT);
+    policy_tile<SAMPLE, HS, VEC, WIDE>(F0, F1, W2, x, ldx, H, IN, cols, h1out, h2out, o, log_std, act, act_cols,
+                                       logp, value, deterministic, step, step_base, seed,
+                                       (int64_t)xcd_block() * FT);
 }
 
 // The forward and sample of several sims' observations in one launch:
@@ -330,7 +356,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_fused_batch(
     int k = 0;
     while (k + 1 < tab.count && b >= tab.start[k + 1]) ++k;
     const racer_policy_slot& q = tab.slot[k];
-    policy_tile<true, HS, false>(F0, F1, W2, q.x, q.ldx, H, IN, q.cols, q.h1, q.h2, q.o, log_std, q.act, q.act_cols, q.logp,
+    policy_tile<true, HS, false, false>(F0, F1, W2, q.x, q.ldx, H, IN, q.cols, q.h1, q.h2, q.o, log_std, q.act, q.act_cols, q.logp,
                       q.value, deterministic, step, step_base, q.seed, (b - tab.start[k]) * FT);
 }
 
@@ -373,8 +399,10 @@ size_t racer_policy_fused_smem(int H, int IN) { return fused_layout(H, IN).total
 // inputs: 1081 beams) and 2 (113 inputs: 108 beams); other shapes loop.
 template <int HS>
 static int fused_setup_one() {
-    const void* kernels[] = {(const void*)k_policy_fused<true, HS, false>, (const void*)k_policy_fused<false, HS, false>,
-                             (const void*)k_policy_fused<true, HS, true>, (const void*)k_policy_fused<false, HS, true>};
+    const void* kernels[] = {
+        (const void*)k_policy_fused<true, HS, false, false>, (const void*)k_policy_fused<false, HS, false, false>,
+        (const void*)k_policy_fused<true, HS, true, false>,  (const void*)k_policy_fused<false, HS, true, false>,
+        (const void*)k_policy_fused<true, HS, true, true>,   (const void*)k_policy_fused<false, HS, true, true>};
     for (const void* k : kernels)
         RACER_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     RACER_TRY(hipFuncSetAttribute((const void*)k_policy_fused_batch<HS>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -406,6 +434,22 @@ int racer_policy_fused_batch(racer_policy* p, const racer_policy_table& tab, int
     return racer_launch_check("policy_fused_batch");
 }
 
+// The kernel for one launch: the unrolled half (HS) of the input width,
+// 16-byte input loads when the columns allow them, and the wide variant when
+// the grid holds at least WIDE_MIN_TILES workgroups (four per CU).
+constexpr unsigned WIDE_MIN_TILES = 1024;
+template <bool SAMPLE>
+static auto fused_kernel(int hs, bool vec, bool wide) {
+    if (vec && wide)
+        return hs == 17 ? k_policy_fused<SAMPLE, 17, true, true>
+                        : (hs == 2 ? k_policy_fused<SAMPLE, 2, true, true> : k_policy_fused<SAMPLE, 0, true, true>);
+    if (vec)
+        return hs == 17 ? k_policy_fused<SAMPLE, 17, true, false>
+                        : (hs == 2 ? k_policy_fused<SAMPLE, 2, true, false> : k_policy_fused<SAMPLE, 0, true, false>);
+    return hs == 17 ? k_policy_fused<SAMPLE, 17, false, false>
+                    : (hs == 2 ? k_policy_fused<SAMPLE, 2, false, false> : k_policy_fused<SAMPLE, 0, false, false>);
+}
+
 int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
                        float* act, float* logp, float* value, int deterministic, int64_t step, int seed) {
     racer_device* dev = p->dev;
@@ -414,27 +458,14 @@ int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_ac
     const float* W2 = p->theta->ptr + p->W2;
     const float* log_std = p->theta->ptr + p->LOGSTD;
     dim3 grid((unsigned)((cols + FT - 1) / FT)), block(64 * WAVES);
-    size_t shmem = racer_policy_fused_smem(p->hidden, (int)p->IN);
     const int hs = unrolled_half((int)p->IN);
     const bool vec = x_cols % 4 == 0 && cols % 4 == 0 && (uintptr_t)x % 16 == 0;
-    if (sample) {
-        auto kernel = vec ? (hs == 17 ? k_policy_fused<true, 17, true>
-                                      : (hs == 2 ? k_policy_fused<true, 2, true> : k_policy_fused<true, 0, true>))
-                          : (hs == 17 ? k_policy_fused<true, 17, false>
-                                      : (hs == 2 ? k_policy_fused<true, 2, false> : k_policy_fused<true, 0, false>));
-        racer_timed timer(dev, "policy_fused_sample");
-        hipLaunchKernelGGL(kernel, grid, block, shmem, dev->stream, f0, f1, W2, x, x_cols, p->hidden, (int)p->IN,
-                           (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr, log_std, act, (int64_t)cols, logp, value,
-                           deterministic, step, dev->step_base, seed);
-    } else {
-        auto kernel = vec ? (hs == 17 ? k_policy_fused<false, 17, true>
-                                      : (hs == 2 ? k_policy_fused<false, 2, true> : k_policy_fused<false, 0, true>))
-                          : (hs == 17 ? k_policy_fused<false, 17, false>
-                                      : (hs == 2 ? k_policy_fused<false, 2, false> : k_policy_fused<false, 0, false>));
-        racer_timed timer(dev, "policy_fused");
-        hipLaunchKernelGGL(kernel, grid, block, shmem, dev->stream, f0, f1, W2, x, x_cols, p->hidden, (int)p->IN,
-                           (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr, log_std, act, (int64_t)cols, logp, value,
-                           deterministic, step, dev->step_base, seed);
-    }
+    const bool wide = vec && grid.x >= WIDE_MIN_TILES;
+    const size_t shmem = fused_layout(p->hidden, (int)p->IN, wide).total;
+    auto kernel = sample ? fused_kernel<true>(hs, vec, wide) : fused_kernel<false>(hs, vec, wide);
+    racer_timed timer(dev, sample ? "policy_fused_sample" : "policy_fused");
+    hipLaunchKernelGGL(kernel, grid, block, shmem, dev->stream, f0, f1, W2, x, x_cols, p->hidden, (int)p->IN,
+                       (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr, log_std, act, (int64_t)cols, logp, value,
+                       deterministic, step, dev->step_base, seed);
     return racer_launch_check("policy_fused");
 }
