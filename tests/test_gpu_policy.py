@@ -196,29 +196,31 @@ def test_bf16_weight_gradients_at_batch_scale(dev, cols):
     assert np.array_equal(gg[o.W2: o.LOGSTD], rgrad[o.W2: o.LOGSTD])
 
 
-def test_fused_forward_wide_launch_matches_narrow(dev):
-    """A launch of 1024 or more 16-column workgroups takes the fused forward's
-    wide variant (two-step fragment ring, aliased LDS, three workgroups per
-    CU); its arithmetic is the narrow variant's, so the outputs of a 16384
-    column launch equal, bit for bit, those of 4096-column launches over the
-    same observations."""
-    print("fused forward wide vs narrow", flush=True)
+@pytest.mark.parametrize("big,blocks", [(16384, ((0, 4096), (8192, 4096))),
+                                        (65556, ((0, 4096), (61440, 4096), (65536, 20)))])
+def test_fused_forward_large_launch_matches_narrow(dev, big, blocks):
+    """Large bf16 forwards take other variants of the same arithmetic: 1024
+    or more 16-column workgroups the wide one (two-step fragment ring, aliased
+    LDS, three workgroups per CU), 65536 or more columns without the sample
+    the tiled one (64 columns per workgroup, inputs staged through LDS). Their
+    outputs equal, bit for bit, those of narrow launches over the same
+    observations; 65556 columns leave the tiled launch a ragged last tile."""
+    print(f"fused forward large launch {big}", flush=True)
     cfg = R.Config(beams=1081, hidden=256, precision=1)
-    big, small = 16384, 4096
     pol = R.Policy(dev, 7, cfg)
     rng = np.random.default_rng(3)
     x = rng.uniform(-1.0, 1.0, (cfg.IN, big)).astype(np.float32)
     x[-1] = 1.0
     xb = dev.alloc(cfg.IN * big)
     xb.write(x.ravel())
-    wide = R.Activations(dev, big, cfg)
-    pol.forward(R.View(xb, cfg.IN, big), wide)
-    h1, h2, o = (getattr(wide, n).read().reshape(-1, big) for n in ("h1", "h2", "o"))
-    narrow = R.Activations(dev, small, cfg)
-    xs = dev.alloc(cfg.IN * small)
-    for c0 in (0, 8192):
-        xs.write(np.ascontiguousarray(x[:, c0:c0 + small]).ravel())
-        pol.forward(R.View(xs, cfg.IN, small), narrow)
+    large = R.Activations(dev, big, cfg)
+    pol.forward(R.View(xb, cfg.IN, big), large)
+    h1, h2, o = (getattr(large, n).read().reshape(-1, big) for n in ("h1", "h2", "o"))
+    for c0, n in blocks:
+        narrow = R.Activations(dev, n, cfg)
+        xs = dev.alloc(cfg.IN * n)
+        xs.write(np.ascontiguousarray(x[:, c0:c0 + n]).ravel())
+        pol.forward(R.View(xs, cfg.IN, n), narrow)
         for name, ref in (("h1", h1), ("h2", h2), ("o", o)):
-            got = getattr(narrow, name).read().reshape(-1, small)
-            assert np.array_equal(got, ref[:, c0:c0 + small]), (name, c0)
+            got = getattr(narrow, name).read().reshape(-1, n)
+            assert np.array_equal(got, ref[:, c0:c0 + n]), (name, c0)

@@ -376,6 +376,243 @@ __global__ void k_pack_fragments(const float* __restrict__ src, int64_t ld_src, 
     dst[idx] = (row < rows && k < cols) ? bf16_of(src[(int64_t)row * ld_src + k]) : (uint16_t)0;
 }
 
+
+// ---- the tiled forward for large launches (the PPO update's minibatches) ----
+//
+// Every 16-column workgroup above streams all of W0's fragments (557 KB at
+// 1086 inputs) for its 16 samples, so at 262144 samples the weight stream,
+// not the samples' 4.3 KB each, sets the time. Here a workgroup owns TC = 64
+// columns, so each weight fragment serves four 16-column MFMA tiles, and the
+// input slice (too large for LDS at 64 columns) is staged through LDS in
+// double-buffered blocks of KB k-steps per k half. The arithmetic is the
+// fused kernel's: the same MFMA on the same operand layouts, each k half
+// accumulated over its k-steps in order and the halves added (half 0 +
+// half 1), the same tanh, the second product over all its k-steps in order,
+// and the output layer with fmaf over k in order; so the outputs equal the
+// fused kernel's bit for bit, and the update sees the values the rollout saw.
+constexpr int TC = 64;           // columns per workgroup
+constexpr int TCT = TC / 16;     // 16-column MFMA tiles per workgroup
+constexpr int TKB = 2;           // k-steps per staged block and k half
+constexpr int TXLD = TKB * 32 + 8;  // bf16 per staged column (row pitch, padded)
+constexpr int TRING = 2;         // k-steps of weight fragments in flight per wavefront
+
+struct TiledLayout {
+    int HA, ha32, ldh;
+    size_t xst, h2f, h1s, w2s, total;
+};
+
+__host__ __device__ inline TiledLayout tiled_layout(int H) {
+    TiledLayout L;
+    L.HA = H + 1;
+    L.ha32 = (L.HA + 31) & ~31;
+    L.ldh = L.ha32 + 8;
+    const size_t xst_bytes = (size_t)2 * 2 * TC * TXLD * 2;       // [buffer][half][column][k]
+    const size_t red_bytes = (size_t)4 * 4 * 2 * 64 * 4 * 4;      // one round: [4][4 row tiles][2 column tiles][64][4]
+    const size_t h2_bytes = (size_t)L.HA * TC * 4;                // [HA][TC]
+    size_t r0 = xst_bytes > red_bytes ? xst_bytes : red_bytes;
+    r0 = r0 > h2_bytes ? r0 : h2_bytes;
+    L.xst = 0;
+    L.h2f = 0;  // the staged inputs, the halves' exchange and h2 share region 0 in turn
+    L.h1s = r0;
+    L.w2s = L.h1s + (size_t)TC * L.ldh * 2;
+    L.total = L.w2s + (size_t)3 * L.HA * 4;
+    return L;
+}
+
+template <int HS>
+__global__ void __launch_bounds__(64 * WAVES) k_policy_tiled(
+    const uint16_t* __restrict__ F0, const uint16_t* __restrict__ F1, const float* __restrict__ W2,
+    const float* __restrict__ x, int64_t ldx, int H, int IN, int64_t cols, float* __restrict__ h1out,
+    float* __restrict__ h2out, float* __restrict__ o) {
+    static_assert(HS > 0, "the tiled forward needs equal unrolled k halves");
+    constexpr int NB = (HS + TKB - 1) / TKB;  // staged blocks per k half
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const TiledLayout Ly = tiled_layout(H);
+    const int HA = Ly.HA;
+    uint16_t* xst = reinterpret_cast<uint16_t*>(smem + Ly.xst);   // [2][2][TC][TXLD]
+    float* red = reinterpret_cast<float*>(smem + Ly.xst);         // [4][4][2][64][4] per round
+    float* h2f = reinterpret_cast<float*>(smem + Ly.h2f);         // [HA][TC]
+    uint16_t* h1s = reinterpret_cast<uint16_t*>(smem + Ly.h1s);   // [TC][ldh]
+    float* w2s = reinterpret_cast<float*>(smem + Ly.w2s);         // [3][HA]
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, kg = (lane >> 4) * 8;  // MFMA operand lane layout
+    const int64_t c0 = (int64_t)xcd_block() * TC;
+    const int steps0 = 2 * HS, steps1 = Ly.ha32 / 32;
+    const auto xres = racer_rsrc(x, ((uint64_t)(IN - 1) * ldx + cols) * 4);
+    const auto w0res = racer_rsrc(F0, (uint64_t)FRAG_TILES * steps0 * 1024);
+    const auto w1res = racer_rsrc(F1, (uint64_t)FRAG_TILES * steps1 * 1024);
+    for (int idx = tid; idx < 3 * HA; idx += blockDim.x) w2s[idx] = W2[idx];
+
+    // staging: thread tid covers k-half tid / 256, 4 consecutive k (kq) of a
+    // block and 4 adjacent columns (cq), as 4 float4 loads
+    const int sh = tid >> 8, kq = ((tid & 255) >> 4) * 4, cq = (tid & 15) * 4;
+    const int64_t scol = c0 + cq;
+    float4 xr[4];
+    auto stage_load = [&](int b) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int kin = b * TKB * 32 + kq + e;  // k within the half
+            const int k = sh * HS * 32 + kin;
+            const bool ok = kin < HS * 32 && k < IN && scol < cols;
+            xr[e] = __builtin_bit_cast(float4,
+                                       racer_ld_b128(xres, ok ? (uint32_t)(((int64_t)k * ldx + scol) * 4) : RACER_OOB));
+        }
+    };
+    auto stage_store = [&](int b) {
+        uint16_t* dst = xst + ((size_t)((b & 1) * 2 + sh) * TC + cq) * TXLD + kq;
+        uint16_t v[4][4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v[0][e] = bf16_of(xr[e].x);
+            v[1][e] = bf16_of(xr[e].y);
+            v[2][e] = bf16_of(xr[e].z);
+            v[3][e] = bf16_of(xr[e].w);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) *reinterpret_cast<uint2*>(dst + q * TXLD) = *reinterpret_cast<uint2*>(v[q]);
+    };
+
+    // ---- first product: wavefront (mg, kh) = rows 64 mg .. + 63 over k half kh, all TC columns ----
+    const int mg = wave & 3, kh = wave >> 2;
+    f32x4 acc[4][TCT];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int c = 0; c < TCT; ++c) acc[t][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    uint4 ring[TRING][4];
+    auto load_step = [&](int j, uint4 (&w)[4]) {  // j: k-step within the half
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int tile = 4 * mg + t;
+            const uint32_t off = j < HS ? (uint32_t)(((tile * steps0 + kh * HS + j) * 64 + lane) * 16) : RACER_OOB;
+            w[t] = racer_ld_b128(w0res, off);
+        }
+    };
+    stage_load(0);
+#pragma unroll
+    for (int u = 0; u < TRING; ++u) load_step(u, ring[u]);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        stage_store(b);
+        if (b + 1 < NB) stage_load(b + 1);
+        __syncthreads();
+        const uint16_t* xb = xst + (size_t)((b & 1) * 2 + kh) * TC * TXLD;
+#pragma unroll
+        for (int i = 0; i < TKB; ++i) {
+            const int j = b * TKB + i;
+            if (j < HS) {
+                const int u = j % TRING;
+                bf16x8 bx[TCT];
+#pragma unroll
+                for (int c = 0; c < TCT; ++c)
+                    bx[c] = *reinterpret_cast<const bf16x8*>(xb + (c * 16 + r) * TXLD + i * 32 + kg);
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int c = 0; c < TCT; ++c)
+                        acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&ring[u][t]),
+                                                                            bx[c], acc[t][c], 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (j + TRING < HS) load_step(j + TRING, ring[u]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+    __syncthreads();  // region 0 now holds the halves' exchange: every read of the staged inputs is done
+
+    // halves: half 0 finishes row tiles 0 and 1, half 1 tiles 2 and 3 (each
+    // element half 0 + half 1), in two rounds of two column tiles
+    // exchange slots are keyed by the absolute row tile: half 0 writes tiles
+    // 2 and 3 and reads 0 and 1 (written by half 1), and the reverse
+    const bool first = kh == 0;
+    const int give_t = first ? 2 : 0, keep_t = first ? 0 : 2;
+    const int jc = lane & 15, ib = (lane >> 4) * 4;
+#pragma unroll
+    for (int rd = 0; rd < TCT / 2; ++rd) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const f32x4 g = first ? acc[2 + t][2 * rd + c] : acc[t][2 * rd + c];
+                *reinterpret_cast<f32x4*>(red + (((mg * 4 + give_t + t) * 2 + c) * 64 + lane) * 4) = g;
+            }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const f32x4 keep = first ? acc[t][2 * rd + c] : acc[2 + t][2 * rd + c];
+                const f32x4 other =
+                    *reinterpret_cast<const f32x4*>(red + (((mg * 4 + keep_t + t) * 2 + c) * 64 + lane) * 4);
+                const int col = (2 * rd + c) * 16 + jc;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = 64 * mg + 16 * (keep_t + t) + ib + q;
+                    if (row < H) {
+                        const float v = racer_tanh_mlp(keep[q] + other[q]);
+                        h1s[col * Ly.ldh + row] = bf16_of(v);
+                        if (c0 + col < cols) h1out[(int64_t)row * cols + c0 + col] = v;
+                    }
+                }
+            }
+        __syncthreads();
+    }
+    // bias row (the constant 1) and zero padding of the second product's k range
+    for (int idx = tid; idx < TC * (Ly.ha32 - H); idx += blockDim.x) {
+        const int j = idx / (Ly.ha32 - H), k = H + idx % (Ly.ha32 - H);
+        h1s[j * Ly.ldh + k] = k == H ? bf16_of(1.0f) : (uint16_t)0;
+    }
+    __syncthreads();
+
+    // ---- second product: wavefront w = rows 32 w .. + 31 over all k, all TC columns ----
+    f32x4 acc2[2][TCT];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int c = 0; c < TCT; ++c) acc2[t][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int s = 0; s < steps1; ++s) {
+        uint4 wf[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int tile = 2 * wave + t;
+            wf[t] = racer_ld_b128(w1res, (uint32_t)(((tile * steps1 + s) * 64 + lane) * 16));
+        }
+#pragma unroll
+        for (int c = 0; c < TCT; ++c) {
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(h1s + (c * 16 + r) * Ly.ldh + 32 * s + kg);
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+                acc2[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&wf[t]), bh,
+                                                                     acc2[t][c], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int c = 0; c < TCT; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = 32 * wave + 16 * t + ib + q, col = c * 16 + jc;
+                if (row < H) {
+                    const float v = racer_tanh_mlp(acc2[t][c][q]);
+                    h2f[row * TC + col] = v;
+                    if (c0 + col < cols) h2out[(int64_t)row * cols + c0 + col] = v;
+                }
+            }
+    for (int j = tid; j < TC; j += blockDim.x) h2f[H * TC + j] = 1.0f;
+    __syncthreads();
+
+    // ---- output layer o = W2 h2 in float32, k in order ----
+    if (tid < 3 * TC) {
+        const int j = tid / TC, c = tid % TC;
+        const float* w = w2s + j * HA;
+        float a = 0.0f;
+        for (int k = 0; k < HA; ++k) a = fmaf(w[k], h2f[k * TC + c], a);
+        if (c0 + c < cols) o[j * cols + c0 + c] = a;
+    }
+}
+
 }  // namespace
 
 size_t racer_fragment_elems(int cols) { return (size_t)FRAG_TILES * ((cols + 31) / 32) * 512; }
@@ -410,7 +647,12 @@ static int fused_setup_one() {
     return 0;
 }
 
-int racer_policy_fused_setup() { return fused_setup_one<0>() || fused_setup_one<2>() || fused_setup_one<17>(); }
+int racer_policy_fused_setup() {
+    if (fused_setup_one<0>() || fused_setup_one<2>() || fused_setup_one<17>()) return 1;
+    for (const void* k : {(const void*)k_policy_tiled<2>, (const void*)k_policy_tiled<17>})
+        RACER_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    return 0;
+}
 
 // HS for a first product of IN inputs: its k-steps split into two equal halves of HS.
 static int unrolled_half(int IN) {
@@ -438,6 +680,9 @@ int racer_policy_fused_batch(racer_policy* p, const racer_policy_table& tab, int
 // 16-byte input loads when the columns allow them, and the wide variant when
 // the grid holds at least WIDE_MIN_TILES workgroups (four per CU).
 constexpr unsigned WIDE_MIN_TILES = 1024;
+// The forward without the sample over at least this many columns (the PPO
+// update's minibatches) takes the tiled kernel.
+constexpr int TILED_MIN_COLS = 65536;
 template <bool SAMPLE>
 static auto fused_kernel(int hs, bool vec, bool wide) {
     if (vec && wide)
@@ -460,6 +705,15 @@ int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_ac
     dim3 grid((unsigned)((cols + FT - 1) / FT)), block(64 * WAVES);
     const int hs = unrolled_half((int)p->IN);
     const bool vec = x_cols % 4 == 0 && cols % 4 == 0 && (uintptr_t)x % 16 == 0;
+    if (!sample && vec && hs > 0 && cols >= TILED_MIN_COLS) {
+        racer_timed timer(dev, "policy_tiled");
+        const dim3 tgrid((unsigned)((cols + TC - 1) / TC));
+        const size_t tshmem = tiled_layout(p->hidden).total;
+        hipLaunchKernelGGL(hs == 17 ? k_policy_tiled<17> : k_policy_tiled<2>, tgrid, block, tshmem, dev->stream, f0,
+                           f1, W2, x, x_cols, p->hidden, (int)p->IN, (int64_t)cols, a->h1->ptr, a->h2->ptr,
+                           a->o->ptr);
+        return racer_launch_check("policy_tiled");
+    }
     const bool wide = vec && grid.x >= WIDE_MIN_TILES;
     const size_t shmem = fused_layout(p->hidden, (int)p->IN, wide).total;
     auto kernel = sample ? fused_kernel<true>(hs, vec, wide) : fused_kernel<false>(hs, vec, wide);
