@@ -395,6 +395,7 @@ constexpr int TCT = TC / 16;     // 16-column MFMA tiles per workgroup
 constexpr int TKB = 2;           // k-steps per staged block and k half
 constexpr int TXLD = TKB * 32 + 8;  // bf16 per staged column (row pitch, padded)
 constexpr int TRING = 2;         // k-steps of weight fragments in flight per wavefront
+constexpr int TDX = 2;           // staged blocks of inputs in flight (register sets)
 
 struct TiledLayout {
     int HA, ha32, ldh;
@@ -444,33 +445,46 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_tiled(
     const auto w1res = racer_rsrc(F1, (uint64_t)FRAG_TILES * steps1 * 1024);
     for (int idx = tid; idx < 3 * HA; idx += blockDim.x) w2s[idx] = W2[idx];
 
-    // staging: thread tid covers k-half tid / 256, 4 consecutive k (kq) of a
-    // block and 4 adjacent columns (cq), as 4 float4 loads
-    const int sh = tid >> 8, kq = ((tid & 255) >> 4) * 4, cq = (tid & 15) * 4;
+    // staging: thread tid covers k-half tid / 256 and 4 adjacent columns
+    // (cq); piece m of a block is 4 consecutive k at kq(m), as 4 float4 loads
+    // (consecutive lanes along the columns: 256 contiguous bytes per k row;
+    // lanes along k instead avoided LDS bank conflicts on the stores but ran
+    // slower)
+    static_assert(TKB % 2 == 0, "512 threads stage a block as TKB / 2 pieces each");
+    constexpr int PIECES = TKB / 2;
+    const int sh = tid >> 8, cq = (tid & 15) * 4;
     const int64_t scol = c0 + cq;
-    float4 xr[4];
+    auto kq = [&](int m) { return (((tid & 255) >> 4) + 16 * m) * 4; };
+    // blocks b .. b + TDX - 1 are in flight in register set b % TDX (HBM
+    // latency is several blocks' worth of MFMA work)
+    float4 xr[TDX][PIECES][4];
     auto stage_load = [&](int b) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int kin = b * TKB * 32 + kq + e;  // k within the half
-            const int k = sh * HS * 32 + kin;
-            const bool ok = kin < HS * 32 && k < IN && scol < cols;
-            xr[e] = __builtin_bit_cast(float4,
-                                       racer_ld_b128(xres, ok ? (uint32_t)(((int64_t)k * ldx + scol) * 4) : RACER_OOB));
-        }
+        for (int m = 0; m < PIECES; ++m)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int kin = b * TKB * 32 + kq(m) + e;  // k within the half
+                const int k = sh * HS * 32 + kin;
+                const bool ok = kin < HS * 32 && k < IN && scol < cols;
+                xr[b % TDX][m][e] = __builtin_bit_cast(
+                    float4, racer_ld_b128(xres, ok ? (uint32_t)(((int64_t)k * ldx + scol) * 4) : RACER_OOB));
+            }
     };
     auto stage_store = [&](int b) {
-        uint16_t* dst = xst + ((size_t)((b & 1) * 2 + sh) * TC + cq) * TXLD + kq;
-        uint16_t v[4][4];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            v[0][e] = bf16_of(xr[e].x);
-            v[1][e] = bf16_of(xr[e].y);
-            v[2][e] = bf16_of(xr[e].z);
-            v[3][e] = bf16_of(xr[e].w);
+        for (int m = 0; m < PIECES; ++m) {
+            uint16_t* dst = xst + ((size_t)((b & 1) * 2 + sh) * TC + cq) * TXLD + kq(m);
+            uint16_t v[4][4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v[0][e] = bf16_of(xr[b % TDX][m][e].x);
+                v[1][e] = bf16_of(xr[b % TDX][m][e].y);
+                v[2][e] = bf16_of(xr[b % TDX][m][e].z);
+                v[3][e] = bf16_of(xr[b % TDX][m][e].w);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) *reinterpret_cast<uint2*>(dst + q * TXLD) = *reinterpret_cast<uint2*>(v[q]);
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) *reinterpret_cast<uint2*>(dst + q * TXLD) = *reinterpret_cast<uint2*>(v[q]);
     };
 
     // ---- first product: wavefront (mg, kh) = rows 64 mg .. + 63 over k half kh, all TC columns ----
@@ -489,13 +503,21 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_tiled(
             w[t] = racer_ld_b128(w0res, off);
         }
     };
+    // Issue order matters: loads retire in order (one vmcnt counter), so a
+    // wait for a k-step's weight fragments (from L2) also waits for every
+    // input load (from HBM) issued before them. Block b + TDX's inputs are
+    // therefore requested at the end of block b, after the block's weight
+    // refills: the inputs a weight wait covers are then those of the block
+    // that consumes the weights, needed there anyway.
     stage_load(0);
 #pragma unroll
     for (int u = 0; u < TRING; ++u) load_step(u, ring[u]);
 #pragma unroll
+    for (int b = 1; b < TDX; ++b)
+        if (b < NB) stage_load(b);
+#pragma unroll
     for (int b = 0; b < NB; ++b) {
         stage_store(b);
-        if (b + 1 < NB) stage_load(b + 1);
         __syncthreads();
         const uint16_t* xb = xst + (size_t)((b & 1) * 2 + kh) * TC * TXLD;
 #pragma unroll
@@ -518,7 +540,21 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_tiled(
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
+        if (b + TDX < NB) stage_load(b + TDX);
+        __builtin_amdgcn_sched_barrier(0);
     }
+    // request the second product's weight fragments now (rows 32 w .. + 31),
+    // so their latency hides behind the halves' exchange and the tanh
+    constexpr int TW1 = W1_STEPS;  // k-steps held (H + 1 <= 288)
+    uint4 w1f[2][TW1];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int s = 0; s < TW1; ++s) {
+            const int tile = 2 * wave + t;
+            const uint32_t off = s < steps1 ? (uint32_t)(((tile * steps1 + s) * 64 + lane) * 16) : RACER_OOB;
+            w1f[t][s] = racer_ld_b128(w1res, off);
+        }
     __syncthreads();  // region 0 now holds the halves' exchange: every read of the staged inputs is done
 
     // halves: half 0 finishes row tiles 0 and 1, half 1 tiles 2 and 3 (each
@@ -571,20 +607,17 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_tiled(
     for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int c = 0; c < TCT; ++c) acc2[t][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int s = 0; s < steps1; ++s) {
-        uint4 wf[2];
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int tile = 2 * wave + t;
-            wf[t] = racer_ld_b128(w1res, (uint32_t)(((tile * steps1 + s) * 64 + lane) * 16));
-        }
+    for (int s = 0; s < TW1; ++s) {
+        if (s < steps1) {
 #pragma unroll
-        for (int c = 0; c < TCT; ++c) {
-            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(h1s + (c * 16 + r) * Ly.ldh + 32 * s + kg);
+            for (int c = 0; c < TCT; ++c) {
+                const bf16x8 bh = *reinterpret_cast<const bf16x8*>(h1s + (c * 16 + r) * Ly.ldh + 32 * s + kg);
 #pragma unroll
-            for (int t = 0; t < 2; ++t)
-                acc2[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&wf[t]), bh,
-                                                                     acc2[t][c], 0, 0, 0);
+                for (int t = 0; t < 2; ++t)
+                    acc2[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<bf16x8*>(&w1f[t][s]), bh,
+                                                                         acc2[t][c], 0, 0, 0);
+            }
         }
     }
 #pragma unroll
