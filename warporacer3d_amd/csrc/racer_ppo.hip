@@ -508,9 +508,13 @@ static int backward_raw(racer_policy* p, const float* x, racer_acts* a, const fl
                           RACER_EPI_MASK, h2, cols))
         return 1;
     if (p->precision == 1) {
+        // rows 0 .. H-1 on MFMA; row H is the bias row, masked by 1 - h1[H]^2 = 0
+        // and read by no weight gradient (they take d1's first H rows), so it
+        // is filled with zeros rather than spending a fifth row tile on it
         if (refresh_bf16(p) ||
-            racer_gemm_nn_bf16(dev, p->w1t_bf16, p->h_pad, d2, cols, d1, cols, (int)HA, (int)cols, H, RACER_EPI_MASK,
-                               h1, cols))
+            racer_gemm_nn_bf16(dev, p->w1t_bf16, p->h_pad, d2, cols, d1, cols, H, (int)cols, H, RACER_EPI_MASK, h1,
+                               cols) ||
+            racer_fill(dev, d1 + (int64_t)H * cols, (size_t)cols, 0.0f))
             return 1;
     } else if (racer_gemm_nn_f32(dev, p->w1t->ptr, H, d2, cols, d1, cols, (int)HA, (int)cols, H, RACER_EPI_MASK, h1,
                                  cols)) {
