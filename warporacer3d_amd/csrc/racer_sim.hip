@@ -757,19 +757,15 @@ __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_
         *clearance = fabsf(r.y - z) < 1e30f ? r.x : -1.0f;
         *height = r.y;
     } else {
+        // Map.probe's loop over the two layers as selects (a short-circuit
+        // test compiles to nested exec-mask branches on every march step)
         float4 r = __builtin_bit_cast(float4, racer_ld_b128(rec, off));
-        float cl = -1.0f, h = r.y, gap = 1e30f;
-        if (r.x > 0.0f && fabsf(r.y - z) < gap) {
-            cl = r.x;
-            h = r.y;
-            gap = fabsf(r.y - z);
-        }
-        if (r.z > 0.0f && fabsf(r.w - z) < gap) {
-            cl = r.z;
-            h = r.w;
-        }
-        *clearance = cl;
-        *height = h;
+        const float d0 = fabsf(r.y - z), d1 = fabsf(r.w - z);
+        const bool take0 = (r.x > 0.0f) & (d0 < 1e30f);
+        const float gap = take0 ? d0 : 1e30f;
+        const bool take1 = (r.z > 0.0f) & (d1 < gap);
+        *clearance = take1 ? r.z : (take0 ? r.x : -1.0f);
+        *height = take1 ? r.w : r.y;
     }
 }
 
