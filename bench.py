@@ -479,7 +479,7 @@ def main() -> int:
                     help="MiB copied once device to device before the run (known-size traffic for counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        print(json.dumps({"error": "this benchmark measures one GPU only; run with --gpus 1"}))
         return 2
 
     import numpy as np

@@ -206,7 +206,7 @@ def load(name: str, cell: float | None = None) -> np.ndarray:
     if "route" in meta:
         route = np.loadtxt(Path(name).parent / meta["route"])[:, :3]
         return ribbon(route, meta.get("half_width", 1.3), meta.get("bank", 0), c, meta.get("ramps", []))
-    raise ValueError("occupancy-image maps need a skeletonizer, which this build does not provide: " + name)
+    raise ValueError("occupancy-image maps need a skeletonizer, which this package does not provide: " + name)
 
 
 def header(track: np.ndarray) -> dict:
