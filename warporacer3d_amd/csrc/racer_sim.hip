@@ -93,10 +93,11 @@ __device__ __forceinline__ float waypoint(const racer_dmap& m, int64_t w, int fi
     return m.d[m.route + 4 * w + field];
 }
 
-// Map.progress (racer/sim.mojo:146-152). Off the map w = -1 reads the floats
-// just before the route table, as the reference does.
-__device__ float map_progress(const racer_dmap& m, int layer, float x, float y) {
-    int64_t w = (int64_t)map_at(m, layer, RACER_NEAREST, x, y);
+// Map.progress (racer/sim.mojo:146-152) with its first read, Map.at(layer,
+// NEAREST, x, y), already done (-1 off the map: then w = -1 reads the floats
+// just before the route table, as the reference does).
+__device__ float map_progress_at(const racer_dmap& m, float nearest, float x, float y) {
+    int64_t w = (int64_t)nearest;
     float heading = waypoint(m, w, 3);
     float along = (x - waypoint(m, w, 0)) * racer_cos(heading) + (y - waypoint(m, w, 1)) * racer_sin(heading);
     return (float)w + along / m.spacing;
@@ -147,11 +148,15 @@ __device__ void spawn(const racer_dmap& m, float* s, int n, int i, int seed, flo
 // are loaded together, every layer's heights at once, and the probed layer's
 // are selected afterwards, so the five loads are one round trip. L = 0 reads
 // the track array layer by layer.
+// nearest (optional): Map.at(layer, NEAREST, x, y) of the probed layer, the
+// progress lookup's first read, loaded with the same round of loads.
 template <int L>
-__device__ __forceinline__ void probe_surface(const racer_dmap& m, float x, float y, float z, Spot* spot, Surf* g) {
+__device__ __forceinline__ void probe_surface(const racer_dmap& m, float x, float y, float z, Spot* spot, Surf* g,
+                                              float* nearest = nullptr) {
     if constexpr (L == 0) {
         *spot = map_probe(m, x, y, z);
         *g = map_surface(m, spot->layer, x, y);
+        if (nearest) *nearest = map_at(m, spot->layer, RACER_NEAREST, x, y);
     } else {
         const float2* rec = reinterpret_cast<const float2*>(m.packed);
         int64_t c = 0;
@@ -165,6 +170,7 @@ __device__ __forceinline__ void probe_surface(const racer_dmap& m, float x, floa
         float ty = racer_clamp(fy - (float)iy, 0.0f, 1.0f);
         const int64_t k = iy * m.nx + ix;
         float2 pr[L], r0[L], r1[L], r2[L], r3[L];
+        float nr[L];
 #pragma unroll
         for (int l = 0; l < L; ++l) {
             pr[l] = on ? rec[c * L + l] : make_float2(-1.0f, -1.0f);
@@ -172,6 +178,7 @@ __device__ __forceinline__ void probe_surface(const racer_dmap& m, float x, floa
             r1[l] = rec[(k + 1) * L + l];
             r2[l] = rec[(k + m.nx) * L + l];
             r3[l] = rec[(k + m.nx + 1) * L + l];
+            if (nearest) nr[l] = on ? grid_at(m, l, RACER_NEAREST, c) : -1.0f;
         }
         Spot best{0, -1.0f, pr[0].y};
         float gap = 1e30f;
@@ -185,6 +192,13 @@ __device__ __forceinline__ void probe_surface(const racer_dmap& m, float x, floa
             }
         }
         *spot = best;
+        if (nearest) {
+            float w = nr[0];
+#pragma unroll
+            for (int l = 1; l < L; ++l)
+                if (best.layer == l) w = nr[l];
+            *nearest = w;
+        }
         float h0 = r0[0].y, h1 = r1[0].y, h2 = r2[0].y, h3 = r3[0].y;
 #pragma unroll
         for (int l = 1; l < L; ++l) {
@@ -208,7 +222,7 @@ __device__ __forceinline__ void probe_surface(const racer_dmap& m, float x, floa
 template <int L>
 __device__ void finish(const racer_dmap& m, float* s, int n, float* reward, float* done, float* obs,
                        int64_t obs_cols, int beams, int i, int seed, float heading_noise, float p[4], float steer,
-                       const Spot& spot, Surf ground, bool crashed);
+                       const Spot& spot, Surf ground, bool crashed, float nearest);
 
 // advance (racer/sim.mojo:184-235): one 60 Hz step of car i.
 template <int L>
@@ -224,6 +238,7 @@ __device__ void advance(const racer_dmap& m, float* s, int n, const float* act, 
     float accel = racer_clamp(act[act_cols + i], -1.0f, 1.0f) * RACER_ACCEL_MAX;
     float grip = RACER_GRIP * s[RACER_FRICTION * n + i];
     bool crashed = false;
+    float nearest = -1.0f;  // the last probe's nearest waypoint (progress)
     for (int sub = 0; sub < RACER_SUBSTEPS; ++sub) {
         float along, across;
         slopes(ground, p[2], &along, &across);
@@ -244,23 +259,26 @@ __device__ void advance(const racer_dmap& m, float* s, int n, const float* act, 
         p[3] = racer_clamp(p[3], -RACER_SPEED_MAX, RACER_SPEED_MAX);
         steer = racer_clamp(full, -RACER_STEER_MAX, RACER_STEER_MAX);
         const float previous = spot.height;  // the re-probe uses the last spot's road height
-        probe_surface<L>(m, p[0], p[1], previous, &spot, &ground);
+        probe_surface<L>(m, p[0], p[1], previous, &spot, &ground,
+                         sub == RACER_SUBSTEPS - 1 ? &nearest : nullptr);
         crashed = crashed || spot.clearance < RACER_HALF_DIAG ||
                   ground.sx * ground.sx + ground.sy * ground.sy > 1.0f;
     }
-    finish<L>(m, s, n, reward, done, obs, obs_cols, beams, i, seed, heading_noise, p, steer, spot, ground, crashed);
+    finish<L>(m, s, n, reward, done, obs, obs_cols, beams, i, seed, heading_noise, p, steer, spot, ground, crashed,
+              nearest);
 }
 
 // The end of advance (racer/sim.mojo:219-235) from the integrated pose p =
-// (x, y, heading, speed), steer, the last probe and the crash flag: heading
-// wrap, progress, reward, done code, state and return, respawn, obs rows.
+// (x, y, heading, speed), steer, the last probe (with its nearest waypoint at
+// (x, y), read in the same round of loads) and the crash flag: heading wrap,
+// progress, reward, done code, state and return, respawn, obs rows.
 template <int L>
 __device__ void finish(const racer_dmap& m, float* s, int n, float* reward, float* done, float* obs,
                        int64_t obs_cols, int beams, int i, int seed, float heading_noise, float p[4], float steer,
-                       const Spot& spot, Surf ground, bool crashed) {
+                       const Spot& spot, Surf ground, bool crashed, float nearest) {
     if (p[2] > RACER_PI) p[2] -= RACER_2PI;
     if (p[2] < -RACER_PI) p[2] += RACER_2PI;
-    float progress = map_progress(m, spot.layer, p[0], p[1]);
+    float progress = map_progress_at(m, nearest, p[0], p[1]);
     float dp = progress - s[RACER_PROGRESS * n + i];
     float halfw = (float)m.waypoints / 2.0f;
     dp = dp > halfw ? dp - 2.0f * halfw : (dp < -halfw ? dp + 2.0f * halfw : dp);
@@ -410,7 +428,8 @@ __device__ __forceinline__ void surrogate_apply(const racer_dmap& m, float* s, i
     const float steer = racer_clamp(c.steer + d[4], -RACER_STEER_MAX, RACER_STEER_MAX);
     Spot spot;
     Surf ground;
-    probe_surface<L>(m, p[0], p[1], c.now.height, &spot, &ground);
+    float nearest = -1.0f;
+    probe_surface<L>(m, p[0], p[1], c.now.height, &spot, &ground, &nearest);
     const bool back = !(spot.clearance >= min_clearance) || spot.layer != c.now.layer;
     if (fallback) fallback[i] = back ? 1.0f : 0.0f;
     if (back) {
@@ -418,7 +437,8 @@ __device__ __forceinline__ void surrogate_apply(const racer_dmap& m, float* s, i
         return;
     }
     const bool crashed = spot.clearance < RACER_HALF_DIAG || ground.sx * ground.sx + ground.sy * ground.sy > 1.0f;
-    finish<L>(m, s, n, reward, done, obs, obs_cols, beams, i, seed, heading_noise, p, steer, spot, ground, crashed);
+    finish<L>(m, s, n, reward, done, obs, obs_cols, beams, i, seed, heading_noise, p, steer, spot, ground, crashed,
+              nearest);
 }
 
 // The surrogate step with the predictions of a separate forward (pred rows ldp apart).
