@@ -158,56 +158,63 @@ __device__ __forceinline__ void probe_surface(const racer_dmap& m, float x, floa
         *g = map_surface(m, spot->layer, x, y);
         if (nearest) *nearest = map_at(m, spot->layer, RACER_NEAREST, x, y);
     } else {
+        // One IEEE quotient per axis serves Map.at's nearest cell floor(q +
+        // 0.5) (racer/sim.mojo:110-111) and Map.surface's bilinear cell and
+        // fraction (:128-133); cells fit 32-bit integers (grids of at most
+        // 2^24 cells), and the tests and clamps are selects, not branches (this
+        // runs five times in each car's serial chain). fmaxf / fminf clamp a
+        // NaN to 0 and +inf to n - 2, as the reference's comparisons do.
         const float2* rec = reinterpret_cast<const float2*>(m.packed);
-        int64_t c = 0;
-        const bool on = cell_of(m, x, y, &c);
-        float fx = (x - m.x0) / m.cell;
-        float fy = (y - m.y0) / m.cell;
-        float cx = floorf(fx), cy = floorf(fy);
-        int64_t ix = !(cx >= 0.0f) ? 0 : (cx > (float)(m.nx - 2) ? m.nx - 2 : (int64_t)cx);
-        int64_t iy = !(cy >= 0.0f) ? 0 : (cy > (float)(m.ny - 2) ? m.ny - 2 : (int64_t)cy);
-        float tx = racer_clamp(fx - (float)ix, 0.0f, 1.0f);
-        float ty = racer_clamp(fy - (float)iy, 0.0f, 1.0f);
-        const int64_t k = iy * m.nx + ix;
+        const float qx = (x - m.x0) / m.cell, qy = (y - m.y0) / m.cell;
+        const float fxn = floorf(qx + 0.5f), fyn = floorf(qy + 0.5f);
+        const bool on = (fxn >= 0.0f) & (fyn >= 0.0f) & (fxn < (float)m.nx) & (fyn < (float)m.ny);
+        const int c = on ? (int)fyn * m.nx + (int)fxn : 0;
+        const int ix = (int)fminf(fmaxf(floorf(qx), 0.0f), (float)(m.nx - 2));
+        const int iy = (int)fminf(fmaxf(floorf(qy), 0.0f), (float)(m.ny - 2));
+        const float tx = racer_clamp(qx - (float)ix, 0.0f, 1.0f);
+        const float ty = racer_clamp(qy - (float)iy, 0.0f, 1.0f);
+        const int k = iy * m.nx + ix;
         float2 pr[L], r0[L], r1[L], r2[L], r3[L];
         float nr[L];
 #pragma unroll
         for (int l = 0; l < L; ++l) {
-            pr[l] = on ? rec[c * L + l] : make_float2(-1.0f, -1.0f);
+            const float2 v = rec[c * L + l];  // c = 0 off the map: a valid record, not used
+            pr[l] = on ? v : make_float2(-1.0f, -1.0f);
             r0[l] = rec[k * L + l];
             r1[l] = rec[(k + 1) * L + l];
             r2[l] = rec[(k + m.nx) * L + l];
             r3[l] = rec[(k + m.nx + 1) * L + l];
-            if (nearest) nr[l] = on ? grid_at(m, l, RACER_NEAREST, c) : -1.0f;
+            if (nearest) {
+                const float w = grid_at(m, l, RACER_NEAREST, c);
+                nr[l] = on ? w : -1.0f;
+            }
         }
         Spot best{0, -1.0f, pr[0].y};
         float gap = 1e30f;
 #pragma unroll
         for (int l = 0; l < L; ++l) {
-            if (pr[l].x > 0.0f && fabsf(pr[l].y - z) < gap) {
-                best.layer = l;
-                best.clearance = pr[l].x;
-                best.height = pr[l].y;
-                gap = fabsf(pr[l].y - z);
-            }
+            const float d = fabsf(pr[l].y - z);
+            const bool take = (pr[l].x > 0.0f) & (d < gap);
+            best.layer = take ? l : best.layer;
+            best.clearance = take ? pr[l].x : best.clearance;
+            best.height = take ? pr[l].y : best.height;
+            gap = take ? d : gap;
         }
         *spot = best;
         if (nearest) {
             float w = nr[0];
 #pragma unroll
-            for (int l = 1; l < L; ++l)
-                if (best.layer == l) w = nr[l];
+            for (int l = 1; l < L; ++l) w = best.layer == l ? nr[l] : w;
             *nearest = w;
         }
         float h0 = r0[0].y, h1 = r1[0].y, h2 = r2[0].y, h3 = r3[0].y;
 #pragma unroll
         for (int l = 1; l < L; ++l) {
-            if (best.layer == l) {
-                h0 = r0[l].y;
-                h1 = r1[l].y;
-                h2 = r2[l].y;
-                h3 = r3[l].y;
-            }
+            const bool sel = best.layer == l;
+            h0 = sel ? r0[l].y : h0;
+            h1 = sel ? r1[l].y : h1;
+            h2 = sel ? r2[l].y : h2;
+            h3 = sel ? r3[l].y : h3;
         }
         float w0 = (1.0f - tx) * (1.0f - ty);
         float w1 = tx * (1.0f - ty);
