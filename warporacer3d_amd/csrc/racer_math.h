@@ -99,17 +99,22 @@ RACER_FN int racer_reduce_pio2(float x, float* r) {
     return ((int)k) & 3;
 }
 
+/* Both short kernels are evaluated and one is selected: on the GPU a choice
+ * between calls compiles to a divergent branch per call, which costs a
+ * serial per-car chain more than the other kernel's few instructions. */
 RACER_FN float racer_sin(float x) {
     float r;
     int q = racer_reduce_pio2(x, &r);
-    float v = (q & 1) ? racer_cos_kernel(r) : racer_sin_kernel(r);
+    const float s = racer_sin_kernel(r), c = racer_cos_kernel(r);
+    float v = (q & 1) ? c : s;
     return (q & 2) ? -v : v;
 }
 
 RACER_FN float racer_cos(float x) {
     float r;
     int q = racer_reduce_pio2(x, &r);
-    float v = (q & 1) ? racer_sin_kernel(r) : racer_cos_kernel(r);
+    const float s = racer_sin_kernel(r), c = racer_cos_kernel(r);
+    float v = (q & 1) ? s : c;
     return ((q + 1) & 2) ? -v : v;
 }
 
