@@ -470,56 +470,73 @@ __global__ void __launch_bounds__(64) k_physics_surrogate(racer_dmap m, float* s
                        out_mean, out_std, min_clearance, fallback, i);
 }
 
-// The whole bf16 surrogate step for 32 cars per workgroup in one launch:
-// inputs (wavefront 0, lanes 0 .. 31, one car each, which keep the start
-// state in registers for the pose update), the three 128-wide tanh layers on
-// v_mfma_f32_32x32x16_bf16 with wavefront w owning rows 32 w .. + 31 of each
-// layer (weights from the fragment-ordered copy, loaded ahead under the
-// previous phase; activations in LDS as bf16 [car][k]), the float32 output
-// layer with k in order (output o on wavefront o % 4, lanes 32 o / 4 ..),
-// then the pose update (wavefront 0). The operands, instruction and k order
-// of each product are those of the separate bf16 and float32 products, so the
+// The whole bf16 surrogate step for 64 cars per workgroup in one launch, on
+// nine wavefronts. Wavefront 8, the car wavefront (one car per lane),
+// computes the inputs while wavefronts 0 .. 7 load the first layer's weight
+// fragments and set up LDS, keeps each car's start state in registers through the network,
+// and applies the pose update at the end. Wavefronts 0 .. 7 run the three
+// 128-wide tanh layers on v_mfma_f32_32x32x16_bf16, wavefront w owning rows
+// 32 (w % 4) .. + 31 of each layer for cars 32 (w / 4) .. + 31 (activations
+// in LDS as bf16 [car][k]), two per SIMD so that one's tanh epilogue runs
+// under the other's products and LDS reads. Wavefronts 0 .. 4 then run the
+// float32 output layer, one lane per output and car, k in order, four k per
+// LDS read from the car-major h3. The operands, instruction and k order of
+// each product are those of the separate bf16 and float32 products, so the
 // step is the same as the unfused one: the only difference is the unfused
 // products' trailing all-zero k-steps, which change an accumulator only when
 // it is exactly -0.
-constexpr int SF_CARS = 32, SF_LDX = 152;  // bf16 row stride: 144 k + 8 (conflict-free 16-byte reads)
+constexpr int SF_CARS = 64, SF_NET_WAVES = 8, SF_THREADS = 64 * (SF_NET_WAVES + 1);
+constexpr int SF_NET_THREADS = 64 * SF_NET_WAVES;
+constexpr int SF_LDX = 152;  // bf16 row stride: 144 k + 8 (conflict-free 16-byte reads)
+constexpr int SF_LDH = 132;  // float row stride of h3 and w3s: 129 k padded to 16 bytes
 
 template <int L>
-__global__ void __launch_bounds__(256) k_surrogate_fused(racer_dmap m, float* s, int n, const float* act,
-                                                         int64_t act_cols, float* reward, float* done, float* obs,
-                                                         int64_t obs_cols, int beams, int seed, float heading_noise,
-                                                         const uint16_t* __restrict__ frags,
-                                                         const float* __restrict__ w3,
-                                                         const float* __restrict__ st, float min_clearance,
-                                                         float* fallback) {
+__global__ void __launch_bounds__(SF_THREADS) k_surrogate_fused(racer_dmap m, float* s, int n, const float* act,
+                                                                int64_t act_cols, float* reward, float* done,
+                                                                float* obs, int64_t obs_cols, int beams, int seed,
+                                                                float heading_noise,
+                                                                const uint16_t* __restrict__ frags,
+                                                                const float* __restrict__ w3,
+                                                                const float* __restrict__ st, float min_clearance,
+                                                                float* fallback) {
     typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
     typedef float f32x16 __attribute__((ext_vector_type(16)));
     __shared__ __attribute__((aligned(16))) uint16_t xa[SF_CARS][SF_LDX];
     __shared__ __attribute__((aligned(16))) uint16_t xb[SF_CARS][SF_LDX];
-    __shared__ float h3[129][SF_CARS];  // float32 input of the output layer, [k][car]
-    __shared__ float w3s[5][129];
+    __shared__ __attribute__((aligned(16))) float h3[SF_CARS][SF_LDH];  // float32 input of the output layer
+    __shared__ __attribute__((aligned(16))) float w3s[5][SF_LDH];
     __shared__ float pred[5][SF_CARS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int car0 = (int)blockIdx.x * SF_CARS;
-    const int j = lane & 31, half = lane >> 5;
-    // this wavefront's row tile of layers 0 and 1, loading under the inputs phase
+    const bool net = wave < SF_NET_WAVES;  // wavefront-uniform
+    const int rt = wave & 3, car = 32 * (wave >> 2) + (lane & 31);  // a network lane's row tile and car
+    const int half = lane >> 5;
+    const int i = (int)blockIdx.x * SF_CARS + lane;  // the car wavefront's car
     const bf16x8* fw = reinterpret_cast<const bf16x8*>(frags) + lane;
-    const bf16x8 f0 = fw[(size_t)wave * 64];
-    bf16x8 f1[RACER_SF_STEPS], f2[RACER_SF_STEPS];
-#pragma unroll
-    for (int k = 0; k < RACER_SF_STEPS; ++k) f1[k] = fw[(size_t)(RACER_SF_BASE1 + wave * RACER_SF_STEPS + k) * 64];
     const uint16_t one = 0x3f80;  // bf16 1.0
-    // constant rows: xa k = 11 (layer 0's bias) and k = 128, xb k = 128; zeros past them
-    for (int idx = tid; idx < SF_CARS * SF_LDX; idx += 256) {
-        const int r = idx / SF_LDX, k = idx % SF_LDX;
-        if (k >= 11) xa[r][k] = (k == 11 || k == 128) ? one : (uint16_t)0;
-        if (k >= 128) xb[r][k] = k == 128 ? one : (uint16_t)0;
-    }
-    for (int idx = tid; idx < 5 * 129; idx += 256) w3s[idx / 129][idx % 129] = w3[idx];
+    constexpr int W3R = (5 * 129 + SF_NET_THREADS - 1) / SF_NET_THREADS;
+    bf16x8 f0, f1[RACER_SF_STEPS], f2[RACER_SF_STEPS];
+    float w3r[W3R];  // this lane's share of the output layer's weights, stored to LDS after the first layer
     SurStart c;
-    const int i = car0 + lane;
-    const bool mine = wave == 0 && lane < SF_CARS && i < n;
-    if (wave == 0 && lane < SF_CARS) {
+    if (net) {
+#pragma unroll
+        for (int r = 0; r < W3R; ++r) {
+            const int idx = tid + SF_NET_THREADS * r;
+            w3r[r] = idx < 5 * 129 ? w3[idx] : 0.0f;
+        }
+        f0 = fw[(size_t)rt * 64];
+        // constant columns: xa k = 11 (layer 0's bias) and k = 128, xb k = 128; zeros past them
+        // (four k per 8-byte store from k = 12; k = 128 is the low half of its first word)
+        const int r = tid >> 3, part = tid & 7;
+        if (part == 0) xa[r][11] = one;
+        for (int t = part; t < (SF_LDX - 12) / 4; t += 8) {
+            const int k = 12 + 4 * t;
+            *reinterpret_cast<uint2*>(&xa[r][k]) = make_uint2(k == 128 ? (uint32_t)one : 0u, 0u);
+        }
+        for (int t = part; t < (SF_LDX - 128) / 4; t += 8) {
+            const int k = 128 + 4 * t;
+            *reinterpret_cast<uint2*>(&xb[r][k]) = make_uint2(k == 128 ? (uint32_t)one : 0u, 0u);
+        }
+    } else {
         float f[11];
         if (i < n) {
             surrogate_inputs<L>(m, s, n, act, act_cols, i, f, &c);
@@ -530,7 +547,7 @@ __global__ void __launch_bounds__(256) k_surrogate_fused(racer_dmap m, float* s,
 #pragma unroll
         for (int r = 0; r < 11; ++r)
             xa[lane][r] = __builtin_bit_cast(uint16_t, (__bf16)((f[r] - st[r]) * st[11 + r]));
-        h3[128][lane] = 1.0f;
+        h3[lane][128] = 1.0f;
     }
     __syncthreads();
     auto tile = [&](const uint16_t (*x)[SF_LDX], const bf16x8* a, int steps) {
@@ -540,41 +557,66 @@ __global__ void __launch_bounds__(256) k_surrogate_fused(racer_dmap m, float* s,
 #pragma unroll
         for (int k = 0; k < RACER_SF_STEPS; ++k) {
             if (k < steps) {
-                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&x[j][16 * k + 8 * half]);
+                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&x[car][16 * k + 8 * half]);
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k], bv, acc, 0, 0, 0);
             }
         }
         return acc;
     };
-    auto row = [&](int q) { return 32 * wave + 8 * (q >> 2) + 4 * half + (q & 3); };
-    {
+    // accumulator entries 4 g .. 4 g + 3 hold rows 32 rt + 8 g + 4 half + 0 .. 3 of this lane's car
+    const int row0 = 32 * rt + 4 * half;
+    auto store_bf16 = [&](uint16_t (*y)[SF_LDX], const f32x16& acc) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            uint32_t v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = __builtin_bit_cast(uint16_t, (__bf16)racer_tanh_mlp(acc[4 * g + e]));
+            *reinterpret_cast<uint2*>(&y[car][row0 + 8 * g]) = make_uint2(v[0] | (v[1] << 16), v[2] | (v[3] << 16));
+        }
+    };
+    if (net) {
+        // the second layer's fragments only now, so that they do not queue ahead of the car wavefront's loads
+#pragma unroll
+        for (int k = 0; k < RACER_SF_STEPS; ++k) f1[k] = fw[(size_t)(RACER_SF_BASE1 + rt * RACER_SF_STEPS + k) * 64];
         const f32x16 acc = tile(xa, &f0, RACER_SF_STEPS0);
 #pragma unroll
-        for (int k = 0; k < RACER_SF_STEPS; ++k) f2[k] = fw[(size_t)(RACER_SF_BASE2 + wave * RACER_SF_STEPS + k) * 64];
+        for (int k = 0; k < RACER_SF_STEPS; ++k) f2[k] = fw[(size_t)(RACER_SF_BASE2 + rt * RACER_SF_STEPS + k) * 64];
+        store_bf16(xb, acc);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) xb[j][row(q)] = __builtin_bit_cast(uint16_t, (__bf16)racer_tanh_mlp(acc[q]));
+        for (int r = 0; r < W3R; ++r) {
+            const int idx = tid + SF_NET_THREADS * r;
+            if (idx < 5 * 129) w3s[idx / 129][idx % 129] = w3r[r];
+        }
     }
     __syncthreads();
-    {
-        const f32x16 acc = tile(xb, f1, RACER_SF_STEPS);
-#pragma unroll
-        for (int q = 0; q < 16; ++q) xa[j][row(q)] = __builtin_bit_cast(uint16_t, (__bf16)racer_tanh_mlp(acc[q]));
-    }
+    if (net) store_bf16(xa, tile(xb, f1, RACER_SF_STEPS));
     __syncthreads();
-    {
+    if (net) {
         const f32x16 acc = tile(xa, f2, RACER_SF_STEPS);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) h3[row(q)][j] = racer_tanh_mlp(acc[q]);
+        for (int g = 0; g < 4; ++g)
+            *reinterpret_cast<float4*>(&h3[car][row0 + 8 * g]) =
+                make_float4(racer_tanh_mlp(acc[4 * g]), racer_tanh_mlp(acc[4 * g + 1]),
+                            racer_tanh_mlp(acc[4 * g + 2]), racer_tanh_mlp(acc[4 * g + 3]));
     }
     __syncthreads();
-    const int o = wave + 4 * half;
-    if (o < 5) {
+    if (wave < 5) {
+        // output `wave` of car `lane`
+        const float4* wv = reinterpret_cast<const float4*>(w3s[wave]);
+        const float4* hv = reinterpret_cast<const float4*>(h3[lane]);
         float acc = 0.0f;
-        for (int k = 0; k < 129; ++k) acc = fmaf(w3s[o][k], h3[k][j], acc);
-        pred[o][j] = acc;
+#pragma unroll
+        for (int kb = 0; kb < 32; ++kb) {
+            const float4 a = wv[kb], x = hv[kb];
+            acc = fmaf(a.x, x.x, acc);
+            acc = fmaf(a.y, x.y, acc);
+            acc = fmaf(a.z, x.z, acc);
+            acc = fmaf(a.w, x.w, acc);
+        }
+        pred[wave][lane] = fmaf(w3s[wave][128], h3[lane][128], acc);
     }
     __syncthreads();
-    if (mine) {
+    if (!net && i < n) {
         float q[5];
 #pragma unroll
         for (int r = 0; r < 5; ++r) q[r] = pred[r][lane];
@@ -1323,7 +1365,7 @@ int racer_launch_surrogate_fused(racer_sim* sim, const uint16_t* frags, const fl
                                        : (sim->map.layers == 2 ? k_surrogate_fused<2> : k_surrogate_fused<0>);
     {
         racer_timed timer(sim->dev, "surrogate_fused");
-        hipLaunchKernelGGL(kernel, dim3((n + SF_CARS - 1) / SF_CARS), dim3(256), 0, sim->dev->stream, sim->map,
+        hipLaunchKernelGGL(kernel, dim3((n + SF_CARS - 1) / SF_CARS), dim3(SF_THREADS), 0, sim->dev->stream, sim->map,
                            sim->state->ptr, n, sim->actions->ptr, (int64_t)n, sim->reward->ptr, sim->done->ptr,
                            sim->obs->ptr, (int64_t)n, sim->beams, sim->seed, sim->heading_noise, frags, w3, stats,
                            min_clearance, fallback);
