@@ -56,10 +56,10 @@ extern "C" int racer_batch_create(racer_device* dev, racer_sim** sims, int count
 
 extern "C" void racer_batch_destroy(racer_batch* b) {
     if (!b) return;
-    if (b->dev) hipStreamSynchronize(b->dev->stream);
+    if (b->dev) (void)hipStreamSynchronize(b->dev->stream);
     for (auto* a : b->acts) racer_acts_destroy(a);
     racer_free(b->scratch);
-    if (b->counter) hipFree(b->counter);
+    if (b->counter) (void)hipFree(b->counter);
     racer_device* dev = b->dev;
     delete b;
     racer_device_release(dev);
@@ -141,32 +141,37 @@ extern "C" int racer_batch_waves_per_car(const racer_batch* b) { return b->waves
 extern "C" int racer_batch_autotune(racer_batch* b, int* chosen) {
     racer_clear_error();
     RACER_REQUIRE(!b->dev->capturing, "cannot autotune while capturing a graph");
-    hipEvent_t e0, e1;
-    RACER_TRY(hipEventCreate(&e0));
-    RACER_TRY(hipEventCreate(&e1));
+    racer_event_pair ev;
+    RACER_TRY(hipEventCreate(&ev.a));
+    RACER_TRY(hipEventCreate(&ev.b));
     const bool counting = b->count_iterations;
+    const int before = b->waves_per_car;
     b->count_iterations = false;
     float best = 1e30f;
-    int pick = b->waves_per_car;
-    for (int w : {1, 2, 4}) {
-        b->waves_per_car = w;
-        if (racer_launch_batch_lidar(b)) return 1;  // warm
-        RACER_TRY(hipEventRecord(e0, b->dev->stream));
-        for (int r = 0; r < 3; ++r)
-            if (racer_launch_batch_lidar(b)) return 1;
-        RACER_TRY(hipEventRecord(e1, b->dev->stream));
-        RACER_TRY(hipEventSynchronize(e1));
-        float ms = 0.0f;
-        RACER_TRY(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-            best = ms;
-            pick = w;
+    int pick = before;
+    auto time_each = [&]() -> int {
+        for (int w : {1, 2, 4}) {
+            b->waves_per_car = w;
+            if (racer_launch_batch_lidar(b)) return 1;  // warm
+            RACER_TRY(hipEventRecord(ev.a, b->dev->stream));
+            for (int r = 0; r < 3; ++r)
+                if (racer_launch_batch_lidar(b)) return 1;
+            RACER_TRY(hipEventRecord(ev.b, b->dev->stream));
+            RACER_TRY(hipEventSynchronize(ev.b));
+            float ms = 0.0f;
+            RACER_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+            if (ms < best) {
+                best = ms;
+                pick = w;
+            }
         }
-    }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+        return 0;
+    };
+    const int failed = time_each();
+    // on failure the batch keeps its previous count
     b->count_iterations = counting;
-    b->waves_per_car = pick;
+    b->waves_per_car = failed ? before : pick;
+    if (failed) return 1;
     if (chosen) *chosen = pick;
     return 0;
 }

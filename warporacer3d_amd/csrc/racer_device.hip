@@ -80,16 +80,16 @@ void racer_device_retain(racer_device* dev) {
 
 void racer_device_release(racer_device* dev) {
     if (!dev || --dev->refs > 0) return;
-    hipStreamSynchronize(dev->stream);
+    (void)hipStreamSynchronize(dev->stream);
     for (auto& p : dev->pending) {
-        hipEventDestroy(p.start);
-        hipEventDestroy(p.stop);
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
     }
-    for (auto e : dev->event_pool) hipEventDestroy(e);
-    if (dev->scratch) hipFree(dev->scratch);
-    if (dev->graph_base) hipFree(dev->graph_base);
-    if (dev->sums) hipFree(dev->sums);
-    hipStreamDestroy(dev->stream);
+    for (auto e : dev->event_pool) (void)hipEventDestroy(e);
+    if (dev->scratch) (void)hipFree(dev->scratch);
+    if (dev->graph_base) (void)hipFree(dev->graph_base);
+    if (dev->sums) (void)hipFree(dev->sums);
+    (void)hipStreamDestroy(dev->stream);
     delete dev;
 }
 
@@ -166,8 +166,8 @@ extern "C" int racer_alloc(racer_device* dev, size_t count, float fill, racer_bu
 extern "C" void racer_free(racer_buf* buf) {
     if (!buf) return;
     if (buf->ptr) {
-        hipStreamSynchronize(buf->dev->stream);
-        hipFree(buf->ptr);
+        (void)hipStreamSynchronize(buf->dev->stream);
+        (void)hipFree(buf->ptr);
     }
     racer_device* dev = buf->dev;
     delete buf;
@@ -245,7 +245,7 @@ static hipEvent_t take_event(racer_device* dev) {
         return e;
     }
     hipEvent_t e = nullptr;
-    hipEventCreate(&e);
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
     return e;
 }
 
@@ -259,26 +259,36 @@ static int kernel_index(racer_device* dev, const char* name) {
 racer_timed::racer_timed(racer_device* d, const char* name) : dev(d) {
     if (!dev || !dev->profiling || dev->capturing) return;
     if (!dev->profile_only.empty() && dev->profile_only != name) return;
-    kernel = kernel_index(dev, name);
     start = take_event(dev);
     stop = take_event(dev);
-    hipEventRecord(start, dev->stream);
+    // a launch whose events cannot be made or recorded is left out of the samples
+    if (start && stop && hipEventRecord(start, dev->stream) == hipSuccess) {
+        kernel = kernel_index(dev, name);
+        return;
+    }
+    if (start) dev->event_pool.push_back(start);
+    if (stop) dev->event_pool.push_back(stop);
 }
 
 racer_timed::~racer_timed() {
     if (kernel < 0) return;
-    hipEventRecord(stop, dev->stream);
-    dev->pending.push_back({kernel, start, stop});
+    if (hipEventRecord(stop, dev->stream) == hipSuccess) {
+        dev->pending.push_back({kernel, start, stop});
+        return;
+    }
+    dev->event_pool.push_back(start);
+    dev->event_pool.push_back(stop);
 }
 
 static void resolve_pending(racer_device* dev) {
     if (dev->pending.empty()) return;
-    hipStreamSynchronize(dev->stream);
+    (void)hipStreamSynchronize(dev->stream);  // an error shows in the elapsed times below
     for (auto& p : dev->pending) {
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, p.start, p.stop);
-        dev->samples[p.kernel].launches += 1;
-        dev->samples[p.kernel].total_ms += ms;
+        if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+            dev->samples[p.kernel].launches += 1;
+            dev->samples[p.kernel].total_ms += ms;
+        }
         dev->event_pool.push_back(p.start);
         dev->event_pool.push_back(p.stop);
     }

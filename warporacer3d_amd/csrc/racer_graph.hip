@@ -53,7 +53,7 @@ int capture(racer_device* dev, racer_policy* pol, racer_graph* g, F body) {
     dev->capturing = false;
     dev->step_base = nullptr;
     if (failed) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return 1;
     }
     RACER_TRY(ended);
@@ -151,8 +151,8 @@ extern "C" int racer_trainer_rollout_graph(racer_trainer* tr, racer_graph* g, in
 
 extern "C" void racer_graph_destroy(racer_graph* g) {
     if (!g) return;
-    if (g->exec) hipGraphExecDestroy(g->exec);
-    if (g->graph) hipGraphDestroy(g->graph);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
     racer_device* dev = g->dev;
     delete g;
     racer_device_release(dev);

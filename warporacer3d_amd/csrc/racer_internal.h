@@ -92,6 +92,18 @@ struct racer_buf {
 };
 
 // Launch bracket for the profiler: records events around a launch when enabled.
+// Two timing events, destroyed on every way out of the scope that made them.
+struct racer_event_pair {
+    hipEvent_t a = nullptr, b = nullptr;
+    racer_event_pair() = default;
+    racer_event_pair(const racer_event_pair&) = delete;
+    racer_event_pair& operator=(const racer_event_pair&) = delete;
+    ~racer_event_pair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
 struct racer_timed {
     racer_device* dev;
     int kernel = -1;

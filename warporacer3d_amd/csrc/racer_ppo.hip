@@ -377,10 +377,10 @@ extern "C" void racer_policy_destroy(racer_policy* p) {
     racer_free(p->theta);
     racer_free(p->w1t);
     racer_free(p->w2t);
-    if (p->theta_bf16) hipFree(p->theta_bf16);
-    if (p->w1t_bf16) hipFree(p->w1t_bf16);
-    if (p->frag0) hipFree(p->frag0);
-    if (p->frag1) hipFree(p->frag1);
+    if (p->theta_bf16) (void)hipFree(p->theta_bf16);
+    if (p->w1t_bf16) (void)hipFree(p->w1t_bf16);
+    if (p->frag0) (void)hipFree(p->frag0);
+    if (p->frag1) (void)hipFree(p->frag1);
     racer_device* dev = p->dev;
     delete p;
     racer_device_release(dev);
@@ -599,7 +599,7 @@ extern "C" void racer_trainer_destroy(racer_trainer* tr) {
                          tr->d3, tr->d2, tr->d1, tr->grad, tr->first, tr->second, tr->stats, tr->xall};
     for (auto* b : bufs) racer_free(b);
     racer_acts_destroy(tr->acts);
-    if (tr->dstats) hipFree(tr->dstats);
+    if (tr->dstats) (void)hipFree(tr->dstats);
     racer_device* dev = tr->dev;
     delete tr;
     racer_device_release(dev);

@@ -816,8 +816,9 @@ __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_
     // 0 <= f < n as one unsigned compare of the bit patterns: f is a floor
     // of a sum with 0.5, never -0.0, so a set sign bit means negative, and
     // NaN and +inf patterns lie above every finite n
-    const bool on = (__float_as_uint(fx) < __float_as_uint((float)m.nx)) &
-                    (__float_as_uint(fy) < __float_as_uint((float)m.ny));
+    const bool on_x = __float_as_uint(fx) < __float_as_uint((float)m.nx);
+    const bool on_y = __float_as_uint(fy) < __float_as_uint((float)m.ny);
+    const bool on = on_x & on_y;
     // cell = fy nx + fx, exact in float (both below 2^12, the sum below 2^24)
     const uint32_t off = on ? (uint32_t)fmaf(fy, (float)m.nx, fx) * (uint32_t)(8 * L) : (uint32_t)(m.plane * 8 * L);
     if constexpr (L == 1) {
@@ -1495,16 +1496,16 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
 
 extern "C" void racer_sim_destroy(racer_sim* sim) {
     if (!sim) return;
-    if (sim->dev) hipStreamSynchronize(sim->dev->stream);
+    if (sim->dev) (void)hipStreamSynchronize(sim->dev->stream);  // destroy reports no errors
     racer_free(sim->data);
     racer_free(sim->state);
     racer_free(sim->obs);
     racer_free(sim->actions);
     racer_free(sim->reward);
     racer_free(sim->done);
-    if (sim->packed) hipFree(sim->packed);
-    if (sim->dirs) hipFree(sim->dirs);
-    if (sim->counter) hipFree(sim->counter);
+    if (sim->packed) (void)hipFree(sim->packed);
+    if (sim->dirs) (void)hipFree(sim->dirs);
+    if (sim->counter) (void)hipFree(sim->counter);
     racer_device* dev = sim->dev;
     delete sim;
     racer_device_release(dev);
@@ -1574,32 +1575,37 @@ extern "C" int racer_sim_waves_per_car(const racer_sim* sim) { return sim->waves
 extern "C" int racer_sim_autotune(racer_sim* sim, int* chosen) {
     racer_clear_error();
     RACER_REQUIRE(!sim->dev->capturing, "cannot autotune while capturing a graph");
-    hipEvent_t a, b;
-    RACER_TRY(hipEventCreate(&a));
-    RACER_TRY(hipEventCreate(&b));
+    racer_event_pair ev;
+    RACER_TRY(hipEventCreate(&ev.a));
+    RACER_TRY(hipEventCreate(&ev.b));
     const bool counting = sim->count_iterations;
+    const int before = sim->waves_per_car;
     sim->count_iterations = false;
     float best = 1e30f;
-    int pick = sim->waves_per_car;
-    for (int w : {1, 2, 4}) {
-        sim->waves_per_car = w;
-        launch_lidar_any(sim, sim->obs->ptr, sim->n);  // warm
-        RACER_TRY(hipEventRecord(a, sim->dev->stream));
-        for (int r = 0; r < 3; ++r) launch_lidar_any(sim, sim->obs->ptr, sim->n);
-        RACER_TRY(hipEventRecord(b, sim->dev->stream));
-        if (racer_launch_check("autotune lidar")) return 1;
-        RACER_TRY(hipEventSynchronize(b));
-        float ms = 0.0f;
-        RACER_TRY(hipEventElapsedTime(&ms, a, b));
-        if (ms < best) {
-            best = ms;
-            pick = w;
+    int pick = before;
+    auto time_each = [&]() -> int {
+        for (int w : {1, 2, 4}) {
+            sim->waves_per_car = w;
+            launch_lidar_any(sim, sim->obs->ptr, sim->n);  // warm
+            RACER_TRY(hipEventRecord(ev.a, sim->dev->stream));
+            for (int r = 0; r < 3; ++r) launch_lidar_any(sim, sim->obs->ptr, sim->n);
+            RACER_TRY(hipEventRecord(ev.b, sim->dev->stream));
+            if (racer_launch_check("autotune lidar")) return 1;
+            RACER_TRY(hipEventSynchronize(ev.b));
+            float ms = 0.0f;
+            RACER_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+            if (ms < best) {
+                best = ms;
+                pick = w;
+            }
         }
-    }
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+        return 0;
+    };
+    const int failed = time_each();
+    // on failure the sim keeps its previous count
     sim->count_iterations = counting;
-    sim->waves_per_car = pick;
+    sim->waves_per_car = failed ? before : pick;
+    if (failed) return 1;
     if (chosen) *chosen = pick;
     return 0;
 }

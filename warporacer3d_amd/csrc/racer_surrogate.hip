@@ -216,18 +216,18 @@ struct racer_dataset {
 // ---------------------------------------------------------------- surrogate
 extern "C" void racer_surrogate_destroy(racer_surrogate* s) {
     if (!s) return;
-    if (s->dev) hipStreamSynchronize(s->dev->stream);
+    if (s->dev) (void)hipStreamSynchronize(s->dev->stream);
     racer_buf* bufs[] = {s->theta, s->grad, s->first, s->second, s->wt, s->stats, s->y, s->mask};
     for (auto* b : bufs) racer_free(b);
     for (int l = 0; l <= S_LAYERS; ++l) {
         racer_free(s->act[l]);
         racer_free(s->delta[l]);
     }
-    if (s->theta_bf16) hipFree(s->theta_bf16);
-    if (s->frags) hipFree(s->frags);
-    if (s->doff) hipFree(s->doff);
-    if (s->partial) hipFree(s->partial);
-    if (s->acc) hipFree(s->acc);
+    if (s->theta_bf16) (void)hipFree(s->theta_bf16);
+    if (s->frags) (void)hipFree(s->frags);
+    if (s->doff) (void)hipFree(s->doff);
+    if (s->partial) (void)hipFree(s->partial);
+    if (s->acc) (void)hipFree(s->acc);
     racer_device* dev = s->dev;
     delete s;
     racer_device_release(dev);
