@@ -61,7 +61,7 @@ def lib():
             "o_scan_one": (F, [P, C.c_size_t, P, L, P, L, L, I, I, P]),
             "o_check_nearest_index": (L, [F, F, F]),
             "o_check_nearest_index_lim": (L, [F, F, F]),
-            "o_check_min_quotient": (L, [L, C.c_uint64]),
+            "o_check_div_by_reciprocal": (L, [L, C.c_uint64]),
             "o_surrogate_features": (None, [P, C.c_size_t, P, L, P, L, P, L]),
             "o_surrogate_targets": (None, [P, P, P, L, P, L, P]),
             "o_mlp_forward": (None, [P, P, I, P, L, P]),

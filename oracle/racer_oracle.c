@@ -638,20 +638,60 @@ long o_check_nearest_index_lim(float b, float lo, float hi) {
 
 /* `count` seeded random (cap, num, den) triples: mismatches of
  * racer_min_quotient against racer_min(cap, num / den). */
-long o_check_min_quotient(long count, uint64_t seed) {
+/* A float with the given unbiased exponent (-126 .. 127) and 23-bit fraction. */
+static float o_make_float(int exponent, uint32_t fraction, int negative) {
+    return racer_u2f(((uint32_t)negative << 31) | ((uint32_t)(exponent + 127) << 23) | (fraction & 0x7fffffu));
+}
+
+/* racer_div_by_reciprocal(a, b, RN(1 / b)) against the IEEE quotient a / b,
+ * bit for bit, over its stated range. Cases: random a and b (exponents
+ * -60 .. 60); a quotient near a rounding midpoint (a = RN(m b) for a midpoint
+ * m between two floats, nudged by -2 .. +2 ulps); hard-to-round quotients,
+ * within r 2^-25 ulp of a midpoint for r = +-1, +-3 (divisor significand B
+ * odd, midpoint significand N = r B^-1 mod 2^25, then N B - r = 2^25 A and
+ * a = A scaled; these catch a reciprocal that is one ulp off); divisors with
+ * an all-ones fraction or a power of two; a = +0. */
+long o_check_div_by_reciprocal(long count, uint64_t seed) {
     long bad = 0;
 #pragma omp parallel for schedule(static) num_threads(g_threads) reduction(+ : bad)
     for (long k = 0; k < count; ++k) {
         racer_u32x4 r = racer_random(seed, (uint64_t)k, 0);
-        float cap = (racer_uniform(r.v[0]) - 0.2f) * 3.0f;
-        float num = racer_uniform(r.v[1]) * 0.5f;
-        float den = racer_uniform(r.v[2]) * 0.3f;
-        if ((r.v[3] & 15u) == 0) den = 0.0f;
-        if ((r.v[3] & 240u) == 16u) num = 0.0f;
-        if ((r.v[3] & 3840u) == 256u && den > 0.0f) cap = num / den; /* ties at the boundary */
-        float inv = 1.0f / den;
-        float want = racer_min(cap, num / den);
-        float got = racer_min_quotient(cap, num, den, inv);
+        racer_u32x4 r2 = racer_random(seed, (uint64_t)k, 1);
+        int kind = (int)(r.v[3] & 7u);
+        int eb = (int)(r.v[0] % 121u) - 60;
+        float b = o_make_float(eb, r.v[1], 0);
+        float a = o_make_float((int)(r2.v[0] % 121u) - 60, r2.v[1], (int)(r2.v[2] & 1u));
+        if (kind == 3) {
+            /* quotient exponent chosen so that |a| = |m b| stays within [2^-59, 2^59] */
+            int eq = (int)(r2.v[0] % 117u) - 58 - eb;
+            if (eq < -58) eq = -58;
+            if (eq > 58) eq = 58;
+            if (eq + eb < -58 || eq + eb > 58) eq = -eb;
+            float q = o_make_float(eq, r2.v[1], (int)(r2.v[2] & 1u));
+            double m = (double)q + (q > 0 ? 0.5 : -0.5) * ((double)racer_u2f(racer_f2u(fabsf(q)) + 1u) - fabs((double)q));
+            a = (float)(m * (double)b);
+            int nudge = (int)(r2.v[3] % 5u) - 2;
+            a = racer_u2f((uint32_t)((int32_t)racer_f2u(a) + nudge));
+        } else if (kind == 4 || kind == 5) {
+            uint32_t B = (r.v[1] & 0x7fffffu) | 0x800001u, x = B;
+            for (int i = 0; i < 5; ++i) x *= 2u - B * x; /* B^-1 mod 2^32 */
+            static const int rs[4] = {1, -1, 3, -3};
+            int rr = rs[r2.v[3] & 3u];
+            uint32_t N = ((uint32_t)rr * x) & 0x1ffffffu;
+            int64_t A = ((int64_t)((uint64_t)N * B) - rr) >> 25;
+            if (N < 0x1000000u || A < (1 << 23) || A >= (1 << 24)) continue; /* not a normal 25-bit midpoint */
+            int e1 = (int)(r2.v[0] % 61u) - 30, e2 = (int)(r2.v[1] % 61u) - 30;
+            b = ldexpf((float)B, e1 - 23);
+            a = ldexpf((float)A, e1 + e2 - 22);
+            if (r2.v[2] & 1u) a = -a;
+        } else if (kind == 6) {
+            b = o_make_float(eb, (r.v[3] & 8u) ? 0x7fffffu : 0u, 0);
+        } else if (kind == 7) {
+            a = 0.0f;
+        }
+        float y = 1.0f / b;
+        float want = a / b;
+        float got = racer_div_by_reciprocal(a, b, y);
         if (racer_f2u(want) != racer_f2u(got)) ++bad;
     }
     return bad;

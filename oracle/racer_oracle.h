@@ -23,7 +23,9 @@
  *  - Philox4x32-10: the Random123 known-answer vectors; uniform and Box-Muller
  *    normals: their range and moment statistics;
  *  - sin, cos, exp, log, tanh: ULP bounds against float64 libm and special
- *    values; the two division shortcuts: exhaustive and sampled checks.
+ *    values; the nearest-cell division shortcut: exhaustive checks; the
+ *    lidar's quotient from a reciprocal: sampled checks over its range,
+ *    rounding midpoints included, on top of its proof (Markstein's theorem).
  * Parity unpinned (no fixture or closed form fixes the bits, and the Mojo
  * standard library that defines them is absent): the mapping of Mojo's
  * Random(seed, subsequence, offset) onto the Philox counter and key and its
@@ -130,10 +132,11 @@ float o_progress(const float* d, size_t len, int layer, float x, float y);
 float o_scan_one(const float* d, size_t len, const float* s, long n, const float* obs, long obs_cols, long i, int beam,
                  int beams, long* iterations);
 
-/* Exhaustive / sampled checks of the shared header's division shortcuts. */
+/* Exhaustive / sampled checks of the shared header's division shortcuts
+ * (each returns the number of mismatches). */
 long o_check_nearest_index(float b, float lo, float hi);
 long o_check_nearest_index_lim(float b, float lo, float hi);
-long o_check_min_quotient(long count, uint64_t seed);
+long o_check_div_by_reciprocal(long count, uint64_t seed);
 
 /* Threads used by the parallel loops (1 = serial). */
 void o_set_threads(int threads);

@@ -189,6 +189,38 @@ def test_autotune_keeps_results(dev):
     _assert_same(g, o, "step after autotune")
 
 
+@pytest.mark.parametrize("name", ["ramp10", "bridge8"])
+def test_lidar_quotient_paths_bit_identical(dev, name):
+    """The march step takes fl / descent from the beam's reciprocal only for
+    cars whose height and slopes keep that correctly rounded, and divides for
+    the others. Cars pushed past those bounds (|z| > 2^40, |along| + |across|
+    > 2^20) and cars just inside them share one launch; every range against
+    the oracle's scan of the same beam."""
+    print(f"lidar quotient paths {name}", flush=True)
+    n, beams = 64, 256
+    g, _ = _pair(dev, name, n, 5, beams)
+    g.reset()
+    s = g.state.read().reshape(11, n).copy()
+    obs = g.obs.read().reshape(beams + 5, n).copy()
+    s[2, 0:8] += 2.0**41  # z beyond 2^40: divides
+    obs[2, 8:16] = 2.0**21  # along beyond 2^20: divides
+    obs[3, 16:20] = -(2.0**21)  # across beyond 2^20: divides
+    s[2, 20:28] += 2.0**39  # z inside: reciprocal, with large dividends
+    obs[2, 28:32] = 2.0**19  # steep beams inside: reciprocal, with small divisors
+    g.state.write(s)
+    g.obs.write(obs)
+    g.autotune()  # runs the lidar alone on the current state
+    got = g.obs.read().reshape(beams + 5, n)[4:4 + beams]
+    tr, s, obs = O.f32(track(name)), O.f32(s), O.f32(obs)
+    want = np.empty((beams, n), np.float32)
+    for i in range(n):
+        for b in range(beams):
+            r = O.lib().o_scan_one(O.ptr(tr), tr.size, O.ptr(s), n, O.ptr(obs), n, i, b, beams, None)
+            want[b, i] = np.float32(r) / np.float32(10.0)  # RACER_RANGE
+    bad = np.argwhere(got != want)
+    assert len(bad) == 0, f"{len(bad)} ranges differ, first (beam, car) {bad[:3].tolist()}"
+
+
 def test_objects_outlive_the_device_handle():
     """Destroying the device handle first (a garbage collector may finalise a
     cycle in any order) leaves the objects made on it working; the device goes

@@ -97,6 +97,9 @@ def test_nearest_index_with_run_limit_exhaustive(cell):
         assert O.lib().o_check_nearest_index_lim(cell, lo, hi) == 0, (lo, hi)
 
 
-def test_min_quotient_shortcut_sampled():
+def test_div_by_reciprocal_is_correctly_rounded():
+    """The lidar march step's quotient from the beam's RN(1 / descent): bit
+    for bit the IEEE division over its stated range (random operands, quotients
+    next to rounding midpoints, all-ones and power-of-two divisors, +0)."""
     O.set_threads(8)
-    assert O.lib().o_check_min_quotient(4_000_000, 11) == 0
+    assert O.lib().o_check_div_by_reciprocal(16_000_000, 11) == 0

@@ -125,6 +125,7 @@ struct racer_dmap {
     int nx, ny, layers, waypoints;
     float x0, y0, cell, spacing, slope;
     float inv_cell;        // RN(1 / cell), for the division shortcut of the nearest-cell index
+    int quotient_ok;       // heights and slope bounded: the lidar may divide by reciprocals
     int64_t route;         // offset of the waypoint table
     int64_t plane;         // nx * ny
 };

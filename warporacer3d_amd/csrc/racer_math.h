@@ -265,17 +265,26 @@ RACER_FN float racer_nearest_index_lim(float a, float b, float inv_b, float lim)
     return floorf(a / b + 0.5f);
 }
 
-/* racer_min(cap, num / den) with num / den correctly rounded, skipping the
- * division when the reciprocal estimate is clearly above cap (then so is the
- * exact quotient: |q0 - q| <= 2^-22 |q|). den >= 0; inv_den = RN(1/den),
- * +inf when den is 0 or subnormal (then num / den is +inf or NaN, and
- * racer_min returns cap either way when the estimate is +inf). */
-RACER_FN float racer_min_quotient(float cap, float num, float den, float inv_den) {
-    float q0 = num * inv_den;
-    /* |cap| 2^-20 is exact above the subnormal range, so the fused form equals
-     * cap + |cap| 2^-20 there; below it the 2^-126 term dominates either way */
-    if (q0 > fmaf(racer_abs(cap), 0x1p-20f, cap) + 0x1p-126f) return cap;
-    return racer_min(cap, num / den);
+/* a / b correctly rounded (to nearest even) from y = RN(1 / b), with one
+ * multiply and four FMAs, for b in [2^-60, 2^60] and a = +0 or |a| in
+ * [2^-60, 2^60] (no step then overflows, and every residual is a normal
+ * number). This is Markstein's FMA division (P. Markstein, "Computation of
+ * elementary functions on the IBM RISC System/6000 processor", IBM J. Res.
+ * Dev. 34(1), 1990; J.-M. Muller et al., Handbook of Floating-Point
+ * Arithmetic, FMA-based division):
+ *  - q0 = RN(a y) is within 2^-23 (1 + 2^-24) relative of a / b (y's and the
+ *    product's roundings), so within two ulps of it;
+ *  - q0 + RN(a - b q0) y differs from a / b by (a / b - q0)(e1 + e3 + e1 e3),
+ *    |e1|, |e3| <= 2^-24: at most 2^-45.9 relative, so its rounding q1 is a
+ *    faithful rounding of a / b (one of the two floats around it);
+ *  - with q1 faithful and y within half an ulp of 1 / b, r = a - b q1 is
+ *    exactly representable and RN(q1 + r y) = RN(a / b) (Markstein's
+ *    theorem).
+ * a = -0 would give +0 where the division gives -0. */
+RACER_FN float racer_div_by_reciprocal(float a, float b, float y) {
+    const float q0 = a * y;
+    const float q1 = fmaf(fmaf(-q0, b, a), y, q0);
+    return fmaf(fmaf(-q1, b, a), y, q1);
 }
 
 #if defined(__HIPCC__)

@@ -6,6 +6,7 @@
 // with -ffp-contract=off, using the shared float32 elementary functions, so
 // every result is bit-identical to the CPU restatement.
 #include <cstring>
+#include <type_traits>
 
 #include "racer_internal.h"
 
@@ -754,7 +755,6 @@ __global__ void k_pack(const float* d, int64_t plane, int layers, float* packed)
     }
 }
 
-// Map.probe on the packed records, returning (clearance, height) only.
 // Map.probe on the packed records, returning (clearance, height) only. The
 // records are read with 32-bit buffer offsets (the descriptor's size makes any
 // out-of-range offset read zeros instead of faulting).
@@ -839,7 +839,7 @@ __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_
     }
 }
 
-constexpr int LIDAR_RECORD = 4;               // dx, dy, dz, descent
+constexpr int LIDAR_RECORD = 5;               // dx, dy, dz, descent, RN(1 / descent)
 constexpr int LIDAR_RING = 2 * LIDAR_RECORD * 64;  // two chunks of 64 beam records
 constexpr int LIDAR_COOP_CARS = 8;                  // cars per workgroup with cooperative stores
 constexpr size_t LIDAR_COOP_MAX_SHMEM = 80 * 1024;  // two such workgroups per CU (160 KB of LDS)
@@ -903,6 +903,19 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     const float qx = (fabsf(ox - m.x0) + (RACER_RANGE + 1.0f)) * m.inv_cell * 1.001f;
     const float qy = (fabsf(oy - m.y0) + (RACER_RANGE + 1.0f)) * m.inv_cell * 1.001f;
     const float limx = racer_nearest_limit(qx), limy = racer_nearest_limit(qy);
+    // The march step's quotient fl / descent comes from the beam's RN(1 /
+    // descent) (racer_div_by_reciprocal, correctly rounded for a divisor in
+    // [2^-60, 2^60] and a dividend of +0 or magnitude in [2^-60, 2^60]) when
+    // this car's numbers keep it in that range: map heights within 2^40 and
+    // |oz| <= 2^40 bound |fl| = |(pz - height) - 0.03| by 2^42, and fl is +0
+    // or at least 2^-30 (a float minus 0.03f is a multiple of 2^-30 unless
+    // the float is below 2^-7; fl = -0 would need 0.03f = +0); |along| +
+    // |across| <= 2^20 keeps the horizontal part of each beam above 2^-20.1,
+    // so descent >= 1.5 slope 2^-20.1 >= 2^-41 for slope >= 2^-20, and
+    // descent <= 1.5 slope + 1 <= 2^21. Otherwise the step divides.
+    const bool fast_q =
+        __builtin_amdgcn_readfirstlane((int)((m.quotient_ok != 0) & (fabsf(oz) <= 0x1p40f) &
+                                             (fabsf(along) + fabsf(across) <= 0x1p20f))) != 0;
 
     // Set-up of beams 64 c .. 64 c + 63, one per lane, into ring half c % 2:
     // field f of beam b lives at ring[128 f + b % 128].
@@ -922,9 +935,10 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
             slot[1 * 128] = dy;
             slot[2 * 128] = dz;
             slot[3 * 128] = descent;
+            slot[4 * 128] = fast_q ? 1.0f / descent : 0.0f;
         }
     };
-    float dx = 0.0f, dy = 0.0f, dz = 0.0f, descent = 0.0f;
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f, descent = 0.0f, inv_descent = 0.0f;
     float t = 0.0f, last = 0.0f, last_floor = 0.0f;
     auto take = [&](int b) {
         const float* slot = ring + (b & 127);
@@ -932,6 +946,7 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
         dy = slot[1 * 128];
         dz = slot[2 * 128];
         descent = slot[3 * 128];
+        inv_descent = slot[4 * 128];
         t = 0.0f;
         last = 0.0f;
         last_floor = 0.0f;
@@ -953,56 +968,69 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     // instruction, on every trip. The step's quotient is always divided: some
     // live lane needs the exact division on almost every trip (the floor
     // bound is the smaller one on about one iteration in nine), so a
-    // shortcut around it only adds its test.
-    while (__ballot(b < beams) != 0ull) {
-        const bool live = b < beams;
-        if (COUNT) iterations += live ? 1 : 0;
-        const float px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
-        float clearance, height;
-        if constexpr (L == 1 || L == 2)
-            probe_flat<L>(m, rec, px, py, pz, limx, limy, live, &clearance, &height);
-        else
-            probe_packed<L>(m, rec, px, py, pz, &clearance, &height);
-        const float fl = (pz - height) - 0.03f;
-        const float step = racer_min(clearance - m.cell, fl / descent);
-        // fl < 0: the floor is hit between last and t; else step < cell / 2:
-        // stop at t; else advance, leaving the range at t + step >= RANGE
-        const bool hit = fl < 0.0f;
-        const bool advance = !hit & !(step < half_cell);
-        const float next_t = t + step;
-        last = advance ? t : last;
-        last_floor = advance ? fl : last_floor;
-        t = advance ? next_t : t;
-        const bool finished = live & (!advance | !(t < RACER_RANGE));
-        float range = racer_min(t, RACER_RANGE);
-        if (live & hit) range = racer_min(last + ((t - last) * last_floor) / (last_floor - fl), RACER_RANGE);
-        if (COUNT && lane == 0) ++trips;
-        const uint64_t done_mask = __ballot(finished);
-        if (done_mask != 0ull) {
-            const int count = (int)__popcll(done_mask);
-            // The ring slot reused here held chunk `chunks - 2`, whose beams
-            // are all below `next` and already taken.
-            if (next + count > chunks * 64 && chunks * 64 < beams) {
-                setup_chunk(chunks);
-                ++chunks;
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // shortcut around it only adds its test. The loop is compiled twice, for
+    // the reciprocal form of that quotient and for the division (fast_q is
+    // uniform), so the choice costs nothing per trip.
+    auto march = [&](auto reciprocal) {
+        while (__ballot(b < beams) != 0ull) {
+            const bool live = b < beams;
+            if (COUNT) iterations += live ? 1 : 0;
+            const float px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
+            float clearance, height;
+            if constexpr (L == 1 || L == 2)
+                probe_flat<L>(m, rec, px, py, pz, limx, limy, live, &clearance, &height);
+            else
+                probe_packed<L>(m, rec, px, py, pz, &clearance, &height);
+            const float fl = (pz - height) - 0.03f;
+            float q;
+            if constexpr (decltype(reciprocal)::value)
+                q = racer_div_by_reciprocal(fl, descent, inv_descent);
+            else
+                q = fl / descent;
+            const float step = racer_min(clearance - m.cell, q);
+            // fl < 0: the floor is hit between last and t; else step < cell / 2:
+            // stop at t; else advance, leaving the range at t + step >= RANGE
+            const bool hit = fl < 0.0f;
+            const bool advance = !hit & !(step < half_cell);
+            const float next_t = t + step;
+            last = advance ? t : last;
+            last_floor = advance ? fl : last_floor;
+            t = advance ? next_t : t;
+            const bool finished = live & (!advance | !(t < RACER_RANGE));
+            float range = racer_min(t, RACER_RANGE);
+            if (live & hit) range = racer_min(last + ((t - last) * last_floor) / (last_floor - fl), RACER_RANGE);
+            if (COUNT && lane == 0) ++trips;
+            const uint64_t done_mask = __ballot(finished);
+            if (done_mask != 0ull) {
+                const int count = (int)__popcll(done_mask);
+                // The ring slot reused here held chunk `chunks - 2`, whose beams
+                // are all below `next` and already taken.
+                if (next + count > chunks * 64 && chunks * 64 < beams) {
+                    setup_chunk(chunks);
+                    ++chunks;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+                // finished lanes below this one
+                const int claimed = next + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(done_mask >> 32),
+                                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)done_mask, 0u));
+                // one exec region: store the range, then take the next beam (a
+                // lane whose claim is past the share reads a stale ring slot and
+                // stays idle, since b >= beams)
+                if (finished) {
+                    ranges[b] = range;
+                    b = claimed;
+                    take(claimed);
+                }
+                next += count;
             }
-            // finished lanes below this one
-            const int claimed = next + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(done_mask >> 32),
-                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)done_mask, 0u));
-            // one exec region: store the range, then take the next beam (a
-            // lane whose claim is past the share reads a stale ring slot and
-            // stays idle, since b >= beams)
-            if (finished) {
-                ranges[b] = range;
-                b = claimed;
-                take(claimed);
-            }
-            next += count;
         }
-    }
+    };
+    if (fast_q)
+        march(std::integral_constant<bool, true>{});
+    else
+        march(std::integral_constant<bool, false>{});
     if constexpr (!COOP) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1465,6 +1493,18 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
     m.layers = layers;
     m.slope = track[8];
     m.inv_cell = 1.0f / m.cell;
+    // The lidar's reciprocal quotient needs every height finite within 2^40
+    // and the slope within [2^-20, 2^20] (the bounds are explained where the
+    // lidar decides per car).
+    m.quotient_ok = m.slope >= 0x1p-20f && m.slope <= 0x1p20f;
+    for (int l = 0; m.quotient_ok && l < layers; ++l) {
+        const float* height = track + RACER_HEADER + (int64_t)(3 * l + RACER_HEIGHT) * plane;
+        for (int64_t c = 0; c < plane; ++c)
+            if (!(fabsf(height[c]) <= 0x1p40f)) {
+                m.quotient_ok = 0;
+                break;
+            }
+    }
     m.route = route;
     m.plane = plane;
     m.packed = nullptr;
