@@ -697,6 +697,27 @@ long o_check_div_by_reciprocal(long count, uint64_t seed) {
     return bad;
 }
 
+/* The lidar's nearest cell through the reciprocal, floorf(RN-quotient + 0.5f)
+ * with the quotient from racer_div_by_reciprocal(a, cell, RN(1 / cell)),
+ * against floorf(a / cell + 0.5f) for the floats a whose bit patterns are
+ * multiples of `stride` (1: all 2^32, NaNs and infinities included): the same
+ * index wherever either lies in [0, 2^24), both outside it (or NaN)
+ * elsewhere. */
+long o_check_nearest_by_reciprocal(float cell, long stride) {
+    const float inv = 1.0f / cell;
+    long bad = 0;
+    const int64_t count = 0xffffffffll / stride + 1;
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(+ : bad)
+    for (int64_t k = 0; k < count; ++k) {
+        float a = racer_u2f((uint32_t)(k * stride));
+        float want = floorf(a / cell + 0.5f);
+        float got = floorf(racer_div_by_reciprocal(a, cell, inv) + 0.5f);
+        int in_want = want >= 0.0f && want < 0x1p24f, in_got = got >= 0.0f && got < 0x1p24f;
+        if (in_want != in_got || (in_want && racer_f2u(want) != racer_f2u(got))) ++bad;
+    }
+    return bad;
+}
+
 /* ---------------- Learned dynamics surrogate (no reference counterpart) ---------------- */
 
 /* The surrogate's 11 inputs of car i and its constant-1 row (the HIP feature

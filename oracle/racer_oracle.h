@@ -137,6 +137,7 @@ float o_scan_one(const float* d, size_t len, const float* s, long n, const float
 long o_check_nearest_index(float b, float lo, float hi);
 long o_check_nearest_index_lim(float b, float lo, float hi);
 long o_check_div_by_reciprocal(long count, uint64_t seed);
+long o_check_nearest_by_reciprocal(float cell, long stride);
 
 /* Threads used by the parallel loops (1 = serial). */
 void o_set_threads(int threads);

@@ -97,6 +97,16 @@ def test_nearest_index_with_run_limit_exhaustive(cell):
         assert O.lib().o_check_nearest_index_lim(cell, lo, hi) == 0, (lo, hi)
 
 
+@pytest.mark.parametrize("cell,stride", [(0.025, 1), (0.05, 13), (1.0 / 3.0, 13), (7.3, 13)])
+def test_nearest_cell_by_reciprocal(cell, stride):
+    """The lidar's nearest cell from the reciprocal quotient: the same index as
+    floorf(a / cell + 0.5f) for every float a on the grid, off the grid where
+    that is (all 2^32 patterns for the built-in tracks' cell, every 13th
+    pattern for the others)."""
+    O.set_threads(8)
+    assert O.lib().o_check_nearest_by_reciprocal(cell, stride) == 0
+
+
 def test_div_by_reciprocal_is_correctly_rounded():
     """The lidar march step's quotient from the beam's RN(1 / descent): bit
     for bit the IEEE division over its stated range (random operands, quotients

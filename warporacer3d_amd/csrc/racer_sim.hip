@@ -800,18 +800,31 @@ __device__ __forceinline__ void probe_packed(const racer_dmap& m, __amdgpu_buffe
 // for the rare division of either (a lane whose shortcut holds gets the same
 // value from the division, which is the formula the shortcut reproduces);
 // `live` = false lanes (their results unused) never take it.
-template <int L>
+template <int L, bool RECIP>
 __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_rsrc_t rec, float x, float y, float z,
                                            float limx, float limy, bool live, float* clearance, float* height) {
-    // limx, limy: racer_nearest_limit of bounds on |x - x0| / cell and |y - y0| / cell
     const float ax = x - m.x0, ay = y - m.y0;
-    const float sx = ax * m.inv_cell + 0.5f, sy = ay * m.inv_cell + 0.5f;
-    float fx = floorf(sx), fy = floorf(sy);
-    // non-short-circuit & and |: one exec mask, not a nest of them
-    const bool slow_x = !(racer_abs((sx - fx) - 0.5f) < limx), slow_y = !(racer_abs((sy - fy) - 0.5f) < limy);
-    if (live & (slow_x | slow_y)) {
-        fx = floorf(ax / m.cell + 0.5f);
-        fy = floorf(ay / m.cell + 0.5f);
+    float fx, fy;
+    if constexpr (RECIP) {
+        // floorf(a / cell + 0.5f) through racer_div_by_reciprocal, for a cell
+        // in [2^-20, 2^20]: exact for |a| in [2^-60, 2^60] (the quotient is
+        // RN(a / cell)); below 2^-60 both quotients are under 2^-36 in
+        // magnitude (the computed one is at most 9 |a| / cell), so both sums
+        // round to 0.5 and the index is 0; above 2^60 both are beyond 2^40 in
+        // magnitude with a's sign, or NaN, all off the grid.
+        fx = floorf(racer_div_by_reciprocal(ax, m.cell, m.inv_cell) + 0.5f);
+        fy = floorf(racer_div_by_reciprocal(ay, m.cell, m.inv_cell) + 0.5f);
+    } else {
+        // limx, limy: racer_nearest_limit of bounds on |x - x0| / cell and |y - y0| / cell
+        const float sx = ax * m.inv_cell + 0.5f, sy = ay * m.inv_cell + 0.5f;
+        fx = floorf(sx);
+        fy = floorf(sy);
+        // non-short-circuit & and |: one exec mask, not a nest of them
+        const bool slow_x = !(racer_abs((sx - fx) - 0.5f) < limx), slow_y = !(racer_abs((sy - fy) - 0.5f) < limy);
+        if (live & (slow_x | slow_y)) {
+            fx = floorf(ax / m.cell + 0.5f);
+            fy = floorf(ay / m.cell + 0.5f);
+        }
     }
     // 0 <= f < n as one unsigned compare of the bit patterns: f is a floor
     // of a sum with 0.5, never -0.0, so a set sign bit means negative, and
@@ -905,14 +918,16 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     const float limx = racer_nearest_limit(qx), limy = racer_nearest_limit(qy);
     // The march step's quotient fl / descent comes from the beam's RN(1 /
     // descent) (racer_div_by_reciprocal, correctly rounded for a divisor in
-    // [2^-60, 2^60] and a dividend of +0 or magnitude in [2^-60, 2^60]) when
-    // this car's numbers keep it in that range: map heights within 2^40 and
-    // |oz| <= 2^40 bound |fl| = |(pz - height) - 0.03| by 2^42, and fl is +0
-    // or at least 2^-30 (a float minus 0.03f is a multiple of 2^-30 unless
-    // the float is below 2^-7; fl = -0 would need 0.03f = +0); |along| +
-    // |across| <= 2^20 keeps the horizontal part of each beam above 2^-20.1,
-    // so descent >= 1.5 slope 2^-20.1 >= 2^-41 for slope >= 2^-20, and
-    // descent <= 1.5 slope + 1 <= 2^21. Otherwise the step divides.
+    // [2^-60, 2^60] and a dividend of +0 or magnitude in [2^-60, 2^60]), and
+    // the probes' nearest cells from RN(1 / cell) (probe_flat), when this
+    // car's numbers keep them in range. The map's flag holds the cell and the
+    // slope within [2^-20, 2^20] and its heights within 2^40; with |oz| <=
+    // 2^40 that bounds |fl| = |(pz - height) - 0.03| by 2^42, and fl is +0 or
+    // at least 2^-30 (a float minus 0.03f is a multiple of 2^-30 unless the
+    // float is below 2^-7; fl = -0 would need 0.03f = +0); |along| + |across|
+    // <= 2^20 keeps the horizontal part of each beam above 2^-20.1, so
+    // descent >= 1.5 slope 2^-20.1 >= 2^-41, and descent <= 1.5 slope + 1 <=
+    // 2^21. Otherwise the step divides.
     const bool fast_q =
         __builtin_amdgcn_readfirstlane((int)((m.quotient_ok != 0) & (fabsf(oz) <= 0x1p40f) &
                                              (fabsf(along) + fabsf(across) <= 0x1p20f))) != 0;
@@ -978,7 +993,8 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
             const float px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
             float clearance, height;
             if constexpr (L == 1 || L == 2)
-                probe_flat<L>(m, rec, px, py, pz, limx, limy, live, &clearance, &height);
+                probe_flat<L, decltype(reciprocal)::value>(m, rec, px, py, pz, limx, limy, live, &clearance,
+                                                           &height);
             else
                 probe_packed<L>(m, rec, px, py, pz, &clearance, &height);
             const float fl = (pz - height) - 0.03f;
@@ -1493,10 +1509,10 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
     m.layers = layers;
     m.slope = track[8];
     m.inv_cell = 1.0f / m.cell;
-    // The lidar's reciprocal quotient needs every height finite within 2^40
-    // and the slope within [2^-20, 2^20] (the bounds are explained where the
-    // lidar decides per car).
-    m.quotient_ok = m.slope >= 0x1p-20f && m.slope <= 0x1p20f;
+    // The lidar's reciprocal quotients need the cell and the slope within
+    // [2^-20, 2^20] and every height finite within 2^40 (the bounds are
+    // explained where the lidar decides per car).
+    m.quotient_ok = m.slope >= 0x1p-20f && m.slope <= 0x1p20f && m.cell >= 0x1p-20f && m.cell <= 0x1p20f;
     for (int l = 0; m.quotient_ok && l < layers; ++l) {
         const float* height = track + RACER_HEADER + (int64_t)(3 * l + RACER_HEIGHT) * plane;
         for (int64_t c = 0; c < plane; ++c)
