@@ -1013,8 +1013,6 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
             last_floor = advance ? fl : last_floor;
             t = advance ? next_t : t;
             const bool finished = live & (!advance | !(t < RACER_RANGE));
-            float range = racer_min(t, RACER_RANGE);
-            if (live & hit) range = racer_min(last + ((t - last) * last_floor) / (last_floor - fl), RACER_RANGE);
             if (COUNT && lane == 0) ++trips;
             const uint64_t done_mask = __ballot(finished);
             if (done_mask != 0ull) {
@@ -1031,11 +1029,13 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
                 // finished lanes below this one
                 const int claimed = next + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(done_mask >> 32),
                                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)done_mask, 0u));
-                // one exec region: store the range, then take the next beam (a
-                // lane whose claim is past the share reads a stale ring slot and
-                // stays idle, since b >= beams)
+                // one exec region: the range (interpolated between the last two
+                // steps on a floor hit), its store, then the next beam (a lane
+                // whose claim is past the share reads a stale ring slot and stays
+                // idle, since b >= beams)
                 if (finished) {
-                    ranges[b] = range;
+                    ranges[b] = hit ? racer_min(last + ((t - last) * last_floor) / (last_floor - fl), RACER_RANGE)
+                                    : racer_min(t, RACER_RANGE);
                     b = claimed;
                     take(claimed);
                 }
