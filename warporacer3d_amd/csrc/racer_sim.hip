@@ -837,7 +837,9 @@ __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_
     if constexpr (L == 1) {
         // the stored clearance is -1 unless positive
         float2 r = racer_ld_f32x2(rec, off);
-        *clearance = fabsf(r.y - z) < 1e30f ? r.x : -1.0f;
+        // |z - height| = |height - z| exactly; this form shares the
+        // subtraction with the caller's floor distance (z - height) - 0.03
+        *clearance = fabsf(z - r.y) < 1e30f ? r.x : -1.0f;
         *height = r.y;
     } else {
         // Map.probe's loop over the two layers as selects (a short-circuit
@@ -968,13 +970,11 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     };
     setup_chunk(0);
     setup_chunk(1);
-    int chunks = 2;  // beams below 64 * chunks have their set-up in the ring
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int b = lane;
     if (b < beams) take(b);
-    int next = 64;
     uint64_t iterations = 0, trips = 0;
     // The march step below runs on every lane with selects in place of
     // branches (lanes past the share's end compute on their last beam's
@@ -987,6 +987,10 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     // the reciprocal form of that quotient and for the division (fast_q is
     // uniform), so the choice costs nothing per trip.
     auto march = [&](auto reciprocal) {
+        // declared here rather than captured: the compiler then keeps them in
+        // scalar registers in both copies of the loop
+        int chunks = 2;  // beams below 64 * chunks have their set-up in the ring
+        int next = 64;   // the next unclaimed beam
         while (__ballot(b < beams) != 0ull) {
             const bool live = b < beams;
             if (COUNT) iterations += live ? 1 : 0;
@@ -1014,7 +1018,9 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
             t = advance ? next_t : t;
             const bool finished = live & (!advance | !(t < RACER_RANGE));
             if (COUNT && lane == 0) ++trips;
-            const uint64_t done_mask = __ballot(finished);
+            // the builtin on the bool itself: __ballot's int argument costs a
+            // select and a compare to turn the lane mask back into one
+            const uint64_t done_mask = __builtin_amdgcn_ballot_w64(finished);
             if (done_mask != 0ull) {
                 const int count = (int)__popcll(done_mask);
                 // The ring slot reused here held chunk `chunks - 2`, whose beams
