@@ -864,8 +864,8 @@ constexpr size_t LIDAR_COOP_MAX_SHMEM = 80 * 1024;  // two such workgroups per C
 // Each lane marches one beam at a time; a lane whose beam ends takes the next
 // unclaimed beam of the car (ballot + popcount), so the wavefront stays full
 // until the car's last beams. Beam set-up (direction, normalisation, floor
-// descent rate) is done 64 beams at a time by the whole wavefront into a
-// two-chunk LDS ring, so a refill is four LDS reads rather
+// descent rate and its reciprocal) is done 64 beams at a time by the whole
+// wavefront into a two-chunk LDS ring, so a refill is five LDS reads rather
 // than a divergent pass of divisions and square roots. Ranges collect in LDS
 // and are scaled and stored in one full-lane pass at the end.
 //
