@@ -59,6 +59,7 @@ def lib():
             "o_surface": (I, [P, C.c_size_t, I, F, F, P]),
             "o_progress": (F, [P, C.c_size_t, I, F, F]),
             "o_scan_one": (F, [P, C.c_size_t, P, L, P, L, L, I, I, P]),
+            "o_spawn_one": (I, [P, C.c_size_t, P, L, L, I, F]),
             "o_check_nearest_index": (L, [F, F, F]),
             "o_check_nearest_index_lim": (L, [F, F, F]),
             "o_check_div_by_reciprocal": (L, [L, C.c_uint64]),
@@ -250,6 +251,14 @@ class Map:
 
     def progress(self, layer, x, y) -> float:
         return float(lib().o_progress(ptr(self.d), self.d.size, layer, x, y))
+
+
+def spawn(track, state: np.ndarray, i: int, seed: int, heading_noise: float = 0.0) -> None:
+    """spawn (racer/sim.mojo:170-181) of car i of state [11, n], in place."""
+    t = f32(track)
+    assert state.dtype == np.float32 and state.flags["C_CONTIGUOUS"]
+    if lib().o_spawn_one(ptr(t), t.size, ptr(state), state.shape[1], int(i), int(seed), float(heading_noise)):
+        raise ValueError("not a track array")
 
 
 def elementary(name: str):

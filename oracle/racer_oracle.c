@@ -595,6 +595,13 @@ float o_scan_one(const float* d, size_t len, const float* s, long n, const float
     return o_scan(&m, d, s, n, obs, obs_cols, i, beam, beams, iterations);
 }
 
+int o_spawn_one(const float* d, size_t len, float* s, long n, long i, int seed, float heading_noise) {
+    omap m;
+    if (om_init(&m, d, len)) return -1;
+    o_spawn(s, n, &m, d, i, seed, heading_noise);
+    return 0;
+}
+
 /* ---------------- checks of the shared header's division shortcuts ---------------- */
 
 /* Every float a in [lo, hi] (walking the ordered bit patterns): number of a

@@ -131,6 +131,8 @@ int o_surface(const float* d, size_t len, int layer, float x, float y, float out
 float o_progress(const float* d, size_t len, int layer, float x, float y);
 float o_scan_one(const float* d, size_t len, const float* s, long n, const float* obs, long obs_cols, long i, int beam,
                  int beams, long* iterations);
+/* spawn (sim.mojo:170-181) of car i of the state s [11, n], in place. */
+int o_spawn_one(const float* d, size_t len, float* s, long n, long i, int seed, float heading_noise);
 
 /* Exhaustive / sampled checks of the shared header's division shortcuts
  * (each returns the number of mismatches). */

@@ -107,3 +107,12 @@ def test_checkpoint_round_trip(dev, tmp_path, pair):
     np.save(tmp_path / "bad.npy", np.zeros(10, np.float32))
     with pytest.raises(R.RacerError, match="checkpoint does not match the network size"):
         R.load(fresh, tmp_path / "bad.npy")
+
+
+def test_bf16_trainer_refuses_minibatches_beyond_32bit_offsets(dev):
+    """The bf16 forward reads minibatch inputs through 32-bit buffer offsets: a
+    minibatch of 1086 x 1048576 floats (4.2 GiB) is refused at creation, before
+    anything is allocated, instead of wrapping its offsets."""
+    cfg = R.Config(beams=1081, hidden=256, rollout=256, minibatches=4, precision=1)
+    with pytest.raises(R.RacerError, match="4 GiB"):
+        R.Trainer(dev, 16384, cfg)

@@ -24,6 +24,18 @@ void racer_clear_error();
         }                                                                            \
     } while (0)
 
+// RACER_TRY for a half-built object: `cleanup` (its destroy call) runs before
+// the error return, so a failed create leaks nothing.
+#define RACER_TRY_OR(expr, cleanup)                                                  \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            racer_set_error(std::string(#expr) + ": " + hipGetErrorString(e_));      \
+            cleanup;                                                                 \
+            return 1;                                                                \
+        }                                                                            \
+    } while (0)
+
 #define RACER_REQUIRE(cond, msg)   \
     do {                           \
         if (!(cond)) {             \
@@ -292,6 +304,11 @@ inline int64_t racer_pad32(int64_t v) { return (v + 31) / 32 * 32; }
 size_t racer_fragment_elems(int cols);
 int racer_pack_fragments(racer_device* dev, const float* src, int64_t ld_src, int rows, int cols, uint16_t* dst);
 // The fused bf16 policy forward (and optionally Sim.sample) over `cols` columns of x.
+// Its input is read through a buffer descriptor with 32-bit byte offsets: the
+// span of IN rows of stride x_cols must stay below 4 GiB.
+inline bool racer_fused_reach(int IN, int64_t x_cols, int64_t cols) {
+    return x_cols >= 0 && cols >= 0 && ((uint64_t)(IN - 1) * (uint64_t)x_cols + (uint64_t)cols) * 4 < 0xFFFFFF00ull;
+}
 int racer_policy_fused_setup();
 int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
                        float* act, float* logp, float* value, int deterministic, int64_t step, int seed);

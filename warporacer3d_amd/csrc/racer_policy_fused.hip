@@ -696,6 +696,9 @@ static int unrolled_half(int IN) {
 int racer_policy_fused_batch(racer_policy* p, const racer_policy_table& tab, int deterministic, int64_t step) {
     racer_device* dev = p->dev;
     if (tab.count <= 0 || tab.start[tab.count] <= 0) return 0;
+    for (int k = 0; k < tab.count; ++k)
+        RACER_REQUIRE(racer_fused_reach((int)p->IN, tab.slot[k].ldx, tab.slot[k].cols),
+                      "bf16 policy input exceeds the 32-bit offsets of the buffer loads (4 GiB)");
     dim3 grid((unsigned)tab.start[tab.count]), block(64 * WAVES);
     size_t shmem = racer_policy_fused_smem(p->hidden, (int)p->IN);
     {
@@ -731,6 +734,8 @@ static auto fused_kernel(int hs, bool vec, bool wide) {
 int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
                        float* act, float* logp, float* value, int deterministic, int64_t step, int seed) {
     racer_device* dev = p->dev;
+    RACER_REQUIRE(racer_fused_reach((int)p->IN, x_cols, cols),
+                  "bf16 policy input exceeds the 32-bit offsets of the buffer loads (4 GiB)");
     const uint16_t* f0 = p->frag0;
     const uint16_t* f1 = p->frag1;
     const float* W2 = p->theta->ptr + p->W2;

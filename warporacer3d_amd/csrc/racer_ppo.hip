@@ -556,6 +556,12 @@ extern "C" int racer_trainer_create(racer_device* dev, int n, const racer_config
     RACER_REQUIRE(n > 0 && cfg->rollout > 0 && cfg->minibatches > 0 && cfg->epochs > 0, "bad trainer shape");
     RACER_REQUIRE(((int64_t)n * cfg->rollout) % cfg->minibatches == 0,
                   "cars x rollout must be a multiple of the minibatch count");
+    // The bf16 forward reads its inputs through buffer loads with 32-bit byte
+    // offsets: a minibatch's [beams + 5, batch] floats must stay below 4 GiB.
+    const int64_t batch = (int64_t)n * cfg->rollout / cfg->minibatches;
+    RACER_REQUIRE(cfg->precision != 1 || racer_fused_reach(cfg->beams + 5, batch, batch),
+                  "a bf16 minibatch of (beams + 5) x cars x rollout / minibatches floats must stay below 4 GiB "
+                  "(32-bit buffer offsets): use more minibatches or fewer cars");
     auto* tr = new racer_trainer();
     tr->dev = dev;
     racer_device_retain(dev);
@@ -582,7 +588,7 @@ extern "C" int racer_trainer_create(racer_device* dev, int n, const racer_config
         racer_trainer_destroy(tr);
         return 1;
     }
-    RACER_TRY(hipMalloc(&tr->dstats, 8 * sizeof(double)));
+    RACER_TRY_OR(hipMalloc(&tr->dstats, 8 * sizeof(double)), racer_trainer_destroy(tr));
     const int MB = tr->minibatches;
     if ((MB == 1 || MB == 2 || MB == 4 || MB == 8) && N % MB == 0 &&
         racer_alloc(dev, (size_t)(R * tr->IN * N), 0.0f, &tr->xall)) {

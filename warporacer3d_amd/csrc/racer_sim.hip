@@ -1531,10 +1531,13 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
     m.plane = plane;
     m.packed = nullptr;
     if (layers <= 2) {
-        RACER_TRY(hipMalloc(&sim->packed, (size_t)(plane + 1) * layers * 2 * sizeof(float)));
+        RACER_TRY_OR(hipMalloc(&sim->packed, (size_t)(plane + 1) * layers * 2 * sizeof(float)), racer_sim_destroy(sim));
         hipLaunchKernelGGL(k_pack, dim3((unsigned)((plane + 1 + 255) / 256)), dim3(256), 0, dev->stream, m.d, plane,
                            layers, sim->packed);
-        if (racer_launch_check("pack")) return 1;
+        if (racer_launch_check("pack")) {
+            racer_sim_destroy(sim);
+            return 1;
+        }
         m.packed = sim->packed;
     }
     // Beam directions: azimuth = -FOV/2 + b FOV/(BEAMS-1) (racer/sim.mojo:246).
@@ -1543,11 +1546,12 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
         float az = -RACER_FOV / 2.0f + (float)b * (RACER_FOV / (float)(cfg->beams - 1));
         dirs[b] = make_float2(racer_cos(az), racer_sin(az));
     }
-    RACER_TRY(hipMalloc(&sim->dirs, dirs.size() * sizeof(float2)));
-    RACER_TRY(hipMemcpyAsync(sim->dirs, dirs.data(), dirs.size() * sizeof(float2), hipMemcpyHostToDevice,
-                             dev->stream));
-    RACER_TRY(hipMalloc(&sim->counter, 2 * sizeof(unsigned long long)));
-    RACER_TRY(hipStreamSynchronize(dev->stream));
+    RACER_TRY_OR(hipMalloc(&sim->dirs, dirs.size() * sizeof(float2)), racer_sim_destroy(sim));
+    RACER_TRY_OR(hipMemcpyAsync(sim->dirs, dirs.data(), dirs.size() * sizeof(float2), hipMemcpyHostToDevice,
+                                dev->stream),
+                 racer_sim_destroy(sim));
+    RACER_TRY_OR(hipMalloc(&sim->counter, 2 * sizeof(unsigned long long)), racer_sim_destroy(sim));
+    RACER_TRY_OR(hipStreamSynchronize(dev->stream), racer_sim_destroy(sim));
     if (lidar_coop_setup()) {
         racer_sim_destroy(sim);
         return 1;
