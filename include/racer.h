@@ -132,6 +132,20 @@ int racer_policy_forward(racer_policy* pol, const racer_view* x, racer_acts* act
 int racer_policy_backward(racer_policy* pol, const racer_view* x, racer_acts* acts, const racer_view* d3,
                           racer_buf* d2, racer_buf* d1, racer_buf* grad);
 
+/* racer_policy_backward without allocating (no counterpart in the reference):
+ * for callers that cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation). The weight gradients'
+ * split-K partial products go to a fixed scratch array of 16 MiB (4194304
+ * floats) that each GPU the library runs on holds until the process ends,
+ * instead of the device's grow-only scratch; the kernels and their order are
+ * those of racer_policy_backward, so the results are bitwise the same. Shapes
+ * whose partials exceed the array fail with an error naming the need (about
+ * 61 K columns at 1081 beams and H 256 in float32, twice that in bf16).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (other host threads, other device handles or streams) race. */
+int racer_policy_backward_noalloc(racer_policy* pol, const racer_view* x, racer_acts* acts, const racer_view* d3,
+                                  racer_buf* d2, racer_buf* d1, racer_buf* grad);
+
 /* ---------------- Trainer: racer/ppo.mojo:141-337 ---------------- */
 int racer_trainer_create(racer_device* dev, int n, const racer_config* cfg, racer_trainer** out);
 void racer_trainer_destroy(racer_trainer* tr);

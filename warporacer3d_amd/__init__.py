@@ -121,6 +121,7 @@ def lib() -> C.CDLL:
         "racer_policy_transpose": (i, [vp]),
         "racer_policy_forward": (i, [vp, VP, vp]),
         "racer_policy_backward": (i, [vp, VP, vp, VP, vp, vp, vp]),
+        "racer_policy_backward_noalloc": (i, [vp, VP, vp, VP, vp, vp, vp]),
         "racer_trainer_create": (i, [vp, i, C.POINTER(Config), cp]),
         "racer_trainer_destroy": (None, [vp]),
         "racer_trainer_rollout": (i, [vp, vp, vp, vp, i, C.POINTER(C.c_float)]),
@@ -485,6 +486,11 @@ class Policy:
 
     def backward(self, x: View, acts: Activations, d3: View, d2: Buffer, d1: Buffer, grad: Buffer) -> None:
         _check(lib().racer_policy_backward(self.handle, x.c, acts.handle, d3.c, d2.handle, d1.handle, grad.handle))
+
+    def backward_noalloc(self, x: View, acts: Activations, d3: View, d2: Buffer, d1: Buffer, grad: Buffer) -> None:
+        """backward with the split-K partials in the GPU's fixed 16 MiB scratch: no allocation."""
+        _check(lib().racer_policy_backward_noalloc(self.handle, x.c, acts.handle, d3.c, d2.handle, d1.handle,
+                                                   grad.handle))
 
     def __del__(self):
         try:

@@ -54,6 +54,13 @@ int racer_check_view(const racer_view& v, int64_t rows, int64_t cols, const char
 }
 
 // ---------------------------------------------------------------- device
+// The fixed scratch of the allocation-free entry points: one copy per GPU, at
+// its own address there, held until the process ends. Host code never passes
+// this name to a kernel (on the host it is only a stand-in for the symbol);
+// racer_device_create looks up the GPU's address and keeps it in
+// racer_device::fixed_scratch.
+static __device__ __align__(256) float scratch[RACER_FIXED_SCRATCH_FLOATS];
+
 extern "C" int racer_device_create(int gpu, racer_device** out) {
     racer_clear_error();
     *out = nullptr;
@@ -62,14 +69,28 @@ extern "C" int racer_device_create(int gpu, racer_device** out) {
     hipError_t e = hipGetDeviceCount(&count);
     RACER_REQUIRE(e == hipSuccess && count > 0, "no HIP device is available");
     auto* dev = new racer_device();
-    RACER_TRY(hipGetDevice(&dev->id));
+    auto drop = [dev] {
+        if (dev->sums) (void)hipFree(dev->sums);
+        if (dev->stream) (void)hipStreamDestroy(dev->stream);
+        delete dev;
+    };
+    RACER_TRY_OR(hipGetDevice(&dev->id), drop());
     hipDeviceProp_t prop;
-    RACER_TRY(hipGetDeviceProperties(&prop, dev->id));
+    RACER_TRY_OR(hipGetDeviceProperties(&prop, dev->id), drop());
     dev->name = prop.name;
     if (prop.gcnArchName[0]) dev->name += std::string(" (") + prop.gcnArchName + ")";
     dev->cus = prop.multiProcessorCount;
-    RACER_TRY(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking));
-    RACER_TRY(hipMalloc(&dev->sums, RACER_SUM_PARTIALS * sizeof(double)));
+    RACER_TRY_OR(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking), drop());
+    RACER_TRY_OR(hipMalloc(&dev->sums, RACER_SUM_PARTIALS * sizeof(double)), drop());
+    // with this device current: its copy of the fixed scratch array
+    void* fixed = nullptr;
+    RACER_TRY_OR(hipGetSymbolAddress(&fixed, HIP_SYMBOL(scratch)), drop());
+    if (!fixed) {
+        drop();
+        racer_set_error("the fixed scratch array has no device address");
+        return 1;
+    }
+    dev->fixed_scratch = static_cast<float*>(fixed);
     *out = dev;
     return 0;
 }
@@ -100,6 +121,17 @@ extern "C" const char* racer_device_name(racer_device* dev) { return dev ? dev->
 extern "C" int racer_sync(racer_device* dev) {
     racer_clear_error();
     RACER_TRY(hipStreamSynchronize(dev->stream));
+    return 0;
+}
+
+int racer_partials(racer_device* dev, size_t floats, float* ws, size_t ws_len, float** out) {
+    if (ws) {
+        RACER_REQUIRE(floats <= ws_len, "the split-K partial products exceed the workspace");
+        *out = ws;
+        return 0;
+    }
+    if (racer_ensure_scratch(dev, floats)) return 1;
+    *out = dev->scratch;
     return 0;
 }
 

@@ -249,24 +249,34 @@ int racer_gemm_nn_f32(racer_device* dev, const float* A, int64_t lda, const floa
     return racer_launch_check("gemm_nn_f32");
 }
 
+// The batch splits into chunks of 4096; each chunk's partial product fills one
+// [M, ldc] slab of the workspace.
+constexpr int NT_F32_CHUNK = 4096;
+
+size_t racer_gemm_nt_f32_partials(int M, int64_t ldc, int K) {
+    int chunks = K <= 0 ? 1 : (K + NT_F32_CHUNK - 1) / NT_F32_CHUNK;
+    return (size_t)M * (size_t)ldc * (size_t)chunks;
+}
+
 int racer_gemm_nt_f32(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
-                      int64_t ldc, int M, int N, int K) {
+                      int64_t ldc, int M, int N, int K, float* ws, size_t ws_len) {
     if (M <= 0 || N <= 0) return 0;
-    const int chunk = 4096;
+    const int chunk = NT_F32_CHUNK;
     int chunks = K <= 0 ? 1 : (K + chunk - 1) / chunk;
     int64_t stride = (int64_t)M * ldc;
-    if (racer_ensure_scratch(dev, (size_t)(stride * chunks))) return 1;
+    float* partial = nullptr;
+    if (racer_partials(dev, racer_gemm_nt_f32_partials(M, ldc, K), ws, ws_len, &partial)) return 1;
     const bool reach = ((uint64_t)(M - 1) * lda + K) * 4 < 0xFFFFFF00ull && ((uint64_t)(N - 1) * ldb + K) * 4 < 0xFFFFFF00ull;
     const bool aligned4 = ldb % 4 == 0 && K % 4 == 0 && (uintptr_t)B % 16 == 0;  // float4 loads of B
     if (M <= 4 && reach && aligned4) {
         racer_timed timer(dev, "gemm_nt_f32_skinny");
         dim3 grid((N + SK_N - 1) / SK_N, 1, chunks), block(256);
-        hipLaunchKernelGGL(k_gemm_nt_f32_skinny, grid, block, 0, dev->stream, A, lda, B, ldb, dev->scratch, stride,
+        hipLaunchKernelGGL(k_gemm_nt_f32_skinny, grid, block, 0, dev->stream, A, lda, B, ldb, partial, stride,
                            ldc, M, N, K, chunk);
     } else {
         racer_timed timer(dev, "gemm_nt_f32");
         dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, chunks), block(256);
-        hipLaunchKernelGGL(k_gemm_nt_f32, grid, block, 0, dev->stream, A, lda, B, ldb, dev->scratch, stride, ldc, M,
+        hipLaunchKernelGGL(k_gemm_nt_f32, grid, block, 0, dev->stream, A, lda, B, ldb, partial, stride, ldc, M,
                            N, K, chunk);
     }
     if (racer_launch_check("gemm_nt_f32")) return 1;
@@ -274,7 +284,7 @@ int racer_gemm_nt_f32(racer_device* dev, const float* A, int64_t lda, const floa
     {
         racer_timed timer(dev, "sum_chunks");
         hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, dev->stream,
-                           dev->scratch, stride, chunks, C, ldc, M, N);
+                           partial, stride, chunks, C, ldc, M, N);
     }
     return racer_launch_check("sum_chunks");
 }

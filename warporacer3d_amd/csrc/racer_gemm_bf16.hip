@@ -359,22 +359,32 @@ int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const 
     return racer_launch_check("gemm_nn_bf16");
 }
 
+// The batch splits into chunks of 8192 (both kernels); each chunk's partial
+// product fills one [M, ldc] slab of the workspace.
+constexpr int NT_BF16_CHUNK = 8192;
+
+size_t racer_gemm_nt_bf16_partials(int M, int64_t ldc, int K) {
+    int chunks = K <= 0 ? 1 : (K + NT_BF16_CHUNK - 1) / NT_BF16_CHUNK;
+    return (size_t)M * (size_t)ldc * (size_t)chunks;
+}
+
 int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
-                       int64_t ldc, int M, int N, int K) {
+                       int64_t ldc, int M, int N, int K, float* ws, size_t ws_len) {
     if (M <= 0 || N <= 0) return 0;
+    float* partial = nullptr;
+    if (racer_partials(dev, racer_gemm_nt_bf16_partials(M, ldc, K), ws, ws_len, &partial)) return 1;
     // the MFMA tile kernel: 16-byte aligned rows and operands within the reach of 32-bit offsets
     const bool aligned = lda % 4 == 0 && ldb % 4 == 0 && K % 4 == 0 && (uintptr_t)A % 16 == 0 && (uintptr_t)B % 16 == 0;
     const bool reach = ((uint64_t)(M - 1) * lda + K) * 4 < 0xFFFFFF00ull && ((uint64_t)(N - 1) * ldb + K) * 4 < 0xFFFFFF00ull;
     if (aligned && reach && K > 0) {
-        const int chunk = 8192;
+        const int chunk = NT_BF16_CHUNK;
         int chunks = (K + chunk - 1) / chunk;
         int64_t stride = (int64_t)M * ldc;
-        if (racer_ensure_scratch(dev, (size_t)(stride * chunks))) return 1;
         const int tx = (N + WG_T - 1) / WG_T, ty = (M + WG_T - 1) / WG_T;
         dim3 grid((unsigned)(tx * ty * chunks)), block(256);
         {
             racer_timed timer(dev, "wgrad_bf16");
-            hipLaunchKernelGGL(k_wgrad_bf16, grid, block, 0, dev->stream, A, lda, B, ldb, dev->scratch, ldc, stride,
+            hipLaunchKernelGGL(k_wgrad_bf16, grid, block, 0, dev->stream, A, lda, B, ldb, partial, ldc, stride,
                                M, N, K, chunk, tx, ty, chunks);
         }
         if (racer_launch_check("wgrad_bf16")) return 1;
@@ -382,26 +392,25 @@ int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const flo
         {
             racer_timed timer(dev, "sum_chunks");
             hipLaunchKernelGGL(k_sum_chunks_bf16, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, dev->stream,
-                               dev->scratch, stride, chunks, C, ldc, M, N);
+                               partial, stride, chunks, C, ldc, M, N);
         }
         return racer_launch_check("sum_chunks");
     }
-    const int chunk = 8192;
+    const int chunk = NT_BF16_CHUNK;
     int chunks = K <= 0 ? 1 : (K + chunk - 1) / chunk;
     int64_t stride = (int64_t)M * ldc;
-    if (racer_ensure_scratch(dev, (size_t)(stride * chunks))) return 1;
     dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, chunks), block(256);
     {
         racer_timed timer(dev, "gemm_nt_bf16");
         hipLaunchKernelGGL((k_gemm_bf16<false, true, RACER_EPI_NONE>), grid, block, 0, dev->stream, A, lda, B, ldb,
-                           dev->scratch, ldc, stride, M, N, K, chunk);
+                           partial, ldc, stride, M, N, K, chunk);
     }
     if (racer_launch_check("gemm_nt_bf16")) return 1;
     int64_t total = (int64_t)M * N;
     {
         racer_timed timer(dev, "sum_chunks");
         hipLaunchKernelGGL(k_sum_chunks_bf16, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, dev->stream,
-                           dev->scratch, stride, chunks, C, ldc, M, N);
+                           partial, stride, chunks, C, ldc, M, N);
     }
     return racer_launch_check("sum_chunks");
 }

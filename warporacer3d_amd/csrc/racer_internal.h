@@ -58,6 +58,10 @@ struct racer_pending_event {
     hipEvent_t start, stop;
 };
 
+// Size of the fixed per-GPU scratch array of the allocation-free entry points
+// (16 MiB of float32).
+constexpr size_t RACER_FIXED_SCRATCH_FLOATS = (size_t)4 << 20;
+
 struct racer_device {
     int id = 0;
     hipStream_t stream = nullptr;
@@ -69,6 +73,9 @@ struct racer_device {
     std::vector<hipEvent_t> event_pool;
     float* scratch = nullptr;  // split-K partial products (grown on demand)
     size_t scratch_len = 0;
+    // Device address of this GPU's fixed scratch array (RACER_FIXED_SCRATCH_FLOATS
+    // floats, held until the process ends) for the allocation-free entry points.
+    float* fixed_scratch = nullptr;
     double* sums = nullptr;    // RACER_SUM_PARTIALS float64 partial sums of the row reductions
     std::string profile_only;  // when set, the profiler times only the kernel of this name
     // Graph capture: while a graph is captured, sampling kernels add the device
@@ -127,6 +134,9 @@ struct racer_timed {
 int racer_check_view(const racer_view& v, int64_t rows, int64_t cols, const char* what);
 inline float* view_ptr(const racer_view& v) { return v.buf->ptr + v.offset; }
 int racer_ensure_scratch(racer_device* dev, size_t floats);
+// The workspace of a split-K product: `ws` when the caller brings one (it must
+// hold `floats`), else the device scratch grown to `floats`.
+int racer_partials(racer_device* dev, size_t floats, float* ws, size_t ws_len, float** out);
 int racer_launch_check(const char* what);
 
 // ---------------------------------------------------------------- map (device side)
@@ -285,15 +295,21 @@ enum { RACER_EPI_NONE = 0, RACER_EPI_TANH = 1, RACER_EPI_MASK = 2, RACER_EPI_TAN
 // multiplies by (1 - h^2) with h = H[m, n].
 int racer_gemm_nn_f32(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
                       int64_t ldc, int M, int N, int K, int epi, const float* H, int64_t ldh);
-// C[M, N] = A[M, K] B[N, K]^T (weight gradients); K is the long batch dimension.
+// C[M, N] = A[M, K] B[N, K]^T (weight gradients); K is the long batch dimension,
+// split into chunks whose partial products land in a workspace and are then
+// added in chunk order. The workspace is `ws` (ws_len floats, at least
+// racer_gemm_nt_*_partials) when given, else the device's grow-only scratch.
+// Every partial the sum reads is written first, so `ws` needs no clearing.
+size_t racer_gemm_nt_f32_partials(int M, int64_t ldc, int K);
 int racer_gemm_nt_f32(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
-                      int64_t ldc, int M, int N, int K);
+                      int64_t ldc, int M, int N, int K, float* ws = nullptr, size_t ws_len = 0);
 // bf16 MFMA variants (operands rounded to bf16, float32 accumulation)
 // A is a bf16 copy with rows padded to a multiple of 8 values (lda % 8 == 0, zeros beyond K).
 int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const float* B, int64_t ldb, float* C,
                        int64_t ldc, int M, int N, int K, int epi, const float* H = nullptr, int64_t ldh = 0);
+size_t racer_gemm_nt_bf16_partials(int M, int64_t ldc, int K);
 int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
-                       int64_t ldc, int M, int N, int K);
+                       int64_t ldc, int M, int N, int K, float* ws = nullptr, size_t ws_len = 0);
 // dst[r, c] = bf16(src[r, c]) for c < cols, 0 for cols <= c < ld_dst.
 int racer_pack_bf16(racer_device* dev, const float* src, int64_t ld_src, uint16_t* dst, int64_t ld_dst, int rows,
                     int cols);

@@ -1,6 +1,7 @@
 // Policy, activations and the PPO trainer on gfx950 (racer/ppo.mojo), the
 // .npy checkpoint format (racer/ppo.mojo:341-355, racer/device.mojo:137-143)
 // and the Runner step/benchmark loop (main.mojo:183-235).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -495,8 +496,20 @@ extern "C" int racer_policy_forward(racer_policy* p, const racer_view* x_p, race
     return racer_policy_forward_raw(p, view_ptr(x), x.cols, a, a->cols);
 }
 
+// Workspace floats the three weight gradients of backward_raw need (they run
+// one after another on the stream, so they share it).
+static size_t backward_partials(const racer_policy* p, int64_t cols) {
+    const int H = p->hidden, K = (int)cols;
+    size_t w01 = p->precision == 1
+                     ? std::max(racer_gemm_nt_bf16_partials(H, p->IN, K), racer_gemm_nt_bf16_partials(H, p->HA, K))
+                     : std::max(racer_gemm_nt_f32_partials(H, p->IN, K), racer_gemm_nt_f32_partials(H, p->HA, K));
+    return std::max(w01, racer_gemm_nt_f32_partials(RACER_OUT, p->HA, K));
+}
+
+// ws: the weight gradients' split-K workspace (ws_len floats), or null for the
+// device's grow-only scratch.
 static int backward_raw(racer_policy* p, const float* x, racer_acts* a, const float* d3, float* d2, float* d1,
-                        float* grad, int64_t cols) {
+                        float* grad, int64_t cols, float* ws = nullptr, size_t ws_len = 0) {
     racer_device* dev = p->dev;
     int H = p->hidden;
     int64_t HA = p->HA, IN = p->IN;
@@ -522,15 +535,15 @@ static int backward_raw(racer_policy* p, const float* x, racer_acts* a, const fl
     }
     // weight gradients (racer/ppo.mojo:121-123)
     if (p->precision == 1) {
-        if (racer_gemm_nt_bf16(dev, d1, cols, x, cols, grad, IN, H, (int)IN, (int)cols) ||
-            racer_gemm_nt_bf16(dev, d2, cols, h1, cols, grad + p->W1, HA, H, (int)HA, (int)cols))
+        if (racer_gemm_nt_bf16(dev, d1, cols, x, cols, grad, IN, H, (int)IN, (int)cols, ws, ws_len) ||
+            racer_gemm_nt_bf16(dev, d2, cols, h1, cols, grad + p->W1, HA, H, (int)HA, (int)cols, ws, ws_len))
             return 1;
     } else {
-        if (racer_gemm_nt_f32(dev, d1, cols, x, cols, grad, IN, H, (int)IN, (int)cols) ||
-            racer_gemm_nt_f32(dev, d2, cols, h1, cols, grad + p->W1, HA, H, (int)HA, (int)cols))
+        if (racer_gemm_nt_f32(dev, d1, cols, x, cols, grad, IN, H, (int)IN, (int)cols, ws, ws_len) ||
+            racer_gemm_nt_f32(dev, d2, cols, h1, cols, grad + p->W1, HA, H, (int)HA, (int)cols, ws, ws_len))
             return 1;
     }
-    return racer_gemm_nt_f32(dev, d3, cols, h2, cols, grad + p->W2, HA, RACER_OUT, (int)HA, (int)cols);
+    return racer_gemm_nt_f32(dev, d3, cols, h2, cols, grad + p->W2, HA, RACER_OUT, (int)HA, (int)cols, ws, ws_len);
 }
 
 extern "C" int racer_policy_backward(racer_policy* p, const racer_view* x_p, racer_acts* a, const racer_view* d3_p,
@@ -544,6 +557,33 @@ extern "C" int racer_policy_backward(racer_policy* p, const racer_view* x_p, rac
                       grad->len >= (size_t)p->P,
                   "backward buffers are too small");
     return backward_raw(p, view_ptr(x), a, view_ptr(d3), d2->ptr, d1->ptr, grad->ptr, cols);
+}
+
+// racer_policy_backward with the weight gradients' split-K partials in the
+// GPU's fixed scratch array instead of the grow-only device scratch: the same
+// kernels in the same order (so bitwise the same results), and no allocation.
+// Calls that overlap on one GPU share that array and race.
+extern "C" int racer_policy_backward_noalloc(racer_policy* p, const racer_view* x_p, racer_acts* a,
+                                             const racer_view* d3_p, racer_buf* d2, racer_buf* d1, racer_buf* grad) {
+    racer_clear_error();
+    RACER_REQUIRE(p && x_p && a && d3_p, "null argument");
+    const racer_view &x = *x_p, &d3 = *d3_p;
+    int64_t cols = a->cols;
+    if (racer_check_view(x, p->IN, cols, "policy input") || racer_check_view(d3, RACER_OUT, cols, "d3")) return 1;
+    RACER_REQUIRE(d2 && d2->len >= (size_t)(p->HA * cols) && d1 && d1->len >= (size_t)(p->HA * cols) && grad &&
+                      grad->len >= (size_t)p->P,
+                  "backward buffers are too small");
+    const size_t need = backward_partials(p, cols);
+    if (need > RACER_FIXED_SCRATCH_FLOATS) {
+        racer_set_error("the allocation-free backward needs " + std::to_string(need) +
+                        " floats of split-K scratch for " + std::to_string(cols) + " columns; its fixed scratch holds " +
+                        std::to_string(RACER_FIXED_SCRATCH_FLOATS) +
+                        " (16 MiB): use racer_policy_backward, or fewer columns per call");
+        return 1;
+    }
+    RACER_REQUIRE(p->dev->fixed_scratch, "the device has no fixed scratch array");
+    return backward_raw(p, view_ptr(x), a, view_ptr(d3), d2->ptr, d1->ptr, grad->ptr, cols, p->dev->fixed_scratch,
+                        RACER_FIXED_SCRATCH_FLOATS);
 }
 
 // ---------------------------------------------------------------- trainer
