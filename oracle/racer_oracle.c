@@ -710,6 +710,19 @@ long o_check_div_by_reciprocal(long count, uint64_t seed) {
  * multiples of `stride` (1: all 2^32, NaNs and infinities included): the same
  * index wherever either lies in [0, 2^24), both outside it (or NaN)
  * elsewhere. */
+/* racer_tanh_select against racer_tanh on every `stride`-th float bit
+ * pattern (stride 1: all 2^32): the number of results that differ. */
+long o_check_tanh_select(long stride) {
+    long bad = 0;
+    if (stride < 1) stride = 1;
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(+ : bad)
+    for (long i = 0; i <= 0xffffffffL; i += stride) {
+        float x = racer_u2f((uint32_t)i);
+        bad += racer_f2u(racer_tanh(x)) != racer_f2u(racer_tanh_select(x));
+    }
+    return bad;
+}
+
 long o_check_nearest_by_reciprocal(float cell, long stride) {
     const float inv = 1.0f / cell;
     long bad = 0;

@@ -15,7 +15,13 @@ def _policy_pair(dev, beams, hidden, precision=0, seed=42):
     return R.Policy(dev, seed, cfg), O.Policy(seed, beams, hidden), cfg
 
 
-@pytest.mark.parametrize("beams,hidden,cols", [(108, 64, 200), (1081, 256, 300), (13, 7, 5)])
+# shapes of the fused float32 forward: unrolled (1081/256, 108/64), 16-byte
+# staging (cols % 4 == 0) and scalar staging, the generic loop with a partial
+# row group (200 beams, H 100), ragged workgroups, and past its column limit
+# (20000 columns: the tiled products)
+@pytest.mark.parametrize("beams,hidden,cols", [(108, 64, 200), (1081, 256, 300), (13, 7, 5), (1081, 256, 4096),
+                                               (108, 64, 1024), (1081, 256, 301), (200, 100, 48), (108, 256, 77),
+                                               (1081, 256, 20000)])
 def test_init_and_forward_float32_bit_identical(dev, beams, hidden, cols):
     print(f"forward f32 {beams}/{hidden}/{cols}", flush=True)
     g, o, cfg = _policy_pair(dev, beams, hidden)
@@ -28,6 +34,22 @@ def test_init_and_forward_float32_bit_identical(dev, beams, hidden, cols):
     h1, h2, out = o.forward(x)
     assert np.array_equal(acts.h1.read().reshape(cfg.HA, cols), h1)
     assert np.array_equal(acts.h2.read().reshape(cfg.HA, cols), h2)
+    assert np.array_equal(acts.o.read().reshape(3, cols), out)
+
+
+def test_forward_float32_sees_direct_theta_writes(dev):
+    """A write into theta (no transpose) reaches the next forward, as it does
+    in the reference, where the forward reads theta itself."""
+    beams, hidden, cols = 108, 64, 64
+    g, o, cfg = _policy_pair(dev, beams, hidden)
+    x = np.random.default_rng(5).uniform(0, 1, (cfg.IN, cols)).astype(np.float32)
+    xb = dev.upload(x)
+    acts = R.Activations(dev, cols, cfg)
+    g.forward(R.mat(xb, cfg.IN, cols), acts)
+    o.theta[:] = np.random.default_rng(6).normal(0, 0.1, o.theta.size).astype(np.float32)
+    g.theta.write(o.theta)
+    g.forward(R.mat(xb, cfg.IN, cols), acts)
+    _, _, out = o.forward(x)
     assert np.array_equal(acts.o.read().reshape(3, cols), out)
 
 

@@ -113,3 +113,11 @@ def test_div_by_reciprocal_is_correctly_rounded():
     next to rounding midpoints, all-ones and power-of-two divisors, +0)."""
     O.set_threads(8)
     assert O.lib().o_check_div_by_reciprocal(16_000_000, 11) == 0
+
+
+def test_branch_free_tanh_is_racer_tanh():
+    """racer_tanh_select (both paths evaluated, one selected; the float32 fused
+    forward's epilogue) equals racer_tanh bit for bit: every 61st float bit
+    pattern here (all 2^32 agree as well, checked with stride 1)."""
+    O.set_threads(8)
+    assert O.lib().o_check_tanh_select(61) == 0

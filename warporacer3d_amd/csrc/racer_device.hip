@@ -222,6 +222,7 @@ extern "C" int racer_write_range(racer_buf* buf, size_t offset, size_t count, co
     racer_clear_error();
     RACER_REQUIRE(buf && offset + count <= buf->len, "write exceeds the buffer");
     if (!count) return 0;
+    ++buf->gen;
     RACER_TRY(hipMemcpyAsync(buf->ptr + offset, host, count * sizeof(float), hipMemcpyHostToDevice, buf->dev->stream));
     RACER_TRY(hipStreamSynchronize(buf->dev->stream));
     return 0;
@@ -242,6 +243,7 @@ extern "C" int racer_copy(racer_buf* dst, size_t dst_offset, racer_buf* src, siz
     RACER_REQUIRE(dst && src && dst_offset + count <= dst->len && src_offset + count <= src->len,
                   "copy exceeds a buffer");
     if (!count) return 0;
+    ++dst->gen;
     float* d = dst->ptr + dst_offset;
     const float* s = src->ptr + src_offset;
     if (d < s + count && s < d + count) {  // overlapping ranges of one buffer: the runtime's copy

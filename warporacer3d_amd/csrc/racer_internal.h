@@ -108,6 +108,10 @@ struct racer_buf {
     racer_device* dev = nullptr;
     float* ptr = nullptr;
     size_t len = 0;
+    // Bumped by every write through the C ABI (racer_write*, racer_copy into
+    // it), so objects that keep derived copies of a buffer (the policy's packed
+    // weights) know to rebuild them.
+    uint64_t gen = 0;
 };
 
 // Launch bracket for the profiler: records events around a launch when enabled.
@@ -181,6 +185,11 @@ struct racer_policy {
     uint16_t* w1t_bf16 = nullptr;  // bf16 w1t [HA, pad8(H)] for the backward's through product
     int64_t h_pad = 0;
     bool bf16_stale = true;
+    uint64_t bf16_gen = 0;  // theta's write generation the bf16 copies were made from
+    // float32 W0, W1 in the K = 1 MFMA operand order (float32 fused forward, H <= 256)
+    float *frag0_f32 = nullptr, *frag1_f32 = nullptr;
+    bool f32_stale = true;
+    uint64_t f32_gen = 0;
 };
 
 struct racer_acts {
@@ -326,6 +335,14 @@ inline bool racer_fused_reach(int IN, int64_t x_cols, int64_t cols) {
     return x_cols >= 0 && cols >= 0 && ((uint64_t)(IN - 1) * (uint64_t)x_cols + (uint64_t)cols) * 4 < 0xFFFFFF00ull;
 }
 int racer_policy_fused_setup();
+// The float32 forward (and optionally Sim.sample) in one kernel on K = 1
+// float32 MFMA, bitwise the float32 products; for H <= 256 within the LDS.
+bool racer_policy_f32_fits(int H, int IN);
+int racer_policy_f32_setup();
+size_t racer_f32_fragment_floats(int K);
+int racer_pack_f32_fragments(racer_device* dev, const float* src, int64_t ld_src, int rows, int cols, float* dst);
+int racer_policy_f32(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
+                     float* act, float* logp, float* value, int deterministic, int64_t step, int seed);
 int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
                        float* act, float* logp, float* value, int deterministic, int64_t step, int seed);
 

@@ -124,10 +124,8 @@ RACER_FN float racer_cos(float x) {
 #define RACER_LN2_3 (-0x1.05c61p-29f)
 #define RACER_INV_LN2 0x1.715476p+0f
 
-RACER_FN float racer_exp(float x) {
-    if (!(x == x)) return x;
-    if (x > 88.7228394f) return racer_u2f(0x7f800000u);
-    if (x < -103.972084f) return 0.0f;
+/* e^x for finite x in [-103.972084, 88.7228394] (racer_exp's main path). */
+RACER_FN float racer_exp_main(float x) {
     float k = racer_rint(x * RACER_INV_LN2);
     float r = ((x - k * RACER_LN2_1) - k * RACER_LN2_2) - k * RACER_LN2_3;
     float p = (float)(1.0 / 40320.0);
@@ -143,6 +141,13 @@ RACER_FN float racer_exp(float x) {
     int k1 = ki / 2;
     int k2 = ki - k1;
     return (e * racer_u2f((uint32_t)(k1 + 127) << 23)) * racer_u2f((uint32_t)(k2 + 127) << 23);
+}
+
+RACER_FN float racer_exp(float x) {
+    if (!(x == x)) return x;
+    if (x > 88.7228394f) return racer_u2f(0x7f800000u);
+    if (x < -103.972084f) return 0.0f;
+    return racer_exp_main(x);
 }
 
 RACER_FN float racer_log(float x) {
@@ -196,6 +201,32 @@ RACER_FN float racer_tanh(float x) {
     } else {
         y = (a == a) ? 1.0f : a;
     }
+    return x < 0.0f ? -y : y;
+}
+
+/* racer_tanh with no branches: both finite-range paths (racer_tanh's own
+ * operations) are evaluated and one is selected, so the result is racer_tanh's
+ * bit for bit, and many independent evaluations can be interleaved by the
+ * compiler (where racer_tanh's branches serialise them). The exp path's
+ * argument is clamped to 18.2 (NaN included), which changes nothing for the
+ * |x| < 9.1 it serves and keeps racer_exp_main in its range otherwise. */
+RACER_FN float racer_tanh_select(float x) {
+    const float a = racer_abs(x);
+    const float x2 = a * a;
+    float p = (float)(-443861162.0 / 1856156927625.0);
+    p = p * x2 + (float)(6404582.0 / 10854718875.0);
+    p = p * x2 + (float)(-929569.0 / 638512875.0);
+    p = p * x2 + (float)(21844.0 / 6081075.0);
+    p = p * x2 + (float)(-1382.0 / 155925.0);
+    p = p * x2 + (float)(62.0 / 2835.0);
+    p = p * x2 + (float)(-17.0 / 315.0);
+    p = p * x2 + (float)(2.0 / 15.0);
+    p = p * x2 + (float)(-1.0 / 3.0);
+    const float small = a + a * (x2 * p);
+    float e2 = 2.0f * a;
+    e2 = e2 < 18.2f ? e2 : 18.2f;
+    const float middle = 1.0f - 2.0f / (racer_exp_main(e2) + 1.0f);
+    const float y = a < 0.55f ? small : (a < 9.1f ? middle : ((a == a) ? 1.0f : a));
     return x < 0.0f ? -y : y;
 }
 

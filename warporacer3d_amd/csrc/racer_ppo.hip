@@ -364,6 +364,11 @@ extern "C" int racer_policy_create(racer_device* dev, int seed, const racer_conf
         RACER_TRY(hipMalloc(&p->frag1, racer_fragment_elems((int)p->HA) * sizeof(uint16_t)));
         if (p->precision == 1 && racer_policy_fused_setup()) return 1;
     }
+    if (p->precision == 0 && racer_policy_f32_fits((int)H, (int)p->IN)) {
+        RACER_TRY(hipMalloc(&p->frag0_f32, racer_f32_fragment_floats((int)p->IN) * sizeof(float)));
+        RACER_TRY(hipMalloc(&p->frag1_f32, racer_f32_fragment_floats((int)p->HA) * sizeof(float)));
+        if (racer_policy_f32_setup()) return 1;
+    }
     hipLaunchKernelGGL(k_policy_init, dim3((unsigned)((p->P + 255) / 256)), dim3(256), 0, dev->stream, p->theta->ptr,
                        p->P, p->IN, p->IN - 1, H, p->HA, p->W1, p->W2, p->LOGSTD, seed);
     if (racer_launch_check("policy init")) return 1;
@@ -382,6 +387,8 @@ extern "C" void racer_policy_destroy(racer_policy* p) {
     if (p->w1t_bf16) (void)hipFree(p->w1t_bf16);
     if (p->frag0) (void)hipFree(p->frag0);
     if (p->frag1) (void)hipFree(p->frag1);
+    if (p->frag0_f32) (void)hipFree(p->frag0_f32);
+    if (p->frag1_f32) (void)hipFree(p->frag1_f32);
     racer_device* dev = p->dev;
     delete p;
     racer_device_release(dev);
@@ -409,12 +416,13 @@ extern "C" int racer_policy_transpose(racer_policy* p) {
                            p->theta->ptr, p->W1, p->W2, p->hidden, (int)p->HA, p->w1t->ptr, p->w2t->ptr);
     }
     p->bf16_stale = true;
+    p->f32_stale = true;
     return racer_launch_check("transpose");
 }
 
 // The bf16 weight copies used by the MFMA products, refreshed after updates.
 static int refresh_bf16(racer_policy* p) {
-    if (!p->bf16_stale) return 0;
+    if (!p->bf16_stale && p->bf16_gen == p->theta->gen) return 0;
     const float* th = p->theta->ptr;
     int H = p->hidden;
     uint16_t* w1 = p->theta_bf16 + (int64_t)H * p->in_pad;
@@ -426,10 +434,35 @@ static int refresh_bf16(racer_policy* p) {
         return 1;
     if (racer_pack_bf16(p->dev, p->w1t->ptr, H, p->w1t_bf16, p->h_pad, (int)p->HA, H)) return 1;
     p->bf16_stale = false;
+    p->bf16_gen = p->theta->gen;
     return 0;
 }
 
-int racer_policy_refresh(racer_policy* p) { return p->precision == 1 ? refresh_bf16(p) : 0; }
+// The float32 operand-order weight copies of the fused float32 forward,
+// rebuilt when an update or a write changed theta.
+static int refresh_f32(racer_policy* p) {
+    if (!p->f32_stale && p->f32_gen == p->theta->gen) return 0;
+    const float* th = p->theta->ptr;
+    if (racer_pack_f32_fragments(p->dev, th, p->IN, p->hidden, (int)p->IN, p->frag0_f32) ||
+        racer_pack_f32_fragments(p->dev, th + p->W1, p->HA, p->hidden, (int)p->HA, p->frag1_f32))
+        return 1;
+    p->f32_stale = false;
+    p->f32_gen = p->theta->gen;
+    return 0;
+}
+
+// The float32 forward runs as one kernel on K = 1 float32 MFMA (bitwise the
+// float32 products) up to this many columns; past it every 16-column
+// workgroup re-streaming all of W0 costs more than the tiled products.
+constexpr int F32_FUSED_MAX_COLS = 16384;
+static bool use_f32_fused(const racer_policy* p, int cols) {
+    return p->precision == 0 && p->frag0_f32 && cols <= F32_FUSED_MAX_COLS;
+}
+
+int racer_policy_refresh(racer_policy* p) {
+    if (p->precision == 1) return refresh_bf16(p);
+    return p->frag0_f32 ? refresh_f32(p) : 0;
+}
 
 // Hidden layers of the forward pass (racer/ppo.mojo:109-110) as two products.
 static int forward_hidden(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols) {
@@ -463,6 +496,10 @@ int racer_policy_forward_raw(racer_policy* p, const float* x, int64_t x_cols, ra
         if (refresh_bf16(p)) return 1;
         return racer_policy_fused(p, x, x_cols, a, cols, false, nullptr, nullptr, nullptr, 0, 0, 0);
     }
+    if (use_f32_fused(p, cols)) {
+        if (refresh_f32(p)) return 1;
+        return racer_policy_f32(p, x, x_cols, a, cols, false, nullptr, nullptr, nullptr, 0, 0, 0);
+    }
     if (forward_hidden(p, x, x_cols, a, cols)) return 1;
     if (p->precision == 2)  // the reference's decomposition: the output layer as a third product
         return racer_gemm_nn_f32(p->dev, p->theta->ptr + p->W2, p->HA, a->h2->ptr, cols, a->o->ptr, cols, RACER_OUT,
@@ -477,6 +514,10 @@ int racer_policy_act(racer_policy* p, const float* x, int64_t x_cols, racer_acts
     if (use_fused(p)) {
         if (refresh_bf16(p)) return 1;
         return racer_policy_fused(p, x, x_cols, a, cols, true, act, logp, value, deterministic, step, seed);
+    }
+    if (use_f32_fused(p, cols)) {
+        if (refresh_f32(p)) return 1;
+        return racer_policy_f32(p, x, x_cols, a, cols, true, act, logp, value, deterministic, step, seed);
     }
     if (p->precision == 2) {  // three products, then a separate sample kernel (racer/sim.mojo:323-348)
         if (racer_policy_forward_raw(p, x, x_cols, a, cols)) return 1;
