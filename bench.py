@@ -6,10 +6,13 @@ ramp ribbon of radius 10 m with road height 0.3 sin(3 theta), 0.025 m grids,
 4096 cars x 1081 beams over 270 degrees, random-init 2 x 256 tanh policy,
 spawn heading noise U(-0.1, 0.1). Inputs are resident in HBM before timing.
 
+The headline `value` is the float32 step (float32 policy products, bitwise
+the CPU restatement); the bf16-policy step is reported beside it (`bf16`).
 Prints one JSON line with env-steps/s, the roofline of the dominant kernel
-(measured with HIP events on the library's stream during the timed steps) and
-the CPU baseline (the oracle's float32 restatement, timed on a bounded sample
-of the same workload on the host cores).
+(measured with HIP events on the library's stream right after the timed
+steps), the policy forward's MFMA roofline and the CPU baseline (the oracle's
+float32 restatement of the same workload, timed on the host's threads and on
+one).
 """
 from __future__ import annotations
 
@@ -46,16 +49,17 @@ def policy_flops(cars: int, beams: int, hidden: int) -> float:
     return 2.0 * cars * (hidden * IN + hidden * HA + 3 * HA)
 
 
-def cpu_baseline(track, beams: int, hidden: int, threads: int, seconds: float) -> dict:
-    """The oracle's Runner step (policy forward, sample, physics, lidar) on the
-    host, bounded to about `seconds` of work."""
+def cpu_baseline(track, beams: int, hidden: int, cars: int, threads: int, steps: int, steps_1core: int) -> dict:
+    """The oracle's Runner step (float32 policy forward, sample, physics,
+    lidar) on the host at the benchmark's own size, timed like the reference's
+    `racer benchmark ... cpu` (main.mojo:225-235: a warm-up step, then timed
+    steps): `steps` steps on `threads` threads, then `steps_1core` on one."""
     import numpy as np
 
     import oracle as O
 
-    O.set_threads(threads)
-    cars = 256
     sim = O.Sim(track, cars, 42, beams=beams, heading_noise=0.1)
+    O.set_threads(threads)
     sim.reset()
     pol = O.Policy(42, beams, hidden)
     logp, value = np.zeros(cars, np.float32), np.zeros(cars, np.float32)
@@ -65,16 +69,24 @@ def cpu_baseline(track, beams: int, hidden: int, threads: int, seconds: float) -
         sim.sample(out, pol.log_std(), sim.actions, logp, value, False, t)
         sim.step()
 
+    def timed(n, t0_step):
+        t0 = time.perf_counter()
+        for k in range(n):
+            step(t0_step + k)
+        return time.perf_counter() - t0
+
     step(0)
-    t0 = time.perf_counter()
-    steps = 0
-    while time.perf_counter() - t0 < seconds:
-        step(steps + 1)
-        steps += 1
-    dt = time.perf_counter() - t0
-    return {"value": cars * steps / dt, "unit": "env-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{cars} cars x {beams} beams, 2x{hidden} float32 policy, ramp10, {steps} Runner steps "
-                      f"({dt:.1f} s) on {threads} host threads"}
+    dt = timed(steps, 1)
+    O.set_threads(1)
+    dt1 = timed(steps_1core, 1 + steps)
+    O.set_threads(threads)
+    return {"value": cars * steps / dt, "unit": "env-steps/s", "cores": threads, "nproc": os.cpu_count(),
+            "kind": "port", "value_1core": cars * steps_1core / dt1,
+            "ms_per_step": dt * 1000.0 / steps, "ms_per_step_1core": dt1 * 1000.0 / steps_1core,
+            "sample": f"the benchmark workload itself ({cars} cars x {beams} beams, 2x{hidden} float32 policy, "
+                      f"ramp10): {steps} Runner steps on {threads} host threads ({dt:.1f} s) and "
+                      f"{steps_1core} on 1 ({dt1:.1f} s), after one warm-up step; host has {os.cpu_count()} "
+                      f"logical CPUs"}
 
 
 def ppo_flops(cars: int, beams: int, hidden: int, rollout: int, epochs: int) -> float:
@@ -452,60 +464,21 @@ def plain_baseline(R, dev, track, args, steps: int) -> dict:
             "schedule": "physics 1 thread/car; lidar 1 thread/(car, beam); 3 float32 products; sample kernel"}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cars", type=int, default=4096)
-    ap.add_argument("--beams", type=int, default=1081)
-    ap.add_argument("--hidden", type=int, default=256)
-    ap.add_argument("--track", default="ramp10")
-    ap.add_argument("--precision", choices=["bf16", "fp32"], default="bf16")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU decomposition baseline")
-    ap.add_argument("--surrogate-transitions", type=int, default=10_000_000,
-                    help="transitions recorded for the learned dynamics surrogate (0 skips it)")
-    ap.add_argument("--surrogate-epochs", type=int, default=2)
-    ap.add_argument("--batch-steps", type=int, default=100,
-                    help="timed steps of the eight-instance batched launch (0 skips it)")
-    ap.add_argument("--ppo-iterations", type=int, default=3,
-                    help="timed PPO iterations per beam count on the bridge (0 skips the PPO leg)")
-    ap.add_argument("--train-parity", type=int, default=0,
-                    help="PPO iterations of the bf16 / fp32 training-curve comparison (0: skip; minutes per 300)")
-    ap.add_argument("--train-parity-seeds", type=int, default=2)
-    ap.add_argument("--calibrate-copy", type=int, default=0,
-                    help="MiB copied once device to device before the run (known-size traffic for counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark measures one GPU only; run with --gpus 1"}))
-        return 2
-
+def runner_leg(R, dev, arr, args, precision: str, layers: int) -> dict:
+    """Runner steps (policy forward with sample, physics, lidar) at the
+    benchmark's size: the timed region is `steps` steps replayed from a
+    captured HIP graph, bracketed by device synchronizations; kernel durations
+    come from HIP events in eager passes of the same steps right after."""
     import numpy as np
 
-    import warporacer3d_amd as R
-    from warporacer3d_amd import track as T
-
-    arr = T.cached(args.track)
-    layers = int(arr[7])
-    cfg = R.Config(beams=args.beams, hidden=args.hidden, precision=1 if args.precision == "bf16" else 0,
+    cfg = R.Config(beams=args.beams, hidden=args.hidden, precision=1 if precision == "bf16" else 0,
                    heading_noise=0.1)
-    dev = R.Device(True)
-    log("device:", dev.name)
-    if args.calibrate_copy:
-        count = args.calibrate_copy * (1 << 20) // 4
-        src, dst = dev.alloc(count, 1.0), dev.alloc(count)
-        dev.copy(dst, 0, src, 0, count)
-        dev.synchronize()
-        log(f"calibration copy: {count * 4} bytes read and written")
-        del src, dst
     run = R.Runner(arr, args.cars, cfg, seed=42, device=dev)
     for t in range(args.warmup):
         run.step(False, t)
     dev.synchronize()
     waves_per_car, t = waves_by_replay(dev, run.sim.set_waves_per_car, run.capture, args.warmup)
-    log("lidar wavefronts per car (fastest graph replay of 1, 2, 4):", waves_per_car)
+    log(f"[{precision}] lidar wavefronts per car (fastest graph replay of 1, 2, 4):", waves_per_car)
     # march iterations of the lidar, counted by the instrumented variant on the warm state (untimed)
     run.sim.count_iterations(True)
     counts, trips = [], []
@@ -515,10 +488,7 @@ def main() -> int:
         trips.append(run.sim.last_trips)
     run.sim.count_iterations(False)
     dev.synchronize()
-    log("lidar iterations per step:", counts, "wavefront loop trips per step:", trips)
     t += 3
-    # Timed region: the Runner steps replayed from a captured HIP graph (the
-    # same kernels and arguments as eager steps, bitwise the same results).
     chunk = math.gcd(args.steps, 100)
     graph = run.capture(chunk)
     graph.launch(t)  # untimed first replay
@@ -532,8 +502,9 @@ def main() -> int:
     t += args.steps
     ms = dt * 1000.0 / args.steps
     value = args.cars * 1000.0 / ms
-    log(f"graph replay: {ms:.4f} ms/step, {value:.4g} env-steps/s")
+    log(f"[{precision}] graph replay: {ms:.4f} ms/step, {value:.4g} env-steps/s")
     assert np.isfinite(run.sim.obs.read()).all(), "non-finite observations"
+    del graph
     # The same number of eager steps with HIP events around each lidar launch
     # only (one event pair per step): the lidar's launch duration.
     dev.profile_reset()
@@ -548,7 +519,20 @@ def main() -> int:
     t += args.steps
     lidar_n, lidar_ms = dev.profile_results().get("lidar", (0, 0.0))
     lidar_us = lidar_ms / max(lidar_n, 1) * 1000.0
-    log(f"eager steps with lidar events: {eager_ms:.4f} ms/step; lidar {lidar_us:.2f} us over {lidar_n} launches")
+    # the same for the policy forward (with the sample) alone
+    policy_kernel = "policy_fused_sample" if precision == "bf16" else "policy_f32_sample"
+    dev.profile_reset()
+    dev.profile_only(policy_kernel)
+    dev.profile(True)
+    for k in range(args.steps):
+        run.step(False, t + k)
+    dev.synchronize()
+    dev.profile(False)
+    t += args.steps
+    pol_n, pol_ms = dev.profile_results().get(policy_kernel, (0, 0.0))
+    pol_us = pol_ms / max(pol_n, 1) * 1000.0
+    log(f"[{precision}] eager steps: {eager_ms:.4f} ms/step; lidar {lidar_us:.2f} us, {policy_kernel} "
+        f"{pol_us:.2f} us")
     # untimed: iteration count on the final state, to bracket the measured steps
     run.sim.count_iterations(True)
     run.step(False, t)
@@ -568,7 +552,6 @@ def main() -> int:
         log(f"  {k:20s} launches {n:6d} avg {tot / max(n, 1) * 1000:9.2f} us  total {tot:9.2f} ms")
     iters = float(np.mean(counts))
     lbytes = lidar_bytes(iters, args.cars, args.beams, layers)
-    gemm_ms = sum(tot for k, (n, tot) in prof.items() if k.startswith(("gemm", "policy", "head")))
     flops = policy_flops(args.cars, args.beams, args.hidden)
     traffic = None
     tfile = ROOT / "profiles" / "lidar_traffic.json"
@@ -583,12 +566,67 @@ def main() -> int:
         "algorithmic_bytes_per_launch": lbytes, "march_iterations_per_launch": iters,
         "avg_launch_us": lidar_us,
     }
-    peak = MFMA_BF16_PEAK_TFLOPS if args.precision == "bf16" else FP32_PEAK_TFLOPS
-    gemm_tflops = flops * min(args.steps, 100) / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
-    policy = {"bound": "mfma" if args.precision == "bf16" else "fp32", "achieved": gemm_tflops, "peak": peak,
-              "unit": "TFLOP/s", "frac": gemm_tflops / peak if gemm_tflops else None,
-              "flops_per_step": flops}
+    peak = MFMA_BF16_PEAK_TFLOPS if precision == "bf16" else FP32_PEAK_TFLOPS
+    achieved = flops / (pol_us * 1e-6) / 1e12 if pol_us > 0 else None
+    policy = {"bound": "mfma", "kernel": policy_kernel, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+              "frac": achieved / peak if achieved else None, "flops_per_launch": flops, "avg_launch_us": pol_us,
+              "peak_is": "dense bf16 MFMA" if precision == "bf16" else
+                         "dense float32 MFMA (the K=1 float32 MFMA the kernel runs on)"}
     kernels = {k: {"launches": n, "avg_us": tot / max(n, 1) * 1000.0} for k, (n, tot) in prof.items()}
+    return {"value": value, "ms_per_step": ms, "eager_ms_per_step": eager_ms, "roofline": roofline,
+            "policy_roofline": policy, "kernels": kernels, "lidar_waves_per_car": waves_per_car,
+            "lidar_wavefront_trips": trips}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cars", type=int, default=4096)
+    ap.add_argument("--beams", type=int, default=1081)
+    ap.add_argument("--hidden", type=int, default=256)
+    ap.add_argument("--track", default="ramp10")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the bf16-policy Runner step")
+    ap.add_argument("--cpu-steps", type=int, default=30, help="oracle Runner steps on all host threads")
+    ap.add_argument("--cpu-steps-1core", type=int, default=3, help="oracle Runner steps on one thread")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU decomposition baseline")
+    ap.add_argument("--surrogate-transitions", type=int, default=10_000_000,
+                    help="transitions recorded for the learned dynamics surrogate (0 skips it)")
+    ap.add_argument("--surrogate-epochs", type=int, default=2)
+    ap.add_argument("--batch-steps", type=int, default=100,
+                    help="timed steps of the eight-instance batched launch (0 skips it)")
+    ap.add_argument("--ppo-iterations", type=int, default=3,
+                    help="timed PPO iterations per beam count on the bridge (0 skips the PPO leg)")
+    ap.add_argument("--train-parity", type=int, default=0,
+                    help="PPO iterations of the bf16 / fp32 training-curve comparison (0: skip; minutes per 300)")
+    ap.add_argument("--train-parity-seeds", type=int, default=2)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="MiB copied once device to device before the run (known-size traffic for counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark measures one GPU: the build runs on one GPU only, so "
+                                   "--gpus above 1 is not supported; run with --gpus 1"}))
+        return 2
+
+    import warporacer3d_amd as R
+    from warporacer3d_amd import track as T
+
+    arr = T.cached(args.track)
+    layers = int(arr[7])
+    dev = R.Device(True)
+    log("device:", dev.name)
+    if args.calibrate_copy:
+        count = args.calibrate_copy * (1 << 20) // 4
+        src, dst = dev.alloc(count, 1.0), dev.alloc(count)
+        dev.copy(dst, 0, src, 0, count)
+        dev.synchronize()
+        log(f"calibration copy: {count * 4} bytes read and written")
+        del src, dst
+    fp32 = runner_leg(R, dev, arr, args, "fp32", layers)
+    bf16 = None if args.no_bf16 else runner_leg(R, dev, arr, args, "bf16", layers)
+    value, ms = fp32["value"], fp32["ms_per_step"]
     plain = None
     if not args.no_plain:
         plain = plain_baseline(R, dev, arr, args, min(args.steps, 50))
@@ -604,35 +642,42 @@ def main() -> int:
     ppo = None
     if args.ppo_iterations > 0:
         ppo = {}
-        for beams in (1081, 108):
-            ppo[f"beams{beams}"] = ppo_leg(R, T, dev, beams, args.ppo_iterations)
-            log(f"PPO iteration (bridge8, 16384 cars, {beams} beams):",
-                {k: v for k, v in ppo[f'beams{beams}'].items() if k != "kernels_ms_per_iteration"})
+        for beams, precision in ((1081, 1), (108, 1), (108, 0)):
+            key = f"beams{beams}" + ("" if precision == 1 else "_fp32")
+            ppo[key] = ppo_leg(R, T, dev, beams, args.ppo_iterations, precision=precision)
+            log(f"PPO iteration (bridge8, 16384 cars, {beams} beams, precision {precision}):",
+                {k: v for k, v in ppo[key].items() if k != "kernels_ms_per_iteration"})
     parity = None
     if args.train_parity > 0:
         parity = train_parity_leg(R, T, dev, args.train_parity, args.train_parity_seeds)
         log("training parity:", {k: v for k, v in parity.items() if k != "runs"})
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline on {threads} threads ...")
-        cpu = cpu_baseline(arr, args.beams, args.hidden, threads, args.cpu_seconds)
+        threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
+        log(f"cpu baseline: {args.cars} cars x {args.beams} beams on {threads} threads and on 1 ...")
+        cpu = cpu_baseline(arr, args.beams, args.hidden, args.cars, threads, args.cpu_steps, args.cpu_steps_1core)
         log("cpu baseline:", cpu)
     out = {
         "metric": METRIC, "value": value, "unit": "env-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "bf16+fp32" if args.precision == "bf16" else "fp32",
+        "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic (procedural ramp ribbon, random-init policy)",
         "config": {"workload": f"Runner step: {args.track} ribbon R=10 m, height 0.3 sin 3t, 0.025 m grid; "
-                               f"{args.cars} cars x {args.beams} beams; 2x{args.hidden} tanh policy "
-                               f"({args.precision} GEMMs, fp32 dynamics and lidar)",
+                               f"{args.cars} cars x {args.beams} beams; 2x{args.hidden} tanh policy, float32 "
+                               "products (bitwise the CPU restatement), float32 dynamics and lidar",
                    "cars": args.cars, "beams": args.beams, "hidden": args.hidden, "track": args.track,
-                   "policy_precision": args.precision, "parallelism": "single GPU",
-                   "lidar_waves_per_car": waves_per_car},
-        "eager_ms_per_step": eager_ms, "timing": "graph replays of captured Runner steps; kernel durations from "
-                                                 "HIP events in eager passes of the same steps right after",
-        "roofline": roofline, "policy_roofline": policy, "kernels": kernels, "cpu_baseline": cpu,
+                   "policy_precision": "fp32", "parallelism": "single GPU",
+                   "lidar_waves_per_car": fp32["lidar_waves_per_car"]},
+        "eager_ms_per_step": fp32["eager_ms_per_step"],
+        "timing": "graph replays of captured Runner steps; kernel durations from HIP events in eager passes of "
+                  "the same steps right after",
+        "roofline": fp32["roofline"], "policy_roofline": fp32["policy_roofline"], "kernels": fp32["kernels"],
+        "value_bf16": bf16["value"] if bf16 else None,
+        "bf16": ({k: bf16[k] for k in ("value", "ms_per_step", "eager_ms_per_step", "roofline", "policy_roofline",
+                                       "kernels", "lidar_waves_per_car")} if bf16 else None),
+        "cpu_baseline": cpu,
         "plain_gpu_baseline": plain, "speedup_vs_plain_gpu": value / plain["value"] if plain else None,
+        "speedup_bf16_vs_plain_gpu": bf16["value"] / plain["value"] if (plain and bf16) else None,
         "ppo_iteration": ppo, "batch": batched, "surrogate": surrogate,
     }
     if parity is not None:

@@ -362,15 +362,25 @@ void o_policy_init(float* theta, int beams, int hidden, int seed) {
     }
 }
 
-/* c[j, col] = sum_k a[j, k] b[k, col] accumulated with fmaf in increasing k. */
+/* c[j, col] = sum_k a[j, k] b[k, col] accumulated with fmaf in increasing k.
+ * Columns go 16 at a time so the b reads are contiguous; each output is still
+ * its own fmaf chain over k in order. */
+#define MM_COLS 16
 static void mm(const float* a, long lda, const float* b, long ldb, float* c, long ldc, long rows, long inner, long cols,
                int squash) {
+    long blocks = (cols + MM_COLS - 1) / MM_COLS;
 #pragma omp parallel for schedule(static) num_threads(g_threads)
-    for (long col = 0; col < cols; ++col) {
+    for (long blk = 0; blk < blocks; ++blk) {
+        long c0 = blk * MM_COLS, n = cols - c0 < MM_COLS ? cols - c0 : MM_COLS;
         for (long j = 0; j < rows; ++j) {
-            float acc = 0.0f;
-            for (long k = 0; k < inner; ++k) acc = fmaf(a[j * lda + k], b[k * ldb + col], acc);
-            c[j * ldc + col] = squash ? racer_tanh(acc) : acc;
+            float acc[MM_COLS] = {0.0f};
+            const float* arow = a + j * lda;
+            for (long k = 0; k < inner; ++k) {
+                const float w = arow[k];
+                const float* brow = b + k * ldb + c0;
+                for (long q = 0; q < n; ++q) acc[q] = fmaf(w, brow[q], acc[q]);
+            }
+            for (long q = 0; q < n; ++q) c[j * ldc + c0 + q] = squash ? racer_tanh(acc[q]) : acc[q];
         }
     }
 }
