@@ -109,6 +109,10 @@ racer_buf* racer_sim_reward(racer_sim* sim);  /* [n] */
 racer_buf* racer_sim_done(racer_sim* sim);    /* [n] */
 int racer_sim_cars(const racer_sim* sim);
 int racer_sim_beams(const racer_sim* sim);
+/* The viewer's snapshot of every buffer (main.mojo:117-122): state [11, n],
+ * obs [beams+5, n], actions [2, n], reward [n], done [n] copied back to back
+ * into host (count >= (11 + beams + 5 + 4) n floats); synchronous. */
+int racer_sim_snapshot(racer_sim* sim, float* host, size_t count);
 
 /* ---------------- Policy and Activations: racer/ppo.mojo:47-138 ---------------- */
 /* Activations(device, cols) (ppo.mojo:56-60): h1, h2 [H+1, cols] filled with 1, o [3, cols]. */

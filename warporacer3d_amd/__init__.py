@@ -59,6 +59,17 @@ class Config(C.Structure):
     def params(self) -> int:
         return self.hidden * self.IN + self.hidden * self.HA + OUT * self.HA + 2
 
+    @classmethod
+    def from_env(cls) -> "Config":
+        """The shapes the reference fixes at compile time (-D BEAMS, racer/ppo.mojo:17-26), from the
+        environment as the CLI reads them: RACER_BEAMS, RACER_HIDDEN, RACER_ROLLOUT, RACER_EPOCHS,
+        RACER_MINIBATCHES and RACER_PRECISION (fp32 or bf16)."""
+        e = os.environ
+        return cls(beams=int(e.get("RACER_BEAMS", 108)), hidden=int(e.get("RACER_HIDDEN", 64)),
+                   rollout=int(e.get("RACER_ROLLOUT", 32)), epochs=int(e.get("RACER_EPOCHS", 4)),
+                   minibatches=int(e.get("RACER_MINIBATCHES", 4)),
+                   precision=1 if e.get("RACER_PRECISION", "fp32") == "bf16" else 0)
+
 
 class _View(C.Structure):
     _fields_ = [("buf", C.c_void_p), ("offset", C.c_size_t), ("rows", C.c_int64), ("cols", C.c_int64)]
@@ -108,6 +119,7 @@ def lib() -> C.CDLL:
         "racer_sim_done": (vp, [vp]),
         "racer_sim_cars": (i, [vp]),
         "racer_sim_beams": (i, [vp]),
+        "racer_sim_snapshot": (i, [vp, vp, sz]),
         "racer_acts_create": (i, [vp, i, C.POINTER(Config), cp]),
         "racer_acts_destroy": (None, [vp]),
         "racer_acts_h1": (vp, [vp]),
@@ -381,6 +393,12 @@ class Sim:
                step: int) -> None:
         _check(lib().racer_sim_sample(self.handle, output.c, log_std.c, actions.c, logp.c, value.c,
                                       int(deterministic), int(step)))
+
+    def snapshot(self) -> np.ndarray:
+        """state, obs, actions, reward and done back to back, as the viewer reads them (main.mojo:117-122)."""
+        out = np.empty((STATE + self.IN + 4) * self.n, np.float32)
+        _check(lib().racer_sim_snapshot(self.handle, _ptr(out), out.size))
+        return out
 
     def count_iterations(self, on: bool = True) -> None:
         _check(lib().racer_sim_count_iterations(self.handle, int(on)))

@@ -1,13 +1,23 @@
 // racer: the command-line driver of main.mojo:1-149 over the C ABI.
 //
-//   racer train     TRACK.npy [auto|gpu|cpu] [cars] [iterations] [checkpoint.npy] [resume.npy]
-//   racer eval      TRACK.npy [auto|gpu|cpu] [cars] [steps] [checkpoint.npy]
-//   racer benchmark TRACK.npy [auto|gpu|cpu] [cars] [steps]
+//   racer train     TRACK [auto|gpu|cpu] [cars] [iterations] [checkpoint.npy] [resume.npy]
+//   racer eval      TRACK [auto|gpu|cpu] [cars] [steps] [checkpoint.npy]
+//   racer benchmark TRACK [auto|gpu|cpu] [cars] [steps]
+//   racer serve     TRACK [auto|gpu|cpu] [cars] [port] [checkpoint.npy|-]
 //
-// TRACK.npy is a float32 track array (python -m warporacer3d_amd.track NAME
-// TRACK.npy writes one). Shape parameters come from the environment:
-// RACER_BEAMS (108), RACER_HIDDEN (64), RACER_ROLLOUT (32), RACER_EPOCHS (4),
-// RACER_MINIBATCHES (4), RACER_PRECISION (fp32|bf16).
+// TRACK is a built-in name (flat, ramp, bank, bridge, ring10, ramp10, bank10,
+// bridge8, random<k>), a map YAML, or a float32 .npy track array. As the
+// reference does (main.mojo:133, through Python), names and YAML maps are
+// compiled by the Python track compiler, here in a child process
+// (python3 -m warporacer3d_amd.track TRACK FILE.npy) run before the GPU is
+// touched; `serve` is the Python server (python3 -m warporacer3d_amd.serve),
+// also a child. Shape parameters come from the environment: RACER_BEAMS (108),
+// RACER_HIDDEN (64), RACER_ROLLOUT (32), RACER_EPOCHS (4), RACER_MINIBATCHES
+// (4), RACER_PRECISION (fp32|bf16); RACER_PYTHON names the interpreter.
+// `cpu` fails: the library is the GPU path only.
+#include <sys/wait.h>
+#include <unistd.h>
+
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -21,10 +31,48 @@
 #include "racer.h"
 
 static const char* USAGE =
-    "racer train     TRACK.npy [auto|gpu|cpu] [cars] [iterations] [checkpoint.npy] [resume.npy]\n"
-    "racer eval      TRACK.npy [auto|gpu|cpu] [cars] [steps] [checkpoint.npy]\n"
-    "racer benchmark TRACK.npy [auto|gpu|cpu] [cars] [steps]\n"
-    "TRACK.npy is a float32 track array (python -m warporacer3d_amd.track NAME TRACK.npy).";
+    "racer train     TRACK [auto|gpu|cpu] [cars] [iterations] [checkpoint.npy] [resume.npy]\n"
+    "racer eval      TRACK [auto|gpu|cpu] [cars] [steps] [checkpoint.npy]\n"
+    "racer benchmark TRACK [auto|gpu|cpu] [cars] [steps]\n"
+    "racer serve     TRACK [auto|gpu|cpu] [cars] [port] [checkpoint.npy|-]\n"
+    "TRACK is flat, ramp, bank, bridge, ring10, ramp10, bank10, bridge8, random<k>, a map YAML, or a .npy track "
+    "array.";
+
+// The directory holding the package (this binary sits in it), for PYTHONPATH.
+static std::string package_parent() {
+    char buf[4096];
+    ssize_t len = readlink("/proc/self/exe", buf, sizeof(buf) - 1);
+    if (len <= 0) return ".";
+    std::string exe(buf, (size_t)len);
+    std::string dir = exe.substr(0, exe.rfind('/'));  // .../warporacer3d_amd
+    return dir.substr(0, dir.rfind('/'));
+}
+
+// Runs python3 -m <module> <args...> as a child (fork + exec in the child,
+// before this process has touched the GPU); returns its exit status.
+static int run_python(const std::string& module, const std::vector<std::string>& args) {
+    const char* py = std::getenv("RACER_PYTHON");
+    std::string python = py && *py ? py : "python3";
+    std::string path = package_parent();
+    const char* old = std::getenv("PYTHONPATH");
+    std::string pp = old && *old ? path + ":" + old : path;
+    pid_t pid = fork();
+    if (pid < 0) return 127;
+    if (pid == 0) {
+        setenv("PYTHONPATH", pp.c_str(), 1);
+        std::vector<char*> av;
+        av.push_back(const_cast<char*>(python.c_str()));
+        av.push_back(const_cast<char*>("-m"));
+        av.push_back(const_cast<char*>(module.c_str()));
+        for (const auto& a : args) av.push_back(const_cast<char*>(a.c_str()));
+        av.push_back(nullptr);
+        execvp(av[0], av.data());
+        _exit(127);
+    }
+    int status = 0;
+    if (waitpid(pid, &status, 0) < 0) return 127;
+    return WIFEXITED(status) ? WEXITSTATUS(status) : 128 + (WIFSIGNALED(status) ? WTERMSIG(status) : 0);
+}
 
 static int fail(const char* what) {
     std::fprintf(stderr, "error: %s: %s\n", what, racer_last_error());
@@ -76,19 +124,46 @@ static int env_int(const char* name, int def) {
 
 static std::string arg(int argc, char** argv, int i, const char* def) { return argc > i ? argv[i] : def; }
 
+static bool ends_with(const std::string& s, const char* tail) {
+    size_t n = std::strlen(tail);
+    return s.size() >= n && s.compare(s.size() - n, n, tail) == 0;
+}
+
+// TRACK as a float32 array: a .npy file directly, anything else through the track compiler.
+static bool load_track(const std::string& name, std::vector<float>& track, std::string& err) {
+    if (ends_with(name, ".npy")) return read_npy(name, track, err);
+    char tmpl[] = "/tmp/racer_track_XXXXXX.npy";
+    int fd = mkstemps(tmpl, 4);
+    if (fd < 0) return err = "cannot create a temporary file", false;
+    close(fd);
+    int rc = run_python("warporacer3d_amd.track", {name, tmpl});
+    bool ok = rc == 0 && read_npy(tmpl, track, err);
+    if (rc != 0) err = "the track compiler failed on " + name;
+    unlink(tmpl);
+    return ok;
+}
+
 int main(int argc, char** argv) {
     std::string mode = arg(argc, argv, 1, "");
-    if (argc < 3 || (mode != "train" && mode != "eval" && mode != "benchmark")) {
+    if (argc < 3 || (mode != "train" && mode != "eval" && mode != "benchmark" && mode != "serve")) {
         std::puts(USAGE);
         return argc < 3 ? 0 : 2;
     }
+    if (mode == "serve") {  // main.mojo:94-124 and racer/serve.py, in Python over the same library
+        std::vector<std::string> rest(argv + 2, argv + argc);
+        return run_python("warporacer3d_amd.serve", rest);
+    }
+    std::string choice = arg(argc, argv, 3, "auto");
+    if (choice == "cpu") {
+        std::fprintf(stderr, "error: libracer_hip runs on the GPU only; the CPU device is not provided\n");
+        return 1;
+    }
     std::vector<float> track;
     std::string err;
-    if (!read_npy(argv[2], track, err)) {
+    if (!load_track(argv[2], track, err)) {
         std::fprintf(stderr, "error: %s\n", err.c_str());
         return 1;
     }
-    std::string choice = arg(argc, argv, 3, "auto");
     int cars = std::atoi(arg(argc, argv, 4, "1024").c_str());
     int count = std::atoi(arg(argc, argv, 5, "1000").c_str());
     std::string path = arg(argc, argv, 6, "agent.npy");
@@ -103,7 +178,7 @@ int main(int argc, char** argv) {
     cfg.precision = prec && std::strcmp(prec, "bf16") == 0 ? 1 : 0;
 
     racer_device* dev = nullptr;
-    if (racer_device_create(choice == "cpu" ? 0 : 1, &dev)) return fail("device");
+    if (racer_device_create(1, &dev)) return fail("device");
     std::printf("Device: %s\n", racer_device_name(dev));
     racer_sim* sim = nullptr;
     racer_policy* pol = nullptr;

@@ -1617,6 +1617,23 @@ extern "C" racer_buf* racer_sim_actions(racer_sim* sim) { return sim->actions; }
 extern "C" racer_buf* racer_sim_reward(racer_sim* sim) { return sim->reward; }
 extern "C" racer_buf* racer_sim_done(racer_sim* sim) { return sim->done; }
 extern "C" int racer_sim_cars(const racer_sim* sim) { return sim->n; }
+
+// The viewer's snapshot (main.mojo:117-122): state, obs, actions, reward and
+// done, each feature-major, back to back in one host array; waits for the
+// step that produced them.
+extern "C" int racer_sim_snapshot(racer_sim* sim, float* host, size_t count) {
+    racer_clear_error();
+    RACER_REQUIRE(sim && host, "null argument");
+    const size_t n = (size_t)sim->n, need = (RACER_STATE + (size_t)(sim->beams + 5) + 4) * n;
+    RACER_REQUIRE(count >= need, "the snapshot needs (11 + beams + 5 + 4) x cars floats");
+    size_t at = 0;
+    for (racer_buf* b : {sim->state, sim->obs, sim->actions, sim->reward, sim->done}) {
+        RACER_TRY(hipMemcpyAsync(host + at, b->ptr, b->len * sizeof(float), hipMemcpyDeviceToHost, sim->dev->stream));
+        at += b->len;
+    }
+    RACER_TRY(hipStreamSynchronize(sim->dev->stream));
+    return 0;
+}
 extern "C" int racer_sim_beams(const racer_sim* sim) { return sim->beams; }
 
 extern "C" int racer_sim_count_iterations(racer_sim* sim, int on) {
