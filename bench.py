@@ -149,7 +149,7 @@ def ppo_leg(R, T, dev, beams: int, iterations: int, cars: int = 16384, precision
     pol = R.Policy(dev, 42, cfg)
     acts = R.Activations(dev, cars, cfg)
     tr = R.Trainer(dev, cars, cfg)
-    waves_by_replay(dev, sim.set_waves_per_car, lambda k: tr.capture_rollout(sim, pol, acts), 0)
+    R.autotune_runner(sim, pol, acts, dev.alloc(2 * cars))  # launch shapes of the rollout's steps
     graph = tr.capture_rollout(sim, pol, acts)
     tr.rollout_graph(graph, 0)  # untimed first iteration
     tr.update(pol)
@@ -243,30 +243,6 @@ BATCH_TRACKS = ("ring10", "ramp10", "bank10", "bridge8", "random0", "random1", "
 BATCH_CARS = (256, 512, 1024, 2048, 4096, 4096, 8192, 8192)
 
 
-def waves_by_replay(dev, set_waves, capture, t: int, steps: int = 10) -> tuple[int, int]:
-    """The lidar's wavefronts per car (1, 2 or 4) with the fastest replay of a
-    short captured graph of the real steps (the lidar-only autotuner times
-    back-to-back lidar launches, whose cache state differs from a step's).
-    Returns the choice and the next free step index."""
-    best, pick = None, 2
-    for w in (1, 2, 4):
-        set_waves(w)
-        g = capture(steps)
-        g.launch(t)
-        t += steps
-        dev.synchronize()
-        t0 = time.perf_counter()
-        g.launch(t)
-        t += steps
-        dev.synchronize()
-        dt = time.perf_counter() - t0
-        if best is None or dt < best:
-            best, pick = dt, w
-        del g
-    set_waves(pick)
-    return pick, t
-
-
 def batch_leg(R, T, dev, steps: int) -> dict:
     """Eight uneven instances (flat, ramp, bank, bridge and four random routes;
     256 .. 8192 cars, 1081 beams, one 2 x 256 bf16 policy): Runner steps of all
@@ -290,7 +266,8 @@ def batch_leg(R, T, dev, steps: int) -> dict:
     batch = R.Batch(dev, sims, cfg)
     for t in range(5):
         batch.runner_step(pol, False, t)
-    wpc, t = waves_by_replay(dev, batch.set_waves_per_car, lambda k: batch.capture(pol, k), 5)
+    wpc, t = batch.autotune(), 5
+    log("batch lidar wavefronts per car (library autotuner):", wpc)
     graph = batch.capture(pol, chunk)
     graph.launch(t)
     dev.synchronize()
@@ -308,9 +285,7 @@ def batch_leg(R, T, dev, steps: int) -> dict:
         scratch = dev.alloc(2 * s.n)
         for t in range(5):
             R.runner_step(s, pol, acts, scratch, False, t)
-        _, t_s = waves_by_replay(dev, s.set_waves_per_car,
-                                 lambda k, s=s, acts=acts, scratch=scratch: R.capture_runner(s, pol, acts, scratch, k), 5)
-        t_own = max(t_own, t_s)
+        R.autotune_runner(s, pol, acts, scratch)
         own.append(R.capture_runner(s, pol, acts, scratch, chunk))
     for g in own:
         g.launch(t_own)
@@ -464,11 +439,25 @@ def plain_baseline(R, dev, track, args, steps: int) -> dict:
             "schedule": "physics 1 thread/car; lidar 1 thread/(car, beam); 3 float32 products; sample kernel"}
 
 
+ACT_NAMES = {0: "auto", 1: "fused", 2: "wide", 3: "tiled"}
+SIM_KERNELS = ("physics", "lidar", "step_fused", "lidar_flat")
+
+
+def physics_bytes(cars: int, layers: int) -> float:
+    """Algorithmic bytes of the physics of one step: the state read and written
+    (11 floats each way), the two actions, reward and done, five obs rows, and
+    per car five probes of five packed records (8 B per layer), the nearest
+    waypoint cell and the waypoint row it names."""
+    return cars * (88.0 + 8.0 + 8.0 + 20.0 + 5 * 5 * 8.0 * layers + 4.0 * layers + 16.0)
+
+
 def runner_leg(R, dev, arr, args, precision: str, layers: int) -> dict:
     """Runner steps (policy forward with sample, physics, lidar) at the
-    benchmark's size: the timed region is `steps` steps replayed from a
-    captured HIP graph, bracketed by device synchronizations; kernel durations
-    come from HIP events in eager passes of the same steps right after."""
+    benchmark's size: the launch shapes come from the library's autotuner
+    (graph replays of every choice from one saved state), the timed region is
+    `steps` steps replayed from a captured HIP graph, bracketed by device
+    synchronizations; kernel durations come from HIP events in eager passes of
+    the same steps right after."""
     import numpy as np
 
     cfg = R.Config(beams=args.beams, hidden=args.hidden, precision=1 if precision == "bf16" else 0,
@@ -477,8 +466,11 @@ def runner_leg(R, dev, arr, args, precision: str, layers: int) -> dict:
     for t in range(args.warmup):
         run.step(False, t)
     dev.synchronize()
-    waves_per_car, t = waves_by_replay(dev, run.sim.set_waves_per_car, run.capture, args.warmup)
-    log(f"[{precision}] lidar wavefronts per car (fastest graph replay of 1, 2, 4):", waves_per_car)
+    tune = run.autotune(10)
+    t = args.warmup
+    log(f"[{precision}] launch shapes (library autotuner): lidar wavefronts per car {tune['waves_per_car']}, "
+        f"{'fused' if tune['fused_step'] else 'two-launch'} sim step, {ACT_NAMES[tune['act_kernel']]} forward")
+    sim_kernel = "step_fused" if run.sim.fused_step else "lidar"
     # march iterations of the lidar, counted by the instrumented variant on the warm state (untimed)
     run.sim.count_iterations(True)
     counts, trips = [], []
@@ -505,10 +497,10 @@ def runner_leg(R, dev, arr, args, precision: str, layers: int) -> dict:
     log(f"[{precision}] graph replay: {ms:.4f} ms/step, {value:.4g} env-steps/s")
     assert np.isfinite(run.sim.obs.read()).all(), "non-finite observations"
     del graph
-    # The same number of eager steps with HIP events around each lidar launch
-    # only (one event pair per step): the lidar's launch duration.
+    # The same number of eager steps with HIP events around the dominant sim
+    # launch only (one event pair per step): its launch duration.
     dev.profile_reset()
-    dev.profile_only("lidar")
+    dev.profile_only(sim_kernel)
     dev.profile(True)
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -517,64 +509,60 @@ def runner_leg(R, dev, arr, args, precision: str, layers: int) -> dict:
     eager_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
     dev.profile(False)
     t += args.steps
-    lidar_n, lidar_ms = dev.profile_results().get("lidar", (0, 0.0))
-    lidar_us = lidar_ms / max(lidar_n, 1) * 1000.0
-    # the same for the policy forward (with the sample) alone
-    policy_kernel = "policy_fused_sample" if precision == "bf16" else "policy_f32_sample"
-    dev.profile_reset()
-    dev.profile_only(policy_kernel)
-    dev.profile(True)
-    for k in range(args.steps):
-        run.step(False, t + k)
-    dev.synchronize()
-    dev.profile(False)
-    t += args.steps
-    pol_n, pol_ms = dev.profile_results().get(policy_kernel, (0, 0.0))
-    pol_us = pol_ms / max(pol_n, 1) * 1000.0
-    log(f"[{precision}] eager steps: {eager_ms:.4f} ms/step; lidar {lidar_us:.2f} us, {policy_kernel} "
-        f"{pol_us:.2f} us")
+    sim_n, sim_ms = dev.profile_results().get(sim_kernel, (0, 0.0))
+    sim_us = sim_ms / max(sim_n, 1) * 1000.0
     # untimed: iteration count on the final state, to bracket the measured steps
     run.sim.count_iterations(True)
     run.step(False, t)
     t += 1
     counts.append(run.sim.last_iterations)
     run.sim.count_iterations(False)
-    # every kernel, in a shorter profiled pass (an event pair per launch)
+    # every kernel, in a shorter profiled pass (an event pair per launch); the
+    # forward's time per step is that of every launch that is not the sim's
     dev.profile_only(None)
     dev.profile_reset()
     dev.profile(True)
-    for k in range(min(args.steps, 100)):
+    prof_steps = min(args.steps, 100)
+    for k in range(prof_steps):
         run.step(False, t + k)
     dev.synchronize()
     dev.profile(False)
     prof = dev.profile_results()
     for k, (n, tot) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
         log(f"  {k:20s} launches {n:6d} avg {tot / max(n, 1) * 1000:9.2f} us  total {tot:9.2f} ms")
+    policy_names = sorted(k for k in prof if k not in SIM_KERNELS)
+    pol_us = sum(tot for k, (n, tot) in prof.items() if k not in SIM_KERNELS) * 1000.0 / prof_steps
+    log(f"[{precision}] eager steps: {eager_ms:.4f} ms/step; {sim_kernel} {sim_us:.2f} us, forward "
+        f"({' + '.join(policy_names)}) {pol_us:.2f} us")
     iters = float(np.mean(counts))
     lbytes = lidar_bytes(iters, args.cars, args.beams, layers)
+    if sim_kernel == "step_fused":
+        lbytes += physics_bytes(args.cars, layers)
     flops = policy_flops(args.cars, args.beams, args.hidden)
     traffic = None
     tfile = ROOT / "profiles" / "lidar_traffic.json"
-    if tfile.exists():
+    if tfile.exists() and sim_kernel == "lidar":
         try:
             traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
         except Exception:
             traffic = None
     roofline = {
-        "bound": "hbm", "kernel": "lidar", "achieved": lbytes / (lidar_us * 1e-6) / 1e9, "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": lbytes / (lidar_us * 1e-6) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
+        "bound": "hbm", "kernel": sim_kernel, "achieved": lbytes / (sim_us * 1e-6) / 1e9, "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": lbytes / (sim_us * 1e-6) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
         "algorithmic_bytes_per_launch": lbytes, "march_iterations_per_launch": iters,
-        "avg_launch_us": lidar_us,
+        "avg_launch_us": sim_us,
     }
     peak = MFMA_BF16_PEAK_TFLOPS if precision == "bf16" else FP32_PEAK_TFLOPS
     achieved = flops / (pol_us * 1e-6) / 1e12 if pol_us > 0 else None
-    policy = {"bound": "mfma", "kernel": policy_kernel, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-              "frac": achieved / peak if achieved else None, "flops_per_launch": flops, "avg_launch_us": pol_us,
+    policy = {"bound": "mfma", "kernel": "+".join(policy_names), "achieved": achieved, "peak": peak,
+              "unit": "TFLOP/s", "frac": achieved / peak if achieved else None, "flops_per_launch": flops,
+              "avg_launch_us": pol_us,
               "peak_is": "dense bf16 MFMA" if precision == "bf16" else
                          "dense float32 MFMA (the K=1 float32 MFMA the kernel runs on)"}
     kernels = {k: {"launches": n, "avg_us": tot / max(n, 1) * 1000.0} for k, (n, tot) in prof.items()}
     return {"value": value, "ms_per_step": ms, "eager_ms_per_step": eager_ms, "roofline": roofline,
-            "policy_roofline": policy, "kernels": kernels, "lidar_waves_per_car": waves_per_car,
+            "policy_roofline": policy, "kernels": kernels, "lidar_waves_per_car": tune["waves_per_car"],
+            "fused_step": tune["fused_step"], "act_kernel": ACT_NAMES[tune["act_kernel"]],
             "lidar_wavefront_trips": trips}
 
 
@@ -667,14 +655,16 @@ def main() -> int:
                                "products (bitwise the CPU restatement), float32 dynamics and lidar",
                    "cars": args.cars, "beams": args.beams, "hidden": args.hidden, "track": args.track,
                    "policy_precision": "fp32", "parallelism": "single GPU",
-                   "lidar_waves_per_car": fp32["lidar_waves_per_car"]},
+                   "lidar_waves_per_car": fp32["lidar_waves_per_car"], "fused_step": fp32["fused_step"],
+                   "act_kernel": fp32["act_kernel"]},
         "eager_ms_per_step": fp32["eager_ms_per_step"],
         "timing": "graph replays of captured Runner steps; kernel durations from HIP events in eager passes of "
                   "the same steps right after",
         "roofline": fp32["roofline"], "policy_roofline": fp32["policy_roofline"], "kernels": fp32["kernels"],
         "value_bf16": bf16["value"] if bf16 else None,
         "bf16": ({k: bf16[k] for k in ("value", "ms_per_step", "eager_ms_per_step", "roofline", "policy_roofline",
-                                       "kernels", "lidar_waves_per_car")} if bf16 else None),
+                                       "kernels", "lidar_waves_per_car", "fused_step", "act_kernel")}
+                 if bf16 else None),
         "cpu_baseline": cpu,
         "plain_gpu_baseline": plain, "speedup_vs_plain_gpu": value / plain["value"] if plain else None,
         "speedup_bf16_vs_plain_gpu": bf16["value"] / plain["value"] if (plain and bf16) else None,

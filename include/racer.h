@@ -102,6 +102,10 @@ int racer_sim_step(racer_sim* sim, const racer_view* actions, const racer_view* 
 /* Sim.sample (sim.mojo:323-348): out [3, n], log_std [1, 2]. */
 int racer_sim_sample(racer_sim* sim, const racer_view* out, const racer_view* log_std, const racer_view* actions,
                      const racer_view* logp, const racer_view* value, int deterministic, int64_t step);
+/* The lidar pass of Sim.step alone (the second launch of sim.mojo:314-321):
+ * every beam of every car from the current state and the obs slope rows
+ * (2, 3) into the sim's obs range rows. */
+int racer_sim_scan(racer_sim* sim);
 racer_buf* racer_sim_state(racer_sim* sim);   /* [11, n] */
 racer_buf* racer_sim_obs(racer_sim* sim);     /* [beams+5, n] */
 racer_buf* racer_sim_actions(racer_sim* sim); /* [2, n] */
@@ -191,6 +195,33 @@ int racer_runner_step(racer_sim* sim, racer_policy* pol, racer_acts* acts, racer
 int racer_benchmark(racer_sim* sim, racer_policy* pol, racer_acts* acts, racer_buf* scratch, int steps,
                     double* ms_per_step);
 
+/* Launch-shape autotuning (no counterpart in the reference, whose launch
+ * shapes are fixed in racer/device.mojo:56-98). Every choice gives bitwise the
+ * same results; only the time differs.
+ *
+ * The forward kernel of racer_policy_act (the Runner step's and the
+ * rollout's forward with the sample): RACER_ACT_AUTO picks by size;
+ * RACER_ACT_FUSED is the one-kernel forward of 16 columns per workgroup (bf16,
+ * or float32 on K = 1 MFMA); RACER_ACT_WIDE its bf16 variant for many
+ * workgroups per CU; RACER_ACT_TILED the 64-column tiled bf16 forward (or the
+ * float32 products) followed by the sample kernel. A choice that does not
+ * apply to a launch falls back to RACER_ACT_AUTO for it. */
+#define RACER_ACT_AUTO 0
+#define RACER_ACT_FUSED 1
+#define RACER_ACT_WIDE 2
+#define RACER_ACT_TILED 3
+int racer_policy_set_act_kernel(racer_policy* pol, int kernel);
+int racer_policy_act_kernel(const racer_policy* pol);
+/* Times `steps` Runner steps replayed from a captured graph for every
+ * combination of lidar wavefronts per car (1, 2, 4), fused or two-launch sim
+ * step (where the fused one applies) and forward kernel (those that apply to
+ * the policy's precision), from the same saved sim state each time (best of
+ * three replays); keeps the fastest in sim and pol and restores the sim's
+ * state, obs, actions, reward and done. Writes the choice (any pointer may be
+ * null). Not while capturing; the profiler must be off. */
+int racer_runner_autotune(racer_sim* sim, racer_policy* pol, racer_acts* acts, racer_buf* scratch, int steps,
+                          int* waves_per_car, int* fused_step, int* act_kernel);
+
 /* ---------------- Batches (no reference counterpart) ---------------- */
 /* The reference steps one Sim per launch (racer/sim.mojo:314-321). A batch
  * groups 1 .. 16 existing sims (any tracks and car counts, one beam count) and
@@ -211,8 +242,9 @@ int racer_batch_step(racer_batch* b);
  * probabilities, then values) start at 2 (n_0 + .. + n_(k-1)). */
 int racer_batch_runner_step(racer_batch* b, racer_policy* pol, int deterministic, int64_t t);
 racer_buf* racer_batch_scratch(racer_batch* b);
-/* The batched lidar's wavefronts per car (1, 2, 4), its autotuner, and its
- * march-iteration counter (as the Sim ones). */
+/* The batched lidar's wavefronts per car (1, 2, 4), its autotuner (replays
+ * of captured graphs of batched sim steps from the same saved states, as
+ * racer_sim_autotune), and its march-iteration counter (as the Sim ones). */
 int racer_batch_set_waves_per_car(racer_batch* b, int waves);
 int racer_batch_waves_per_car(const racer_batch* b);
 int racer_batch_autotune(racer_batch* b, int* chosen);
@@ -334,9 +366,20 @@ int racer_sim_schedule(const racer_sim* sim);
  * results are identical for every choice). */
 int racer_sim_set_waves_per_car(racer_sim* sim, int waves);
 int racer_sim_waves_per_car(const racer_sim* sim);
-/* Autotuner: times the lidar with 1, 2 and 4 wavefronts per car on the current
- * state and keeps the fastest; writes the choice. Call after reset or a step,
- * not while capturing. */
+/* One launch for the whole step (physics, then the lidar, in the same
+ * workgroups, with each car's neighbourhood of the track staged in LDS) in
+ * place of the two launches of racer/sim.mojo:314-321. It applies with the
+ * default lidar schedule, one or two track layers and 1 or 2 wavefronts per
+ * car; racer_sim_fused_step returns 1 when the next step will run fused.
+ * Results are identical either way. */
+int racer_sim_set_fused_step(racer_sim* sim, int on);
+int racer_sim_fused_step(const racer_sim* sim);
+/* Autotuner: times `racer_sim_step`s replayed from captured graphs with 1, 2
+ * and 4 wavefronts per car, each fused and two-launch where the fused step
+ * applies, from the same saved state each time; keeps the fastest (waves per
+ * car written to `chosen`, the fused choice read by racer_sim_fused_step) and
+ * restores the sim's state, obs, actions, reward and done. Not while
+ * capturing; the profiler must be off. */
 int racer_sim_autotune(racer_sim* sim, int* chosen);
 
 #ifdef __cplusplus

@@ -209,7 +209,7 @@ def test_lidar_quotient_paths_bit_identical(dev, name):
     obs[2, 28:32] = 2.0**19  # steep beams inside: reciprocal, with small divisors
     g.state.write(s)
     g.obs.write(obs)
-    g.autotune()  # runs the lidar alone on the current state
+    g.scan()  # the lidar alone on the current state
     got = g.obs.read().reshape(beams + 5, n)[4:4 + beams]
     tr, s, obs = O.f32(track(name)), O.f32(s), O.f32(obs)
     want = np.empty((beams, n), np.float32)

@@ -28,6 +28,8 @@ CRASH, TIMEOUT = 1, 2
 RANGE = 10.0
 FOV = 4.71238898
 HORIZON = 3000
+# Forward kernels of Runner steps and rollouts (Policy.set_act_kernel; racer.h RACER_ACT_*).
+ACT_AUTO, ACT_FUSED, ACT_WIDE, ACT_TILED = range(4)
 SPEED_MAX = 5.0
 STEER_MAX = 0.4189
 GAMMA, LAMBDA = 0.99, 0.95
@@ -154,6 +156,13 @@ def lib() -> C.CDLL:
         "racer_sim_set_schedule": (i, [vp, i]),
         "racer_sim_set_waves_per_car": (i, [vp, i]),
         "racer_sim_waves_per_car": (i, [vp]),
+        "racer_sim_set_fused_step": (i, [vp, i]),
+        "racer_sim_scan": (i, [vp]),
+        "racer_policy_set_act_kernel": (i, [vp, i]),
+        "racer_policy_act_kernel": (i, [vp]),
+        "racer_runner_autotune": (i, [vp, vp, vp, vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
+        "racer_sim_fused_step": (i, [vp]),
         "racer_sim_autotune": (i, [vp, C.POINTER(C.c_int)]),
         "racer_profile_only": (i, [vp, C.c_char_p]),
         "racer_sim_schedule": (i, [vp]),
@@ -419,8 +428,22 @@ class Sim:
     def waves_per_car(self) -> int:
         return int(lib().racer_sim_waves_per_car(self.handle))
 
+    def scan(self) -> None:
+        """The lidar pass alone on the current state and obs slope rows (into obs)."""
+        _check(lib().racer_sim_scan(self.handle))
+
+    def set_fused_step(self, on: bool) -> None:
+        """Run physics and lidar as one launch (k_step_fused) where it applies; results are identical."""
+        _check(lib().racer_sim_set_fused_step(self.handle, int(on)))
+
+    @property
+    def fused_step(self) -> bool:
+        """Whether the next step runs as one launch."""
+        return bool(lib().racer_sim_fused_step(self.handle))
+
     def autotune(self) -> int:
-        """Time the lidar with 1, 2 and 4 wavefronts per car; keep and return the fastest."""
+        """Time replays of captured sim steps with 1, 2 and 4 wavefronts per car, fused and two-launch,
+        from the same saved state; keep the fastest (restoring the state) and return its wavefronts."""
         w = C.c_int()
         _check(lib().racer_sim_autotune(self.handle, C.byref(w)))
         return int(w.value)
@@ -505,6 +528,15 @@ class Policy:
     def backward(self, x: View, acts: Activations, d3: View, d2: Buffer, d1: Buffer, grad: Buffer) -> None:
         _check(lib().racer_policy_backward(self.handle, x.c, acts.handle, d3.c, d2.handle, d1.handle, grad.handle))
 
+    def set_act_kernel(self, kernel: int) -> None:
+        """The forward kernel of Runner steps and rollouts: ACT_AUTO, ACT_FUSED, ACT_WIDE or ACT_TILED
+        (identical results)."""
+        _check(lib().racer_policy_set_act_kernel(self.handle, int(kernel)))
+
+    @property
+    def act_kernel(self) -> int:
+        return int(lib().racer_policy_act_kernel(self.handle))
+
     def backward_noalloc(self, x: View, acts: Activations, d3: View, d2: Buffer, d1: Buffer, grad: Buffer) -> None:
         """backward with the split-K partials in the GPU's fixed 16 MiB scratch: no allocation."""
         _check(lib().racer_policy_backward_noalloc(self.handle, x.c, acts.handle, d3.c, d2.handle, d1.handle,
@@ -583,6 +615,17 @@ def runner_step(sim: "Sim", policy: "Policy", acts: "Activations", scratch: Buff
     sim.actions and scratch (log probabilities, then values), physics, lidar."""
     _check(lib().racer_runner_step(sim.handle, policy.handle, acts.handle, scratch.handle, int(deterministic),
                                    int(t)))
+
+
+def autotune_runner(sim: "Sim", policy: "Policy", acts: "Activations", scratch: Buffer, steps: int = 8) -> dict:
+    """Time `steps` Runner steps replayed from graphs for every launch-shape
+    choice (lidar wavefronts per car, fused sim step, forward kernel) from the
+    same saved sim state; keep the fastest in sim and policy and restore the
+    sim. Returns {"waves_per_car", "fused_step", "act_kernel"}."""
+    w, f, k = C.c_int(), C.c_int(), C.c_int()
+    _check(lib().racer_runner_autotune(sim.handle, policy.handle, acts.handle, scratch.handle, int(steps),
+                                       C.byref(w), C.byref(f), C.byref(k)))
+    return {"waves_per_car": int(w.value), "fused_step": bool(f.value), "act_kernel": int(k.value)}
 
 
 def capture_runner(sim: "Sim", policy: "Policy", acts: "Activations", scratch: Buffer, steps: int,
@@ -792,6 +835,10 @@ class Runner:
     def step(self, deterministic: bool, t: int) -> None:
         _check(lib().racer_runner_step(self.sim.handle, self.policy.handle, self.acts.handle, self.scratch.handle,
                                        int(deterministic), int(t)))
+
+    def autotune(self, steps: int = 8) -> dict:
+        """autotune_runner on this runner's objects."""
+        return autotune_runner(self.sim, self.policy, self.acts, self.scratch, steps)
 
     def capture(self, steps: int, deterministic: bool = False) -> Graph:
         """`steps` Runner steps as one HIP graph; graph.launch(t0) runs steps

@@ -136,42 +136,3 @@ extern "C" int racer_batch_set_waves_per_car(racer_batch* b, int waves) {
 }
 extern "C" int racer_batch_waves_per_car(const racer_batch* b) { return b->waves_per_car; }
 
-// Times the batched lidar (which only rewrites range rows) with 1, 2 and 4
-// wavefronts per car on the current states; keeps and writes the fastest.
-extern "C" int racer_batch_autotune(racer_batch* b, int* chosen) {
-    racer_clear_error();
-    RACER_REQUIRE(!b->dev->capturing, "cannot autotune while capturing a graph");
-    racer_event_pair ev;
-    RACER_TRY(hipEventCreate(&ev.a));
-    RACER_TRY(hipEventCreate(&ev.b));
-    const bool counting = b->count_iterations;
-    const int before = b->waves_per_car;
-    b->count_iterations = false;
-    float best = 1e30f;
-    int pick = before;
-    auto time_each = [&]() -> int {
-        for (int w : {1, 2, 4}) {
-            b->waves_per_car = w;
-            if (racer_launch_batch_lidar(b)) return 1;  // warm
-            RACER_TRY(hipEventRecord(ev.a, b->dev->stream));
-            for (int r = 0; r < 3; ++r)
-                if (racer_launch_batch_lidar(b)) return 1;
-            RACER_TRY(hipEventRecord(ev.b, b->dev->stream));
-            RACER_TRY(hipEventSynchronize(ev.b));
-            float ms = 0.0f;
-            RACER_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
-            if (ms < best) {
-                best = ms;
-                pick = w;
-            }
-        }
-        return 0;
-    };
-    const int failed = time_each();
-    // on failure the batch keeps its previous count
-    b->count_iterations = counting;
-    b->waves_per_car = failed ? before : pick;
-    if (failed) return 1;
-    if (chosen) *chosen = pick;
-    return 0;
-}

@@ -37,7 +37,7 @@ int capture(racer_device* dev, racer_policy* pol, racer_graph* g, F body) {
     RACER_REQUIRE(!dev->capturing, "a capture is already in progress on this device");
     if (!dev->graph_base) RACER_TRY(hipMalloc(&dev->graph_base, sizeof(int64_t)));
     // weights first, outside the graph: a replay refreshes them itself when stale
-    if (racer_policy_refresh(pol)) return 1;
+    if (pol && racer_policy_refresh(pol)) return 1;
     RACER_TRY(hipStreamSynchronize(dev->stream));
     dev->capturing = true;
     dev->step_base = dev->graph_base;
@@ -112,6 +112,49 @@ extern "C" int racer_graph_batch(racer_batch* b, racer_policy* pol, int steps, i
     return 0;
 }
 
+// Sim steps (physics and lidar, no policy) as one graph, for the autotuners.
+int racer_graph_sim_steps(racer_sim* sim, int steps, racer_graph** out) {
+    *out = nullptr;
+    RACER_REQUIRE(steps >= 1, "a graph needs at least one step");
+    RACER_REQUIRE(!sim->count_iterations, "turn the lidar iteration counter off before capturing");
+    auto* g = new racer_graph();
+    g->dev = sim->dev;
+    racer_device_retain(g->dev);
+    int rc = capture(sim->dev, nullptr, g, [&]() -> int {
+        for (int t = 0; t < steps; ++t)
+            if (racer_launch_step(sim, sim->actions->ptr, sim->reward->ptr, sim->done->ptr, sim->obs->ptr, sim->n))
+                return 1;
+        return 0;
+    });
+    if (rc) {
+        racer_graph_destroy(g);
+        return 1;
+    }
+    *out = g;
+    return 0;
+}
+
+// Batched sim steps (racer_batch_step) as one graph, for the autotuners.
+int racer_graph_batch_steps(racer_batch* b, int steps, racer_graph** out) {
+    *out = nullptr;
+    RACER_REQUIRE(steps >= 1, "a graph needs at least one step");
+    RACER_REQUIRE(!b->count_iterations, "turn the lidar iteration counter off before capturing");
+    auto* g = new racer_graph();
+    g->dev = b->dev;
+    racer_device_retain(g->dev);
+    int rc = capture(b->dev, nullptr, g, [&]() -> int {
+        for (int t = 0; t < steps; ++t)
+            if (racer_launch_batch_step(b)) return 1;
+        return 0;
+    });
+    if (rc) {
+        racer_graph_destroy(g);
+        return 1;
+    }
+    *out = g;
+    return 0;
+}
+
 extern "C" int racer_graph_rollout(racer_trainer* tr, racer_sim* sim, racer_policy* pol, racer_acts* acts,
                                    racer_graph** out) {
     racer_clear_error();
@@ -136,7 +179,7 @@ extern "C" int racer_graph_rollout(racer_trainer* tr, racer_sim* sim, racer_poli
 extern "C" int racer_graph_launch(racer_graph* g, int64_t t0) {
     racer_clear_error();
     RACER_REQUIRE(g && g->exec, "not a graph");
-    if (racer_policy_refresh(g->policy)) return 1;
+    if (g->policy && racer_policy_refresh(g->policy)) return 1;
     if (set_base(g->dev, t0)) return 1;
     RACER_TRY(hipGraphLaunch(g->exec, g->dev->stream));
     return 0;

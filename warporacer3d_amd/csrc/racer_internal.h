@@ -167,6 +167,7 @@ struct racer_sim {
     racer_dmap map{};
     int schedule = 0;       // lidar launch: 0 wavefront per car with beam refill, 1 thread per (car, beam)
     int waves_per_car = 2;  // schedule 0: wavefronts sharing one car's beams (1, 2 or 4)
+    bool fused_step = false;  // physics and lidar in one launch where that applies
     bool count_iterations = false;
     unsigned long long* counter = nullptr;
     int64_t last_iterations = 0;
@@ -190,6 +191,7 @@ struct racer_policy {
     float *frag0_f32 = nullptr, *frag1_f32 = nullptr;
     bool f32_stale = true;
     uint64_t f32_gen = 0;
+    int act_kernel = RACER_ACT_AUTO;  // the forward of racer_policy_act (Runner steps, rollouts)
 };
 
 struct racer_acts {
@@ -267,6 +269,11 @@ int racer_launch_batch_lidar(racer_batch* b);
 int racer_policy_fused_batch(racer_policy* p, const racer_policy_table& tab, int deterministic, int64_t step);
 int racer_batch_runner_enqueue(racer_batch* b, racer_policy* p, int deterministic, int64_t t);
 
+// Graphs of `steps` sim steps (physics and lidar) or batched sim steps, for
+// the autotuners; replayed with racer_graph_launch.
+int racer_graph_sim_steps(racer_sim* sim, int steps, racer_graph** out);
+int racer_graph_batch_steps(racer_batch* b, int steps, racer_graph** out);
+
 // ---------------------------------------------------------------- surrogate (sim side)
 // Surrogate inputs (11 rows + a constant-1 row) of a sim's cars for the given
 // actions, normalised when mean / inv_std are given.
@@ -343,8 +350,12 @@ size_t racer_f32_fragment_floats(int K);
 int racer_pack_f32_fragments(racer_device* dev, const float* src, int64_t ld_src, int rows, int cols, float* dst);
 int racer_policy_f32(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
                      float* act, float* logp, float* value, int deterministic, int64_t step, int seed);
+// kernel: RACER_ACT_AUTO picks by the launch's size; RACER_ACT_FUSED the
+// 16-column ring kernel, RACER_ACT_WIDE its wide variant, RACER_ACT_TILED the
+// 64-column tiled kernel (without the sample only), where they apply.
 int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
-                       float* act, float* logp, float* value, int deterministic, int64_t step, int seed);
+                       float* act, float* logp, float* value, int deterministic, int64_t step, int seed,
+                       int kernel = RACER_ACT_AUTO);
 
 int racer_policy_forward_raw(racer_policy* pol, const float* x, int64_t x_cols, racer_acts* acts, int cols);
 int racer_policy_act(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, float* act,

@@ -732,7 +732,7 @@ static auto fused_kernel(int hs, bool vec, bool wide) {
 }
 
 int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
-                       float* act, float* logp, float* value, int deterministic, int64_t step, int seed) {
+                       float* act, float* logp, float* value, int deterministic, int64_t step, int seed, int kernel) {
     racer_device* dev = p->dev;
     RACER_REQUIRE(racer_fused_reach((int)p->IN, x_cols, cols),
                   "bf16 policy input exceeds the 32-bit offsets of the buffer loads (4 GiB)");
@@ -743,7 +743,8 @@ int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_ac
     dim3 grid((unsigned)((cols + FT - 1) / FT)), block(64 * WAVES);
     const int hs = unrolled_half((int)p->IN);
     const bool vec = x_cols % 4 == 0 && cols % 4 == 0 && (uintptr_t)x % 16 == 0;
-    if (!sample && vec && hs > 0 && cols >= TILED_MIN_COLS) {
+    const bool tiled_kernel = kernel == RACER_ACT_TILED || (kernel == RACER_ACT_AUTO && cols >= TILED_MIN_COLS);
+    if (!sample && vec && hs > 0 && tiled_kernel) {
         racer_timed timer(dev, "policy_tiled");
         const dim3 tgrid((unsigned)((cols + TC - 1) / TC));
         const size_t tshmem = tiled_layout(p->hidden).total;
@@ -752,11 +753,11 @@ int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_ac
                            a->o->ptr);
         return racer_launch_check("policy_tiled");
     }
-    const bool wide = vec && grid.x >= WIDE_MIN_TILES;
+    const bool wide = vec && (kernel == RACER_ACT_WIDE || (kernel != RACER_ACT_FUSED && grid.x >= WIDE_MIN_TILES));
     const size_t shmem = fused_layout(p->hidden, (int)p->IN, wide).total;
-    auto kernel = sample ? fused_kernel<true>(hs, vec, wide) : fused_kernel<false>(hs, vec, wide);
+    auto launch = sample ? fused_kernel<true>(hs, vec, wide) : fused_kernel<false>(hs, vec, wide);
     racer_timed timer(dev, sample ? "policy_fused_sample" : "policy_fused");
-    hipLaunchKernelGGL(kernel, grid, block, shmem, dev->stream, f0, f1, W2, x, x_cols, p->hidden, (int)p->IN,
+    hipLaunchKernelGGL(launch, grid, block, shmem, dev->stream, f0, f1, W2, x, x_cols, p->hidden, (int)p->IN,
                        (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr, log_std, act, (int64_t)cols, logp, value,
                        deterministic, step, dev->step_base, seed);
     return racer_launch_check("policy_fused");

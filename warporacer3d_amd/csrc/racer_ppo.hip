@@ -511,11 +511,18 @@ int racer_policy_forward_raw(racer_policy* p, const float* x, int64_t x_cols, ra
 // racer/ppo.mojo:192-202).
 int racer_policy_act(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, float* act,
                      float* logp, float* value, int deterministic, int64_t step, int seed) {
+    const int kernel = p->act_kernel;
     if (use_fused(p)) {
         if (refresh_bf16(p)) return 1;
-        return racer_policy_fused(p, x, x_cols, a, cols, true, act, logp, value, deterministic, step, seed);
+        if (kernel == RACER_ACT_TILED) {  // the tiled forward, then Sim.sample on its outputs
+            if (racer_policy_fused(p, x, x_cols, a, cols, false, nullptr, nullptr, nullptr, 0, 0, 0, kernel))
+                return 1;
+            return racer_launch_sample_raw(p->dev, cols, a->o->ptr, cols, p->theta->ptr + p->LOGSTD, act, cols, logp,
+                                           value, deterministic, step, seed);
+        }
+        return racer_policy_fused(p, x, x_cols, a, cols, true, act, logp, value, deterministic, step, seed, kernel);
     }
-    if (use_f32_fused(p, cols)) {
+    if (kernel != RACER_ACT_TILED && use_f32_fused(p, cols)) {
         if (refresh_f32(p)) return 1;
         return racer_policy_f32(p, x, x_cols, a, cols, true, act, logp, value, deterministic, step, seed);
     }

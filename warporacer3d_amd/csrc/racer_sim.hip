@@ -151,9 +151,21 @@ __device__ void spawn(const racer_dmap& m, float* s, int n, int i, int seed, flo
 // the track array layer by layer.
 // nearest (optional): Map.at(layer, NEAREST, x, y) of the probed layer, the
 // progress lookup's first read, loaded with the same round of loads.
-template <int L>
+//
+// TILE (L = 1 or 2): a copy of the records and nearest-waypoint cells of a
+// PHYS_TILE x PHYS_TILE window around the car sits in LDS; a probe whose five
+// cells all lie in the window reads them there, any other reads the track
+// array. The values are the same floats either way.
+constexpr int PHYS_TILE = 10;
+struct PhysTile {
+    const float2* rec;  // [PHYS_TILE * PHYS_TILE][L] records, row-major from (cx0, cy0)
+    const float* near;  // [L][PHYS_TILE * PHYS_TILE] nearest-waypoint cells
+    int cx0, cy0;       // the window's first cell
+};
+
+template <int L, bool TILE = false>
 __device__ __forceinline__ void probe_surface(const racer_dmap& m, float x, float y, float z, Spot* spot, Surf* g,
-                                              float* nearest = nullptr) {
+                                              float* nearest = nullptr, const PhysTile* tile = nullptr) {
     if constexpr (L == 0) {
         *spot = map_probe(m, x, y, z);
         *g = map_surface(m, spot->layer, x, y);
@@ -177,17 +189,47 @@ __device__ __forceinline__ void probe_surface(const racer_dmap& m, float x, floa
         const int k = iy * m.nx + ix;
         float2 pr[L], r0[L], r1[L], r2[L], r3[L];
         float nr[L];
+        bool inside = false;
+        int tc = 0, tk = 0;  // the probe cell and the first corner in the window
+        if constexpr (TILE) {
+            constexpr unsigned T = PHYS_TILE;
+            const int ux = ix - tile->cx0, uy = iy - tile->cy0;
+            // clamped before the conversion; only used when on the map
+            const int vx = (int)fminf(fmaxf(fxn, -1.0f), (float)m.nx) - tile->cx0;
+            const int vy = (int)fminf(fmaxf(fyn, -1.0f), (float)m.ny) - tile->cy0;
+            inside = ((unsigned)ux < T - 1) & ((unsigned)uy < T - 1) &
+                     (!on | (((unsigned)vx < T) & ((unsigned)vy < T)));
+            tc = on ? vy * (int)T + vx : 0;
+            tk = uy * (int)T + ux;
+        }
+        if (inside) {
+            constexpr int T = PHYS_TILE;
 #pragma unroll
-        for (int l = 0; l < L; ++l) {
-            const float2 v = rec[c * L + l];  // c = 0 off the map: a valid record, not used
-            pr[l] = on ? v : make_float2(-1.0f, -1.0f);
-            r0[l] = rec[k * L + l];
-            r1[l] = rec[(k + 1) * L + l];
-            r2[l] = rec[(k + m.nx) * L + l];
-            r3[l] = rec[(k + m.nx + 1) * L + l];
-            if (nearest) {
-                const float w = grid_at(m, l, RACER_NEAREST, c);
-                nr[l] = on ? w : -1.0f;
+            for (int l = 0; l < L; ++l) {
+                const float2 v = tile->rec[tc * L + l];
+                pr[l] = on ? v : make_float2(-1.0f, -1.0f);
+                r0[l] = tile->rec[tk * L + l];
+                r1[l] = tile->rec[(tk + 1) * L + l];
+                r2[l] = tile->rec[(tk + T) * L + l];
+                r3[l] = tile->rec[(tk + T + 1) * L + l];
+                if (nearest) {
+                    const float w = tile->near[l * T * T + tc];
+                    nr[l] = on ? w : -1.0f;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                const float2 v = rec[c * L + l];  // c = 0 off the map: a valid record, not used
+                pr[l] = on ? v : make_float2(-1.0f, -1.0f);
+                r0[l] = rec[k * L + l];
+                r1[l] = rec[(k + 1) * L + l];
+                r2[l] = rec[(k + m.nx) * L + l];
+                r3[l] = rec[(k + m.nx + 1) * L + l];
+                if (nearest) {
+                    const float w = grid_at(m, l, RACER_NEAREST, c);
+                    nr[l] = on ? w : -1.0f;
+                }
             }
         }
         Spot best{0, -1.0f, pr[0].y};
@@ -230,21 +272,36 @@ __device__ __forceinline__ void probe_surface(const racer_dmap& m, float x, floa
 template <int L>
 __device__ void finish(const racer_dmap& m, float* s, int n, float* reward, float* done, float* obs,
                        int64_t obs_cols, int beams, int i, int seed, float heading_noise, float p[4], float steer,
-                       const Spot& spot, Surf ground, bool crashed, float nearest);
+                       const Spot& spot, Surf ground, bool crashed, float nearest, float* pose = nullptr);
 
-// advance (racer/sim.mojo:184-235): one 60 Hz step of car i.
-template <int L>
-__device__ void advance(const racer_dmap& m, float* s, int n, const float* act, int64_t act_cols, float* reward,
-                        float* done, float* obs, int64_t obs_cols, int beams, int i, int seed, float heading_noise) {
+// What a step of car i reads before its first probe: the pose, steer and
+// friction rows of the state and the two action rows.
+struct CarIn {
+    float x, y, z, heading, speed, steer, friction, act_steer, act_throttle;
+};
+
+__device__ __forceinline__ CarIn load_car(const float* s, int n, const float* act, int64_t act_cols, int i) {
+    return {s[RACER_X * n + i],     s[RACER_Y * n + i],     s[RACER_Z * n + i],
+            s[RACER_HEADING * n + i], s[RACER_SPEED * n + i], s[RACER_STEER * n + i],
+            s[RACER_FRICTION * n + i], act[i],               act[act_cols + i]};
+}
+
+// advance (racer/sim.mojo:184-235): one 60 Hz step of car i from its inputs
+// `in`. TILE: the substeps' probes read the LDS window `tile` where it covers
+// them, and the new pose goes to `pose` (finish).
+template <int L, bool TILE = false>
+__device__ void advance(const racer_dmap& m, float* s, int n, const CarIn& in, float* reward, float* done,
+                        float* obs, int64_t obs_cols, int beams, int i, int seed, float heading_noise,
+                        const PhysTile* tile = nullptr, float* pose = nullptr) {
     const float DT = RACER_DT;
-    float p[4] = {s[RACER_X * n + i], s[RACER_Y * n + i], s[RACER_HEADING * n + i], s[RACER_SPEED * n + i]};
+    float p[4] = {in.x, in.y, in.heading, in.speed};
     Spot spot;
     Surf ground;
-    probe_surface<L>(m, p[0], p[1], s[RACER_Z * n + i], &spot, &ground);
-    float steer = s[RACER_STEER * n + i];
-    float rate = racer_clamp(act[i], -1.0f, 1.0f) * RACER_STEER_RATE;
-    float accel = racer_clamp(act[act_cols + i], -1.0f, 1.0f) * RACER_ACCEL_MAX;
-    float grip = RACER_GRIP * s[RACER_FRICTION * n + i];
+    probe_surface<L, TILE>(m, p[0], p[1], in.z, &spot, &ground, nullptr, tile);
+    float steer = in.steer;
+    float rate = racer_clamp(in.act_steer, -1.0f, 1.0f) * RACER_STEER_RATE;
+    float accel = racer_clamp(in.act_throttle, -1.0f, 1.0f) * RACER_ACCEL_MAX;
+    float grip = RACER_GRIP * in.friction;
     bool crashed = false;
     float nearest = -1.0f;  // the last probe's nearest waypoint (progress)
     for (int sub = 0; sub < RACER_SUBSTEPS; ++sub) {
@@ -267,23 +324,25 @@ __device__ void advance(const racer_dmap& m, float* s, int n, const float* act, 
         p[3] = racer_clamp(p[3], -RACER_SPEED_MAX, RACER_SPEED_MAX);
         steer = racer_clamp(full, -RACER_STEER_MAX, RACER_STEER_MAX);
         const float previous = spot.height;  // the re-probe uses the last spot's road height
-        probe_surface<L>(m, p[0], p[1], previous, &spot, &ground,
-                         sub == RACER_SUBSTEPS - 1 ? &nearest : nullptr);
+        probe_surface<L, TILE>(m, p[0], p[1], previous, &spot, &ground,
+                               sub == RACER_SUBSTEPS - 1 ? &nearest : nullptr, tile);
         crashed = crashed || spot.clearance < RACER_HALF_DIAG ||
                   ground.sx * ground.sx + ground.sy * ground.sy > 1.0f;
     }
     finish<L>(m, s, n, reward, done, obs, obs_cols, beams, i, seed, heading_noise, p, steer, spot, ground, crashed,
-              nearest);
+              nearest, pose);
 }
 
 // The end of advance (racer/sim.mojo:219-235) from the integrated pose p =
 // (x, y, heading, speed), steer, the last probe (with its nearest waypoint at
 // (x, y), read in the same round of loads) and the crash flag: heading wrap,
 // progress, reward, done code, state and return, respawn, obs rows.
+// pose (optional): heading, along, across, x, y, z after the step (and the
+// respawn), the lidar's inputs, also written there.
 template <int L>
 __device__ void finish(const racer_dmap& m, float* s, int n, float* reward, float* done, float* obs,
                        int64_t obs_cols, int beams, int i, int seed, float heading_noise, float p[4], float steer,
-                       const Spot& spot, Surf ground, bool crashed, float nearest) {
+                       const Spot& spot, Surf ground, bool crashed, float nearest, float* pose) {
     if (p[2] > RACER_PI) p[2] -= RACER_2PI;
     if (p[2] < -RACER_PI) p[2] += RACER_2PI;
     float progress = map_progress_at(m, nearest, p[0], p[1]);
@@ -317,6 +376,14 @@ __device__ void finish(const racer_dmap& m, float* s, int n, float* reward, floa
     obs[2 * obs_cols + i] = along;
     obs[3 * obs_cols + i] = across;
     obs[(int64_t)(4 + beams) * obs_cols + i] = 1.0f;
+    if (pose) {
+        pose[0] = s[RACER_HEADING * n + i];
+        pose[1] = along;
+        pose[2] = across;
+        pose[3] = s[RACER_X * n + i];
+        pose[4] = s[RACER_Y * n + i];
+        pose[5] = s[RACER_Z * n + i];
+    }
 }
 
 template <int L>
@@ -324,7 +391,9 @@ __global__ void __launch_bounds__(64) k_physics(racer_dmap m, float* s, int n, c
                                                 float* reward, float* done, float* obs, int64_t obs_cols, int beams,
                                                 int seed, float heading_noise) {
     int i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i < n) advance<L>(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, i, seed, heading_noise);
+    if (i < n)
+        advance<L>(m, s, n, load_car(s, n, act, act_cols, i), reward, done, obs, obs_cols, beams, i, seed,
+                   heading_noise);
 }
 
 // ---- learned dynamics surrogate (no reference counterpart) ----
@@ -441,7 +510,8 @@ __device__ __forceinline__ void surrogate_apply(const racer_dmap& m, float* s, i
     const bool back = !(spot.clearance >= min_clearance) || spot.layer != c.now.layer;
     if (fallback) fallback[i] = back ? 1.0f : 0.0f;
     if (back) {
-        advance<L>(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, i, seed, heading_noise);
+        advance<L>(m, s, n, load_car(s, n, act, act_cols, i), reward, done, obs, obs_cols, beams, i, seed,
+                   heading_noise);
         return;
     }
     const bool crashed = spot.clearance < RACER_HALF_DIAG || ground.sx * ground.sx + ground.sy * ground.sy > 1.0f;
@@ -877,11 +947,16 @@ constexpr size_t LIDAR_COOP_MAX_SHMEM = 80 * 1024;  // two such workgroups per C
 // stores them together after one barrier, each obs row as a run of
 // consecutive cars: partial 128-byte lines of 32 bytes (8 cars) instead of one
 // float per line from each wavefront. Otherwise each wavefront stores its own.
-template <int L, bool COUNT, bool COOP>
+//
+// POSE: the car's heading, along and across slopes and position come from
+// pose[8 * (car - wg_car0) ..] (LDS, written by the same workgroup's physics)
+// rather than the state and obs rows.
+template <int L, bool COUNT, bool COOP, bool POSE = false>
 __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __restrict__ s, int n,
                                            float* __restrict__ obs, int64_t obs_cols, int total_beams, int per_wave,
                                            int waves_per_car, const float2* __restrict__ dirs,
-                                           unsigned long long* counter, int global_wave, int wg_car0) {
+                                           unsigned long long* counter, int global_wave, int wg_car0,
+                                           const float* pose = nullptr) {
     extern __shared__ float lds[];
     const int lane = threadIdx.x & 63;
     // wavefront-uniform (the compiler cannot see it through threadIdx.x): the
@@ -903,13 +978,14 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     __amdgpu_buffer_rsrc_t rec = __builtin_amdgcn_make_buffer_rsrc(
         (void*)m.packed, (short)0, (int)((m.plane + 1) * (L > 0 ? L : 1) * 8), 0x00020000);
 
-    const float heading = s[RACER_HEADING * n + car];
-    const float along = obs[2 * obs_cols + car];
-    const float across = obs[3 * obs_cols + car];
+    const float* cp = POSE ? pose + 8 * (car - wg_car0) : nullptr;
+    const float heading = POSE ? cp[0] : s[RACER_HEADING * n + car];
+    const float along = POSE ? cp[1] : obs[2 * obs_cols + car];
+    const float across = POSE ? cp[2] : obs[3 * obs_cols + car];
     const float ch = racer_cos(heading), sh = racer_sin(heading);
-    const float ox = (s[RACER_X * n + car] + RACER_MOUNT_FORWARD * ch) + 0.0f;
-    const float oy = (s[RACER_Y * n + car] + RACER_MOUNT_FORWARD * sh) + 0.0f;
-    const float oz = (s[RACER_Z * n + car] + RACER_MOUNT_FORWARD * along) + RACER_MOUNT_UP;
+    const float ox = ((POSE ? cp[3] : s[RACER_X * n + car]) + RACER_MOUNT_FORWARD * ch) + 0.0f;
+    const float oy = ((POSE ? cp[4] : s[RACER_Y * n + car]) + RACER_MOUNT_FORWARD * sh) + 0.0f;
+    const float oz = ((POSE ? cp[5] : s[RACER_Z * n + car]) + RACER_MOUNT_FORWARD * along) + RACER_MOUNT_UP;
     const float slope15 = 1.5f * m.slope;
     const float half_cell = 0.5f * m.cell;
     // Every probe of the march lies within RACER_RANGE of the origin (t < RANGE,
@@ -1091,6 +1167,95 @@ __global__ void __launch_bounds__(COOP ? 1024 : 256) k_lidar(racer_dmap m, const
                                global_wave, wg_car0);
 }
 
+// One whole step (racer/sim.mojo:314-321, physics then lidar) in one launch,
+// on the cooperative lidar's workgroups of LIDAR_COOP_CARS cars.
+//
+// Each car's first wavefront loads a PHYS_TILE x PHYS_TILE window of records
+// and nearest-waypoint cells around the car into LDS (one round of loads;
+// cells off the map are left unset and never read) while lanes 0..7 of
+// wavefront 0 load the cars' inputs; after a barrier those lanes run the
+// physics of the workgroup's cars, whose substep probes read the windows (a
+// car moves at most 4 * 5 m/s / 240 Hz = 0.083 m a step, 3.4 cells of
+// 0.025 m, inside the window's four cells either side; a probe outside it
+// reads the track array). One wavefront rather than one lane in each car's
+// wavefront: the serial chain is issue-bound when many wavefronts of a SIMD
+// run it at once. The physics hands the new pose to the lidar through LDS;
+// after a second barrier every wavefront marches its share of the beams, over
+// LDS that the windows no longer need. Results equal k_physics then k_lidar.
+//
+// LDS: [0, pose_at): the windows (3 * PHYS_TILE^2 * L floats a car), then the
+// lidar's rings and ranges; [pose_at, pose_at + 8 * LIDAR_COOP_CARS): poses.
+template <int L>
+constexpr int phys_tile_floats() {
+    return 3 * PHYS_TILE * PHYS_TILE * L;
+}
+
+// (eight wavefronts per SIMD, the two workgroups a CU holds: the physics
+// alone would take more scalar registers than that allows)
+template <int L, bool COUNT>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) k_step_fused(racer_dmap m, float* s, int n, const float* act,
+                                                     float* reward, float* done, float* obs, int64_t obs_cols,
+                                                     int total_beams, int per_wave, int waves_per_car,
+                                                     const float2* __restrict__ dirs, unsigned long long* counter,
+                                                     int seed, float heading_noise, int pose_at) {
+    extern __shared__ float lds[];
+    constexpr int T = PHYS_TILE, CELLS = T * T * L;
+    const int block = xcd_block();
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wg_waves = (int)(blockDim.x >> 6);
+    const int global_wave = __builtin_amdgcn_readfirstlane((int)((int64_t)block * wg_waves + wave));
+    const int wg_car0 = (int)((int64_t)block * wg_waves / waves_per_car);
+    const int car = global_wave / waves_per_car;
+    float* pose = lds + pose_at;
+    // the window's first cell: four cells before the car's cell on each axis
+    auto window = [&](float x, float y, int* cx0, int* cy0) {
+        *cx0 = (int)fminf(fmaxf(floorf((x - m.x0) / m.cell), -0x1p20f), 0x1p20f) - 4;
+        *cy0 = (int)fminf(fmaxf(floorf((y - m.y0) / m.cell), -0x1p20f), 0x1p20f) - 4;
+    };
+    // wavefront 0, lane c < the workgroup's cars: car wg_car0 + c's inputs
+    const int mine = wg_car0 + lane;
+    const bool physics = wave == 0 && lane < LIDAR_COOP_CARS && mine < n;
+    CarIn in{};
+    if (physics) in = load_car(s, n, act, n, mine);
+    if (global_wave % waves_per_car == 0 && car < n) {
+        const int slot = car - wg_car0;
+        float2* trec = reinterpret_cast<float2*>(lds + slot * phys_tile_floats<L>());
+        float* tnear = lds + slot * phys_tile_floats<L>() + 2 * CELLS;
+        int cx0, cy0;
+        window(s[RACER_X * n + car], s[RACER_Y * n + car], &cx0, &cy0);
+        const float2* rec = reinterpret_cast<const float2*>(m.packed);
+#pragma unroll
+        for (int r = 0; r < (CELLS + 63) / 64; ++r) {
+            const int j = r * 64 + lane;
+            const int cell = j / L, l = j - cell * L;
+            const int cx = cx0 + cell % T, cy = cy0 + cell / T;
+            if (j < CELLS && (unsigned)cx < (unsigned)m.nx && (unsigned)cy < (unsigned)m.ny)
+                trec[j] = rec[(cy * m.nx + cx) * L + l];
+        }
+#pragma unroll
+        for (int r = 0; r < (CELLS + 63) / 64; ++r) {
+            const int j = r * 64 + lane;
+            const int l = j / (T * T), cell = j - l * T * T;
+            const int cx = cx0 + cell % T, cy = cy0 + cell / T;
+            if (j < CELLS && (unsigned)cx < (unsigned)m.nx && (unsigned)cy < (unsigned)m.ny)
+                tnear[j] = grid_at(m, l, RACER_NEAREST, cy * m.nx + cx);
+        }
+    }
+    __syncthreads();
+    if (physics) {
+        PhysTile tile;
+        tile.rec = reinterpret_cast<const float2*>(lds + lane * phys_tile_floats<L>());
+        tile.near = lds + lane * phys_tile_floats<L>() + 2 * CELLS;
+        window(in.x, in.y, &tile.cx0, &tile.cy0);
+        advance<L, true>(m, s, n, in, reward, done, obs, obs_cols, total_beams, mine, seed, heading_noise, &tile,
+                         pose + 8 * lane);
+    }
+    __syncthreads();
+    lidar_wave<L, COUNT, true, true>(m, s, n, obs, obs_cols, total_beams, per_wave, waves_per_car, dirs, counter,
+                                     global_wave, wg_car0, pose);
+}
+
 // The lidar of several sims in one launch: wavefront g belongs to the
 // instance whose range [start[k], start[k + 1]) of wavefronts holds g.
 template <bool COUNT>
@@ -1123,14 +1288,14 @@ __global__ void __launch_bounds__(64) k_physics_batch(racer_sim_table tab) {
     const racer_sim_slot& q = tab.slot[k];
     const int i = (int)(g - tab.start[k]);
     if (q.map.layers == 1)
-        advance<1>(q.map, q.state, q.n, q.actions, q.n, q.reward, q.done, q.obs, q.n, q.beams, i, q.seed,
-                   q.heading_noise);
+        advance<1>(q.map, q.state, q.n, load_car(q.state, q.n, q.actions, q.n, i), q.reward, q.done, q.obs, q.n,
+                   q.beams, i, q.seed, q.heading_noise);
     else if (q.map.layers == 2)
-        advance<2>(q.map, q.state, q.n, q.actions, q.n, q.reward, q.done, q.obs, q.n, q.beams, i, q.seed,
-                   q.heading_noise);
+        advance<2>(q.map, q.state, q.n, load_car(q.state, q.n, q.actions, q.n, i), q.reward, q.done, q.obs, q.n,
+                   q.beams, i, q.seed, q.heading_noise);
     else
-        advance<0>(q.map, q.state, q.n, q.actions, q.n, q.reward, q.done, q.obs, q.n, q.beams, i, q.seed,
-                   q.heading_noise);
+        advance<0>(q.map, q.state, q.n, load_car(q.state, q.n, q.actions, q.n, i), q.reward, q.done, q.obs, q.n,
+                   q.beams, i, q.seed, q.heading_noise);
 }
 
 // The reference's own lidar launch (racer/sim.mojo:317-321 through the 2-D
@@ -1254,7 +1419,9 @@ static void launch_lidar(racer_sim* sim, float* obs, int64_t obs_cols) {
 static int lidar_coop_setup() {
     const void* kernels[] = {(const void*)k_lidar<0, false, true>, (const void*)k_lidar<0, true, true>,
                              (const void*)k_lidar<1, false, true>, (const void*)k_lidar<1, true, true>,
-                             (const void*)k_lidar<2, false, true>, (const void*)k_lidar<2, true, true>};
+                             (const void*)k_lidar<2, false, true>, (const void*)k_lidar<2, true, true>,
+                             (const void*)k_step_fused<1, false>,  (const void*)k_step_fused<1, true>,
+                             (const void*)k_step_fused<2, false>,  (const void*)k_step_fused<2, true>};
     for (const void* k : kernels)
         RACER_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LIDAR_COOP_MAX_SHMEM));
     return 0;
@@ -1269,8 +1436,42 @@ static void launch_lidar_any(racer_sim* sim, float* obs, int64_t obs_cols) {
         launch_lidar<0>(sim, obs, obs_cols);
 }
 
-int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done, float* obs, int64_t obs_cols) {
-    int n = sim->n;
+// LDS of the fused step (k_step_fused) for this sim, or 0 when it does not
+// apply: the lidar's default schedule with cooperative workgroups, and
+// packed one- or two-layer records.
+static size_t step_fused_shmem(const racer_sim* sim, int* pose_at) {
+    const int layers = sim->map.layers;
+    const int coop_waves = LIDAR_COOP_CARS * sim->waves_per_car;
+    if (sim->schedule != 0 || (layers != 1 && layers != 2) || coop_waves > 16) return 0;
+    const size_t lidar = (size_t)coop_waves * LIDAR_RING + (size_t)LIDAR_COOP_CARS * sim->beams;
+    const size_t tiles = (size_t)LIDAR_COOP_CARS * (layers == 1 ? phys_tile_floats<1>() : phys_tile_floats<2>());
+    const size_t at = lidar > tiles ? lidar : tiles;
+    const size_t bytes = (at + 8 * LIDAR_COOP_CARS) * sizeof(float);
+    if (bytes > LIDAR_COOP_MAX_SHMEM) return 0;
+    *pose_at = (int)at;
+    return bytes;
+}
+
+template <int L>
+static void launch_step_fused(racer_sim* sim, float* actions, float* reward, float* done, float* obs,
+                              int64_t obs_cols, size_t shmem, int pose_at) {
+    const int wpc = sim->waves_per_car, per = (sim->beams + wpc - 1) / wpc;
+    const int coop_waves = LIDAR_COOP_CARS * wpc;
+    const int64_t total = (int64_t)sim->n * wpc;
+    dim3 grid((unsigned)((total + coop_waves - 1) / coop_waves)), block(64 * coop_waves);
+    if (sim->count_iterations)
+        hipLaunchKernelGGL((k_step_fused<L, true>), grid, block, shmem, sim->dev->stream, sim->map, sim->state->ptr,
+                           sim->n, actions, reward, done, obs, obs_cols, sim->beams, per, wpc, sim->dirs, sim->counter,
+                           sim->seed, sim->heading_noise, pose_at);
+    else
+        hipLaunchKernelGGL((k_step_fused<L, false>), grid, block, shmem, sim->dev->stream, sim->map, sim->state->ptr,
+                           sim->n, actions, reward, done, obs, obs_cols, sim->beams, per, wpc, sim->dirs, sim->counter,
+                           sim->seed, sim->heading_noise, pose_at);
+}
+
+// k_physics then the lidar (racer/sim.mojo:314-321).
+static int launch_pair(racer_sim* sim, float* actions, float* reward, float* done, float* obs, int64_t obs_cols) {
+    const int n = sim->n;
     {
         racer_timed timer(sim->dev, "physics");
         auto kernel = sim->map.layers == 1 ? k_physics<1> : (sim->map.layers == 2 ? k_physics<2> : k_physics<0>);
@@ -1290,7 +1491,26 @@ int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done
         racer_timed timer(sim->dev, "lidar");
         launch_lidar_any(sim, obs, obs_cols);
     }
-    if (racer_launch_check("lidar")) return 1;
+    return racer_launch_check("lidar");
+}
+
+int racer_launch_step(racer_sim* sim, float* actions, float* reward, float* done, float* obs, int64_t obs_cols) {
+    int pose_at = 0;
+    const size_t fused_shmem = sim->fused_step ? step_fused_shmem(sim, &pose_at) : 0;
+    if (fused_shmem) {
+        if (sim->count_iterations)
+            RACER_TRY(hipMemsetAsync(sim->counter, 0, 2 * sizeof(unsigned long long), sim->dev->stream));
+        {
+            racer_timed timer(sim->dev, "step_fused");
+            if (sim->map.layers == 1)
+                launch_step_fused<1>(sim, actions, reward, done, obs, obs_cols, fused_shmem, pose_at);
+            else
+                launch_step_fused<2>(sim, actions, reward, done, obs, obs_cols, fused_shmem, pose_at);
+        }
+        if (racer_launch_check("fused step")) return 1;
+    } else if (launch_pair(sim, actions, reward, done, obs, obs_cols)) {
+        return 1;
+    }
     if (sim->count_iterations) {
         unsigned long long count[2] = {0, 0};
         RACER_TRY(hipMemcpyAsync(count, sim->counter, sizeof(count), hipMemcpyDeviceToHost, sim->dev->stream));
@@ -1595,6 +1815,31 @@ extern "C" int racer_sim_step(racer_sim* sim, const racer_view* actions_p, const
     return racer_launch_step(sim, view_ptr(actions), view_ptr(reward), view_ptr(done), view_ptr(obs), n);
 }
 
+// The second launch of Sim.step (racer/sim.mojo:317-321) alone: every beam of
+// every car from the current state and obs slope rows, into the sim's obs.
+extern "C" int racer_sim_scan(racer_sim* sim) {
+    racer_clear_error();
+    RACER_REQUIRE(sim, "null argument");
+    if (sim->count_iterations)
+        RACER_TRY(hipMemsetAsync(sim->counter, 0, 2 * sizeof(unsigned long long), sim->dev->stream));
+    if (sim->schedule == 1) {
+        const int64_t threads = (int64_t)sim->n * sim->beams;
+        hipLaunchKernelGGL(k_lidar_flat, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, sim->dev->stream,
+                           sim->map, sim->state->ptr, sim->n, sim->obs->ptr, (int64_t)sim->n, sim->beams);
+    } else {
+        launch_lidar_any(sim, sim->obs->ptr, sim->n);
+    }
+    if (racer_launch_check("lidar")) return 1;
+    if (sim->count_iterations) {
+        unsigned long long count[2] = {0, 0};
+        RACER_TRY(hipMemcpyAsync(count, sim->counter, sizeof(count), hipMemcpyDeviceToHost, sim->dev->stream));
+        RACER_TRY(hipStreamSynchronize(sim->dev->stream));
+        sim->last_iterations = (int64_t)count[0];
+        sim->last_trips = (int64_t)count[1];
+    }
+    return 0;
+}
+
 extern "C" int racer_sim_sample(racer_sim* sim, const racer_view* out_p, const racer_view* log_std_p,
                                 const racer_view* actions_p, const racer_view* logp_p, const racer_view* value_p,
                                 int deterministic, int64_t step) {
@@ -1652,45 +1897,16 @@ extern "C" int racer_sim_set_waves_per_car(racer_sim* sim, int waves) {
 
 extern "C" int racer_sim_waves_per_car(const racer_sim* sim) { return sim->waves_per_car; }
 
-// Times the lidar on the current state (it only rewrites the range rows of
-// sim->obs, so repeating it is harmless) with 1, 2 and 4 wavefronts per car
-// and keeps the fastest; writes the chosen count.
-extern "C" int racer_sim_autotune(racer_sim* sim, int* chosen) {
+extern "C" int racer_sim_set_fused_step(racer_sim* sim, int on) {
     racer_clear_error();
-    RACER_REQUIRE(!sim->dev->capturing, "cannot autotune while capturing a graph");
-    racer_event_pair ev;
-    RACER_TRY(hipEventCreate(&ev.a));
-    RACER_TRY(hipEventCreate(&ev.b));
-    const bool counting = sim->count_iterations;
-    const int before = sim->waves_per_car;
-    sim->count_iterations = false;
-    float best = 1e30f;
-    int pick = before;
-    auto time_each = [&]() -> int {
-        for (int w : {1, 2, 4}) {
-            sim->waves_per_car = w;
-            launch_lidar_any(sim, sim->obs->ptr, sim->n);  // warm
-            RACER_TRY(hipEventRecord(ev.a, sim->dev->stream));
-            for (int r = 0; r < 3; ++r) launch_lidar_any(sim, sim->obs->ptr, sim->n);
-            RACER_TRY(hipEventRecord(ev.b, sim->dev->stream));
-            if (racer_launch_check("autotune lidar")) return 1;
-            RACER_TRY(hipEventSynchronize(ev.b));
-            float ms = 0.0f;
-            RACER_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
-            if (ms < best) {
-                best = ms;
-                pick = w;
-            }
-        }
-        return 0;
-    };
-    const int failed = time_each();
-    // on failure the sim keeps its previous count
-    sim->count_iterations = counting;
-    sim->waves_per_car = failed ? before : pick;
-    if (failed) return 1;
-    if (chosen) *chosen = pick;
+    RACER_REQUIRE(sim, "no sim");
+    sim->fused_step = on != 0;
     return 0;
+}
+
+extern "C" int racer_sim_fused_step(const racer_sim* sim) {
+    int pose_at = 0;
+    return sim->fused_step && step_fused_shmem(sim, &pose_at) != 0 ? 1 : 0;
 }
 
 extern "C" int racer_sim_set_schedule(racer_sim* sim, int schedule) {
