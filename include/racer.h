@@ -148,7 +148,7 @@ int racer_policy_backward(racer_policy* pol, const racer_view* x, racer_acts* ac
  * instead of the device's grow-only scratch; the kernels and their order are
  * those of racer_policy_backward, so the results are bitwise the same. Shapes
  * whose partials exceed the array fail with an error naming the need (about
- * 61 K columns at 1081 beams and H 256 in float32, twice that in bf16).
+ * 61 K columns at 1081 beams and H 256, in float32 and in bf16).
  * Every call on one GPU shares that GPU's array: calls that overlap on one GPU
  * (other host threads, other device handles or streams) race. */
 int racer_policy_backward_noalloc(racer_policy* pol, const racer_view* x, racer_acts* acts, const racer_view* d3,
@@ -167,6 +167,14 @@ int racer_trainer_minibatch(racer_trainer* tr, racer_policy* pol, int m, float a
 /* Trainer.adam (ppo.mojo:317-337): uses and advances the trainer's step count. */
 int racer_trainer_adam(racer_trainer* tr, racer_policy* pol);
 int racer_trainer_step_count(const racer_trainer* tr);
+/* The update's copy of every minibatch's inputs (no reference counterpart;
+ * the reference gathers each minibatch, ppo.mojo:276-279) is kept in bf16
+ * when the bf16 policy's kernels read them so (bf16 precision, the 64-column
+ * tiled forward, minibatches of at least 65536 aligned columns): half the
+ * bytes, the same results bit for bit. On by default where it applies; off
+ * keeps float32 copies. racer_trainer_bf16_inputs says which is in use. */
+int racer_trainer_set_bf16_inputs(racer_trainer* tr, int on);
+int racer_trainer_bf16_inputs(const racer_trainer* tr);
 racer_buf* racer_trainer_obs(racer_trainer* tr);       /* [(ROLLOUT+1), beams+5, n] */
 racer_buf* racer_trainer_actions(racer_trainer* tr);   /* [ROLLOUT, 2, n] */
 racer_buf* racer_trainer_logp(racer_trainer* tr);      /* [ROLLOUT * n] */

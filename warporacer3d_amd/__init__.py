@@ -158,6 +158,8 @@ def lib() -> C.CDLL:
         "racer_sim_waves_per_car": (i, [vp]),
         "racer_sim_set_fused_step": (i, [vp, i]),
         "racer_sim_scan": (i, [vp]),
+        "racer_trainer_set_bf16_inputs": (i, [vp, i]),
+        "racer_trainer_bf16_inputs": (i, [vp]),
         "racer_policy_set_act_kernel": (i, [vp, i]),
         "racer_policy_act_kernel": (i, [vp]),
         "racer_runner_autotune": (i, [vp, vp, vp, vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -587,6 +589,14 @@ class Trainer:
 
     def adam(self, policy: Policy) -> None:
         _check(lib().racer_trainer_adam(self.handle, policy.handle))
+
+    def set_bf16_inputs(self, on: bool) -> None:
+        """Keep the update's permuted minibatch inputs in bf16 where the bf16 kernels read them so (default)."""
+        _check(lib().racer_trainer_set_bf16_inputs(self.handle, int(on)))
+
+    @property
+    def bf16_inputs(self) -> bool:
+        return bool(lib().racer_trainer_bf16_inputs(self.handle))
 
     def capture_rollout(self, sim: Sim, policy: Policy, acts: Activations) -> "Graph":
         """The whole rollout as one HIP graph (replay with rollout_graph)."""

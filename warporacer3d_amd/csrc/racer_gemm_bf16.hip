@@ -8,6 +8,8 @@
 // LDS read. Activations arrive as float32 and are rounded to bf16
 // (round-to-nearest-even) while staging; weights are pre-rounded once per
 // update into a row-padded bf16 copy (rows padded to a multiple of 8 values).
+#include <type_traits>
+
 #include "racer_internal.h"
 
 namespace {
@@ -153,6 +155,17 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = 0.0f;
     load(0);
+    // the mask's h values, loaded with the first stage (the epilogue then waits for no load)
+    float hm[16];
+    const int jm = n0 + wn * 32 + (lane & 31);
+    if constexpr (EPI == RACER_EPI_MASK) {
+        const auto hres = racer_rsrc(H, ((uint64_t)(M - 1) * ldh + N) * 4);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int i = m0 + wm * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+            hm[q] = racer_ld_f32(hres, (i < M && jm < N) ? (uint32_t)(((int64_t)i * ldh + jm) * 4) : RACER_OOB);
+        }
+    }
     for (int k0 = 0; k0 < K; k0 += SK) {
         *reinterpret_cast<uint4*>(&As[arow][akc]) = ra[0];
         *reinterpret_cast<uint4*>(&As[arow][akc + 8]) = ra[1];
@@ -181,7 +194,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
             if constexpr (EPI == RACER_EPI_TANH) v = racer_tanh(v);
             if constexpr (EPI == RACER_EPI_TANH_MLP) v = racer_tanh_mlp(v);
             if constexpr (EPI == RACER_EPI_MASK) {
-                const float h = H[(int64_t)i * ldh + j];
+                const float h = hm[q];
                 v = v * (1.0f - h * h);
             }
             C[(int64_t)i * ldc + j] = v;
@@ -205,10 +218,14 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
 // every tile of chunk z is placed on XCD z % 8, so the tiles that share a
 // chunk's A and B rows stream them through one L2 (one HBM read of each k
 // stage instead of one per tile); otherwise the tiles go in plain order.
+//
+// BB16: B is already bf16 (rounded by the same conversion), read as 8-byte
+// loads of 4 k: the products are the same bits on half of B's bytes.
 constexpr int WG_T = 128, WG_K = 64, WG_LD = WG_K + 8;
 
+template <bool BB16>
 __global__ void __launch_bounds__(256) k_wgrad_bf16(const float* __restrict__ A, int64_t lda,
-                                                    const float* __restrict__ B, int64_t ldb, float* __restrict__ P,
+                                                    const void* __restrict__ Bp, int64_t ldb, float* __restrict__ P,
                                                     int64_t ldp, int64_t chunk_stride, int M, int N, int K,
                                                     int kchunk, int tiles_x, int tiles_y, int chunks) {
     __shared__ __attribute__((aligned(16))) uint16_t As[WG_T][WG_LD];
@@ -229,9 +246,13 @@ __global__ void __launch_bounds__(256) k_wgrad_bf16(const float* __restrict__ A,
     const int kbeg = z * kchunk;
     const int kend = min(K, kbeg + kchunk);
     // descriptors from the chunk's first k: offsets stay below 4 GB
+    constexpr int BSZ = BB16 ? 2 : 4;  // bytes of one B element
     const auto ares = racer_rsrc(A + kbeg, ((uint64_t)(M - 1) * lda + (kend - kbeg)) * 4);
-    const auto bres = racer_rsrc(B + kbeg, ((uint64_t)(N - 1) * ldb + (kend - kbeg)) * 4);
-    uint4 ra[8], rb[8];
+    const auto bres = racer_rsrc(static_cast<const char*>(Bp) + (int64_t)kbeg * BSZ,
+                                 ((uint64_t)(N - 1) * ldb + (kend - kbeg)) * BSZ);
+    using BReg = std::conditional_t<BB16, uint2, uint4>;
+    uint4 ra[8];
+    BReg rb[8];
     auto load = [&](int k0) {  // k0 relative to kbeg
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
@@ -239,9 +260,12 @@ __global__ void __launch_bounds__(256) k_wgrad_bf16(const float* __restrict__ A,
             const int k = k0 + kq;
             const bool kin = kbeg + k < kend;
             const uint32_t ao = (m0 + r < M && kin) ? (uint32_t)(((int64_t)(m0 + r) * lda + k) * 4) : RACER_OOB;
-            const uint32_t bo = (n0 + r < N && kin) ? (uint32_t)(((int64_t)(n0 + r) * ldb + k) * 4) : RACER_OOB;
+            const uint32_t bo = (n0 + r < N && kin) ? (uint32_t)(((int64_t)(n0 + r) * ldb + k) * BSZ) : RACER_OOB;
             ra[i] = racer_ld_b128(ares, ao);
-            rb[i] = racer_ld_b128(bres, bo);
+            if constexpr (BB16)
+                rb[i] = __builtin_bit_cast(uint2, racer_ld_f32x2(bres, bo));
+            else
+                rb[i] = racer_ld_b128(bres, bo);
         }
     };
     auto stage = [&](int k0) {  // registers -> LDS as bf16, zeroing k at or past the chunk's end
@@ -255,7 +279,10 @@ __global__ void __launch_bounds__(256) k_wgrad_bf16(const float* __restrict__ A,
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 a4[e] = e < lim ? to_bf16(__builtin_bit_cast(float, av[e])) : (uint16_t)0;
-                b4[e] = e < lim ? to_bf16(__builtin_bit_cast(float, bv[e])) : (uint16_t)0;
+                if constexpr (BB16)
+                    b4[e] = e < lim ? (uint16_t)(bv[e >> 1] >> (16 * (e & 1))) : (uint16_t)0;
+                else
+                    b4[e] = e < lim ? to_bf16(__builtin_bit_cast(float, bv[e])) : (uint16_t)0;
             }
             *reinterpret_cast<uint2*>(&As[r][kq]) = *reinterpret_cast<uint2*>(a4);
             *reinterpret_cast<uint2*>(&Bs[r][kq]) = *reinterpret_cast<uint2*>(b4);
@@ -359,13 +386,55 @@ int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const 
     return racer_launch_check("gemm_nn_bf16");
 }
 
-// The batch splits into chunks of 8192 (both kernels); each chunk's partial
-// product fills one [M, ldc] slab of the workspace.
-constexpr int NT_BF16_CHUNK = 8192;
+// The batch splits into chunks of 4096 (both kernels); each chunk's partial
+// product fills one [M, ldc] slab of the workspace. At the PPO update's
+// 262144-column minibatches that is 64 chunks, 1152 workgroups for W0's
+// gradient: the k loop's loads wait on HBM latency, and twice the workgroups
+// of 8192-column chunks hide more of it (15.7 against 19.7 ms of weight
+// gradients per update); 2048 adds more partial sums than it saves.
+constexpr int NT_BF16_CHUNK = 4096;
 
 size_t racer_gemm_nt_bf16_partials(int M, int64_t ldc, int K) {
     int chunks = K <= 0 ? 1 : (K + NT_BF16_CHUNK - 1) / NT_BF16_CHUNK;
     return (size_t)M * (size_t)ldc * (size_t)chunks;
+}
+
+// The MFMA weight-gradient kernel, then the chunks' partial products summed
+// in chunk order into C.
+template <bool BB16>
+static int wgrad_launch(racer_device* dev, const float* A, int64_t lda, const void* B, int64_t ldb, float* C,
+                        int64_t ldc, int M, int N, int K, float* partial) {
+    const int chunk = NT_BF16_CHUNK;
+    int chunks = (K + chunk - 1) / chunk;
+    int64_t stride = (int64_t)M * ldc;
+    const int tx = (N + WG_T - 1) / WG_T, ty = (M + WG_T - 1) / WG_T;
+    dim3 grid((unsigned)(tx * ty * chunks)), block(256);
+    {
+        racer_timed timer(dev, "wgrad_bf16");
+        hipLaunchKernelGGL(k_wgrad_bf16<BB16>, grid, block, 0, dev->stream, A, lda, B, ldb, partial, ldc, stride, M,
+                           N, K, chunk, tx, ty, chunks);
+    }
+    if (racer_launch_check("wgrad_bf16")) return 1;
+    int64_t total = (int64_t)M * N;
+    {
+        racer_timed timer(dev, "sum_chunks");
+        hipLaunchKernelGGL(k_sum_chunks_bf16, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, dev->stream,
+                           partial, stride, chunks, C, ldc, M, N);
+    }
+    return racer_launch_check("sum_chunks");
+}
+
+int racer_gemm_nt_bf16_x16(racer_device* dev, const float* A, int64_t lda, const uint16_t* B, int64_t ldb, float* C,
+                           int64_t ldc, int M, int N, int K, float* ws, size_t ws_len) {
+    if (M <= 0 || N <= 0) return 0;
+    RACER_REQUIRE(K > 0 && lda % 4 == 0 && ldb % 4 == 0 && K % 4 == 0 && (uintptr_t)A % 16 == 0 &&
+                      (uintptr_t)B % 8 == 0,
+                  "bf16 weight-gradient operands must be aligned to 4 elements");
+    RACER_REQUIRE(((uint64_t)(M - 1) * lda + K) * 4 < 0xFFFFFF00ull && ((uint64_t)(N - 1) * ldb + K) * 2 < 0xFFFFFF00ull,
+                  "bf16 product operands exceed the 32-bit offsets of the buffer loads");
+    float* partial = nullptr;
+    if (racer_partials(dev, racer_gemm_nt_bf16_partials(M, ldc, K), ws, ws_len, &partial)) return 1;
+    return wgrad_launch<true>(dev, A, lda, B, ldb, C, ldc, M, N, K, partial);
 }
 
 int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
@@ -376,26 +445,7 @@ int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const flo
     // the MFMA tile kernel: 16-byte aligned rows and operands within the reach of 32-bit offsets
     const bool aligned = lda % 4 == 0 && ldb % 4 == 0 && K % 4 == 0 && (uintptr_t)A % 16 == 0 && (uintptr_t)B % 16 == 0;
     const bool reach = ((uint64_t)(M - 1) * lda + K) * 4 < 0xFFFFFF00ull && ((uint64_t)(N - 1) * ldb + K) * 4 < 0xFFFFFF00ull;
-    if (aligned && reach && K > 0) {
-        const int chunk = NT_BF16_CHUNK;
-        int chunks = (K + chunk - 1) / chunk;
-        int64_t stride = (int64_t)M * ldc;
-        const int tx = (N + WG_T - 1) / WG_T, ty = (M + WG_T - 1) / WG_T;
-        dim3 grid((unsigned)(tx * ty * chunks)), block(256);
-        {
-            racer_timed timer(dev, "wgrad_bf16");
-            hipLaunchKernelGGL(k_wgrad_bf16, grid, block, 0, dev->stream, A, lda, B, ldb, partial, ldc, stride,
-                               M, N, K, chunk, tx, ty, chunks);
-        }
-        if (racer_launch_check("wgrad_bf16")) return 1;
-        int64_t total = (int64_t)M * N;
-        {
-            racer_timed timer(dev, "sum_chunks");
-            hipLaunchKernelGGL(k_sum_chunks_bf16, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, dev->stream,
-                               partial, stride, chunks, C, ldc, M, N);
-        }
-        return racer_launch_check("sum_chunks");
-    }
+    if (aligned && reach && K > 0) return wgrad_launch<false>(dev, A, lda, B, ldb, C, ldc, M, N, K, partial);
     const int chunk = NT_BF16_CHUNK;
     int chunks = K <= 0 ? 1 : (K + chunk - 1) / chunk;
     int64_t stride = (int64_t)M * ldc;

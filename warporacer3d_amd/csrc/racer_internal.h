@@ -209,8 +209,10 @@ struct racer_trainer {
               *done = nullptr, *advantage = nullptr, *x = nullptr, *d3 = nullptr, *d2 = nullptr, *d1 = nullptr,
               *grad = nullptr, *first = nullptr, *second = nullptr, *stats = nullptr;
     // The rollout's observations regrouped by minibatch, [MB][IN][batch], built
-    // once per update so that no minibatch gathers strided columns.
+    // once per update so that no minibatch gathers strided columns; in bf16
+    // (x16, two values a float) when the bf16 update's kernels read them so.
     racer_buf* xall = nullptr;
+    bool x16 = false;
     racer_acts* acts = nullptr;
     double* dstats = nullptr;  // float64 accumulators: advantage sum and sum of squares, reward sum
 };
@@ -326,6 +328,11 @@ int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const 
 size_t racer_gemm_nt_bf16_partials(int M, int64_t ldc, int K);
 int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
                        int64_t ldc, int M, int N, int K, float* ws = nullptr, size_t ws_len = 0);
+// racer_gemm_nt_bf16 with B already in bf16 (MFMA weight-gradient kernel only:
+// rows and K aligned to 4 elements, B 8-byte aligned); the same bits as
+// racer_gemm_nt_bf16 on B's float32 values.
+int racer_gemm_nt_bf16_x16(racer_device* dev, const float* A, int64_t lda, const uint16_t* B, int64_t ldb, float* C,
+                           int64_t ldc, int M, int N, int K, float* ws, size_t ws_len);
 // dst[r, c] = bf16(src[r, c]) for c < cols, 0 for cols <= c < ld_dst.
 int racer_pack_bf16(racer_device* dev, const float* src, int64_t ld_src, uint16_t* dst, int64_t ld_dst, int rows,
                     int cols);
@@ -350,6 +357,12 @@ size_t racer_f32_fragment_floats(int K);
 int racer_pack_f32_fragments(racer_device* dev, const float* src, int64_t ld_src, int rows, int cols, float* dst);
 int racer_policy_f32(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
                      float* act, float* logp, float* value, int deterministic, int64_t step, int seed);
+// The forward of the PPO update's minibatches from bf16 inputs (k_policy_tiled
+// reading the bf16 minibatch copy): bitwise the forward of their float32
+// values. racer_tiled_x16_applies: a bf16 policy of this shape takes the
+// 64-column tiled kernel and the shapes are aligned for it.
+bool racer_tiled_x16_applies(int precision, int hidden, int64_t IN, int64_t x_cols, int64_t cols);
+int racer_policy_tiled_x16(racer_policy* p, const uint16_t* x, int64_t x_cols, racer_acts* a, int cols);
 // kernel: RACER_ACT_AUTO picks by the launch's size; RACER_ACT_FUSED the
 // 16-column ring kernel, RACER_ACT_WIDE its wide variant, RACER_ACT_TILED the
 // 64-column tiled kernel (without the sample only), where they apply.

@@ -21,6 +21,8 @@
 //    fragments requested before phase 1's reduction;
 //  - phase 3: o = W2 h2 in float32 with fmaf over k in order (as the float32
 //    path), then the sample.
+#include <type_traits>
+
 #include "racer_internal.h"
 
 namespace {
@@ -420,10 +422,12 @@ __host__ __device__ inline TiledLayout tiled_layout(int H) {
     return L;
 }
 
-template <int HS>
+// XB16: the inputs are already bf16 (rounded by the same conversion), read as
+// 8-byte loads of 4 columns: the same bits on half of the inputs' bytes.
+template <int HS, bool XB16>
 __global__ void __launch_bounds__(64 * WAVES) k_policy_tiled(
     const uint16_t* __restrict__ F0, const uint16_t* __restrict__ F1, const float* __restrict__ W2,
-    const float* __restrict__ x, int64_t ldx, int H, int IN, int64_t cols, float* __restrict__ h1out,
+    const void* __restrict__ x, int64_t ldx, int H, int IN, int64_t cols, float* __restrict__ h1out,
     float* __restrict__ h2out, float* __restrict__ o) {
     static_assert(HS > 0, "the tiled forward needs equal unrolled k halves");
     constexpr int NB = (HS + TKB - 1) / TKB;  // staged blocks per k half
@@ -440,7 +444,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_tiled(
     const int r = lane & 15, kg = (lane >> 4) * 8;  // MFMA operand lane layout
     const int64_t c0 = (int64_t)xcd_block() * TC;
     const int steps0 = 2 * HS, steps1 = Ly.ha32 / 32;
-    const auto xres = racer_rsrc(x, ((uint64_t)(IN - 1) * ldx + cols) * 4);
+    constexpr int XSZ = XB16 ? 2 : 4;  // bytes of one input element
+    const auto xres = racer_rsrc(x, ((uint64_t)(IN - 1) * ldx + cols) * XSZ);
     const auto w0res = racer_rsrc(F0, (uint64_t)FRAG_TILES * steps0 * 1024);
     const auto w1res = racer_rsrc(F1, (uint64_t)FRAG_TILES * steps1 * 1024);
     for (int idx = tid; idx < 3 * HA; idx += blockDim.x) w2s[idx] = W2[idx];
@@ -457,7 +462,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_tiled(
     auto kq = [&](int m) { return (((tid & 255) >> 4) + 16 * m) * 4; };
     // blocks b .. b + TDX - 1 are in flight in register set b % TDX (HBM
     // latency is several blocks' worth of MFMA work)
-    float4 xr[TDX][PIECES][4];
+    using XReg = std::conditional_t<XB16, uint2, float4>;
+    XReg xr[TDX][PIECES][4];
     auto stage_load = [&](int b) {
 #pragma unroll
         for (int m = 0; m < PIECES; ++m)
@@ -466,8 +472,11 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_tiled(
                 const int kin = b * TKB * 32 + kq(m) + e;  // k within the half
                 const int k = sh * HS * 32 + kin;
                 const bool ok = kin < HS * 32 && k < IN && scol < cols;
-                xr[b % TDX][m][e] = __builtin_bit_cast(
-                    float4, racer_ld_b128(xres, ok ? (uint32_t)(((int64_t)k * ldx + scol) * 4) : RACER_OOB));
+                const uint32_t off = ok ? (uint32_t)(((int64_t)k * ldx + scol) * XSZ) : RACER_OOB;
+                if constexpr (XB16)
+                    xr[b % TDX][m][e] = __builtin_bit_cast(uint2, racer_ld_f32x2(xres, off));
+                else
+                    xr[b % TDX][m][e] = __builtin_bit_cast(float4, racer_ld_b128(xres, off));
             }
     };
     auto stage_store = [&](int b) {
@@ -477,10 +486,18 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_tiled(
             uint16_t v[4][4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                v[0][e] = bf16_of(xr[b % TDX][m][e].x);
-                v[1][e] = bf16_of(xr[b % TDX][m][e].y);
-                v[2][e] = bf16_of(xr[b % TDX][m][e].z);
-                v[3][e] = bf16_of(xr[b % TDX][m][e].w);
+                if constexpr (XB16) {
+                    const uint2 w = xr[b % TDX][m][e];
+                    v[0][e] = (uint16_t)w.x;
+                    v[1][e] = (uint16_t)(w.x >> 16);
+                    v[2][e] = (uint16_t)w.y;
+                    v[3][e] = (uint16_t)(w.y >> 16);
+                } else {
+                    v[0][e] = bf16_of(xr[b % TDX][m][e].x);
+                    v[1][e] = bf16_of(xr[b % TDX][m][e].y);
+                    v[2][e] = bf16_of(xr[b % TDX][m][e].z);
+                    v[3][e] = bf16_of(xr[b % TDX][m][e].w);
+                }
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) *reinterpret_cast<uint2*>(dst + q * TXLD) = *reinterpret_cast<uint2*>(v[q]);
@@ -682,7 +699,8 @@ static int fused_setup_one() {
 
 int racer_policy_fused_setup() {
     if (fused_setup_one<0>() || fused_setup_one<2>() || fused_setup_one<17>()) return 1;
-    for (const void* k : {(const void*)k_policy_tiled<2>, (const void*)k_policy_tiled<17>})
+    for (const void* k : {(const void*)k_policy_tiled<2, false>, (const void*)k_policy_tiled<17, false>,
+                          (const void*)k_policy_tiled<2, true>, (const void*)k_policy_tiled<17, true>})
         RACER_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     return 0;
 }
@@ -731,6 +749,28 @@ static auto fused_kernel(int hs, bool vec, bool wide) {
                     : (hs == 2 ? k_policy_fused<SAMPLE, 2, false, false> : k_policy_fused<SAMPLE, 0, false, false>);
 }
 
+bool racer_tiled_x16_applies(int precision, int hidden, int64_t IN, int64_t x_cols, int64_t cols) {
+    return precision == 1 && hidden <= 256 && unrolled_half((int)IN) > 0 && x_cols % 4 == 0 && cols % 4 == 0 &&
+           cols >= TILED_MIN_COLS && ((uint64_t)(IN - 1) * x_cols + cols) * 2 < 0xFFFFFF00ull;
+}
+
+int racer_policy_tiled_x16(racer_policy* p, const uint16_t* x, int64_t x_cols, racer_acts* a, int cols) {
+    racer_device* dev = p->dev;
+    RACER_REQUIRE(racer_tiled_x16_applies(p->precision, p->hidden, p->IN, x_cols, cols) && (uintptr_t)x % 8 == 0,
+                  "the bf16-input tiled forward does not apply to these shapes");
+    if (racer_policy_refresh(p)) return 1;
+    const int hs = unrolled_half((int)p->IN);
+    const dim3 tgrid((unsigned)((cols + TC - 1) / TC)), block(64 * WAVES);
+    const size_t tshmem = tiled_layout(p->hidden).total;
+    {
+        racer_timed timer(dev, "policy_tiled");
+        hipLaunchKernelGGL((hs == 17 ? k_policy_tiled<17, true> : k_policy_tiled<2, true>), tgrid, block, tshmem,
+                           dev->stream, p->frag0, p->frag1, p->theta->ptr + p->W2, x, x_cols, p->hidden, (int)p->IN,
+                           (int64_t)cols, a->h1->ptr, a->h2->ptr, a->o->ptr);
+    }
+    return racer_launch_check("policy_tiled");
+}
+
 int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_acts* a, int cols, bool sample,
                        float* act, float* logp, float* value, int deterministic, int64_t step, int seed, int kernel) {
     racer_device* dev = p->dev;
@@ -748,7 +788,8 @@ int racer_policy_fused(racer_policy* p, const float* x, int64_t x_cols, racer_ac
         racer_timed timer(dev, "policy_tiled");
         const dim3 tgrid((unsigned)((cols + TC - 1) / TC));
         const size_t tshmem = tiled_layout(p->hidden).total;
-        hipLaunchKernelGGL(hs == 17 ? k_policy_tiled<17> : k_policy_tiled<2>, tgrid, block, tshmem, dev->stream, f0,
+        hipLaunchKernelGGL((hs == 17 ? k_policy_tiled<17, false> : k_policy_tiled<2, false>), tgrid, block, tshmem,
+                           dev->stream, f0,
                            f1, W2, x, x_cols, p->hidden, (int)p->IN, (int64_t)cols, a->h1->ptr, a->h2->ptr,
                            a->o->ptr);
         return racer_launch_check("policy_tiled");

@@ -9,6 +9,8 @@
 #include <fstream>
 #include <sstream>
 
+#include <type_traits>
+
 #include "racer_internal.h"
 
 
@@ -127,7 +129,9 @@ __global__ void k_gather(const float* obs, int64_t n, int64_t IN, int64_t batch,
 // sample s = t n + i has m = i % MB and b = t (n / MB) + i / MB, so a thread
 // reads MB consecutive cars of one obs row and writes one column of each
 // minibatch: reads and writes are both contiguous across the wavefront.
-template <int MB>
+// X16: xall holds bf16 (round to nearest even, the conversion the bf16
+// kernels apply on load), two values a float.
+template <int MB, bool X16>
 __global__ void k_permute(const float* __restrict__ obs, int64_t n, int64_t IN, int64_t rows, int64_t batch,
                           float* __restrict__ xall) {
     const int64_t per_row = n / MB;
@@ -145,8 +149,15 @@ __global__ void k_permute(const float* __restrict__ obs, int64_t n, int64_t IN, 
         for (int e = 0; e < MB; ++e) v[e] = src[e];
     }
     const int64_t b = t * per_row + q;
+    if constexpr (X16) {
+        uint16_t* x16 = reinterpret_cast<uint16_t*>(xall);
 #pragma unroll
-    for (int m = 0; m < MB; ++m) xall[((int64_t)m * IN + r) * batch + b] = v[m];
+        for (int m = 0; m < MB; ++m)
+            x16[((int64_t)m * IN + r) * batch + b] = __builtin_bit_cast(uint16_t, (__bf16)v[m]);
+    } else {
+#pragma unroll
+        for (int m = 0; m < MB; ++m) xall[((int64_t)m * IN + r) * batch + b] = v[m];
+    }
 }
 
 // minibatch.loss (racer/ppo.mojo:281-304): clipped surrogate and Huber value
@@ -555,9 +566,11 @@ static size_t backward_partials(const racer_policy* p, int64_t cols) {
 }
 
 // ws: the weight gradients' split-K workspace (ws_len floats), or null for the
-// device's grow-only scratch.
+// device's grow-only scratch. x16 (bf16 policies): x already in bf16, read by
+// the W0 gradient in place of x (the same bits; x is then unused).
 static int backward_raw(racer_policy* p, const float* x, racer_acts* a, const float* d3, float* d2, float* d1,
-                        float* grad, int64_t cols, float* ws = nullptr, size_t ws_len = 0) {
+                        float* grad, int64_t cols, float* ws = nullptr, size_t ws_len = 0,
+                        const uint16_t* x16 = nullptr) {
     racer_device* dev = p->dev;
     int H = p->hidden;
     int64_t HA = p->HA, IN = p->IN;
@@ -583,7 +596,8 @@ static int backward_raw(racer_policy* p, const float* x, racer_acts* a, const fl
     }
     // weight gradients (racer/ppo.mojo:121-123)
     if (p->precision == 1) {
-        if (racer_gemm_nt_bf16(dev, d1, cols, x, cols, grad, IN, H, (int)IN, (int)cols, ws, ws_len) ||
+        if ((x16 ? racer_gemm_nt_bf16_x16(dev, d1, cols, x16, cols, grad, IN, H, (int)IN, (int)cols, ws, ws_len)
+                 : racer_gemm_nt_bf16(dev, d1, cols, x, cols, grad, IN, H, (int)IN, (int)cols, ws, ws_len)) ||
             racer_gemm_nt_bf16(dev, d2, cols, h1, cols, grad + p->W1, HA, H, (int)HA, (int)cols, ws, ws_len))
             return 1;
     } else {
@@ -635,6 +649,22 @@ extern "C" int racer_policy_backward_noalloc(racer_policy* p, const racer_view* 
 }
 
 // ---------------------------------------------------------------- trainer
+// Whether the update's permuted inputs can be bf16 for this trainer.
+static bool x16_applies(const racer_trainer* tr) {
+    return racer_tiled_x16_applies(tr->precision, tr->hidden, tr->IN, tr->batch, tr->batch);
+}
+
+// (Re)allocates the permuted inputs, float32 or bf16 (two values a float).
+static int alloc_permuted(racer_trainer* tr, bool x16) {
+    const size_t values = (size_t)tr->rollout * tr->IN * tr->n;
+    const size_t floats = x16 ? (values + 1) / 2 : values;
+    if (tr->xall && tr->xall->len == floats && tr->x16 == x16) return 0;
+    racer_free(tr->xall);
+    tr->xall = nullptr;
+    tr->x16 = x16;
+    return racer_alloc(tr->dev, floats, 0.0f, &tr->xall);
+}
+
 extern "C" int racer_trainer_create(racer_device* dev, int n, const racer_config* cfg, racer_trainer** out) {
     racer_clear_error();
     *out = nullptr;
@@ -678,14 +708,22 @@ extern "C" int racer_trainer_create(racer_device* dev, int n, const racer_config
     }
     RACER_TRY_OR(hipMalloc(&tr->dstats, 8 * sizeof(double)), racer_trainer_destroy(tr));
     const int MB = tr->minibatches;
-    if ((MB == 1 || MB == 2 || MB == 4 || MB == 8) && N % MB == 0 &&
-        racer_alloc(dev, (size_t)(R * tr->IN * N), 0.0f, &tr->xall)) {
+    if ((MB == 1 || MB == 2 || MB == 4 || MB == 8) && N % MB == 0 && alloc_permuted(tr, x16_applies(tr))) {
         racer_trainer_destroy(tr);
         return 1;
     }
     *out = tr;
     return 0;
 }
+
+extern "C" int racer_trainer_set_bf16_inputs(racer_trainer* tr, int on) {
+    racer_clear_error();
+    RACER_REQUIRE(tr, "null argument");
+    if (!tr->xall) return 0;  // no permuted copy: each minibatch is gathered in float32
+    return alloc_permuted(tr, on != 0 && x16_applies(tr));
+}
+
+extern "C" int racer_trainer_bf16_inputs(const racer_trainer* tr) { return tr->xall && tr->x16 ? 1 : 0; }
 
 extern "C" void racer_trainer_destroy(racer_trainer* tr) {
     if (!tr) return;
@@ -750,7 +788,8 @@ int racer_trainer_mean_reward(racer_trainer* tr, float* mean_reward) {
     return 0;
 }
 
-static int minibatch_core(racer_trainer* tr, racer_policy* p, int m, float adv_mean, float adv_scale, const float* x);
+static int minibatch_core(racer_trainer* tr, racer_policy* p, int m, float adv_mean, float adv_scale, const float* x,
+                          const uint16_t* x16 = nullptr);
 
 extern "C" int racer_trainer_minibatch(racer_trainer* tr, racer_policy* p, int m, float adv_mean, float adv_scale) {
     racer_clear_error();
@@ -774,32 +813,28 @@ static int permute_all(racer_trainer* tr) {
     int64_t rows = (int64_t)tr->rollout * tr->IN, per_row = tr->n / tr->minibatches;
     unsigned blocks = (unsigned)((rows * per_row + 255) / 256);
     racer_timed timer(dev, "permute");
-    switch (tr->minibatches) {
-        case 1:
-            hipLaunchKernelGGL(k_permute<1>, dim3(blocks), dim3(256), 0, dev->stream, tr->obs->ptr, (int64_t)tr->n,
-                               tr->IN, rows, tr->batch, tr->xall->ptr);
-            break;
-        case 2:
-            hipLaunchKernelGGL(k_permute<2>, dim3(blocks), dim3(256), 0, dev->stream, tr->obs->ptr, (int64_t)tr->n,
-                               tr->IN, rows, tr->batch, tr->xall->ptr);
-            break;
-        case 4:
-            hipLaunchKernelGGL(k_permute<4>, dim3(blocks), dim3(256), 0, dev->stream, tr->obs->ptr, (int64_t)tr->n,
-                               tr->IN, rows, tr->batch, tr->xall->ptr);
-            break;
-        default:
-            hipLaunchKernelGGL(k_permute<8>, dim3(blocks), dim3(256), 0, dev->stream, tr->obs->ptr, (int64_t)tr->n,
-                               tr->IN, rows, tr->batch, tr->xall->ptr);
-            break;
-    }
+    auto kernel = [&](auto mb) {
+        constexpr int MB = decltype(mb)::value;
+        return tr->x16 ? k_permute<MB, true> : k_permute<MB, false>;
+    };
+    auto pick = tr->minibatches == 1   ? kernel(std::integral_constant<int, 1>{})
+                : tr->minibatches == 2 ? kernel(std::integral_constant<int, 2>{})
+                : tr->minibatches == 4 ? kernel(std::integral_constant<int, 4>{})
+                                       : kernel(std::integral_constant<int, 8>{});
+    hipLaunchKernelGGL(pick, dim3(blocks), dim3(256), 0, dev->stream, tr->obs->ptr, (int64_t)tr->n, tr->IN, rows,
+                       tr->batch, tr->xall->ptr);
     return racer_launch_check("permute");
 }
 
-static int minibatch_core(racer_trainer* tr, racer_policy* p, int m, float adv_mean, float adv_scale, const float* x) {
+// x16: the minibatch's inputs in bf16 instead of x (bf16 policies on the
+// tiled forward), the same bits through every product.
+static int minibatch_core(racer_trainer* tr, racer_policy* p, int m, float adv_mean, float adv_scale, const float* x,
+                          const uint16_t* x16) {
     racer_device* dev = tr->dev;
     int64_t B = tr->batch, n = tr->n;
     unsigned blocks = (unsigned)((B + 255) / 256);
-    if (racer_policy_forward_raw(p, x, B, tr->acts, (int)B)) return 1;
+    if (x16 ? racer_policy_tiled_x16(p, x16, B, tr->acts, (int)B) : racer_policy_forward_raw(p, x, B, tr->acts, (int)B))
+        return 1;
     {
         racer_timed timer(dev, "loss");
         hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, dev->stream, B, n, tr->minibatches, m,
@@ -807,7 +842,8 @@ static int minibatch_core(racer_trainer* tr, racer_policy* p, int m, float adv_m
                            p->theta->ptr + p->LOGSTD, adv_mean, adv_scale, tr->d3->ptr);
     }
     if (racer_launch_check("loss")) return 1;
-    if (backward_raw(p, x, tr->acts, tr->d3->ptr, tr->d2->ptr, tr->d1->ptr, tr->grad->ptr, B)) return 1;
+    if (backward_raw(p, x, tr->acts, tr->d3->ptr, tr->d2->ptr, tr->d1->ptr, tr->grad->ptr, B, nullptr, 0, x16))
+        return 1;
     // log-std gradient: row sums of d3[3..4] (racer/ppo.mojo:313)
     if (row_sums(dev, tr->d3->ptr + 3 * B, B, B, 2, 0, nullptr, tr->grad->ptr, p->LOGSTD)) return 1;
     // squared gradient norm (racer/ppo.mojo:314-315)
@@ -846,13 +882,20 @@ extern "C" int racer_trainer_update(racer_trainer* tr, racer_policy* p, float* k
     double var = mom[1] / (double)samples - mean * mean;
     float adv_mean = (float)mean;
     float adv_scale = (float)(1.0 / std::sqrt(std::max(var, 1e-8)));
-    if (tr->xall && permute_all(tr)) return 1;
+    // the permuted copy serves a policy of the precision it was made for: a
+    // bf16 copy only a bf16 policy (a float32 policy then gathers each minibatch)
+    const bool permuted = tr->xall && (!tr->x16 || p->precision == 1);
+    if (permuted && permute_all(tr)) return 1;
     for (int epoch = 0; epoch < tr->epochs; ++epoch)
         for (int m = 0; m < tr->minibatches; ++m) {
             const int mm = (m + epoch) % tr->minibatches;
-            if (tr->xall) {
-                if (minibatch_core(tr, p, mm, adv_mean, adv_scale, tr->xall->ptr + (int64_t)mm * tr->IN * tr->batch))
+            const int64_t at = (int64_t)mm * tr->IN * tr->batch;
+            if (permuted && tr->x16) {
+                if (minibatch_core(tr, p, mm, adv_mean, adv_scale, nullptr,
+                                   reinterpret_cast<const uint16_t*>(tr->xall->ptr) + at))
                     return 1;
+            } else if (permuted) {
+                if (minibatch_core(tr, p, mm, adv_mean, adv_scale, tr->xall->ptr + at)) return 1;
             } else if (racer_trainer_minibatch(tr, p, mm, adv_mean, adv_scale)) {
                 return 1;
             }
