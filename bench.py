@@ -587,6 +587,8 @@ def main() -> int:
                     help="timed steps of the eight-instance batched launch (0 skips it)")
     ap.add_argument("--ppo-iterations", type=int, default=3,
                     help="timed PPO iterations per beam count on the bridge (0 skips the PPO leg)")
+    ap.add_argument("--ppo-legs", default="1081,108,108fp32",
+                    help="PPO legs to run: beam counts, bf16 policy, or fp32 with the suffix fp32")
     ap.add_argument("--train-parity", type=int, default=0,
                     help="PPO iterations of the bf16 / fp32 training-curve comparison (0: skip; minutes per 300)")
     ap.add_argument("--train-parity-seeds", type=int, default=2)
@@ -630,7 +632,9 @@ def main() -> int:
     ppo = None
     if args.ppo_iterations > 0:
         ppo = {}
-        for beams, precision in ((1081, 1), (108, 1), (108, 0)):
+        legs = [(int(leg[:-4]), 0) if leg.endswith("fp32") else (int(leg), 1)
+                for leg in args.ppo_legs.split(",") if leg]
+        for beams, precision in legs:
             key = f"beams{beams}" + ("" if precision == 1 else "_fp32")
             ppo[key] = ppo_leg(R, T, dev, beams, args.ppo_iterations, precision=precision)
             log(f"PPO iteration (bridge8, 16384 cars, {beams} beams, precision {precision}):",

@@ -102,10 +102,10 @@ def test_ragged_products(dev, M, N, K, precision):
     assert np.all(np.abs(h - ref) <= bound)
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_backward_against_oracle(dev, precision):
-    print(f"backward p{precision}", flush=True)
-    beams, hidden, cols = 108, 64, 640
+@pytest.mark.parametrize("precision,cols", [(0, 640), (1, 640), (0, 77), (1, 77)])
+def test_backward_against_oracle(dev, precision, cols):
+    print(f"backward p{precision} cols={cols}", flush=True)
+    beams, hidden = 108, 64
     g, o, cfg = _policy_pair(dev, beams, hidden, precision)
     rng = np.random.default_rng(2)
     x = rng.uniform(-1, 1, (cfg.IN, cols)).astype(np.float32)

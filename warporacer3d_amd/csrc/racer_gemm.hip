@@ -86,6 +86,67 @@ __global__ void __launch_bounds__(256) k_gemm_nn_f32(const float* __restrict__ A
     }
 }
 
+// C[M, N] = (A[M, 3] B[3, N]) (1 - H^2) for the output layer's K = 3 (the
+// backward's d2, racer/ppo.mojo:125-130): one thread per output, row
+// blockIdx.y, each the fmaf chain over k = 0, 1, 2 from 0 (the tiled kernel's
+// order and start, so the same bits), streamed at HBM rate instead of through
+// 64 x 64 tiles of mostly k padding. OUT16: C stored in bf16 (nearest even).
+//
+// VEC: 8 consecutive columns per thread with 16-byte loads and stores (every
+// leading dimension and N a multiple of 8, 16-byte aligned operands), so the
+// launch is an eighth of the workgroups of one thread per output.
+template <bool OUT16>
+__device__ __forceinline__ float mask_k3_one(float a0, float a1, float a2, float b0, float b1, float b2, float h) {
+    float acc = fmaf(a0, b0, 0.0f);
+    acc = fmaf(a1, b1, acc);
+    acc = fmaf(a2, b2, acc);
+    return acc * (1.0f - h * h);
+}
+
+template <bool OUT16, bool VEC>
+__global__ void __launch_bounds__(256) k_mask_k3(const float* __restrict__ A, const float* __restrict__ B,
+                                                 int64_t ldb, void* __restrict__ C, int64_t ldc, int N,
+                                                 const float* __restrict__ H, int64_t ldh) {
+    const int i = blockIdx.y;
+    const float a0 = A[3 * i], a1 = A[3 * i + 1], a2 = A[3 * i + 2];
+    if constexpr (VEC) {
+        const int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
+        if (j >= N) return;
+        float v[8];
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int64_t jj = j + 4 * half;
+            const float4 b0 = *reinterpret_cast<const float4*>(B + jj);
+            const float4 b1 = *reinterpret_cast<const float4*>(B + ldb + jj);
+            const float4 b2 = *reinterpret_cast<const float4*>(B + 2 * ldb + jj);
+            const float4 h = *reinterpret_cast<const float4*>(H + (int64_t)i * ldh + jj);
+            v[4 * half + 0] = mask_k3_one<OUT16>(a0, a1, a2, b0.x, b1.x, b2.x, h.x);
+            v[4 * half + 1] = mask_k3_one<OUT16>(a0, a1, a2, b0.y, b1.y, b2.y, h.y);
+            v[4 * half + 2] = mask_k3_one<OUT16>(a0, a1, a2, b0.z, b1.z, b2.z, h.z);
+            v[4 * half + 3] = mask_k3_one<OUT16>(a0, a1, a2, b0.w, b1.w, b2.w, h.w);
+        }
+        if constexpr (OUT16) {
+            uint16_t o[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) o[e] = __builtin_bit_cast(uint16_t, (__bf16)v[e]);
+            *reinterpret_cast<uint4*>(static_cast<uint16_t*>(C) + (int64_t)i * ldc + j) =
+                *reinterpret_cast<const uint4*>(o);
+        } else {
+            float* c = static_cast<float*>(C) + (int64_t)i * ldc + j;
+            *reinterpret_cast<float4*>(c) = make_float4(v[0], v[1], v[2], v[3]);
+            *reinterpret_cast<float4*>(c + 4) = make_float4(v[4], v[5], v[6], v[7]);
+        }
+    } else {
+        const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (j >= N) return;
+        const float v = mask_k3_one<OUT16>(a0, a1, a2, B[j], B[ldb + j], B[2 * ldb + j], H[(int64_t)i * ldh + j]);
+        if constexpr (OUT16)
+            static_cast<uint16_t*>(C)[(int64_t)i * ldc + j] = __builtin_bit_cast(uint16_t, (__bf16)v);
+        else
+            static_cast<float*>(C)[(int64_t)i * ldc + j] = v;
+    }
+}
+
 // Partial C[chunk][M, N] = A[M, kchunk] B[N, kchunk]^T over one K chunk.
 __global__ void __launch_bounds__(256) k_gemm_nt_f32(const float* __restrict__ A, int64_t lda,
                                                      const float* __restrict__ B, int64_t ldb,
@@ -247,6 +308,21 @@ int racer_gemm_nn_f32(racer_device* dev, const float* A, int64_t lda, const floa
         hipLaunchKernelGGL(k_gemm_nn_f32<RACER_EPI_NONE>, grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc, M, N,
                            K, H, ldh);
     return racer_launch_check("gemm_nn_f32");
+}
+
+int racer_mask_k3(racer_device* dev, const float* A, const float* B, int64_t ldb, void* C, bool out16, int64_t ldc,
+                  int M, int N, const float* H, int64_t ldh) {
+    if (M <= 0 || N <= 0) return 0;
+    RACER_REQUIRE(M <= 65535, "the K = 3 masked product takes at most 65535 rows");
+    const bool vec = N % 8 == 0 && ldb % 8 == 0 && ldc % 8 == 0 && ldh % 8 == 0 && (uintptr_t)B % 16 == 0 &&
+                     (uintptr_t)C % 16 == 0 && (uintptr_t)H % 16 == 0;
+    const int per = vec ? 8 : 1;
+    dim3 grid((unsigned)((N / per + 255) / 256), (unsigned)M), block(256);
+    racer_timed timer(dev, "mask_k3");
+    auto kernel = out16 ? (vec ? k_mask_k3<true, true> : k_mask_k3<true, false>)
+                        : (vec ? k_mask_k3<false, true> : k_mask_k3<false, false>);
+    hipLaunchKernelGGL(kernel, grid, block, 0, dev->stream, A, B, ldb, C, ldc, N, H, ldh);
+    return racer_launch_check("mask_k3");
 }
 
 // The batch splits into chunks of 4096; each chunk's partial product fills one

@@ -115,9 +115,13 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const void* __restrict__ Ap, 
 // registers. Each thread gathers 16 consecutive k of one column (loads that
 // are contiguous across the wavefront), rounds them to bf16 and writes them to
 // LDS as two 16-byte vectors, so LDS holds B as [column][k].
-template <int EPI>
+//
+// IO16 (the PPO update's through product): B is already bf16 and C is stored
+// in bf16, rounded as its consumers round it on load; the same bits on half
+// the bytes.
+template <int EPI, bool IO16 = false>
 __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict__ A, int64_t lda,
-                                                      const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
+                                                      const void* __restrict__ B, int64_t ldb, void* __restrict__ C,
                                                       int64_t ldc, int M, int N, int K, const float* __restrict__ H,
                                                       int64_t ldh) {
     constexpr int SK = 64, LD = SK + 8;
@@ -132,11 +136,13 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
     const int m0 = (t % tiles_m) * TM, n0 = (t / tiles_m) * TN;
     const int arow = tid >> 2, akc = (tid & 3) * 16;  // A: 16 consecutive k of one row
     const int bcol = tid & 63, bkc = (tid >> 6) * 16;  // B: 16 consecutive k of one column
+    constexpr int BSZ = IO16 ? 2 : 4;  // bytes of one B element
     uint4 ra[2];
-    float rb[16];
+    using BVal = std::conditional_t<IO16, uint16_t, float>;
+    BVal rb[16];
     // buffer loads: out-of-range offsets (rows past M, k past K, columns past N) read zeros
     const auto ares = racer_rsrc(A, (uint64_t)M * lda * 2);
-    const auto bres = racer_rsrc(B, ((uint64_t)(K - 1) * ldb + N) * 4);
+    const auto bres = racer_rsrc(B, ((uint64_t)(K - 1) * ldb + N) * BSZ);
     auto load = [&](int k0) {
         const int m = m0 + arow;
 #pragma unroll
@@ -148,7 +154,11 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             const int k = k0 + bkc + e;
-            rb[e] = racer_ld_f32(bres, (n < N && k < K) ? (uint32_t)(((int64_t)k * ldb + n) * 4) : RACER_OOB);
+            const uint32_t off = (n < N && k < K) ? (uint32_t)(((int64_t)k * ldb + n) * BSZ) : RACER_OOB;
+            if constexpr (IO16)
+                rb[e] = racer_ld_u16(bres, off);
+            else
+                rb[e] = racer_ld_f32(bres, off);
         }
     };
     f32x16 acc;
@@ -171,7 +181,12 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
         *reinterpret_cast<uint4*>(&As[arow][akc + 8]) = ra[1];
         uint16_t v[16];
 #pragma unroll
-        for (int e = 0; e < 16; ++e) v[e] = to_bf16(rb[e]);
+        for (int e = 0; e < 16; ++e) {
+            if constexpr (IO16)
+                v[e] = rb[e];
+            else
+                v[e] = to_bf16(rb[e]);
+        }
         *reinterpret_cast<uint4*>(&Bs[bcol][bkc]) = *reinterpret_cast<uint4*>(v);
         *reinterpret_cast<uint4*>(&Bs[bcol][bkc + 8]) = *reinterpret_cast<uint4*>(v + 8);
         __syncthreads();
@@ -197,7 +212,10 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
                 const float h = hm[q];
                 v = v * (1.0f - h * h);
             }
-            C[(int64_t)i * ldc + j] = v;
+            if constexpr (IO16)
+                static_cast<uint16_t*>(C)[(int64_t)i * ldc + j] = to_bf16(v);
+            else
+                static_cast<float*>(C)[(int64_t)i * ldc + j] = v;
         }
     }
 }
@@ -219,12 +237,12 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
 // chunk's A and B rows stream them through one L2 (one HBM read of each k
 // stage instead of one per tile); otherwise the tiles go in plain order.
 //
-// BB16: B is already bf16 (rounded by the same conversion), read as 8-byte
-// loads of 4 k: the products are the same bits on half of B's bytes.
+// AB16 / BB16: A / B are already bf16 (rounded by the same conversion), read
+// as 8-byte loads of 4 k: the products are the same bits on half the bytes.
 constexpr int WG_T = 128, WG_K = 64, WG_LD = WG_K + 8;
 
-template <bool BB16>
-__global__ void __launch_bounds__(256) k_wgrad_bf16(const float* __restrict__ A, int64_t lda,
+template <bool AB16, bool BB16>
+__global__ void __launch_bounds__(256) k_wgrad_bf16(const void* __restrict__ Ap, int64_t lda,
                                                     const void* __restrict__ Bp, int64_t ldb, float* __restrict__ P,
                                                     int64_t ldp, int64_t chunk_stride, int M, int N, int K,
                                                     int kchunk, int tiles_x, int tiles_y, int chunks) {
@@ -246,12 +264,14 @@ __global__ void __launch_bounds__(256) k_wgrad_bf16(const float* __restrict__ A,
     const int kbeg = z * kchunk;
     const int kend = min(K, kbeg + kchunk);
     // descriptors from the chunk's first k: offsets stay below 4 GB
-    constexpr int BSZ = BB16 ? 2 : 4;  // bytes of one B element
-    const auto ares = racer_rsrc(A + kbeg, ((uint64_t)(M - 1) * lda + (kend - kbeg)) * 4);
+    constexpr int ASZ = AB16 ? 2 : 4, BSZ = BB16 ? 2 : 4;  // bytes of one A / B element
+    const auto ares = racer_rsrc(static_cast<const char*>(Ap) + (int64_t)kbeg * ASZ,
+                                 ((uint64_t)(M - 1) * lda + (kend - kbeg)) * ASZ);
     const auto bres = racer_rsrc(static_cast<const char*>(Bp) + (int64_t)kbeg * BSZ,
                                  ((uint64_t)(N - 1) * ldb + (kend - kbeg)) * BSZ);
+    using AReg = std::conditional_t<AB16, uint2, uint4>;
     using BReg = std::conditional_t<BB16, uint2, uint4>;
-    uint4 ra[8];
+    AReg ra[8];
     BReg rb[8];
     auto load = [&](int k0) {  // k0 relative to kbeg
 #pragma unroll
@@ -259,9 +279,12 @@ __global__ void __launch_bounds__(256) k_wgrad_bf16(const float* __restrict__ A,
             const int idx = tid + 256 * i, r = idx >> 4, kq = (idx & 15) * 4;
             const int k = k0 + kq;
             const bool kin = kbeg + k < kend;
-            const uint32_t ao = (m0 + r < M && kin) ? (uint32_t)(((int64_t)(m0 + r) * lda + k) * 4) : RACER_OOB;
+            const uint32_t ao = (m0 + r < M && kin) ? (uint32_t)(((int64_t)(m0 + r) * lda + k) * ASZ) : RACER_OOB;
             const uint32_t bo = (n0 + r < N && kin) ? (uint32_t)(((int64_t)(n0 + r) * ldb + k) * BSZ) : RACER_OOB;
-            ra[i] = racer_ld_b128(ares, ao);
+            if constexpr (AB16)
+                ra[i] = __builtin_bit_cast(uint2, racer_ld_f32x2(ares, ao));
+            else
+                ra[i] = racer_ld_b128(ares, ao);
             if constexpr (BB16)
                 rb[i] = __builtin_bit_cast(uint2, racer_ld_f32x2(bres, bo));
             else
@@ -278,7 +301,10 @@ __global__ void __launch_bounds__(256) k_wgrad_bf16(const float* __restrict__ A,
             uint16_t a4[4], b4[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                a4[e] = e < lim ? to_bf16(__builtin_bit_cast(float, av[e])) : (uint16_t)0;
+                if constexpr (AB16)
+                    a4[e] = e < lim ? (uint16_t)(av[e >> 1] >> (16 * (e & 1))) : (uint16_t)0;
+                else
+                    a4[e] = e < lim ? to_bf16(__builtin_bit_cast(float, av[e])) : (uint16_t)0;
                 if constexpr (BB16)
                     b4[e] = e < lim ? (uint16_t)(bv[e >> 1] >> (16 * (e & 1))) : (uint16_t)0;
                 else
@@ -361,7 +387,7 @@ int racer_pack_bf16(racer_device* dev, const float* src, int64_t ld_src, uint16_
     return racer_launch_check("pack_bf16");
 }
 
-int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const float* B, int64_t ldb, float* C,
+int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const void* B, int64_t ldb, void* C,
                        int64_t ldc, int M, int N, int K, int epi, const float* H, int64_t ldh) {
     if (M <= 0 || N <= 0) return 0;
     RACER_REQUIRE(lda % 8 == 0 && (uint64_t)M * lda * 2 < 0xFFFFFF00ull &&
@@ -386,6 +412,19 @@ int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const 
     return racer_launch_check("gemm_nn_bf16");
 }
 
+int racer_gemm_nn_bf16_mask16(racer_device* dev, const uint16_t* A, int64_t lda, const uint16_t* B, int64_t ldb,
+                              uint16_t* C, int64_t ldc, int M, int N, int K, const float* H, int64_t ldh) {
+    if (M <= 0 || N <= 0) return 0;
+    RACER_REQUIRE(lda % 8 == 0 && (uint64_t)M * lda * 2 < 0xFFFFFF00ull &&
+                      ((uint64_t)(K - 1) * ldb + N) * 2 < 0xFFFFFF00ull && ((uint64_t)(M - 1) * ldh + N) * 4 < 0xFFFFFF00ull,
+                  "bf16 product operands exceed the 32-bit offsets of the buffer loads");
+    dim3 grid((unsigned)(((N + TN - 1) / TN) * ((M + TM - 1) / TM))), block(256);
+    racer_timed timer(dev, "gemm_bf16_mask");
+    hipLaunchKernelGGL((k_gemm_bf16_nn<RACER_EPI_MASK, true>), grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc,
+                       M, N, K, H, ldh);
+    return racer_launch_check("gemm_nn_bf16");
+}
+
 // The batch splits into chunks of 4096 (both kernels); each chunk's partial
 // product fills one [M, ldc] slab of the workspace. At the PPO update's
 // 262144-column minibatches that is 64 chunks, 1152 workgroups for W0's
@@ -401,8 +440,8 @@ size_t racer_gemm_nt_bf16_partials(int M, int64_t ldc, int K) {
 
 // The MFMA weight-gradient kernel, then the chunks' partial products summed
 // in chunk order into C.
-template <bool BB16>
-static int wgrad_launch(racer_device* dev, const float* A, int64_t lda, const void* B, int64_t ldb, float* C,
+template <bool AB16, bool BB16>
+static int wgrad_launch(racer_device* dev, const void* A, int64_t lda, const void* B, int64_t ldb, float* C,
                         int64_t ldc, int M, int N, int K, float* partial) {
     const int chunk = NT_BF16_CHUNK;
     int chunks = (K + chunk - 1) / chunk;
@@ -411,8 +450,8 @@ static int wgrad_launch(racer_device* dev, const float* A, int64_t lda, const vo
     dim3 grid((unsigned)(tx * ty * chunks)), block(256);
     {
         racer_timed timer(dev, "wgrad_bf16");
-        hipLaunchKernelGGL(k_wgrad_bf16<BB16>, grid, block, 0, dev->stream, A, lda, B, ldb, partial, ldc, stride, M,
-                           N, K, chunk, tx, ty, chunks);
+        hipLaunchKernelGGL((k_wgrad_bf16<AB16, BB16>), grid, block, 0, dev->stream, A, lda, B, ldb, partial, ldc,
+                           stride, M, N, K, chunk, tx, ty, chunks);
     }
     if (racer_launch_check("wgrad_bf16")) return 1;
     int64_t total = (int64_t)M * N;
@@ -424,17 +463,22 @@ static int wgrad_launch(racer_device* dev, const float* A, int64_t lda, const vo
     return racer_launch_check("sum_chunks");
 }
 
-int racer_gemm_nt_bf16_x16(racer_device* dev, const float* A, int64_t lda, const uint16_t* B, int64_t ldb, float* C,
-                           int64_t ldc, int M, int N, int K, float* ws, size_t ws_len) {
+int racer_gemm_nt_bf16_io(racer_device* dev, const void* A, bool a16, int64_t lda, const void* B, bool b16,
+                          int64_t ldb, float* C, int64_t ldc, int M, int N, int K, float* ws, size_t ws_len) {
     if (M <= 0 || N <= 0) return 0;
-    RACER_REQUIRE(K > 0 && lda % 4 == 0 && ldb % 4 == 0 && K % 4 == 0 && (uintptr_t)A % 16 == 0 &&
-                      (uintptr_t)B % 8 == 0,
+    const uintptr_t aal = a16 ? 8 : 16, bal = b16 ? 8 : 16;
+    RACER_REQUIRE(K > 0 && lda % 4 == 0 && ldb % 4 == 0 && K % 4 == 0 && (uintptr_t)A % aal == 0 &&
+                      (uintptr_t)B % bal == 0,
                   "bf16 weight-gradient operands must be aligned to 4 elements");
-    RACER_REQUIRE(((uint64_t)(M - 1) * lda + K) * 4 < 0xFFFFFF00ull && ((uint64_t)(N - 1) * ldb + K) * 2 < 0xFFFFFF00ull,
+    RACER_REQUIRE(((uint64_t)(M - 1) * lda + K) * (a16 ? 2 : 4) < 0xFFFFFF00ull &&
+                      ((uint64_t)(N - 1) * ldb + K) * (b16 ? 2 : 4) < 0xFFFFFF00ull,
                   "bf16 product operands exceed the 32-bit offsets of the buffer loads");
     float* partial = nullptr;
     if (racer_partials(dev, racer_gemm_nt_bf16_partials(M, ldc, K), ws, ws_len, &partial)) return 1;
-    return wgrad_launch<true>(dev, A, lda, B, ldb, C, ldc, M, N, K, partial);
+    if (a16 && b16) return wgrad_launch<true, true>(dev, A, lda, B, ldb, C, ldc, M, N, K, partial);
+    if (a16) return wgrad_launch<true, false>(dev, A, lda, B, ldb, C, ldc, M, N, K, partial);
+    if (b16) return wgrad_launch<false, true>(dev, A, lda, B, ldb, C, ldc, M, N, K, partial);
+    return wgrad_launch<false, false>(dev, A, lda, B, ldb, C, ldc, M, N, K, partial);
 }
 
 int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
@@ -445,7 +489,7 @@ int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const flo
     // the MFMA tile kernel: 16-byte aligned rows and operands within the reach of 32-bit offsets
     const bool aligned = lda % 4 == 0 && ldb % 4 == 0 && K % 4 == 0 && (uintptr_t)A % 16 == 0 && (uintptr_t)B % 16 == 0;
     const bool reach = ((uint64_t)(M - 1) * lda + K) * 4 < 0xFFFFFF00ull && ((uint64_t)(N - 1) * ldb + K) * 4 < 0xFFFFFF00ull;
-    if (aligned && reach && K > 0) return wgrad_launch<false>(dev, A, lda, B, ldb, C, ldc, M, N, K, partial);
+    if (aligned && reach && K > 0) return wgrad_launch<false, false>(dev, A, lda, B, ldb, C, ldc, M, N, K, partial);
     const int chunk = NT_BF16_CHUNK;
     int chunks = K <= 0 ? 1 : (K + chunk - 1) / chunk;
     int64_t stride = (int64_t)M * ldc;

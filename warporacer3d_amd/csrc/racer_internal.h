@@ -323,16 +323,26 @@ int racer_gemm_nt_f32(racer_device* dev, const float* A, int64_t lda, const floa
                       int64_t ldc, int M, int N, int K, float* ws = nullptr, size_t ws_len = 0);
 // bf16 MFMA variants (operands rounded to bf16, float32 accumulation)
 // A is a bf16 copy with rows padded to a multiple of 8 values (lda % 8 == 0, zeros beyond K).
-int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const float* B, int64_t ldb, float* C,
+int racer_gemm_nn_bf16(racer_device* dev, const uint16_t* A, int64_t lda, const void* B, int64_t ldb, void* C,
                        int64_t ldc, int M, int N, int K, int epi, const float* H = nullptr, int64_t ldh = 0);
 size_t racer_gemm_nt_bf16_partials(int M, int64_t ldc, int K);
 int racer_gemm_nt_bf16(racer_device* dev, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
                        int64_t ldc, int M, int N, int K, float* ws = nullptr, size_t ws_len = 0);
-// racer_gemm_nt_bf16 with B already in bf16 (MFMA weight-gradient kernel only:
-// rows and K aligned to 4 elements, B 8-byte aligned); the same bits as
-// racer_gemm_nt_bf16 on B's float32 values.
-int racer_gemm_nt_bf16_x16(racer_device* dev, const float* A, int64_t lda, const uint16_t* B, int64_t ldb, float* C,
-                           int64_t ldc, int M, int N, int K, float* ws, size_t ws_len);
+// C[M, N] = (A[M, 3] B[3, N]) (1 - H^2), each output fmaf over k = 0, 1, 2
+// from 0: racer_gemm_nn_f32 with K = 3 and the mask epilogue, bit for bit;
+// out16: C stored in bf16 (nearest even).
+int racer_mask_k3(racer_device* dev, const float* A, const float* B, int64_t ldb, void* C, bool out16, int64_t ldc,
+                  int M, int N, const float* H, int64_t ldh);
+// racer_gemm_nt_bf16 with A and / or B already in bf16 (a16, b16; the MFMA
+// weight-gradient kernel only: rows and K aligned to 4 elements); the same
+// bits as racer_gemm_nt_bf16 on their float32 values.
+int racer_gemm_nt_bf16_io(racer_device* dev, const void* A, bool a16, int64_t lda, const void* B, bool b16,
+                          int64_t ldb, float* C, int64_t ldc, int M, int N, int K, float* ws, size_t ws_len);
+// racer_gemm_nn_bf16 with the mask epilogue, B in bf16 and C stored in bf16
+// (rounded to nearest even): the bits its bf16 consumers would load from the
+// float32 product.
+int racer_gemm_nn_bf16_mask16(racer_device* dev, const uint16_t* A, int64_t lda, const uint16_t* B, int64_t ldb,
+                              uint16_t* C, int64_t ldc, int M, int N, int K, const float* H, int64_t ldh);
 // dst[r, c] = bf16(src[r, c]) for c < cols, 0 for cols <= c < ld_dst.
 int racer_pack_bf16(racer_device* dev, const float* src, int64_t ld_src, uint16_t* dst, int64_t ld_dst, int rows,
                     int cols);

@@ -336,6 +336,9 @@ __device__ static inline uint4 racer_ld_b128(__amdgpu_buffer_rsrc_t r, uint32_t 
 __device__ static inline float2 racer_ld_f32x2(__amdgpu_buffer_rsrc_t r, uint32_t byte_offset) {
     return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)byte_offset, 0, 0));
 }
+__device__ static inline uint16_t racer_ld_u16(__amdgpu_buffer_rsrc_t r, uint32_t byte_offset) {
+    return (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)byte_offset, 0, 0);
+}
 #endif
 
 /* ---- Philox4x32-10 ---- */

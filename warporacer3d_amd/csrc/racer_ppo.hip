@@ -566,8 +566,11 @@ static size_t backward_partials(const racer_policy* p, int64_t cols) {
 }
 
 // ws: the weight gradients' split-K workspace (ws_len floats), or null for the
-// device's grow-only scratch. x16 (bf16 policies): x already in bf16, read by
-// the W0 gradient in place of x (the same bits; x is then unused).
+// device's grow-only scratch. x16 (bf16 policies, the PPO update's aligned
+// minibatches): x already in bf16, read by the W0 gradient in place of x; d2
+// and d1 are then stored in bf16 in their buffers (two values a float), as
+// each of their consumers rounds them on load, and d1's bias row is left
+// unset (no weight gradient reads it): the same weight gradients bit for bit.
 static int backward_raw(racer_policy* p, const float* x, racer_acts* a, const float* d3, float* d2, float* d1,
                         float* grad, int64_t cols, float* ws = nullptr, size_t ws_len = 0,
                         const uint16_t* x16 = nullptr) {
@@ -576,12 +579,16 @@ static int backward_raw(racer_policy* p, const float* x, racer_acts* a, const fl
     int64_t HA = p->HA, IN = p->IN;
     const float* h1 = a->h1->ptr;
     const float* h2 = a->h2->ptr;
+    const bool io16 = x16 != nullptr && p->precision == 1;
     // through (racer/ppo.mojo:125-138): d2 = (1 - h2^2) (w2t d3); d1 = (1 - h1^2) (w1t d2[:H]),
     // the second on MFMA in bf16 precision
-    if (racer_gemm_nn_f32(dev, p->w2t->ptr, RACER_OUT, d3, cols, d2, cols, (int)HA, (int)cols, RACER_OUT,
-                          RACER_EPI_MASK, h2, cols))
-        return 1;
-    if (p->precision == 1) {
+    if (racer_mask_k3(dev, p->w2t->ptr, d3, cols, d2, io16, cols, (int)HA, (int)cols, h2, cols)) return 1;
+    if (io16) {
+        if (refresh_bf16(p) ||
+            racer_gemm_nn_bf16_mask16(dev, p->w1t_bf16, p->h_pad, reinterpret_cast<const uint16_t*>(d2), cols,
+                                      reinterpret_cast<uint16_t*>(d1), cols, H, (int)cols, H, h1, cols))
+            return 1;
+    } else if (p->precision == 1) {
         // rows 0 .. H-1 on MFMA; row H is the bias row, masked by 1 - h1[H]^2 = 0
         // and read by no weight gradient (they take d1's first H rows), so it
         // is filled with zeros rather than spending a fifth row tile on it
@@ -595,9 +602,13 @@ static int backward_raw(racer_policy* p, const float* x, racer_acts* a, const fl
         return 1;
     }
     // weight gradients (racer/ppo.mojo:121-123)
-    if (p->precision == 1) {
-        if ((x16 ? racer_gemm_nt_bf16_x16(dev, d1, cols, x16, cols, grad, IN, H, (int)IN, (int)cols, ws, ws_len)
-                 : racer_gemm_nt_bf16(dev, d1, cols, x, cols, grad, IN, H, (int)IN, (int)cols, ws, ws_len)) ||
+    if (io16) {
+        if (racer_gemm_nt_bf16_io(dev, d1, true, cols, x16, true, cols, grad, IN, H, (int)IN, (int)cols, ws, ws_len) ||
+            racer_gemm_nt_bf16_io(dev, d2, true, cols, h1, false, cols, grad + p->W1, HA, H, (int)HA, (int)cols, ws,
+                                  ws_len))
+            return 1;
+    } else if (p->precision == 1) {
+        if (racer_gemm_nt_bf16(dev, d1, cols, x, cols, grad, IN, H, (int)IN, (int)cols, ws, ws_len) ||
             racer_gemm_nt_bf16(dev, d2, cols, h1, cols, grad + p->W1, HA, H, (int)HA, (int)cols, ws, ws_len))
             return 1;
     } else {
