@@ -399,6 +399,43 @@ def surrogate_leg(R, T, dev, transitions: int, epochs: int, cars: int = 16384) -
     rk4_us = per(fused, ["physics"])
     sur_us = per(fused, ["surrogate_fused"])
     sep_us = per(separate, ["surrogate_features", "gemm_bf16_tanh", "gemm_f32", "physics_surrogate"])
+
+    def restart(s):  # both twins back to the recorded sim's state before each timed window
+        dev.copy(s.state, 0, sim.state, 0, 11 * cars)
+        dev.copy(s.obs, 0, sim.obs, 0, (cfg.beams + 5) * cars)
+
+    def best_us(setup, fn, reps=5):
+        best = None
+        for _ in range(reps):
+            setup()
+            dev.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            dev.synchronize()
+            dt = (time.perf_counter() - t0) * 1e6
+            best = dt if best is None else min(best, dt)
+        return best
+
+    # closed loop: 8 Runner steps (policy forward with the sample, physics, exact lidar)
+    sur_acts = R.Activations(dev, cars, cfg)
+    window = {
+        "surrogate_ms": best_us(lambda: restart(imagined), lambda: [
+            sur.runner_step(imagined, pol, sur_acts, scratch, False, k) for k in range(8)]) / 1000.0,
+        "rk4_ms": best_us(lambda: restart(exact), lambda: [
+            R.runner_step(exact, pol, sur_acts, scratch, False, k) for k in range(8)]) / 1000.0,
+    }
+    window["speedup"] = window["rk4_ms"] / window["surrogate_ms"]
+    # open loop: 8 steps of the physics alone (no lidar, no policy), the actions given
+    acts8 = dev.upload(actions)
+    view8 = R.mat(acts8, 16, cars)
+    opened = {
+        "surrogate_one_launch_us": best_us(lambda: restart(imagined), lambda: sur.imagine(imagined, view8, 8)),
+        "rk4_one_launch_us": best_us(lambda: restart(exact), lambda: exact.imagine(view8, 8)),
+        "rk4_eight_launches_us": best_us(lambda: restart(exact), lambda: [
+            exact.imagine(R.mat(acts8, 2, cars, 2 * k * cars), 1) for k in range(8)]),
+    }
+    opened["speedup_vs_rk4_one_launch"] = opened["rk4_one_launch_us"] / opened["surrogate_one_launch_us"]
+    log("surrogate 8-step windows:", window, opened)
     return {"track": "bank10", "cars": cars, "beams": 1081, "transitions": int(data.count),
             "record_s": record_s, "train_s": train_s, "epochs": epochs, "minibatch": cars, "adam_steps": minibatches,
             "network": "11 -> 3 x 128 tanh -> 5, residual (pose change), bf16 MFMA hidden layers",
@@ -406,7 +443,8 @@ def surrogate_leg(R, T, dev, transitions: int, epochs: int, cars: int = 16384) -
             "fallback_fraction_8_steps": float(fell.mean()), "error_vs_float64_rk4": errors,
             "physics_us_per_step": {"rk4_float32": rk4_us, "surrogate_fused": sur_us,
                                     "surrogate_separate_kernels": sep_us,
-                                    "speedup": rk4_us / sur_us if sur_us else None}}
+                                    "speedup": rk4_us / sur_us if sur_us else None},
+            "runner_window_8_steps": window, "open_loop_8_steps": opened}
 
 
 def plain_baseline(R, dev, track, args, steps: int) -> dict:

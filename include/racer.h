@@ -102,6 +102,12 @@ int racer_sim_step(racer_sim* sim, const racer_view* actions, const racer_view* 
 /* Sim.sample (sim.mojo:323-348): out [3, n], log_std [1, 2]. */
 int racer_sim_sample(racer_sim* sim, const racer_view* out, const racer_view* log_std, const racer_view* actions,
                      const racer_view* logp, const racer_view* value, int deterministic, int64_t step);
+/* `steps` exact physics steps of every car (advance, sim.mojo:184-235) in
+ * one launch without the lidar: open-loop steps for planning, step k taking
+ * actions rows 2k, 2k+1 of the [2 steps, n] view; the state, reward, done and
+ * obs rows 0-3 are those of `steps` Sim.step calls, the obs range rows keep
+ * their values (no counterpart in the reference). */
+int racer_sim_imagine(racer_sim* sim, const racer_view* actions, int steps);
 /* The lidar pass of Sim.step alone (the second launch of sim.mojo:314-321):
  * every beam of every car from the current state and the obs slope rows
  * (2, 3) into the sim's obs range rows. */
@@ -316,6 +322,15 @@ int racer_surrogate_set_fused(racer_surrogate* s, int fused);
  * steer) in 12 n floats and is updated in place; friction comes from the sim's
  * state; no crash, progress or respawn. */
 int racer_sim_step_f64(racer_sim* sim, racer_buf* st, const racer_view* actions);
+/* Open-loop imagined steps (model-based planning): `steps` surrogate steps
+ * of every car in one launch, step k taking actions rows 2k, 2k+1 of the
+ * [2 steps, n] view, the network's weights loaded once; the lidar is not run
+ * (obs range rows keep their values; rows 0-3 and the state, reward, done
+ * are those of the last step, RETURN accumulates). fallback[i] = 1 when car i
+ * took the exact step on any of them. bf16 fused surrogates only. Bitwise
+ * the state of `steps` racer_surrogate_step calls with those actions. */
+int racer_surrogate_imagine(racer_surrogate* s, racer_sim* sim, const racer_view* actions, int steps,
+                            float min_clearance, racer_buf* fallback);
 /* Runner.step (main.mojo:183-191) with racer_surrogate_step. */
 int racer_surrogate_runner_step(racer_surrogate* s, racer_sim* sim, racer_policy* pol, racer_acts* acts,
                                 racer_buf* scratch, int deterministic, int64_t t, float min_clearance,

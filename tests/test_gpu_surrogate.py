@@ -187,3 +187,74 @@ def test_training_gains_over_untrained_network(dev):
           f"served by the network {served:.3f}", flush=True)
     assert e_trained < 0.1 * e_untrained, (e_trained, e_untrained)
     assert served > 0.9, served
+
+
+ROWS = ("state", "reward", "done")
+
+
+def _obs_rows(sim, beams):
+    """obs rows 0-3 and the bias row: what a step writes besides the lidar's ranges."""
+    o = sim.obs.read().reshape(beams + 5, sim.n)
+    return np.concatenate([o[:4], o[-1:]])
+
+
+@pytest.mark.parametrize("name,n", [("bank10", 1000), ("bridge", 333)])
+def test_imagined_steps_match_single_steps(dev, name, n):
+    """Eight open-loop imagined steps in one launch (weights loaded once)
+    against eight single fused surrogate steps with the same actions: the
+    same state, reward, done, obs rows and fallback cars, bit for bit."""
+    print(f"surrogate imagine {name} n={n}", flush=True)
+    beams = 64
+    _, sim, pol, acts, scratch = _setup(dev, n=n, beams=beams, name=name)
+    data = R.Dataset(dev, 24 * n)
+    data.record(sim, pol, acts, scratch, 24, 0)
+    sur = R.Surrogate(dev, seed=2, precision=1, cols=n)
+    sur.fit_normalization(data)
+    sur.train(data, epochs=1, batch=n, lr=2e-3)
+    rng = np.random.default_rng(6)
+    actions = rng.uniform(-1.2, 1.2, (8, 2, n)).astype(np.float32)
+    for clearance in (0.1, 2.0):
+        _, a, _, _, _ = _setup(dev, n=n, beams=beams, name=name, seed=8)
+        _, b, _, _, _ = _setup(dev, n=n, beams=beams, name=name, seed=8)
+        fa, fb = dev.alloc(n), dev.alloc(n)
+        sur.imagine(a, R.mat(dev.upload(actions), 16, n), 8, min_clearance=clearance, fallback=fa)
+        fell = np.zeros(n, bool)
+        for k in range(8):
+            b.actions.write(actions[k])
+            sur.step(b, min_clearance=clearance, fallback=fb)
+            fell |= fb.read() > 0
+        for buf in ROWS:
+            assert np.array_equal(getattr(a, buf).read(), getattr(b, buf).read()), (clearance, buf)
+        assert np.array_equal(_obs_rows(a, beams), _obs_rows(b, beams)), clearance
+        assert np.array_equal(fa.read() > 0, fell), clearance
+        print(f"min clearance {clearance}: fell back {fell.mean():.3f}", flush=True)
+
+
+@pytest.mark.parametrize("name,n", [("ramp10", 700), ("bridge8", 1024)])
+def test_sim_imagine_matches_steps(dev, name, n):
+    """Eight exact physics steps in one launch against eight Sim.step calls."""
+    print(f"sim imagine {name} n={n}", flush=True)
+    beams = 64
+    cfg = R.Config(beams=beams, heading_noise=0.1)
+    a, b = R.Sim(dev, track(name), n, 4, cfg), R.Sim(dev, track(name), n, 4, cfg)
+    a.reset()
+    b.reset()
+    actions = np.random.default_rng(7).uniform(-1.2, 1.2, (8, 2, n)).astype(np.float32)
+    a.imagine(R.mat(dev.upload(actions), 16, n), 8)
+    for k in range(8):
+        b.actions.write(actions[k])
+        b.step()
+    for buf in ROWS:
+        assert np.array_equal(getattr(a, buf).read(), getattr(b, buf).read()), buf
+    assert np.array_equal(_obs_rows(a, beams), _obs_rows(b, beams))
+
+
+def test_imagine_errors(dev):
+    cfg = R.Config(beams=32)
+    sim = R.Sim(dev, track("flat"), 64, 1, cfg)
+    acts = dev.alloc(2 * 64)
+    with pytest.raises(R.RacerError):
+        sim.imagine(R.mat(acts, 2, 64), 2)  # two steps need four action rows
+    sur = R.Surrogate(dev, seed=1, precision=0, cols=64)
+    with pytest.raises(R.RacerError, match="fused bf16"):
+        sur.imagine(sim, R.mat(acts, 2, 64), 1)

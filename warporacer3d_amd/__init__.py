@@ -195,6 +195,8 @@ def lib() -> C.CDLL:
         "racer_surrogate_train": (i, [vp, vp, i, i64, f]),
         "racer_surrogate_loss": (i, [vp, vp, C.POINTER(C.c_double)]),
         "racer_surrogate_step": (i, [vp, vp, f, vp]),
+        "racer_surrogate_imagine": (i, [vp, vp, VP, i, f, vp]),
+        "racer_sim_imagine": (i, [vp, VP, i]),
         "racer_surrogate_set_fused": (i, [vp, i]),
         "racer_sim_step_f64": (i, [vp, vp, VP]),
         "racer_surrogate_runner_step": (i, [vp, vp, vp, vp, vp, i, i64, f, vp]),
@@ -429,6 +431,10 @@ class Sim:
     @property
     def waves_per_car(self) -> int:
         return int(lib().racer_sim_waves_per_car(self.handle))
+
+    def imagine(self, actions: View, steps: int) -> None:
+        """`steps` exact physics steps in one launch (actions [2 steps, n]), no lidar: open-loop planning."""
+        _check(lib().racer_sim_imagine(self.handle, actions.c, int(steps)))
 
     def scan(self) -> None:
         """The lidar pass alone on the current state and obs slope rows (into obs)."""
@@ -782,6 +788,12 @@ class Surrogate:
     def step(self, sim: "Sim", min_clearance: float = 0.1, fallback: Buffer | None = None) -> None:
         _check(lib().racer_surrogate_step(self.handle, sim.handle, float(min_clearance),
                                           fallback.handle if fallback is not None else None))
+
+    def imagine(self, sim: "Sim", actions: View, steps: int, min_clearance: float = 0.1,
+                fallback: Buffer | None = None) -> None:
+        """`steps` open-loop imagined steps in one launch (actions [2 steps, n]); no lidar."""
+        _check(lib().racer_surrogate_imagine(self.handle, sim.handle, actions.c, int(steps), float(min_clearance),
+                                             fallback.handle if fallback is not None else None))
 
     def runner_step(self, sim: "Sim", policy: "Policy", acts: "Activations", scratch: Buffer, deterministic: bool,
                     t: int, min_clearance: float = 0.1, fallback: Buffer | None = None) -> None:

@@ -296,6 +296,14 @@ constexpr int RACER_SF_BASE1 = 4, RACER_SF_BASE2 = 40, RACER_SF_BLOCKS = 76;
 // w3: the output layer [5, 129]; stats: the normalisation (37 floats).
 int racer_launch_surrogate_fused(racer_sim* sim, const uint16_t* frags, const float* w3, const float* stats,
                                  float min_clearance, float* fallback);
+// `steps` of that step in one launch without the lidar (open-loop imagined
+// steps), step k with actions + 2 n k; fallback[i] = 1 if car i took the
+// exact step on any of them.
+int racer_launch_surrogate_imagine(racer_sim* sim, const float* actions, int steps, const uint16_t* frags,
+                                   const float* w3, const float* stats, float min_clearance, float* fallback);
+// `steps` exact physics steps (RK4) in one launch without the lidar, step k
+// with actions + 2 n k.
+int racer_launch_physics_steps(racer_sim* sim, const float* actions, int steps);
 
 // ---------------------------------------------------------------- kernels (launchers)
 int racer_launch_reset(racer_sim* sim);

@@ -487,15 +487,14 @@ __global__ void __launch_bounds__(256) k_surrogate_targets(const float* __restri
 // one (finish): crash from the probe at the predicted position, progress,
 // reward, respawn, obs rows. A car whose prediction lands below
 // min_clearance of the walls or on another layer takes the exact RK4 step
-// instead (fallback[i] = 1 when given).
+// instead; returns whether it did.
 template <int L>
-__device__ __forceinline__ void surrogate_apply(const racer_dmap& m, float* s, int n, const float* act,
+__device__ __forceinline__ bool surrogate_apply(const racer_dmap& m, float* s, int n, const float* act,
                                                 int64_t act_cols, float* reward, float* done, float* obs,
                                                 int64_t obs_cols, int beams, int seed, float heading_noise,
                                                 const float pred[5], const SurStart& c,
                                                 const float* __restrict__ out_mean,
-                                                const float* __restrict__ out_std, float min_clearance,
-                                                float* fallback, int i) {
+                                                const float* __restrict__ out_std, float min_clearance, int i) {
     float d[5];
 #pragma unroll
     for (int o = 0; o < 5; ++o) d[o] = pred[o] * out_std[o] + out_mean[o];
@@ -508,15 +507,15 @@ __device__ __forceinline__ void surrogate_apply(const racer_dmap& m, float* s, i
     float nearest = -1.0f;
     probe_surface<L>(m, p[0], p[1], c.now.height, &spot, &ground, &nearest);
     const bool back = !(spot.clearance >= min_clearance) || spot.layer != c.now.layer;
-    if (fallback) fallback[i] = back ? 1.0f : 0.0f;
     if (back) {
         advance<L>(m, s, n, load_car(s, n, act, act_cols, i), reward, done, obs, obs_cols, beams, i, seed,
                    heading_noise);
-        return;
+        return true;
     }
     const bool crashed = spot.clearance < RACER_HALF_DIAG || ground.sx * ground.sx + ground.sy * ground.sy > 1.0f;
     finish<L>(m, s, n, reward, done, obs, obs_cols, beams, i, seed, heading_noise, p, steer, spot, ground, crashed,
               nearest);
+    return false;
 }
 
 // The surrogate step with the predictions of a separate forward (pred rows ldp apart).
@@ -537,8 +536,9 @@ __global__ void __launch_bounds__(64) k_physics_surrogate(racer_dmap m, float* s
                s[RACER_SPEED * n + i], s[RACER_STEER * n + i], Spot{}};
     Surf ground;
     probe_surface<L>(m, c.x, c.y, c.z, &c.now, &ground);
-    surrogate_apply<L>(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, seed, heading_noise, q, c,
-                       out_mean, out_std, min_clearance, fallback, i);
+    const bool back = surrogate_apply<L>(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, seed,
+                                         heading_noise, q, c, out_mean, out_std, min_clearance, i);
+    if (fallback) fallback[i] = back ? 1.0f : 0.0f;
 }
 
 // The whole bf16 surrogate step for 64 cars per workgroup in one launch, on
@@ -561,22 +561,32 @@ constexpr int SF_NET_THREADS = 64 * SF_NET_WAVES;
 constexpr int SF_LDX = 152;  // bf16 row stride: 144 k + 8 (conflict-free 16-byte reads)
 constexpr int SF_LDH = 132;  // float row stride of h3 and w3s: 129 k padded to 16 bytes
 
-template <int L>
+//
+// MULTI: `steps` imagined steps in one launch (step k with act + k act_stride),
+// the second and third layers' fragments copied to LDS once (72 KB; kept in
+// registers across steps they would spill) and the output layer's weights
+// kept in LDS; each car's state is carried through the state rows it owns.
+// The one-step kernel (MULTI false) keeps the fragments in registers.
+constexpr int SF_FRAGS = 4 * RACER_SF_STEPS * 64;  // bf16x8 fragments of one 128-wide layer
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <int L, bool MULTI>
 __global__ void __launch_bounds__(SF_THREADS) k_surrogate_fused(racer_dmap m, float* s, int n, const float* act,
-                                                                int64_t act_cols, float* reward, float* done,
-                                                                float* obs, int64_t obs_cols, int beams, int seed,
+                                                                int64_t act_cols, int steps, int64_t act_stride,
+                                                                float* reward, float* done, float* obs,
+                                                                int64_t obs_cols, int beams, int seed,
                                                                 float heading_noise,
                                                                 const uint16_t* __restrict__ frags,
                                                                 const float* __restrict__ w3,
                                                                 const float* __restrict__ st, float min_clearance,
                                                                 float* fallback) {
-    typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-    typedef float f32x16 __attribute__((ext_vector_type(16)));
     __shared__ __attribute__((aligned(16))) uint16_t xa[SF_CARS][SF_LDX];
     __shared__ __attribute__((aligned(16))) uint16_t xb[SF_CARS][SF_LDX];
     __shared__ __attribute__((aligned(16))) float h3[SF_CARS][SF_LDH];  // float32 input of the output layer
     __shared__ __attribute__((aligned(16))) float w3s[5][SF_LDH];
     __shared__ float pred[5][SF_CARS];
+    __shared__ bf16x8 wfrag[MULTI ? 2 * SF_FRAGS : 1];  // MULTI: layers 1 and 2, [layer][row tile][k][lane]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool net = wave < SF_NET_WAVES;  // wavefront-uniform
     const int rt = wave & 3, car = 32 * (wave >> 2) + (lane & 31);  // a network lane's row tile and car
@@ -588,6 +598,7 @@ __global__ void __launch_bounds__(SF_THREADS) k_surrogate_fused(racer_dmap m, fl
     bf16x8 f0, f1[RACER_SF_STEPS], f2[RACER_SF_STEPS];
     float w3r[W3R];  // this lane's share of the output layer's weights, stored to LDS after the first layer
     SurStart c;
+    bool fell = false;  // the car wavefront's car took the exact step at least once
     if (net) {
 #pragma unroll
         for (int r = 0; r < W3R; ++r) {
@@ -607,29 +618,16 @@ __global__ void __launch_bounds__(SF_THREADS) k_surrogate_fused(racer_dmap m, fl
             const int k = 128 + 4 * t;
             *reinterpret_cast<uint2*>(&xb[r][k]) = make_uint2(k == 128 ? (uint32_t)one : 0u, 0u);
         }
-    } else {
-        float f[11];
-        if (i < n) {
-            surrogate_inputs<L>(m, s, n, act, act_cols, i, f, &c);
-        } else {
-#pragma unroll
-            for (int r = 0; r < 11; ++r) f[r] = 0.0f;
-        }
-#pragma unroll
-        for (int r = 0; r < 11; ++r)
-            xa[lane][r] = __builtin_bit_cast(uint16_t, (__bf16)((f[r] - st[r]) * st[11 + r]));
-        h3[lane][128] = 1.0f;
     }
-    __syncthreads();
-    auto tile = [&](const uint16_t (*x)[SF_LDX], const bf16x8* a, int steps) {
+    auto tile = [&](const uint16_t (*x)[SF_LDX], const bf16x8* a, int steps_k, int astride_k = 1) {
         f32x16 acc;
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc[q] = 0.0f;
 #pragma unroll
         for (int k = 0; k < RACER_SF_STEPS; ++k) {
-            if (k < steps) {
+            if (k < steps_k) {
                 const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&x[car][16 * k + 8 * half]);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k], bv, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[k * astride_k], bv, acc, 0, 0, 0);
             }
         }
         return acc;
@@ -645,54 +643,115 @@ __global__ void __launch_bounds__(SF_THREADS) k_surrogate_fused(racer_dmap m, fl
             *reinterpret_cast<uint2*>(&y[car][row0 + 8 * g]) = make_uint2(v[0] | (v[1] << 16), v[2] | (v[3] << 16));
         }
     };
-    if (net) {
-        // the second layer's fragments only now, so that they do not queue ahead of the car wavefront's loads
-#pragma unroll
-        for (int k = 0; k < RACER_SF_STEPS; ++k) f1[k] = fw[(size_t)(RACER_SF_BASE1 + rt * RACER_SF_STEPS + k) * 64];
-        const f32x16 acc = tile(xa, &f0, RACER_SF_STEPS0);
-#pragma unroll
-        for (int k = 0; k < RACER_SF_STEPS; ++k) f2[k] = fw[(size_t)(RACER_SF_BASE2 + rt * RACER_SF_STEPS + k) * 64];
-        store_bf16(xb, acc);
-#pragma unroll
-        for (int r = 0; r < W3R; ++r) {
-            const int idx = tid + SF_NET_THREADS * r;
-            if (idx < 5 * 129) w3s[idx / 129][idx % 129] = w3r[r];
+    if constexpr (MULTI) {
+        if (net) {  // both later layers' fragments into LDS, once
+            for (int idx = tid; idx < 2 * SF_FRAGS; idx += SF_NET_THREADS) {
+                const int layer = idx / SF_FRAGS, within = idx % SF_FRAGS;
+                const int block = (layer == 0 ? RACER_SF_BASE1 : RACER_SF_BASE2) + within / 64;
+                wfrag[idx] = reinterpret_cast<const bf16x8*>(frags)[(size_t)block * 64 + within % 64];
+            }
         }
     }
-    __syncthreads();
-    if (net) store_bf16(xa, tile(xb, f1, RACER_SF_STEPS));
-    __syncthreads();
-    if (net) {
-        const f32x16 acc = tile(xa, f2, RACER_SF_STEPS);
+    const int nsteps = MULTI ? steps : 1;
+    for (int step = 0; step < nsteps; ++step) {
+        const float* ak = act + (int64_t)step * act_stride;
+        if (!net) {
+            float f[11];
+            if (i < n) {
+                surrogate_inputs<L>(m, s, n, ak, act_cols, i, f, &c);
+            } else {
 #pragma unroll
-        for (int g = 0; g < 4; ++g)
-            *reinterpret_cast<float4*>(&h3[car][row0 + 8 * g]) =
-                make_float4(racer_tanh_mlp(acc[4 * g]), racer_tanh_mlp(acc[4 * g + 1]),
-                            racer_tanh_mlp(acc[4 * g + 2]), racer_tanh_mlp(acc[4 * g + 3]));
-    }
-    __syncthreads();
-    if (wave < 5) {
-        // output `wave` of car `lane`
-        const float4* wv = reinterpret_cast<const float4*>(w3s[wave]);
-        const float4* hv = reinterpret_cast<const float4*>(h3[lane]);
-        float acc = 0.0f;
+                for (int r = 0; r < 11; ++r) f[r] = 0.0f;
+            }
 #pragma unroll
-        for (int kb = 0; kb < 32; ++kb) {
-            const float4 a = wv[kb], x = hv[kb];
-            acc = fmaf(a.x, x.x, acc);
-            acc = fmaf(a.y, x.y, acc);
-            acc = fmaf(a.z, x.z, acc);
-            acc = fmaf(a.w, x.w, acc);
+            for (int r = 0; r < 11; ++r)
+                xa[lane][r] = __builtin_bit_cast(uint16_t, (__bf16)((f[r] - st[r]) * st[11 + r]));
+            if (step == 0) {
+                h3[lane][128] = 1.0f;
+            } else {  // the second layer overwrote k 11 .. 15 of the inputs' rows
+                xa[lane][11] = one;
+                *reinterpret_cast<uint2*>(&xa[lane][12]) = make_uint2(0u, 0u);
+            }
         }
-        pred[wave][lane] = fmaf(w3s[wave][128], h3[lane][128], acc);
-    }
-    __syncthreads();
-    if (!net && i < n) {
-        float q[5];
+        __syncthreads();
+        if (net) {
+            // the later layers' fragments only now, so that they do not queue ahead of the car wavefront's loads
+            if constexpr (!MULTI) {
 #pragma unroll
-        for (int r = 0; r < 5; ++r) q[r] = pred[r][lane];
-        surrogate_apply<L>(m, s, n, act, act_cols, reward, done, obs, obs_cols, beams, seed, heading_noise, q, c,
-                           st + 22, st + 27, min_clearance, fallback, i);
+                for (int k = 0; k < RACER_SF_STEPS; ++k)
+                    f1[k] = fw[(size_t)(RACER_SF_BASE1 + rt * RACER_SF_STEPS + k) * 64];
+            }
+            const f32x16 acc = tile(xa, &f0, RACER_SF_STEPS0);
+            if constexpr (!MULTI) {
+#pragma unroll
+                for (int k = 0; k < RACER_SF_STEPS; ++k)
+                    f2[k] = fw[(size_t)(RACER_SF_BASE2 + rt * RACER_SF_STEPS + k) * 64];
+            }
+            store_bf16(xb, acc);
+            if (step == 0) {
+#pragma unroll
+                for (int r = 0; r < W3R; ++r) {
+                    const int idx = tid + SF_NET_THREADS * r;
+                    if (idx < 5 * 129) w3s[idx / 129][idx % 129] = w3r[r];
+                }
+            }
+        }
+        __syncthreads();
+        const bf16x8* a1 = MULTI ? wfrag + (size_t)(rt * RACER_SF_STEPS) * 64 + lane : f1;
+        const bf16x8* a2 = MULTI ? wfrag + SF_FRAGS + (size_t)(rt * RACER_SF_STEPS) * 64 + lane : f2;
+        constexpr int astride = MULTI ? 64 : 1;  // fragments of consecutive k-steps
+        if (net) store_bf16(xa, tile(xb, a1, RACER_SF_STEPS, astride));
+        __syncthreads();
+        if (net) {
+            const f32x16 acc = tile(xa, a2, RACER_SF_STEPS, astride);
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                *reinterpret_cast<float4*>(&h3[car][row0 + 8 * g]) =
+                    make_float4(racer_tanh_mlp(acc[4 * g]), racer_tanh_mlp(acc[4 * g + 1]),
+                                racer_tanh_mlp(acc[4 * g + 2]), racer_tanh_mlp(acc[4 * g + 3]));
+        }
+        __syncthreads();
+        if (wave < 5) {
+            // output `wave` of car `lane`
+            const float4* wv = reinterpret_cast<const float4*>(w3s[wave]);
+            const float4* hv = reinterpret_cast<const float4*>(h3[lane]);
+            float acc = 0.0f;
+#pragma unroll
+            for (int kb = 0; kb < 32; ++kb) {
+                const float4 a = wv[kb], x = hv[kb];
+                acc = fmaf(a.x, x.x, acc);
+                acc = fmaf(a.y, x.y, acc);
+                acc = fmaf(a.z, x.z, acc);
+                acc = fmaf(a.w, x.w, acc);
+            }
+            pred[wave][lane] = fmaf(w3s[wave][128], h3[lane][128], acc);
+        }
+        __syncthreads();
+        if (!net && i < n) {
+            float q[5];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) q[r] = pred[r][lane];
+            fell |= surrogate_apply<L>(m, s, n, ak, act_cols, reward, done, obs, obs_cols, beams, seed,
+                                       heading_noise, q, c, st + 22, st + 27, min_clearance, i);
+        }
+    }
+    if (!net && i < n && fallback) fallback[i] = fell ? 1.0f : 0.0f;
+}
+
+// Open-loop steps of the physics alone (racer/sim.mojo:184-235 `steps`
+// times, the lidar not run): step k takes actions act + k act_stride; each car
+// carries its state through the rows it owns.
+template <int L>
+__global__ void __launch_bounds__(64) k_physics_steps(racer_dmap m, float* s, int n, const float* act,
+                                                      int64_t act_cols, int steps, int64_t act_stride,
+                                                      float* reward, float* done, float* obs, int64_t obs_cols,
+                                                      int beams, int seed, float heading_noise) {
+    const int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < steps; ++k) {
+        const float* ak = act + (int64_t)k * act_stride;
+        advance<L>(m, s, n, load_car(s, n, ak, act_cols, i), reward, done, obs, obs_cols, beams, i, seed,
+                   heading_noise);
     }
 }
 
@@ -1629,24 +1688,56 @@ int racer_launch_surrogate_step(racer_sim* sim, const float* pred, int64_t ldp, 
     return racer_launch_check("lidar");
 }
 
+int racer_launch_surrogate_imagine(racer_sim* sim, const float* actions, int steps, const uint16_t* frags,
+                                   const float* w3, const float* stats, float min_clearance, float* fallback) {
+    const int n = sim->n;
+    auto pick = [&](auto multi) {
+        constexpr bool M = decltype(multi)::value;
+        return sim->map.layers == 1 ? k_surrogate_fused<1, M>
+                                    : (sim->map.layers == 2 ? k_surrogate_fused<2, M> : k_surrogate_fused<0, M>);
+    };
+    auto kernel = steps == 1 ? pick(std::false_type{}) : pick(std::true_type{});
+    {
+        racer_timed timer(sim->dev, steps == 1 ? "surrogate_fused" : "surrogate_imagine");
+        hipLaunchKernelGGL(kernel, dim3((n + SF_CARS - 1) / SF_CARS), dim3(SF_THREADS), 0, sim->dev->stream, sim->map,
+                           sim->state->ptr, n, actions, (int64_t)n, steps, 2 * (int64_t)n, sim->reward->ptr,
+                           sim->done->ptr, sim->obs->ptr, (int64_t)n, sim->beams, sim->seed, sim->heading_noise,
+                           frags, w3, stats, min_clearance, fallback);
+    }
+    return racer_launch_check("surrogate fused");
+}
+
+int racer_launch_physics_steps(racer_sim* sim, const float* actions, int steps) {
+    const int n = sim->n;
+    auto kernel = sim->map.layers == 1 ? k_physics_steps<1>
+                                       : (sim->map.layers == 2 ? k_physics_steps<2> : k_physics_steps<0>);
+    {
+        racer_timed timer(sim->dev, "physics_steps");
+        hipLaunchKernelGGL(kernel, dim3((n + 63) / 64), dim3(64), 0, sim->dev->stream, sim->map, sim->state->ptr, n,
+                           actions, (int64_t)n, steps, 2 * (int64_t)n, sim->reward->ptr, sim->done->ptr,
+                           sim->obs->ptr, (int64_t)n, sim->beams, sim->seed, sim->heading_noise);
+    }
+    return racer_launch_check("physics steps");
+}
+
 int racer_launch_surrogate_fused(racer_sim* sim, const uint16_t* frags, const float* w3, const float* stats,
                                  float min_clearance, float* fallback) {
+    if (racer_launch_surrogate_imagine(sim, sim->actions->ptr, 1, frags, w3, stats, min_clearance, fallback))
+        return 1;
     const int n = sim->n;
-    auto kernel = sim->map.layers == 1 ? k_surrogate_fused<1>
-                                       : (sim->map.layers == 2 ? k_surrogate_fused<2> : k_surrogate_fused<0>);
-    {
-        racer_timed timer(sim->dev, "surrogate_fused");
-        hipLaunchKernelGGL(kernel, dim3((n + SF_CARS - 1) / SF_CARS), dim3(SF_THREADS), 0, sim->dev->stream, sim->map,
-                           sim->state->ptr, n, sim->actions->ptr, (int64_t)n, sim->reward->ptr, sim->done->ptr,
-                           sim->obs->ptr, (int64_t)n, sim->beams, sim->seed, sim->heading_noise, frags, w3, stats,
-                           min_clearance, fallback);
-    }
-    if (racer_launch_check("surrogate fused")) return 1;
     {
         racer_timed timer(sim->dev, "lidar");
         launch_lidar_any(sim, sim->obs->ptr, (int64_t)n);
     }
     return racer_launch_check("lidar");
+}
+
+extern "C" int racer_sim_imagine(racer_sim* sim, const racer_view* actions_p, int steps) {
+    racer_clear_error();
+    RACER_REQUIRE(sim && actions_p, "null argument");
+    RACER_REQUIRE(steps >= 1, "imagine at least one step");
+    if (racer_check_view(*actions_p, 2 * (int64_t)steps, sim->n, "actions")) return 1;
+    return racer_launch_physics_steps(sim, view_ptr(*actions_p), steps);
 }
 
 extern "C" int racer_sim_step_f64(racer_sim* sim, racer_buf* st, const racer_view* actions_p) {

@@ -600,6 +600,22 @@ extern "C" int racer_surrogate_step(racer_surrogate* s, racer_sim* sim, float mi
     return racer_surrogate_step_raw(s, sim, min_clearance, fallback ? fallback->ptr : nullptr);
 }
 
+extern "C" int racer_surrogate_imagine(racer_surrogate* s, racer_sim* sim, const racer_view* actions_p, int steps,
+                                       float min_clearance, racer_buf* fallback) {
+    racer_clear_error();
+    RACER_REQUIRE(s && sim && actions_p, "null argument");
+    RACER_REQUIRE(steps >= 1, "imagine at least one step");
+    RACER_REQUIRE(s->precision == 1 && s->fused, "open-loop imagined steps run the fused bf16 surrogate step");
+    RACER_REQUIRE(sim->n <= s->cols, "more cars than the surrogate's working buffers");
+    RACER_REQUIRE(s->dev == sim->dev, "the surrogate lives on another device");
+    RACER_REQUIRE(!fallback || fallback->len >= (size_t)sim->n, "fallback must hold one float per car");
+    if (racer_check_view(*actions_p, 2 * (int64_t)steps, sim->n, "actions")) return 1;
+    if (sur_refresh(s)) return 1;
+    return racer_launch_surrogate_imagine(sim, view_ptr(*actions_p), steps, s->frags,
+                                          s->theta->ptr + s->off[S_LAYERS - 1], s->stats->ptr, min_clearance,
+                                          fallback ? fallback->ptr : nullptr);
+}
+
 // Runner.step (main.mojo:183-191) with the surrogate step.
 extern "C" int racer_surrogate_runner_step(racer_surrogate* s, racer_sim* sim, racer_policy* p, racer_acts* acts,
                                            racer_buf* scratch, int deterministic, int64_t t, float min_clearance,
