@@ -41,9 +41,29 @@ constexpr int FRAG_TILES = 16;  // 16-row tiles of a fragment-order weight copy 
 __device__ __forceinline__ uint16_t bf16_of(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
 struct FusedLayout {
-    int in32, kx, ha32, ldh, HA;
+    int in32, kx, ha32, ldh, HA, ldo;  // ldo: row stride (floats) of h2 [column][k] and w2 [3][k]
     size_t xs, red, h1s, h2f, w2s, outs, total;  // byte offsets
 };
+
+// The output layer o[j][c] = sum_k w[k] h[k] of one output and column in
+// float32, an fmaf chain from 0 with k in order (so the same bits as one
+// fmaf per k), reading 4 consecutive k of the weight row and of the
+// column's h2 at once (both 16-byte aligned, k contiguous).
+__device__ __forceinline__ float output_chain(const float* w, const float* h, int HA) {
+    float a = 0.0f;
+    const int k4 = HA & ~3;
+#pragma unroll 8
+    for (int k = 0; k < k4; k += 4) {
+        const float4 wv = *reinterpret_cast<const float4*>(w + k);
+        const float4 hv = *reinterpret_cast<const float4*>(h + k);
+        a = fmaf(wv.x, hv.x, a);
+        a = fmaf(wv.y, hv.y, a);
+        a = fmaf(wv.z, hv.z, a);
+        a = fmaf(wv.w, hv.w, a);
+    }
+    for (int k = k4; k < HA; ++k) a = fmaf(w[k], h[k], a);
+    return a;
+}
 
 // wide: the reduction buffer and h2 also reuse the input slice's bytes, which
 // the first product's MFMA loop is the last to read (a barrier separates them),
@@ -55,9 +75,10 @@ __host__ __device__ inline FusedLayout fused_layout(int H, int IN, bool wide = f
     L.kx = L.in32 + 8;
     L.ha32 = (L.HA + 31) & ~31;
     L.ldh = L.ha32 + 8;
+    L.ldo = (L.HA + 3) & ~3;
     const size_t xs_bytes = (size_t)FT * L.kx * 2;
     const size_t red_bytes = (size_t)4 * 4 * 64 * 4 * 4;  // [4 row groups][4 tiles][64 lanes][4]
-    const size_t h2_bytes = (size_t)L.HA * FT * 4;
+    const size_t h2_bytes = (size_t)FT * L.ldo * 4;
     const size_t rh_bytes = red_bytes > h2_bytes ? red_bytes : h2_bytes;
     L.xs = 0;
     if (wide) {
@@ -70,7 +91,7 @@ __host__ __device__ inline FusedLayout fused_layout(int H, int IN, bool wide = f
         L.w2s = L.red + rh_bytes;
     }
     L.h2f = L.red;  // h2 reuses the reduction buffer
-    L.outs = L.w2s + (size_t)3 * L.HA * 4;
+    L.outs = L.w2s + (size_t)3 * L.ldo * 4;
     L.total = L.outs + (size_t)3 * FT * 4;
     return L;
 }
@@ -98,8 +119,8 @@ __device__ __forceinline__ void policy_tile(
     uint16_t* xs = reinterpret_cast<uint16_t*>(smem + Ly.xs);     // [FT][kx]
     uint16_t* h1s = reinterpret_cast<uint16_t*>(smem + Ly.h1s);   // [FT][ldh]
     float* red = reinterpret_cast<float*>(smem + Ly.red);         // [4][4][64][4]
-    float* h2f = reinterpret_cast<float*>(smem + Ly.h2f);         // [HA][FT]
-    float* w2s = reinterpret_cast<float*>(smem + Ly.w2s);         // [3][HA]
+    float* h2f = reinterpret_cast<float*>(smem + Ly.h2f);         // [FT][ldo]
+    float* w2s = reinterpret_cast<float*>(smem + Ly.w2s);         // [3][ldo]
     float* outs = reinterpret_cast<float*>(smem + Ly.outs);       // [3][FT]
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -115,30 +136,45 @@ __device__ __forceinline__ void policy_tile(
         // 16-byte loads of 4 adjacent columns (x 16-byte aligned, ldx and cols
         // multiples of 4): a thread takes 8 consecutive k of 4 columns with 8
         // loads and stores 4 columns' 8 k as 16-byte LDS writes
-        const int chunks = Ly.in32 / 8;
-        for (int idx = tid; idx < chunks * (FT / 4); idx += blockDim.x) {
-            const int cq = (idx % (FT / 4)) * 4, kc = (idx / (FT / 4)) * 8;
-            const int64_t col = c0 + cq;
-            float4 f[8];
+        // two items per thread in flight at once (1086 inputs: 544 items for
+        // 512 threads), so the slice costs one HBM round trip, not two
+        const int chunks = Ly.in32 / 8, items = chunks * (FT / 4);
+        const int nthreads = (int)blockDim.x;
+        for (int base = 0; base < items; base += 2 * nthreads) {
+            float4 f[2][8];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int k = kc + e;
-                const uint32_t off = (k < IN && col < cols) ? (uint32_t)(((int64_t)k * ldx + col) * 4) : RACER_OOB;
-                f[e] = __builtin_bit_cast(float4, racer_ld_b128(xres, off));
+            for (int h = 0; h < 2; ++h) {
+                const int idx = base + tid + h * nthreads;
+                const int cq = (idx % (FT / 4)) * 4, kc = (idx / (FT / 4)) * 8;
+                const int64_t col = c0 + cq;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int k = kc + e;
+                    const uint32_t off = (idx < items && k < IN && col < cols)
+                                             ? (uint32_t)(((int64_t)k * ldx + col) * 4) : RACER_OOB;
+                    f[h][e] = __builtin_bit_cast(float4, racer_ld_b128(xres, off));
+                }
             }
-            uint16_t v[4][8];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                v[0][e] = bf16_of(f[e].x);
-                v[1][e] = bf16_of(f[e].y);
-                v[2][e] = bf16_of(f[e].z);
-                v[3][e] = bf16_of(f[e].w);
+            for (int h = 0; h < 2; ++h) {
+                const int idx = base + tid + h * nthreads;
+                if (idx < items) {
+                    const int cq = (idx % (FT / 4)) * 4, kc = (idx / (FT / 4)) * 8;
+                    uint16_t v[4][8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        v[0][e] = bf16_of(f[h][e].x);
+                        v[1][e] = bf16_of(f[h][e].y);
+                        v[2][e] = bf16_of(f[h][e].z);
+                        v[3][e] = bf16_of(f[h][e].w);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        *reinterpret_cast<uint4*>(xs + (cq + q) * Ly.kx + kc) = *reinterpret_cast<uint4*>(v[q]);
+                }
             }
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                *reinterpret_cast<uint4*>(xs + (cq + q) * Ly.kx + kc) = *reinterpret_cast<uint4*>(v[q]);
         }
-        for (int idx = tid; idx < 3 * HA; idx += blockDim.x) w2s[idx] = W2[idx];
+        for (int idx = tid; idx < 3 * HA; idx += blockDim.x) w2s[(idx / HA) * Ly.ldo + idx % HA] = W2[idx];
     } else {
         const int chunks = Ly.in32 / 8;  // 8 consecutive k per chunk
         for (int idx = tid; idx < chunks * FT; idx += blockDim.x) {
@@ -156,7 +192,7 @@ __device__ __forceinline__ void policy_tile(
             for (int e = 0; e < 8; ++e) v[e] = bf16_of(f[e]);
             *reinterpret_cast<uint4*>(xs + c * Ly.kx + kc) = *reinterpret_cast<uint4*>(v);
         }
-        for (int idx = tid; idx < 3 * HA; idx += blockDim.x) w2s[idx] = W2[idx];
+        for (int idx = tid; idx < 3 * HA; idx += blockDim.x) w2s[(idx / HA) * Ly.ldo + idx % HA] = W2[idx];
     }
     __syncthreads();
 
@@ -298,19 +334,17 @@ __device__ __forceinline__ void policy_tile(
             const int row = 32 * wave + 16 * t + ib + q;
             if (row < H) {
                 float v = racer_tanh_mlp(acc2[t][q]);
-                h2f[row * FT + jc] = v;
+                h2f[jc * Ly.ldo + row] = v;
                 if (c0 + jc < cols) h2out[(int64_t)row * cols + c0 + jc] = v;
             }
         }
-    for (int j = tid; j < FT; j += blockDim.x) h2f[H * FT + j] = 1.0f;
+    for (int j = tid; j < FT; j += blockDim.x) h2f[j * Ly.ldo + H] = 1.0f;
     __syncthreads();
 
     // ---- phase 3: output layer o = W2 h2 in float32, k in order ----
     if (tid < 3 * FT) {
         const int j = tid / FT, c = tid % FT;
-        const float* w = w2s + j * HA;
-        float a = 0.0f;
-        for (int k = 0; k < HA; ++k) a = fmaf(w[k], h2f[k * FT + c], a);
+        const float a = output_chain(w2s + j * Ly.ldo, h2f + c * Ly.ldo, HA);
         outs[j * FT + c] = a;
         if (c0 + c < cols) o[j * cols + c0 + c] = a;
     }
@@ -400,7 +434,7 @@ constexpr int TRING = 2;         // k-steps of weight fragments in flight per wa
 constexpr int TDX = 2;           // staged blocks of inputs in flight (register sets)
 
 struct TiledLayout {
-    int HA, ha32, ldh;
+    int HA, ha32, ldh, ldo;  // ldo: row stride (floats) of h2 [column][k] and w2 [3][k]
     size_t xst, h2f, h1s, w2s, total;
 };
 
@@ -409,16 +443,17 @@ __host__ __device__ inline TiledLayout tiled_layout(int H) {
     L.HA = H + 1;
     L.ha32 = (L.HA + 31) & ~31;
     L.ldh = L.ha32 + 8;
+    L.ldo = (L.HA + 3) & ~3;
     const size_t xst_bytes = (size_t)2 * 2 * TC * TXLD * 2;       // [buffer][half][column][k]
     const size_t red_bytes = (size_t)4 * 4 * 2 * 64 * 4 * 4;      // one round: [4][4 row tiles][2 column tiles][64][4]
-    const size_t h2_bytes = (size_t)L.HA * TC * 4;                // [HA][TC]
+    const size_t h2_bytes = (size_t)TC * L.ldo * 4;               // [TC][ldo]
     size_t r0 = xst_bytes > red_bytes ? xst_bytes : red_bytes;
     r0 = r0 > h2_bytes ? r0 : h2_bytes;
     L.xst = 0;
     L.h2f = 0;  // the staged inputs, the halves' exchange and h2 share region 0 in turn
     L.h1s = r0;
     L.w2s = L.h1s + (size_t)TC * L.ldh * 2;
-    L.total = L.w2s + (size_t)3 * L.HA * 4;
+    L.total = L.w2s + (size_t)3 * L.ldo * 4;
     return L;
 }
 
@@ -438,7 +473,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_tiled(
     float* red = reinterpret_cast<float*>(smem + Ly.xst);         // [4][4][2][64][4] per round
     float* h2f = reinterpret_cast<float*>(smem + Ly.h2f);         // [HA][TC]
     uint16_t* h1s = reinterpret_cast<uint16_t*>(smem + Ly.h1s);   // [TC][ldh]
-    float* w2s = reinterpret_cast<float*>(smem + Ly.w2s);         // [3][HA]
+    float* w2s = reinterpret_cast<float*>(smem + Ly.w2s);         // [3][ldo]
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r = lane & 15, kg = (lane >> 4) * 8;  // MFMA operand lane layout
@@ -448,7 +483,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_tiled(
     const auto xres = racer_rsrc(x, ((uint64_t)(IN - 1) * ldx + cols) * XSZ);
     const auto w0res = racer_rsrc(F0, (uint64_t)FRAG_TILES * steps0 * 1024);
     const auto w1res = racer_rsrc(F1, (uint64_t)FRAG_TILES * steps1 * 1024);
-    for (int idx = tid; idx < 3 * HA; idx += blockDim.x) w2s[idx] = W2[idx];
+    for (int idx = tid; idx < 3 * HA; idx += blockDim.x) w2s[(idx / HA) * Ly.ldo + idx % HA] = W2[idx];
 
     // staging: thread tid covers k-half tid / 256 and 4 adjacent columns
     // (cq); piece m of a block is 4 consecutive k at kq(m), as 4 float4 loads
@@ -646,19 +681,17 @@ __global__ void __launch_bounds__(64 * WAVES) k_policy_tiled(
                 const int row = 32 * wave + 16 * t + ib + q, col = c * 16 + jc;
                 if (row < H) {
                     const float v = racer_tanh_mlp(acc2[t][c][q]);
-                    h2f[row * TC + col] = v;
+                    h2f[col * Ly.ldo + row] = v;
                     if (c0 + col < cols) h2out[(int64_t)row * cols + c0 + col] = v;
                 }
             }
-    for (int j = tid; j < TC; j += blockDim.x) h2f[H * TC + j] = 1.0f;
+    for (int j = tid; j < TC; j += blockDim.x) h2f[j * Ly.ldo + H] = 1.0f;
     __syncthreads();
 
     // ---- output layer o = W2 h2 in float32, k in order ----
     if (tid < 3 * TC) {
         const int j = tid / TC, c = tid % TC;
-        const float* w = w2s + j * HA;
-        float a = 0.0f;
-        for (int k = 0; k < HA; ++k) a = fmaf(w[k], h2f[k * TC + c], a);
+        const float a = output_chain(w2s + j * Ly.ldo, h2f + c * Ly.ldo, HA);
         if (c0 + c < cols) o[j * cols + c0 + c] = a;
     }
 }
