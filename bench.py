@@ -243,6 +243,9 @@ BATCH_TRACKS = ("ring10", "ramp10", "bank10", "bridge8", "random0", "random1", "
 BATCH_CARS = (256, 512, 1024, 2048, 4096, 4096, 8192, 8192)
 
 
+STEP_MODES = ("physics and lidar launches", "fused step", "fused step, persistent grid")
+
+
 def batch_leg(R, T, dev, steps: int) -> dict:
     """Eight uneven instances (flat, ramp, bank, bridge and four random routes;
     256 .. 8192 cars, 1081 beams, one 2 x 256 bf16 policy): Runner steps of all
@@ -266,8 +269,43 @@ def batch_leg(R, T, dev, steps: int) -> dict:
     batch = R.Batch(dev, sims, cfg)
     for t in range(5):
         batch.runner_step(pol, False, t)
-    wpc, t = batch.autotune(), 5
-    log("batch lidar wavefronts per car (library autotuner):", wpc)
+    # every launch shape of the batched step: 10 Runner steps replayed from a
+    # graph, three times from the same saved sim states (the cost of a step
+    # drifts as the cars spread over the tracks, so shapes are compared from
+    # one state), the best replay counting
+    names = ("state", "obs", "actions", "reward", "done")
+    dev.synchronize()
+    saved = [{k: getattr(s, k).read().copy() for k in names} for s in sims]
+
+    def restore():
+        for s, d in zip(sims, saved):
+            for k in names:
+                getattr(s, k).write(d[k])
+        dev.synchronize()
+
+    modes = {}
+    for mode in (0, 1, 2):
+        for w in (1, 2, 4):
+            batch.set_waves_per_car(w)
+            batch.set_fused_step(mode)
+            if batch.fused_step != mode:
+                continue
+            g = batch.capture(pol, 10)
+            best = math.inf
+            for r in range(4):
+                restore()
+                t0 = time.perf_counter()
+                g.launch(5)
+                dev.synchronize()
+                if r:
+                    best = min(best, time.perf_counter() - t0)
+            modes[f"{STEP_MODES[mode]}, {w} wavefronts per car"] = best * 1000.0 / 10
+            del g
+    log("batch step launch shapes (ms per Runner step, same state):", {k: round(v, 4) for k, v in modes.items()})
+    restore()
+    tuned = batch.autotune_runner(pol)
+    wpc, mode, t = tuned["waves_per_car"], tuned["fused_step"], 5
+    log("batch lidar wavefronts per car and step mode (library autotuner):", wpc, STEP_MODES[mode])
     graph = batch.capture(pol, chunk)
     graph.launch(t)
     dev.synchronize()
@@ -298,7 +336,8 @@ def batch_leg(R, T, dev, steps: int) -> dict:
     separate_ms = (time.perf_counter() - t0) * 1000.0 / steps
     del own, sims
     return {"instances": [{"track": t, "cars": n} for t, n in zip(BATCH_TRACKS, BATCH_CARS)], "beams": 1081,
-            "cars": total, "steps": steps, "lidar_waves_per_car": wpc,
+            "cars": total, "steps": steps, "lidar_waves_per_car": wpc, "step_mode": STEP_MODES[mode],
+            "launch_shapes_ms_per_step": modes,
             "batched_graph": {"ms_per_step": batched_ms, "env_steps_per_s": total * 1000.0 / batched_ms},
             "separate_graphs": {"ms_per_step": separate_ms, "env_steps_per_s": total * 1000.0 / separate_ms},
             "speedup": separate_ms / batched_ms}

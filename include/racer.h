@@ -262,6 +262,25 @@ racer_buf* racer_batch_scratch(racer_batch* b);
 int racer_batch_set_waves_per_car(racer_batch* b, int waves);
 int racer_batch_waves_per_car(const racer_batch* b);
 int racer_batch_autotune(racer_batch* b, int* chosen);
+/* The batched step's physics and lidar in one launch (the fused step of
+ * racer_sim_set_fused_step over workgroups of 8 cars of one sim; it applies
+ * when every sim's track has one or two layers and the lidar runs 1 or 2
+ * wavefronts per car). mode 0: two launches; 1: the fused step, one
+ * workgroup per 8 cars; 2: the fused step on a persistent grid of the
+ * workgroups the GPU holds at once, each running a fixed, XCD-grouped run of
+ * the 8-car blocks (no queue). Results are bitwise the same in every mode.
+ * racer_batch_autotune times every applicable mode with every wavefront
+ * count; racer_batch_fused_step reports the mode that runs (0 when the fused
+ * step does not apply). */
+int racer_batch_set_fused_step(racer_batch* b, int mode);
+int racer_batch_fused_step(const racer_batch* b);
+/* racer_runner_autotune for a batch: times `steps` batched Runner steps with
+ * `pol` (captured as racer_graph_batch graphs, replayed from the sims' saved
+ * states) for every wavefront count, step mode and, when the instances run
+ * racer_policy_act, forward kernel; keeps the fastest in b and pol, restores
+ * the sims' state, obs, actions, reward and done, and writes the choice (any
+ * pointer may be null). Not while capturing; the profiler must be off. */
+int racer_batch_runner_autotune(racer_batch* b, racer_policy* pol, int steps, int* waves_per_car, int* fused_step);
 int racer_batch_count_iterations(racer_batch* b, int on);
 int64_t racer_batch_last_iterations(const racer_batch* b);
 

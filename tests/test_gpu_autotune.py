@@ -108,12 +108,42 @@ def test_batch_autotune_restores_every_instance(dev):
     before = [{k: getattr(s, k).read().copy() for k in BUFFERS} for s in sims]
     chosen = batch.autotune()
     assert chosen == batch.waves_per_car in (1, 2, 4)
+    assert batch.fused_step in ((0, 1, 2) if chosen < 4 else (0,))
     for s, b in zip(sims, before):
         for k in BUFFERS:
             assert np.array_equal(b[k], getattr(s, k).read(), equal_nan=True), k
     batch.step()
     for s in alone:
         s.step()
+    for s, o in zip(sims, alone):
+        for k in BUFFERS:
+            assert np.array_equal(getattr(s, k).read(), getattr(o, k).read(), equal_nan=True), k
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_runner_autotune_restores_every_instance(dev, precision):
+    print(f"batch runner autotune precision={precision}", flush=True)
+    cfg = R.Config(beams=1081, hidden=64, precision=precision, heading_noise=0.1)
+    pol = R.Policy(dev, 42, cfg)
+    names, cars = ("ring10", "ramp10", "bridge8"), (256, 333, 700)
+    sims = [R.Sim(dev, track(k), n, 10 + i, cfg) for i, (k, n) in enumerate(zip(names, cars))]
+    alone = [R.Sim(dev, track(k), n, 10 + i, cfg) for i, (k, n) in enumerate(zip(names, cars))]
+    for s in sims + alone:
+        s.reset()
+    batch = R.Batch(dev, sims, cfg)
+    before = [{k: getattr(s, k).read().copy() for k in BUFFERS} for s in sims]
+    chosen = batch.autotune_runner(pol, 4)
+    assert chosen["waves_per_car"] == batch.waves_per_car in (1, 2, 4)
+    assert chosen["fused_step"] == batch.fused_step in (0, 1, 2)
+    for s, b in zip(sims, before):
+        for k in BUFFERS:
+            assert np.array_equal(b[k], getattr(s, k).read(), equal_nan=True), k
+    acts = [R.Activations(dev, s.n, cfg) for s in alone]
+    scratch = [dev.alloc(2 * s.n) for s in alone]
+    for t in range(2):
+        batch.runner_step(pol, False, t)
+        for s, a, sc in zip(alone, acts, scratch):
+            R.runner_step(s, pol, a, sc, False, t)
     for s, o in zip(sims, alone):
         for k in BUFFERS:
             assert np.array_equal(getattr(s, k).read(), getattr(o, k).read(), equal_nan=True), k

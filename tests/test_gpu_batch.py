@@ -104,6 +104,51 @@ def test_batch_iteration_count_and_autotune(dev):
     _assert_same(alone, together, "counted step")
 
 
+@pytest.mark.parametrize("waves", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batch_fused_step_matches_separate(dev, mode, waves):
+    """The fused batched step (one workgroup per 8 cars, or a persistent grid)
+    on 1- and 2-layer tracks with car counts that are not multiples of 8."""
+    print(f"batch fused step mode={mode} waves={waves}", flush=True)
+    cfg = R.Config(beams=300, hidden=64, precision=0, heading_noise=0.1)
+    pol = R.Policy(dev, 42, cfg)
+    alone, together = _instances(dev, cfg), _instances(dev, cfg)
+    acts = [R.Activations(dev, s.n, cfg) for s in alone]
+    scratch = [dev.alloc(2 * s.n) for s in alone]
+    batch = R.Batch(dev, together, cfg)
+    batch.set_waves_per_car(waves)
+    batch.set_fused_step(mode)
+    assert batch.fused_step == mode
+    for t in range(4):
+        for s, a, sc in zip(alone, acts, scratch):
+            R.runner_step(s, pol, a, sc, False, t)
+        batch.runner_step(pol, False, t)
+        _assert_same(alone, together, f"mode {mode} step {t}")
+    total = 0
+    for s in alone:
+        s.count_iterations(True)
+        s.step()
+        s.count_iterations(False)
+        total += s.last_iterations
+    batch.count_iterations(True)
+    batch.step()
+    batch.count_iterations(False)
+    assert batch.last_iterations == total > 0
+    _assert_same(alone, together, f"mode {mode} counted step")
+
+
+def test_batch_fused_step_applies_to_one_or_two_waves(dev):
+    cfg = R.Config(beams=64, hidden=16)
+    batch = R.Batch(dev, _instances(dev, cfg), cfg)
+    batch.set_fused_step(2)
+    batch.set_waves_per_car(4)
+    assert batch.fused_step == 0
+    batch.set_waves_per_car(2)
+    assert batch.fused_step == 2
+    with pytest.raises(R.RacerError, match="mode"):
+        batch.set_fused_step(3)
+
+
 def test_batch_errors(dev):
     cfg = R.Config(beams=32)
     a = R.Sim(dev, track("flat"), 8, 1, cfg)

@@ -202,3 +202,33 @@ def test_eight_instance_batch_matches_oracle(dev):
             assert np.array_equal(s.obs.read().reshape(beams + 5, o.n), o.obs), (t, k, "obs")
             lp, v = batch.logp_value(k)
             assert np.array_equal(lp, logp) and np.array_equal(v, value), (t, k, "logp/value")
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_eight_instance_fused_batch_matches_two_launches(dev, mode):
+    """The eight-instance shapes (3552 workgroups of 8 cars, several rounds of
+    the persistent grid) in each fused mode against the two-launch batch,
+    which the test above pins to the oracle."""
+    print(f"eight-instance batch, fused mode {mode}, 28416 cars x 256 beams", flush=True)
+    cfg = R.Config(beams=256, hidden=64, precision=0, heading_noise=0.1)
+    pol = R.Policy(dev, 42, cfg)
+
+    def make():
+        sims = []
+        for k, (name, n) in enumerate(zip(BATCH_TRACKS, BATCH_CARS)):
+            s = R.Sim(dev, track(name), n, 100 + k, cfg)
+            s.reset()
+            sims.append(s)
+        return sims, R.Batch(dev, sims, cfg)
+
+    sims0, b0 = make()
+    sims1, b1 = make()
+    b0.set_fused_step(0)
+    b1.set_fused_step(mode)
+    assert b1.fused_step == mode
+    for t in range(3):
+        b0.runner_step(pol, False, t)
+        b1.runner_step(pol, False, t)
+        for k, (a, b) in enumerate(zip(sims0, sims1)):
+            for name in ("state", "obs", "actions", "reward", "done"):
+                assert np.array_equal(getattr(a, name).read(), getattr(b, name).read()), (t, k, name)

@@ -210,6 +210,9 @@ def lib() -> C.CDLL:
         "racer_batch_set_waves_per_car": (i, [vp, i]),
         "racer_batch_waves_per_car": (i, [vp]),
         "racer_batch_autotune": (i, [vp, C.POINTER(C.c_int)]),
+        "racer_batch_set_fused_step": (i, [vp, i]),
+        "racer_batch_runner_autotune": (i, [vp, vp, i, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "racer_batch_fused_step": (i, [vp]),
         "racer_batch_count_iterations": (i, [vp, i]),
         "racer_batch_last_iterations": (i64, [vp]),
     }
@@ -689,10 +692,30 @@ class Batch:
     def waves_per_car(self) -> int:
         return int(lib().racer_batch_waves_per_car(self.handle))
 
+    def set_fused_step(self, mode: int) -> None:
+        """0: physics and lidar as two launches; 1: one fused launch, a workgroup per 8 cars;
+        2: the fused launch on a persistent grid (bitwise the same results in every mode)."""
+        _check(lib().racer_batch_set_fused_step(self.handle, int(mode)))
+
+    @property
+    def fused_step(self) -> int:
+        """The step mode that runs (0 when the fused step does not apply to these sims)."""
+        return int(lib().racer_batch_fused_step(self.handle))
+
     def autotune(self) -> int:
+        """Time replays of batched sim steps for every wavefront count and step mode from the
+        same saved states; keep the fastest (restoring the states) and return its wavefronts."""
         w = C.c_int()
         _check(lib().racer_batch_autotune(self.handle, C.byref(w)))
         return int(w.value)
+
+    def autotune_runner(self, policy: "Policy", steps: int = 8) -> dict:
+        """Time `steps` batched Runner steps with `policy` replayed from graphs for every
+        wavefront count, step mode and (for per-instance forwards) forward kernel, from the
+        same saved states; keep the fastest and restore the states."""
+        w, f = C.c_int(), C.c_int()
+        _check(lib().racer_batch_runner_autotune(self.handle, policy.handle, int(steps), C.byref(w), C.byref(f)))
+        return {"waves_per_car": int(w.value), "fused_step": int(f.value), "act_kernel": policy.act_kernel}
 
     def count_iterations(self, on: bool = True) -> None:
         _check(lib().racer_batch_count_iterations(self.handle, int(on)))

@@ -136,3 +136,15 @@ extern "C" int racer_batch_set_waves_per_car(racer_batch* b, int waves) {
 }
 extern "C" int racer_batch_waves_per_car(const racer_batch* b) { return b->waves_per_car; }
 
+extern "C" int racer_batch_set_fused_step(racer_batch* b, int mode) {
+    racer_clear_error();
+    RACER_REQUIRE(b && mode >= 0 && mode <= 2, "the fused step mode must be 0, 1 or 2");
+    b->fused_step = mode;
+    return 0;
+}
+
+extern "C" int racer_batch_fused_step(const racer_batch* b) {
+    int pose_at = 0;
+    return b->fused_step && racer_batch_fused_shmem(b, b->waves_per_car, &pose_at) != 0 ? b->fused_step : 0;
+}
+

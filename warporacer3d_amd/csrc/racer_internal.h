@@ -256,6 +256,9 @@ struct racer_batch {
     int beams = 0, hidden = 0;
     int64_t cars = 0;
     int waves_per_car = 2;
+    // 0: physics and lidar as two launches; 1: the fused step, one workgroup
+    // per 8 cars; 2: the fused step on a persistent grid (racer_batch_set_fused_step)
+    int fused_step = 0;
     std::vector<racer_acts*> acts;  // per instance, written by the batched forward
     racer_buf* scratch = nullptr;   // log probabilities then values: instance k's 2 n_k floats at 2 start_k
     std::vector<int64_t> car_start;
@@ -267,6 +270,8 @@ struct racer_batch {
 // Physics, then the lidar, of every instance of the batch (each reads its own
 // sim's actions and writes its own sim's reward, done, state and obs).
 int racer_launch_batch_step(racer_batch* b);
+// LDS bytes of the batched fused step, or 0 when it does not apply to the batch.
+size_t racer_batch_fused_shmem(const racer_batch* b, int waves_per_car, int* pose_at);
 int racer_launch_batch_lidar(racer_batch* b);
 int racer_policy_fused_batch(racer_policy* p, const racer_policy_table& tab, int deterministic, int64_t step);
 int racer_batch_runner_enqueue(racer_batch* b, racer_policy* p, int deterministic, int64_t t);

@@ -226,7 +226,11 @@ RACER_FN float racer_tanh_select(float x) {
     float e2 = 2.0f * a;
     e2 = e2 < 18.2f ? e2 : 18.2f;
     const float middle = 1.0f - 2.0f / (racer_exp_main(e2) + 1.0f);
-    const float y = a < 0.55f ? small : (a < 9.1f ? middle : ((a == a) ? 1.0f : a));
+    /* one select per statement: a conditional expression nested in another is
+     * emitted as control flow, and the paths' arithmetic then sinks into it */
+    const float big = (a == a) ? 1.0f : a;
+    const float upper = a < 9.1f ? middle : big;
+    const float y = a < 0.55f ? small : upper;
     return x < 0.0f ? -y : y;
 }
 

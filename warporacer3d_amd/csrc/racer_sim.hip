@@ -1251,15 +1251,18 @@ constexpr int phys_tile_floats() {
 
 // (eight wavefronts per SIMD, the two workgroups a CU holds: the physics
 // alone would take more scalar registers than that allows)
+//
+// step_block runs workgroup `block` of one sim's grid (cars 8 block ..);
+// k_step_fused launches one workgroup per block, k_step_fused_batch the blocks
+// of several sims.
 template <int L, bool COUNT>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) k_step_fused(racer_dmap m, float* s, int n, const float* act,
-                                                     float* reward, float* done, float* obs, int64_t obs_cols,
-                                                     int total_beams, int per_wave, int waves_per_car,
-                                                     const float2* __restrict__ dirs, unsigned long long* counter,
-                                                     int seed, float heading_noise, int pose_at) {
+__device__ __forceinline__ void step_block(const racer_dmap& m, float* s, int n, const float* act, float* reward,
+                                           float* done, float* obs, int64_t obs_cols, int total_beams, int per_wave,
+                                           int waves_per_car, const float2* __restrict__ dirs,
+                                           unsigned long long* counter, int seed, float heading_noise, int pose_at,
+                                           int block) {
     extern __shared__ float lds[];
     constexpr int T = PHYS_TILE, CELLS = T * T * L;
-    const int block = xcd_block();
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wg_waves = (int)(blockDim.x >> 6);
@@ -1313,6 +1316,51 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) 
     __syncthreads();
     lidar_wave<L, COUNT, true, true>(m, s, n, obs, obs_cols, total_beams, per_wave, waves_per_car, dirs, counter,
                                      global_wave, wg_car0, pose);
+}
+
+template <int L, bool COUNT>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) k_step_fused(racer_dmap m, float* s, int n, const float* act,
+                                                     float* reward, float* done, float* obs, int64_t obs_cols,
+                                                     int total_beams, int per_wave, int waves_per_car,
+                                                     const float2* __restrict__ dirs, unsigned long long* counter,
+                                                     int seed, float heading_noise, int pose_at) {
+    step_block<L, COUNT>(m, s, n, act, reward, done, obs, obs_cols, total_beams, per_wave, waves_per_car, dirs,
+                         counter, seed, heading_noise, pose_at, xcd_block());
+}
+
+// The fused step of several sims (one or two track layers each) in one
+// launch. tab.start holds the prefix sums of the instances' workgroups of
+// LIDAR_COOP_CARS cars, so a workgroup never spans two sims. With PERSIST the
+// grid is what the GPU holds at once and each workgroup runs blocks b, b +
+// gridDim.x, ... of a static order in which XCD x takes a contiguous run of
+// every gridDim.x blocks (no queue: every workgroup knows its blocks from its
+// index); otherwise one workgroup per block in the same XCD-grouped order.
+template <bool COUNT, bool PERSIST>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
+k_step_fused_batch(racer_sim_table tab, int per_wave, int waves_per_car, unsigned long long* counter, int pose_at) {
+    const int64_t total = tab.start[tab.count];
+    // the persistent order: round r covers blocks [G r, G r + G) (G = gridDim.x),
+    // workgroup w's block in it the (w / 8)-th of XCD w % 8's contiguous share;
+    // blocks grow with the round, so a workgroup past the end is done
+    const int G = gridDim.x;
+    const int64_t slot = PERSIST ? ((G & 7) == 0 ? (int64_t)(blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3)
+                                                 : (int64_t)blockIdx.x)
+                                 : (int64_t)xcd_block();
+    for (int64_t r0 = 0; r0 < total; r0 += PERSIST ? G : total) {
+        const int64_t b = r0 + slot;
+        if (b >= total) break;
+        int k = 0;
+        while (k + 1 < tab.count && b >= tab.start[k + 1]) ++k;
+        const racer_sim_slot& q = tab.slot[k];
+        const int local = (int)(b - tab.start[k]);
+        if (PERSIST && r0 > 0) __syncthreads();  // the previous block's range stores read LDS
+        if (q.map.layers == 1)
+            step_block<1, COUNT>(q.map, q.state, q.n, q.actions, q.reward, q.done, q.obs, q.n, q.beams, per_wave,
+                                 waves_per_car, q.dirs, counter, q.seed, q.heading_noise, pose_at, local);
+        else
+            step_block<2, COUNT>(q.map, q.state, q.n, q.actions, q.reward, q.done, q.obs, q.n, q.beams, per_wave,
+                                 waves_per_car, q.dirs, counter, q.seed, q.heading_noise, pose_at, local);
+    }
 }
 
 // The lidar of several sims in one launch: wavefront g belongs to the
@@ -1480,7 +1528,11 @@ static int lidar_coop_setup() {
                              (const void*)k_lidar<1, false, true>, (const void*)k_lidar<1, true, true>,
                              (const void*)k_lidar<2, false, true>, (const void*)k_lidar<2, true, true>,
                              (const void*)k_step_fused<1, false>,  (const void*)k_step_fused<1, true>,
-                             (const void*)k_step_fused<2, false>,  (const void*)k_step_fused<2, true>};
+                             (const void*)k_step_fused<2, false>,  (const void*)k_step_fused<2, true>,
+                             (const void*)k_step_fused_batch<false, false>,
+                             (const void*)k_step_fused_batch<true, false>,
+                             (const void*)k_step_fused_batch<false, true>,
+                             (const void*)k_step_fused_batch<true, true>};
     for (const void* k : kernels)
         RACER_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LIDAR_COOP_MAX_SHMEM));
     return 0;
@@ -1604,8 +1656,95 @@ static racer_sim_table sim_table(const racer_batch* b, int per_car) {
     return tab;
 }
 
+size_t racer_batch_fused_shmem(const racer_batch* b, int waves_per_car, int* pose_at) {
+    const int coop_waves = LIDAR_COOP_CARS * waves_per_car;
+    if (coop_waves > 16) return 0;
+    int layers = 1;
+    for (const racer_sim* sim : b->sims) {
+        if (sim->map.layers != 1 && sim->map.layers != 2) return 0;
+        layers = sim->map.layers > layers ? sim->map.layers : layers;
+    }
+    const size_t lidar = (size_t)coop_waves * LIDAR_RING + (size_t)LIDAR_COOP_CARS * b->beams;
+    const size_t tiles = (size_t)LIDAR_COOP_CARS * (layers == 1 ? phys_tile_floats<1>() : phys_tile_floats<2>());
+    const size_t at = lidar > tiles ? lidar : tiles;
+    const size_t bytes = (at + 8 * LIDAR_COOP_CARS) * sizeof(float);
+    if (bytes > LIDAR_COOP_MAX_SHMEM) return 0;
+    *pose_at = (int)at;
+    return bytes;
+}
+
+// The fused step of every instance: one launch over the instances'
+// workgroups of LIDAR_COOP_CARS cars (persistent when b->fused_step == 2).
+static int launch_batch_fused(racer_batch* b, size_t shmem, int pose_at) {
+    racer_device* dev = b->dev;
+    const int wpc = b->waves_per_car, per = (b->beams + wpc - 1) / wpc;
+    racer_sim_table tab{};
+    tab.count = (int)b->sims.size();
+    tab.start[0] = 0;
+    for (int k = 0; k < tab.count; ++k) {
+        const racer_sim* sim = b->sims[k];
+        racer_sim_slot& q = tab.slot[k];
+        q.map = sim->map;
+        q.state = sim->state->ptr;
+        q.obs = sim->obs->ptr;
+        q.actions = sim->actions->ptr;
+        q.reward = sim->reward->ptr;
+        q.done = sim->done->ptr;
+        q.dirs = sim->dirs;
+        q.n = sim->n;
+        q.seed = sim->seed;
+        q.beams = sim->beams;
+        q.heading_noise = sim->heading_noise;
+        tab.start[k + 1] = tab.start[k] + (sim->n + LIDAR_COOP_CARS - 1) / LIDAR_COOP_CARS;
+    }
+    const int64_t total = tab.start[tab.count];
+    const bool persist = b->fused_step == 2;
+    int64_t grid = total;
+    if (persist) {
+        const void* kernel = b->count_iterations ? (const void*)k_step_fused_batch<true, true>
+                                                 : (const void*)k_step_fused_batch<false, true>;
+        int per_cu = 0;
+        RACER_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * LIDAR_COOP_CARS * wpc, shmem));
+        const int64_t resident = (int64_t)(per_cu > 0 ? per_cu : 1) * dev->cus;
+        grid = total < resident ? total : (resident >= 8 ? resident & ~(int64_t)7 : resident);
+    }
+    const dim3 block(64 * LIDAR_COOP_CARS * wpc);
+    racer_timed timer(dev, persist ? "step_fused_batch_persistent" : "step_fused_batch");
+    if (b->count_iterations) {
+        if (persist)
+            hipLaunchKernelGGL((k_step_fused_batch<true, true>), dim3((unsigned)grid), block, shmem, dev->stream, tab,
+                               per, wpc, b->counter, pose_at);
+        else
+            hipLaunchKernelGGL((k_step_fused_batch<true, false>), dim3((unsigned)grid), block, shmem, dev->stream,
+                               tab, per, wpc, b->counter, pose_at);
+    } else {
+        if (persist)
+            hipLaunchKernelGGL((k_step_fused_batch<false, true>), dim3((unsigned)grid), block, shmem, dev->stream,
+                               tab, per, wpc, b->counter, pose_at);
+        else
+            hipLaunchKernelGGL((k_step_fused_batch<false, false>), dim3((unsigned)grid), block, shmem, dev->stream,
+                               tab, per, wpc, b->counter, pose_at);
+    }
+    return 0;
+}
+
 int racer_launch_batch_step(racer_batch* b) {
     racer_device* dev = b->dev;
+    int pose_at = 0;
+    const size_t fused = b->fused_step ? racer_batch_fused_shmem(b, b->waves_per_car, &pose_at) : 0;
+    if (fused) {
+        if (b->count_iterations)
+            RACER_TRY(hipMemsetAsync(b->counter, 0, 2 * sizeof(unsigned long long), dev->stream));
+        if (launch_batch_fused(b, fused, pose_at)) return 1;
+        if (racer_launch_check("fused batch step")) return 1;
+        if (b->count_iterations) {
+            unsigned long long count[2] = {0, 0};
+            RACER_TRY(hipMemcpyAsync(count, b->counter, sizeof(count), hipMemcpyDeviceToHost, dev->stream));
+            RACER_TRY(hipStreamSynchronize(dev->stream));
+            b->last_iterations = (int64_t)count[0];
+        }
+        return 0;
+    }
     {
         const racer_sim_table tab = sim_table(b, 1);
         racer_timed timer(dev, "physics");

@@ -217,33 +217,94 @@ extern "C" int racer_batch_autotune(racer_batch* b, int* chosen) {
     racer_event_pair ev;
     if (make_events(ev)) return 1;
     const bool counting = b->count_iterations;
-    const int waves0 = b->waves_per_car;
+    const int waves0 = b->waves_per_car, fused0 = b->fused_step;
     b->count_iterations = false;
-    int best_waves = waves0;
+    int best_waves = waves0, best_fused = fused0;
     float best = 1e30f;
     Saved saved;
     auto run = [&]() -> int {
         if (saved.save(dev, b->sims)) return 1;
-        for (int w : {1, 2, 4}) {
-            b->waves_per_car = w;
-            float ms = 0.0f;
-            if (time_candidate(dev, saved, ev, [&](racer_graph** g) { return racer_graph_batch_steps(b, 8, g); },
-                               &ms))
-                return 1;
-            if (ms < best) {
-                best = ms;
-                best_waves = w;
+        for (int w : {1, 2, 4})
+            for (int f : {0, 1, 2}) {
+                int pose_at = 0;
+                if (f && !racer_batch_fused_shmem(b, w, &pose_at)) continue;
+                b->waves_per_car = w;
+                b->fused_step = f;
+                float ms = 0.0f;
+                if (time_candidate(dev, saved, ev, [&](racer_graph** g) { return racer_graph_batch_steps(b, 8, g); },
+                                   &ms))
+                    return 1;
+                if (ms < best) {
+                    best = ms;
+                    best_waves = w;
+                    best_fused = f;
+                }
             }
-        }
         return 0;
     };
     const int failed = run();
     const int restored = saved.restore();
     b->count_iterations = counting;
     b->waves_per_car = failed ? waves0 : best_waves;
+    b->fused_step = failed ? fused0 : best_fused;
     if (failed || restored) return 1;
     RACER_TRY(hipStreamSynchronize(dev->stream));
     if (chosen) *chosen = best_waves;
+    return 0;
+}
+
+extern "C" int racer_batch_runner_autotune(racer_batch* b, racer_policy* pol, int steps, int* waves_per_car,
+                                           int* fused_step) {
+    racer_clear_error();
+    RACER_REQUIRE(b && pol, "null argument");
+    RACER_REQUIRE(steps >= 1, "autotuning needs at least one step");
+    RACER_REQUIRE(pol->dev == b->dev, "the policy lives on another device");
+    racer_device* dev = b->dev;
+    if (tune_preconditions(dev)) return 1;
+    racer_event_pair ev;
+    if (make_events(ev)) return 1;
+    const bool counting = b->count_iterations;
+    const int waves0 = b->waves_per_car, fused0 = b->fused_step, kernel0 = pol->act_kernel;
+    b->count_iterations = false;
+    int best_waves = waves0, best_fused = fused0, best_kernel = kernel0;
+    float best = 1e30f;
+    // the batched bf16 forward has one kernel; otherwise each instance runs racer_policy_act
+    const bool batched_forward = pol->precision == 1 && pol->hidden <= 256;
+    const std::vector<int> kernels = batched_forward ? std::vector<int>{kernel0} : act_candidates(pol);
+    Saved saved;
+    auto run = [&]() -> int {
+        if (saved.save(dev, b->sims)) return 1;
+        for (int k : kernels)
+            for (int w : {1, 2, 4})
+                for (int f : {0, 1, 2}) {
+                    int pose_at = 0;
+                    if (f && !racer_batch_fused_shmem(b, w, &pose_at)) continue;
+                    pol->act_kernel = k;
+                    b->waves_per_car = w;
+                    b->fused_step = f;
+                    float ms = 0.0f;
+                    if (time_candidate(dev, saved, ev,
+                                       [&](racer_graph** g) { return racer_graph_batch(b, pol, steps, 0, g); }, &ms))
+                        return 1;
+                    if (ms < best) {
+                        best = ms;
+                        best_waves = w;
+                        best_fused = f;
+                        best_kernel = k;
+                    }
+                }
+        return 0;
+    };
+    const int failed = run();
+    const int restored = saved.restore();
+    b->count_iterations = counting;
+    b->waves_per_car = failed ? waves0 : best_waves;
+    b->fused_step = failed ? fused0 : best_fused;
+    pol->act_kernel = failed ? kernel0 : best_kernel;
+    if (failed || restored) return 1;
+    RACER_TRY(hipStreamSynchronize(dev->stream));
+    if (waves_per_car) *waves_per_car = best_waves;
+    if (fused_step) *fused_step = best_fused;
     return 0;
 }
 
