@@ -22,10 +22,14 @@ struct Surf {
     float height, sx, sy;
 };
 
-// Nearest cell of (x, y) (racer/sim.mojo:108-114); false off the map.
+// Nearest cell of (x, y) (racer/sim.mojo:108-114); false off the map. DIV:
+// floor((x - x0) / cell + 0.5) with the division, as the reference writes it
+// (the reference decomposition's lidar); otherwise racer_nearest_index, the
+// same index from the reciprocal with the division only near a half-integer.
+template <bool DIV = false>
 __device__ __forceinline__ bool cell_of(const racer_dmap& m, float x, float y, int64_t* cell) {
-    float fx = racer_nearest_index(x - m.x0, m.cell, m.inv_cell);
-    float fy = racer_nearest_index(y - m.y0, m.cell, m.inv_cell);
+    float fx = DIV ? floorf((x - m.x0) / m.cell + 0.5f) : racer_nearest_index(x - m.x0, m.cell, m.inv_cell);
+    float fy = DIV ? floorf((y - m.y0) / m.cell + 0.5f) : racer_nearest_index(y - m.y0, m.cell, m.inv_cell);
     if (!(fx >= 0.0f) || !(fy >= 0.0f) || fx >= (float)m.nx || fy >= (float)m.ny) return false;
     *cell = (int64_t)fy * m.nx + (int64_t)fx;
     return true;
@@ -42,9 +46,10 @@ __device__ __forceinline__ float map_at(const racer_dmap& m, int layer, int whic
 }
 
 // Map.probe (racer/sim.mojo:116-124): the drivable layer whose height is nearest z.
+template <bool DIV = false>
 __device__ Spot map_probe(const racer_dmap& m, float x, float y, float z) {
     int64_t c = 0;
-    bool on = cell_of(m, x, y, &c);
+    bool on = cell_of<DIV>(m, x, y, &c);
     Spot best{0, -1.0f, on ? grid_at(m, 0, RACER_HEIGHT, c) : -1.0f};
     float gap = 1e30f;
     for (int l = 0; l < m.layers; ++l) {
@@ -1407,8 +1412,9 @@ __global__ void __launch_bounds__(64) k_physics_batch(racer_sim_table tab) {
 
 // The reference's own lidar launch (racer/sim.mojo:317-321 through the 2-D
 // Device.run of racer/device.mojo:67-79): one thread per (car, beam), beams
-// fastest, every thread recomputing its car's frame and its beam's azimuth.
-// Same arithmetic as k_lidar; kept as the schedule of the plain baseline.
+// fastest, every thread recomputing its car's frame and its beam's azimuth,
+// and every probe's nearest cell by Map.at's division. The same results as
+// k_lidar; kept as the schedule of the plain baseline.
 __global__ void __launch_bounds__(256) k_lidar_flat(racer_dmap m, const float* __restrict__ s, int n,
                                                     float* __restrict__ obs, int64_t obs_cols, int beams) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1434,7 +1440,7 @@ __global__ void __launch_bounds__(256) k_lidar_flat(racer_dmap m, const float* _
     bool done = false;
     while (t < RACER_RANGE) {
         float pz = oz + t * dz;
-        Spot spot = map_probe(m, ox + t * dx, oy + t * dy, pz);
+        Spot spot = map_probe<true>(m, ox + t * dx, oy + t * dy, pz);
         float fl = (pz - spot.height) - 0.03f;
         if (fl < 0.0f) {
             range = racer_min(last + ((t - last) * last_floor) / (last_floor - fl), RACER_RANGE);
