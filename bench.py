@@ -183,7 +183,7 @@ def ppo_leg(R, T, dev, beams: int, iterations: int, cars: int = 16384, precision
             "config": {"track": "bridge8 (two 8 m lobes, 2 layers)", "cars": cars, "beams": beams, "hidden": 256,
                        "rollout": 128, "epochs": 4, "minibatches": 8,
                        "precision": {0: "fp32", 1: "bf16+fp32", 2: "plain fp32"}[precision],
-                       "lidar_waves_per_car": sim.waves_per_car},
+                       "lidar_waves_per_car": sim.waves_per_car, "lidar_coop_stores": sim.lidar_coop},
             "kernels_ms_per_iteration": {k: tot for k, (n, tot) in top},
             "forward_error_vs_float64": precision_check}
 
@@ -546,6 +546,7 @@ def runner_leg(R, dev, arr, args, precision: str, layers: int) -> dict:
     tune = run.autotune(10)
     t = args.warmup
     log(f"[{precision}] launch shapes (library autotuner): lidar wavefronts per car {tune['waves_per_car']}, "
+        f"{'cooperative' if tune['lidar_coop'] else 'per-wavefront'} range stores, "
         f"{'fused' if tune['fused_step'] else 'two-launch'} sim step, {ACT_NAMES[tune['act_kernel']]} forward")
     sim_kernel = "step_fused" if run.sim.fused_step else "lidar"
     # march iterations of the lidar, counted by the instrumented variant on the warm state (untimed)
@@ -639,6 +640,7 @@ def runner_leg(R, dev, arr, args, precision: str, layers: int) -> dict:
     kernels = {k: {"launches": n, "avg_us": tot / max(n, 1) * 1000.0} for k, (n, tot) in prof.items()}
     return {"value": value, "ms_per_step": ms, "eager_ms_per_step": eager_ms, "roofline": roofline,
             "policy_roofline": policy, "kernels": kernels, "lidar_waves_per_car": tune["waves_per_car"],
+            "lidar_coop_stores": tune["lidar_coop"],
             "fused_step": tune["fused_step"], "act_kernel": ACT_NAMES[tune["act_kernel"]],
             "lidar_wavefront_trips": trips}
 
@@ -736,15 +738,16 @@ def main() -> int:
                                "products (bitwise the CPU restatement), float32 dynamics and lidar",
                    "cars": args.cars, "beams": args.beams, "hidden": args.hidden, "track": args.track,
                    "policy_precision": "fp32", "parallelism": "single GPU",
-                   "lidar_waves_per_car": fp32["lidar_waves_per_car"], "fused_step": fp32["fused_step"],
-                   "act_kernel": fp32["act_kernel"]},
+                   "lidar_waves_per_car": fp32["lidar_waves_per_car"], "lidar_coop_stores": fp32["lidar_coop_stores"],
+                   "fused_step": fp32["fused_step"], "act_kernel": fp32["act_kernel"]},
         "eager_ms_per_step": fp32["eager_ms_per_step"],
         "timing": "graph replays of captured Runner steps; kernel durations from HIP events in eager passes of "
                   "the same steps right after",
         "roofline": fp32["roofline"], "policy_roofline": fp32["policy_roofline"], "kernels": fp32["kernels"],
         "value_bf16": bf16["value"] if bf16 else None,
         "bf16": ({k: bf16[k] for k in ("value", "ms_per_step", "eager_ms_per_step", "roofline", "policy_roofline",
-                                       "kernels", "lidar_waves_per_car", "fused_step", "act_kernel")}
+                                       "kernels", "lidar_waves_per_car", "lidar_coop_stores", "fused_step",
+                                       "act_kernel")}
                  if bf16 else None),
         "cpu_baseline": cpu,
         "plain_gpu_baseline": plain, "speedup_vs_plain_gpu": value / plain["value"] if plain else None,

@@ -227,8 +227,9 @@ int racer_benchmark(racer_sim* sim, racer_policy* pol, racer_acts* acts, racer_b
 int racer_policy_set_act_kernel(racer_policy* pol, int kernel);
 int racer_policy_act_kernel(const racer_policy* pol);
 /* Times `steps` Runner steps replayed from a captured graph for every
- * combination of lidar wavefronts per car (1, 2, 4), fused or two-launch sim
- * step (where the fused one applies) and forward kernel (those that apply to
+ * combination of lidar wavefronts per car (1, 2, 4), cooperative or
+ * per-wavefront range stores or the fused sim step (where it applies) and
+ * forward kernel (those that apply to
  * the policy's precision), from the same saved sim state each time (best of
  * three replays); keeps the fastest in sim and pol and restores the sim's
  * state, obs, actions, reward and done. Writes the choice (any pointer may be
@@ -408,6 +409,13 @@ int racer_sim_schedule(const racer_sim* sim);
  * results are identical for every choice). */
 int racer_sim_set_waves_per_car(racer_sim* sim, int waves);
 int racer_sim_waves_per_car(const racer_sim* sim);
+/* Range stores of the default schedule: 1 (default) = cooperative, each
+ * workgroup of 8 wavefronts (8 / waves-per-car cars) writes its cars' ranges
+ * together as runs of consecutive cars; 0 = each wavefront writes its own.
+ * No reference counterpart (a launch-shape choice under racer/sim.mojo:317-321);
+ * results are identical either way. */
+int racer_sim_set_lidar_coop(racer_sim* sim, int on);
+int racer_sim_lidar_coop(const racer_sim* sim);
 /* One launch for the whole step (physics, then the lidar, in the same
  * workgroups, with each car's neighbourhood of the track staged in LDS) in
  * place of the two launches of racer/sim.mojo:314-321. It applies with the
@@ -417,11 +425,12 @@ int racer_sim_waves_per_car(const racer_sim* sim);
 int racer_sim_set_fused_step(racer_sim* sim, int on);
 int racer_sim_fused_step(const racer_sim* sim);
 /* Autotuner: times `racer_sim_step`s replayed from captured graphs with 1, 2
- * and 4 wavefronts per car, each fused and two-launch where the fused step
- * applies, from the same saved state each time; keeps the fastest (waves per
- * car written to `chosen`, the fused choice read by racer_sim_fused_step) and
- * restores the sim's state, obs, actions, reward and done. Not while
- * capturing; the profiler must be off. */
+ * and 4 wavefronts per car, each with cooperative and per-wavefront range
+ * stores and fused where the fused step applies, from the same saved state
+ * each time; keeps the fastest (waves per car written to `chosen`, the other
+ * choices read by racer_sim_fused_step and racer_sim_lidar_coop) and restores
+ * the sim's state, obs, actions, reward and done. Not while capturing; the
+ * profiler must be off. */
 int racer_sim_autotune(racer_sim* sim, int* chosen);
 
 #ifdef __cplusplus

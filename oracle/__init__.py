@@ -64,6 +64,8 @@ def lib():
             "o_check_nearest_index_lim": (L, [F, F, F]),
             "o_check_div_by_reciprocal": (L, [L, C.c_uint64]),
             "o_check_nearest_by_reciprocal": (L, [C.c_float, L]),
+            "o_check_nearest_by_reciprocal1": (L, [C.c_float, L]),
+            "o_nearest_form_bad": (L, [C.c_float, C.c_int, C.c_int, C.c_int]),
             "o_check_tanh_select": (L, [L]),
             "o_surrogate_features": (None, [P, C.c_size_t, P, L, P, L, P, L]),
             "o_surrogate_targets": (None, [P, P, P, L, P, L, P]),

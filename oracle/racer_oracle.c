@@ -748,6 +748,30 @@ long o_check_nearest_by_reciprocal(float cell, long stride) {
     return bad;
 }
 
+/* The lidar's one-step nearest cell (racer_nearest_by_reciprocal1) against
+ * floorf(a / cell + 0.5f) over every stride-th float pattern: the same index
+ * wherever either is a number below 2^22 in magnitude (any grid with its
+ * padding). Elsewhere both are off every grid: beyond 2^22 or infinite, or
+ * NaN where the quotient overflows (q0 = inf makes the residual NaN); the
+ * unpadded probe's on-grid test rejects all of these alike. */
+long o_check_nearest_by_reciprocal1(float cell, long stride) {
+    const float inv = 1.0f / cell;
+    long bad = 0;
+    const int64_t count = 0xffffffffll / stride + 1;
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(+ : bad)
+    for (int64_t k = 0; k < count; ++k) {
+        float a = racer_u2f((uint32_t)(k * stride));
+        float want = floorf(a / cell + 0.5f);
+        float got = racer_nearest_by_reciprocal1(a, cell, inv);
+        const int in_want = racer_abs(want) < 0x1p22f, in_got = racer_abs(got) < 0x1p22f;  /* false for NaN */
+        if ((in_want || in_got) && racer_f2u(want) != racer_f2u(got)) ++bad;
+    }
+    return bad;
+}
+
+/* The per-map check the library runs at sim creation (racer_nearest_form_bad). */
+long o_nearest_form_bad(float cell, int nlo, int nhi, int form) { return racer_nearest_form_bad(cell, nlo, nhi, form); }
+
 /* ---------------- Learned dynamics surrogate (no reference counterpart) ---------------- */
 
 /* The surrogate's 11 inputs of car i and its constant-1 row (the HIP feature

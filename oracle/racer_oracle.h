@@ -140,6 +140,8 @@ long o_check_nearest_index(float b, float lo, float hi);
 long o_check_nearest_index_lim(float b, float lo, float hi);
 long o_check_div_by_reciprocal(long count, uint64_t seed);
 long o_check_nearest_by_reciprocal(float cell, long stride);
+long o_check_nearest_by_reciprocal1(float cell, long stride);
+long o_nearest_form_bad(float cell, int nlo, int nhi, int form);
 long o_check_tanh_select(long stride);
 
 /* Threads used by the parallel loops (1 = serial). */

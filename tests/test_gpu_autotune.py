@@ -1,7 +1,7 @@
 """Launch-shape choices and the autotuners that pick among them: every choice
-(lidar wavefronts per car, fused or two-launch sim step, forward kernel of the
-Runner step) gives bitwise the same Runner steps, and the autotuners leave the
-sim's buffers as they found them."""
+(lidar wavefronts per car, cooperative or per-wavefront range stores, fused or
+two-launch sim step, forward kernel of the Runner step) gives bitwise the same
+Runner steps, and the autotuners leave the sim's buffers as they found them."""
 import numpy as np
 import pytest
 
@@ -34,20 +34,21 @@ def test_every_launch_choice_gives_the_same_steps(dev, precision, n, beams, hidd
         (R.ACT_AUTO, R.ACT_FUSED, R.ACT_TILED)
     for kernel in kernels:
         for waves in (1, 2, 4):
-            for fused in (False, True):
+            for shape in ("coop", "per-wavefront", "fused"):
                 run = R.Runner(track("ramp10"), n, cfg, seed=9, device=dev)
                 for t in range(3):
                     run.step(False, t)
                 run.policy.set_act_kernel(kernel)
                 run.sim.set_waves_per_car(waves)
-                run.sim.set_fused_step(fused)
+                run.sim.set_fused_step(shape == "fused")
+                run.sim.set_lidar_coop(shape == "coop")
                 for t in range(3, 7):
                     run.step(False, t)
                 got = _snap(run)
                 if ref is None:
                     ref = got
                 else:
-                    _same(ref, got, f"kernel {kernel} waves {waves} fused {fused}")
+                    _same(ref, got, f"kernel {kernel} waves {waves} {shape}")
                 del run
 
 
@@ -64,6 +65,7 @@ def test_runner_autotune_restores_the_sim(dev, precision):
     choice = a.autotune(steps=4)
     assert choice["waves_per_car"] in (1, 2, 4) and a.sim.waves_per_car == choice["waves_per_car"]
     assert a.sim.fused_step == choice["fused_step"] and a.policy.act_kernel == choice["act_kernel"]
+    assert a.sim.lidar_coop == choice["lidar_coop"]
     assert choice["act_kernel"] in ((R.ACT_FUSED, R.ACT_WIDE, R.ACT_TILED) if precision else
                                     (R.ACT_FUSED, R.ACT_TILED))
     _same(before, _snap(a), "after autotune", BUFFERS)

@@ -107,6 +107,32 @@ def test_nearest_cell_by_reciprocal(cell, stride):
     assert O.lib().o_check_nearest_by_reciprocal(cell, stride) == 0
 
 
+@pytest.mark.parametrize("cell,stride", [(0.025, 1), (0.05, 13), (0.1, 13), (1.0 / 3.0, 13), (7.3, 13)])
+def test_nearest_cell_by_one_step_reciprocal(cell, stride):
+    """The padded lidar march's nearest cell (one correction step from RN(1 /
+    cell)): the index floorf(a / cell + 0.5f) for every float a whose index is
+    a number below 2^22 in magnitude, and off every grid where it is not (all
+    2^32 patterns for the built-in tracks' cell, every 13th for the others)."""
+    O.set_threads(8)
+    assert O.lib().o_check_nearest_by_reciprocal1(cell, stride) == 0
+
+
+@pytest.mark.parametrize("cell", [0.025, 0.05, 0.1, 0.0625, 0.03, 1.0 / 3.0])
+def test_per_map_nearest_form_check(cell):
+    """The check the library runs at sim creation, which decides whether the
+    lidar may take the one-step nearest cell: it passes the one-step form over
+    a padded grid's index range, and it finds the boundary cells where the
+    multiply-only form RN(a RN(1 / cell)) takes the wrong index (for 1/16 m,
+    whose reciprocal is exact, that form has none)."""
+    lib = O.lib()
+    assert lib.o_nearest_form_bad(cell, -460, 1900, 1) == 0
+    bad2 = lib.o_nearest_form_bad(cell, -460, 1900, 2)
+    if cell == 0.0625:
+        assert bad2 == 0
+    else:
+        assert bad2 > 0
+
+
 def test_div_by_reciprocal_is_correctly_rounded():
     """The lidar march step's quotient from the beam's RN(1 / descent): bit
     for bit the IEEE division over its stated range (random operands, quotients

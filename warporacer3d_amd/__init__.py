@@ -156,6 +156,8 @@ def lib() -> C.CDLL:
         "racer_sim_set_schedule": (i, [vp, i]),
         "racer_sim_set_waves_per_car": (i, [vp, i]),
         "racer_sim_waves_per_car": (i, [vp]),
+        "racer_sim_set_lidar_coop": (i, [vp, i]),
+        "racer_sim_lidar_coop": (i, [vp]),
         "racer_sim_set_fused_step": (i, [vp, i]),
         "racer_sim_scan": (i, [vp]),
         "racer_trainer_set_bf16_inputs": (i, [vp, i]),
@@ -435,6 +437,14 @@ class Sim:
     def waves_per_car(self) -> int:
         return int(lib().racer_sim_waves_per_car(self.handle))
 
+    def set_lidar_coop(self, on: bool) -> None:
+        """Cooperative (workgroup) or per-wavefront range stores for the lidar; results are identical."""
+        _check(lib().racer_sim_set_lidar_coop(self.handle, int(on)))
+
+    @property
+    def lidar_coop(self) -> bool:
+        return bool(lib().racer_sim_lidar_coop(self.handle))
+
     def imagine(self, actions: View, steps: int) -> None:
         """`steps` exact physics steps in one launch (actions [2 steps, n]), no lidar: open-loop planning."""
         _check(lib().racer_sim_imagine(self.handle, actions.c, int(steps)))
@@ -453,8 +463,9 @@ class Sim:
         return bool(lib().racer_sim_fused_step(self.handle))
 
     def autotune(self) -> int:
-        """Time replays of captured sim steps with 1, 2 and 4 wavefronts per car, fused and two-launch,
-        from the same saved state; keep the fastest (restoring the state) and return its wavefronts."""
+        """Time replays of captured sim steps with 1, 2 and 4 wavefronts per car, cooperative or
+        per-wavefront range stores or fused, from the same saved state; keep the fastest (restoring the
+        state) and return its wavefronts."""
         w = C.c_int()
         _check(lib().racer_sim_autotune(self.handle, C.byref(w)))
         return int(w.value)
@@ -638,13 +649,14 @@ def runner_step(sim: "Sim", policy: "Policy", acts: "Activations", scratch: Buff
 
 def autotune_runner(sim: "Sim", policy: "Policy", acts: "Activations", scratch: Buffer, steps: int = 8) -> dict:
     """Time `steps` Runner steps replayed from graphs for every launch-shape
-    choice (lidar wavefronts per car, fused sim step, forward kernel) from the
+    choice (lidar wavefronts per car, range stores, fused sim step, forward kernel) from the
     same saved sim state; keep the fastest in sim and policy and restore the
-    sim. Returns {"waves_per_car", "fused_step", "act_kernel"}."""
+    sim. Returns {"waves_per_car", "fused_step", "lidar_coop", "act_kernel"}."""
     w, f, k = C.c_int(), C.c_int(), C.c_int()
     _check(lib().racer_runner_autotune(sim.handle, policy.handle, acts.handle, scratch.handle, int(steps),
                                        C.byref(w), C.byref(f), C.byref(k)))
-    return {"waves_per_car": int(w.value), "fused_step": bool(f.value), "act_kernel": int(k.value)}
+    return {"waves_per_car": int(w.value), "fused_step": bool(f.value), "lidar_coop": sim.lidar_coop,
+            "act_kernel": int(k.value)}
 
 
 def capture_runner(sim: "Sim", policy: "Policy", acts: "Activations", scratch: Buffer, steps: int,

@@ -151,10 +151,25 @@ struct racer_dmap {
     int nx, ny, layers, waypoints;
     float x0, y0, cell, spacing, slope;
     float inv_cell;        // RN(1 / cell), for the division shortcut of the nearest-cell index
-    int quotient_ok;       // heights and slope bounded: the lidar may divide by reciprocals
+    int quotient_ok;       // heights, slope and cell bounded and the one-step nearest cell exact over the
+                           // padded grid's index range: the lidar may divide by reciprocals
     int64_t route;         // offset of the waypoint table
     int64_t plane;         // nx * ny
+    // The packed records again on a grid with `pad` cells of all -1 records on
+    // every side (row stride pnx = nx + 2 pad), or null: a lidar origin within
+    // [pad_x0, pad_x1] x [pad_y0, pad_y1] keeps every probe of its march inside
+    // it, so the march needs no on-map test.
+    const float* padded;
+    int pad, pnx;
+    uint32_t pad_base;     // byte offset of cell (0, 0)'s record
+    uint32_t pad_bytes;    // bytes of padded records
+    float pad_x0, pad_x1, pad_y0, pad_y1;
+    // The sim's lidar order keys, RACER_LIDAR_KEYS bytes a car: the most march
+    // iterations of any beam of each piece of the fan in the previous step
+    // (written by every lidar launch; they order beams, never change a range).
+    uint8_t* lidar_keys;
 };
+constexpr int RACER_LIDAR_KEYS = 32;
 
 struct racer_sim {
     racer_device* dev = nullptr;
@@ -163,11 +178,14 @@ struct racer_sim {
     racer_buf *data = nullptr, *state = nullptr, *obs = nullptr, *actions = nullptr, *reward = nullptr,
               *done = nullptr;
     float* packed = nullptr;
+    float* padded = nullptr;
+    uint8_t* lidar_keys = nullptr;
     float2* dirs = nullptr;  // (cos, sin) of every beam's azimuth
     racer_dmap map{};
     int schedule = 0;       // lidar launch: 0 wavefront per car with beam refill, 1 thread per (car, beam)
     int waves_per_car = 2;  // schedule 0: wavefronts sharing one car's beams (1, 2 or 4)
     bool fused_step = false;  // physics and lidar in one launch where that applies
+    bool lidar_coop = true;   // schedule 0: cooperative range stores from workgroups of 8 wavefronts
     bool count_iterations = false;
     unsigned long long* counter = nullptr;
     int64_t last_iterations = 0;

@@ -322,6 +322,47 @@ RACER_FN float racer_div_by_reciprocal(float a, float b, float y) {
     return fmaf(fmaf(-q1, b, a), y, q1);
 }
 
+/* Map.at's nearest cell floorf(a / b + 0.5f) from y = RN(1 / b) with one
+ * correction step: q1 = RN(q0 + RN(a - b q0) y) is a faithful rounding of a /
+ * b (the first two steps above; b in [2^-20, 2^20], |a| <= 2^60), so it is
+ * RN(a / b) or the float on the other side of a / b. g(x) = floorf(x + 0.5f)
+ * is non-decreasing, so g(q1) and g(RN(a / b)) can differ only where a jump
+ * of g lies between those two neighbours, that is for a / b within one ulp of
+ * a jump point; every jump point is within two ulps of a half-integer N - 0.5.
+ * racer_nearest_form_bad checks every float a whose quotient lies near N -
+ * 0.5 for a range of N; where it finds none that differs, this form is the
+ * division's index for every a with a / b inside that range of half-integers
+ * (and outside it on the same side of the range, since q1 is faithful). */
+RACER_FN float racer_nearest_by_reciprocal1(float a, float b, float y) {
+    const float q0 = a * y;
+    return floorf(fmaf(fmaf(-q0, b, a), y, q0) + 0.5f);
+}
+
+/* The count of floats a with (N - 0.5) b - 256 ulp <= a <= (N - 0.5) b + 256
+ * ulp, for N in [nlo, nhi], whose index by racer_nearest_by_reciprocal1 (form
+ * 1) or by RN(a y) alone (form 2, kept to show that the check finds the
+ * failures of a form that has them) differs from floorf(a / b + 0.5f). The
+ * window is far wider than the two ulps of quotient that matter: a / b moves
+ * by at most two ulps of the quotient per ulp of a. */
+RACER_FN long racer_nearest_form_bad(float b, int nlo, int nhi, int form) {
+    const float y = 1.0f / b;
+    long bad = 0;
+    for (int N = nlo; N <= nhi; ++N) {
+        const float c = (float)(((double)N - 0.5) * (double)b);
+        /* step through the floats from 256 below c to 256 above it */
+        for (int k = -256; k <= 256; ++k) {
+            const uint32_t u = racer_f2u(c);
+            const int64_t o = (u & 0x80000000u) ? -(int64_t)(u & 0x7fffffffu) : (int64_t)u;
+            const int64_t ok = o + k;
+            const float a = racer_u2f(ok < 0 ? (uint32_t)(-ok) | 0x80000000u : (uint32_t)ok);
+            const float want = floorf(a / b + 0.5f);
+            const float got = form == 1 ? racer_nearest_by_reciprocal1(a, b, y) : floorf(a * y + 0.5f);
+            if (racer_f2u(want) != racer_f2u(got)) ++bad;
+        }
+    }
+    return bad;
+}
+
 #if defined(__HIPCC__)
 /* Branch-free global loads through gfx9 buffer descriptors: an offset at or
  * beyond `bytes` reads as zero in hardware, so guarded loads need no branch

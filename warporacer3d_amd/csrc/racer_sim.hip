@@ -5,6 +5,8 @@
 // Float32 arithmetic is written in the reference's evaluation order and built
 // with -ffp-contract=off, using the shared float32 elementary functions, so
 // every result is bit-identical to the CPU restatement.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <type_traits>
 
@@ -889,6 +891,27 @@ __global__ void k_pack(const float* d, int64_t plane, int layers, float* packed)
     }
 }
 
+// The same records on the padded grid: cell (x, y) of the map, -pad <= x <
+// nx + pad and -pad <= y < ny + pad, at (y + pad) pnx + x + pad, all -1 off the map.
+__global__ void k_pack_padded(const float* d, int nx, int ny, int pad, int layers, float* padded) {
+    const int pnx = nx + 2 * pad;
+    const int64_t cells = (int64_t)pnx * (ny + 2 * pad);
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const int x = (int)(c % pnx) - pad, y = (int)(c / pnx) - pad;
+    const bool on = x >= 0 && x < nx && y >= 0 && y < ny;
+    const int64_t plane = (int64_t)nx * ny, cell = on ? (int64_t)y * nx + x : 0;
+    for (int l = 0; l < layers; ++l) {
+        float cl = -1.0f, h = -1.0f;
+        if (on) {
+            cl = d[RACER_HEADER + (int64_t)(3 * l + RACER_CLEARANCE) * plane + cell];
+            h = d[RACER_HEADER + (int64_t)(3 * l + RACER_HEIGHT) * plane + cell];
+        }
+        padded[(c * layers + l) * 2 + 0] = cl > 0.0f ? cl : -1.0f;
+        padded[(c * layers + l) * 2 + 1] = h;
+    }
+}
+
 // Map.probe on the packed records, returning (clearance, height) only. The
 // records are read with 32-bit buffer offsets (the descriptor's size makes any
 // out-of-range offset read zeros instead of faulting).
@@ -928,26 +951,26 @@ __device__ __forceinline__ void probe_packed(const racer_dmap& m, __amdgpu_buffe
 }
 
 // Map.probe's (clearance, height) from the packed records with the load
-// always issued: off the map it reads the all -1 record after the grid.
+// always issued: off the map it reads an all -1 record.
 //
-// The nearest cell is racer_nearest_index_lim on both axes with one branch
-// for the rare division of either (a lane whose shortcut holds gets the same
-// value from the division, which is the formula the shortcut reproduces);
-// `live` = false lanes (their results unused) never take it.
-template <int L, bool RECIP>
+// RECIP: the nearest cell on both axes by racer_nearest_by_reciprocal1, which
+// the map's quotient_ok flag has checked against the division over every cell
+// boundary of the padded index range. Otherwise racer_nearest_index_lim with
+// one branch for the rare division of either axis (a lane whose shortcut holds
+// gets the same value from the division, which is the formula the shortcut
+// reproduces); `live` = false lanes (their results unused) never take it.
+//
+// PAD: the probe lies on the padded grid (the caller has checked its origin),
+// so its record is read there with no on-map test; else a probe off the map
+// reads the all -1 record kept after the packed grid.
+template <int L, bool RECIP, bool PAD>
 __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_rsrc_t rec, float x, float y, float z,
                                            float limx, float limy, bool live, float* clearance, float* height) {
     const float ax = x - m.x0, ay = y - m.y0;
     float fx, fy;
     if constexpr (RECIP) {
-        // floorf(a / cell + 0.5f) through racer_div_by_reciprocal, for a cell
-        // in [2^-20, 2^20]: exact for |a| in [2^-60, 2^60] (the quotient is
-        // RN(a / cell)); below 2^-60 both quotients are under 2^-36 in
-        // magnitude (the computed one is at most 9 |a| / cell), so both sums
-        // round to 0.5 and the index is 0; above 2^60 both are beyond 2^40 in
-        // magnitude with a's sign, or NaN, all off the grid.
-        fx = floorf(racer_div_by_reciprocal(ax, m.cell, m.inv_cell) + 0.5f);
-        fy = floorf(racer_div_by_reciprocal(ay, m.cell, m.inv_cell) + 0.5f);
+        fx = racer_nearest_by_reciprocal1(ax, m.cell, m.inv_cell);
+        fy = racer_nearest_by_reciprocal1(ay, m.cell, m.inv_cell);
     } else {
         // limx, limy: racer_nearest_limit of bounds on |x - x0| / cell and |y - y0| / cell
         const float sx = ax * m.inv_cell + 0.5f, sy = ay * m.inv_cell + 0.5f;
@@ -960,14 +983,22 @@ __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_
             fy = floorf(ay / m.cell + 0.5f);
         }
     }
-    // 0 <= f < n as one unsigned compare of the bit patterns: f is a floor
-    // of a sum with 0.5, never -0.0, so a set sign bit means negative, and
-    // NaN and +inf patterns lie above every finite n
-    const bool on_x = __float_as_uint(fx) < __float_as_uint((float)m.nx);
-    const bool on_y = __float_as_uint(fy) < __float_as_uint((float)m.ny);
-    const bool on = on_x & on_y;
-    // cell = fy nx + fx, exact in float (both below 2^12, the sum below 2^24)
-    const uint32_t off = on ? (uint32_t)fmaf(fy, (float)m.nx, fx) * (uint32_t)(8 * L) : (uint32_t)(m.plane * 8 * L);
+    uint32_t off;
+    if constexpr (PAD) {
+        // cell fy pnx + fx relative to cell (0, 0), exact in float (the padded
+        // grid has fewer than 2^24 cells) and negative left of or below it;
+        // the byte offset wraps modulo 2^32 to the record's
+        off = (uint32_t)(int)fmaf(fy, (float)m.pnx, fx) * (uint32_t)(8 * L) + m.pad_base;
+    } else {
+        // 0 <= f < n as one unsigned compare of the bit patterns: f is a floor
+        // of a sum with 0.5, never -0.0, so a set sign bit means negative, and
+        // NaN and +inf patterns lie above every finite n
+        const bool on_x = __float_as_uint(fx) < __float_as_uint((float)m.nx);
+        const bool on_y = __float_as_uint(fy) < __float_as_uint((float)m.ny);
+        const bool on = on_x & on_y;
+        // cell = fy nx + fx, exact in float (both below 2^12, the sum below 2^24)
+        off = on ? (uint32_t)fmaf(fy, (float)m.nx, fx) * (uint32_t)(8 * L) : (uint32_t)(m.plane * 8 * L);
+    }
     if constexpr (L == 1) {
         // the stored clearance is -1 unless positive
         float2 r = racer_ld_f32x2(rec, off);
@@ -990,27 +1021,53 @@ __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_
 
 constexpr int LIDAR_RECORD = 5;               // dx, dy, dz, descent, RN(1 / descent)
 constexpr int LIDAR_RING = 2 * LIDAR_RECORD * 64;  // two chunks of 64 beam records
-constexpr int LIDAR_COOP_CARS = 8;                  // cars per workgroup with cooperative stores
+constexpr int LIDAR_PIECES = 24;                    // most pieces a wavefront's share is ordered over
+// a wavefront's LDS: the ring, each record's beam (16 bits, 128 of them), the pieces' iteration maxima
+constexpr int LIDAR_WAVE_LDS = LIDAR_RING + 64 + LIDAR_PIECES;
+constexpr int LIDAR_COOP_CARS = 8;                  // cars per workgroup of the fused step
+constexpr int LIDAR_COOP_WAVES = 8;                 // wavefronts per workgroup of the cooperative lidar
 constexpr size_t LIDAR_COOP_MAX_SHMEM = 80 * 1024;  // two such workgroups per CU (160 KB of LDS)
+
+// The share of a car's beams that wavefront `global_wave` marches: the
+// car's wavefronts take the shares in a rotated order, (w + w / 4) mod
+// waves_per_car for the workgroup's w-th wavefront, so that the four SIMDs of
+// a CU (wavefront w runs on SIMD w mod 4) each get every share equally often.
+// The shares differ in work (the beams ahead of a car along a curve march
+// longest), and the SIMD with most of it sets a launch's time.
+__device__ __forceinline__ int lidar_share(int global_wave, int waves_per_car) {
+    return (global_wave + global_wave / 4) % waves_per_car;
+}
 
 // scan (racer/sim.mojo:238-266) for every beam of one car per wavefront.
 //
 // Each lane marches one beam at a time; a lane whose beam ends takes the next
-// unclaimed beam of the car (ballot + popcount), so the wavefront stays full
-// until the car's last beams. Beam set-up (direction, normalisation, floor
+// unclaimed beam of the share (ballot + popcount), so the wavefront stays full
+// until the share's last beams. Beam set-up (direction, normalisation, floor
 // descent rate and its reciprocal) is done 64 beams at a time by the whole
-// wavefront into a two-chunk LDS ring, so a refill is five LDS reads rather
+// wavefront into a two-chunk LDS ring, so a refill is six LDS reads rather
 // than a divergent pass of divisions and square roots. Ranges collect in LDS
 // and are scaled and stored in one full-lane pass at the end.
+//
+// Order: a share is cut into pieces, its parts of the track-independent
+// 64-beam chunks of the fan, and the pieces are set up longest first by the
+// most march iterations any of their beams took in the sim's previous step
+// (one byte a piece, `m.lidar_keys`, which this launch rewrites). A
+// wavefront's time is set by its longest beams (up to 100+ iterations where
+// the mean is 6) when they start late; started first they run beside the
+// short ones. Every piece takes a whole set-up chunk: a lane past the piece's
+// end gets a spare record that finishes at its first trip (it starts at t =
+// NaN) and stores its range into a spare slot. The order changes which lane marches a beam and when,
+// never the beam's own arithmetic, so the ranges are those of any order.
 //
 // lidar_wave marches wavefront `global_wave`'s share (car global_wave /
 // waves_per_car); the kernels below map workgroups to wavefronts.
 //
 // COOP: the workgroup's cars (consecutive, from wg_car0) keep all their
-// ranges in LDS ([car][beam] after the wavefronts' rings) and the workgroup
-// stores them together after one barrier, each obs row as a run of
-// consecutive cars: partial 128-byte lines of 32 bytes (8 cars) instead of one
-// float per line from each wavefront. Otherwise each wavefront stores its own.
+// ranges in LDS ([car][beam] after the wavefronts' rings, total_beams + 1 a
+// car) and the workgroup stores them together after one barrier, each obs row
+// as a run of consecutive cars: partial 128-byte lines of 32 bytes (8 cars)
+// instead of one float per line from each wavefront. Otherwise each wavefront
+// stores its own.
 //
 // POSE: the car's heading, along and across slopes and position come from
 // pose[8 * (car - wg_car0) ..] (LDS, written by the same workgroup's physics)
@@ -1028,19 +1085,58 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     global_wave = __builtin_amdgcn_readfirstlane(global_wave);
     const int car = global_wave / waves_per_car;
+    const int share = lidar_share(global_wave, waves_per_car);
     // This wavefront's share of the car's beams: [lo, lo + beams).
-    const int lo = (global_wave % waves_per_car) * per_wave;
+    const int lo = share * per_wave;
     const int beams = min(total_beams, lo + per_wave) - lo;
     const bool active = car < n && beams > 0;
     if constexpr (!COOP) {
         if (!active) return;
     }
     const int wg_waves = (int)(blockDim.x >> 6);
-    float* ring = COOP ? lds + wave * LIDAR_RING : lds + wave * (LIDAR_RING + per_wave);
-    float* ranges = COOP ? lds + wg_waves * LIDAR_RING + (int64_t)(car - wg_car0) * total_beams + lo : ring + LIDAR_RING;
+    float* ring = COOP ? lds + wave * LIDAR_WAVE_LDS : lds + wave * (LIDAR_WAVE_LDS + per_wave + 1);
+    uint16_t* ring_beam = reinterpret_cast<uint16_t*>(ring + LIDAR_RING);
+    uint32_t* piece_max = reinterpret_cast<uint32_t*>(ring + LIDAR_RING + 64);
+    float* ranges = COOP ? lds + wg_waves * LIDAR_WAVE_LDS + (int64_t)(car - wg_car0) * (total_beams + 1) + lo
+                         : ring + LIDAR_WAVE_LDS;
+    // the spare range slot of lanes past a piece's end
+    const int spare = COOP ? total_beams - lo : per_wave;
     if (active) {
     __amdgpu_buffer_rsrc_t rec = __builtin_amdgcn_make_buffer_rsrc(
         (void*)m.packed, (short)0, (int)((m.plane + 1) * (L > 0 ? L : 1) * 8), 0x00020000);
+    __amdgpu_buffer_rsrc_t rec_pad =
+        __builtin_amdgcn_make_buffer_rsrc((void*)m.padded, (short)0, (int)m.pad_bytes, 0x00020000);
+
+    // Pieces: chunk k0 + p of the fan cut to [lo, lo + beams), p < pieces; the
+    // keys of car `car` hold piece p of share `share` at byte k0 + share + p.
+    const int k0 = lo >> 6;
+    const int pieces = ((lo + beams - 1) >> 6) - k0 + 1;
+    const bool ordered = m.lidar_keys != nullptr && pieces <= LIDAR_PIECES && k0 + share + pieces <= RACER_LIDAR_KEYS;
+    uint8_t* keys = ordered ? m.lidar_keys + (int64_t)car * RACER_LIDAR_KEYS + k0 + share : nullptr;
+    // rank of piece `lane` in the order: longer first, then by index
+    int rank = lane;
+    if (ordered) {
+        const uint32_t key = lane < pieces ? keys[lane] : 0u;
+        if (lane < pieces) {
+            rank = 0;
+            for (int q = 0; q < pieces; ++q) {
+                const uint32_t kq = __builtin_amdgcn_readlane(key, q);
+                rank += (int)((kq > key) | ((kq == key) & (q < lane)));
+            }
+        }
+        if (lane < LIDAR_PIECES) piece_max[lane] = 0u;
+        // Issue priority by the longest piece's key: a wavefront whose beams
+        // march longest sets the launch's end unless the SIMD serves it first,
+        // while the short ones fill the issue slots it leaves.
+        const int top = __builtin_amdgcn_readfirstlane(
+            (int)__builtin_amdgcn_readlane(key, (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(rank == 0))));
+        if (top >= 40)
+            __builtin_amdgcn_s_setprio(3);
+        else if (top >= 20)
+            __builtin_amdgcn_s_setprio(2);
+        else if (top >= 10)
+            __builtin_amdgcn_s_setprio(1);
+    }
 
     const float* cp = POSE ? pose + 8 * (car - wg_car0) : nullptr;
     const float heading = POSE ? cp[0] : s[RACER_HEADING * n + car];
@@ -1071,14 +1167,23 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     // descent >= 1.5 slope 2^-20.1 >= 2^-41, and descent <= 1.5 slope + 1 <=
     // 2^21. Otherwise the step divides.
     const bool fast_q =
-        __builtin_amdgcn_readfirstlane((int)((m.quotient_ok != 0) & (fabsf(oz) <= 0x1p40f) &
-                                             (fabsf(along) + fabsf(across) <= 0x1p20f))) != 0;
+        __builtin_amdgcn_readfirstlane((int)(m.quotient_ok != 0) & (int)(fabsf(oz) <= 0x1p40f) &
+                                       (int)(fabsf(along) + fabsf(across) <= 0x1p20f)) != 0;
+    // An origin inside the padded grid's box keeps every probe on that grid
+    // (the box leaves RACER_RANGE and a margin of cells to its edges).
+    const bool pad_q = fast_q && m.padded != nullptr &&
+                       __builtin_amdgcn_readfirstlane((int)(ox >= m.pad_x0) & (int)(ox <= m.pad_x1) &
+                                                      (int)(oy >= m.pad_y0) & (int)(oy <= m.pad_y1)) != 0;
 
-    // Set-up of beams 64 c .. 64 c + 63, one per lane, into ring half c % 2:
-    // field f of beam b lives at ring[128 f + b % 128].
+    // Set-up of the c-th piece in the order, one beam per lane, into ring half
+    // c % 2: field f of record r lives at ring[128 f + r % 128], its beam (relative
+    // to lo, or the spare slot) at ring_beam[r % 128].
     auto setup_chunk = [&](int c) {
-        int b = c * 64 + lane;
-        if (b < beams) {
+        const int piece = ordered ? (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(rank == c)) : c;
+        const int k = k0 + piece;
+        const int b = k * 64 + lane - lo;  // this lane's beam, relative to lo
+        float* slot = ring + (c & 1) * 64 + lane;
+        if ((b >= 0) & (b < beams)) {
             float2 cs = dirs[lo + b];
             float d0 = cs.x * ch + cs.y * (-sh);
             float d1 = cs.x * sh + cs.y * ch;
@@ -1087,34 +1192,50 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
             float dx = d0 / norm, dy = d1 / norm, dz = d2 / norm;
             float horizontal = sqrtf(dx * dx + dy * dy);
             float descent = slope15 * horizontal + racer_max(-dz, 0.0f);
-            float* slot = ring + (c & 1) * 64 + lane;
             slot[0 * 128] = dx;
             slot[1 * 128] = dy;
             slot[2 * 128] = dz;
             slot[3 * 128] = descent;
             slot[4 * 128] = fast_q ? 1.0f / descent : 0.0f;
+            ring_beam[(c & 1) * 64 + lane] = (uint16_t)b;
+        } else {
+            // a spare record: its NaN descent starts the march at t = NaN
+            slot[0 * 128] = 0.0f;
+            slot[1 * 128] = 0.0f;
+            slot[2 * 128] = 0.0f;
+            slot[3 * 128] = __builtin_nanf("");
+            slot[4 * 128] = 0.0f;
+            ring_beam[(c & 1) * 64 + lane] = (uint16_t)spare;
         }
     };
     float dx = 0.0f, dy = 0.0f, dz = 0.0f, descent = 0.0f, inv_descent = 0.0f;
     float t = 0.0f, last = 0.0f, last_floor = 0.0f;
-    auto take = [&](int b) {
-        const float* slot = ring + (b & 127);
+    int beam = spare;   // the beam this lane marches (relative to lo)
+    int started = 0;    // the trip it started on
+    auto take = [&](int r) {
+        const float* slot = ring + (r & 127);
         dx = slot[0 * 128];
         dy = slot[1 * 128];
         dz = slot[2 * 128];
         descent = slot[3 * 128];
         inv_descent = slot[4 * 128];
-        t = 0.0f;
+        beam = ring_beam[r & 127];
+        // +0 for a beam (its descent is a number), NaN for a spare record,
+        // which !(t < RANGE) then ends at its first trip; one multiply, as
+        // many instructions as setting t = 0
+        t = fabsf(descent) * 0.0f;
         last = 0.0f;
         last_floor = 0.0f;
     };
     setup_chunk(0);
-    setup_chunk(1);
+    if (pieces > 1) setup_chunk(1);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    int b = lane;
-    if (b < beams) take(b);
+    // records: 64 per piece, pieces * 64 in all
+    const int records = pieces * 64;
+    int b = lane;  // this lane's record
+    take(b);
     uint64_t iterations = 0, trips = 0;
     // The march step below runs on every lane with selects in place of
     // branches (lanes past the share's end compute on their last beam's
@@ -1123,22 +1244,24 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     // instruction, on every trip. The step's quotient is always divided: some
     // live lane needs the exact division on almost every trip (the floor
     // bound is the smaller one on about one iteration in nine), so a
-    // shortcut around it only adds its test. The loop is compiled twice, for
-    // the reciprocal form of that quotient and for the division (fast_q is
-    // uniform), so the choice costs nothing per trip.
-    auto march = [&](auto reciprocal) {
+    // shortcut around it only adds its test. The loop is compiled three
+    // times, for the reciprocal form of that quotient on the padded grid and
+    // on the packed grid, and for the division (fast_q and pad_q are uniform),
+    // so the choice costs nothing per trip.
+    auto march = [&](auto reciprocal, auto padded) {
         // declared here rather than captured: the compiler then keeps them in
-        // scalar registers in both copies of the loop
-        int chunks = 2;  // beams below 64 * chunks have their set-up in the ring
-        int next = 64;   // the next unclaimed beam
-        while (__ballot(b < beams) != 0ull) {
-            const bool live = b < beams;
-            if (COUNT) iterations += live ? 1 : 0;
+        // scalar registers in every copy of the loop
+        int chunks = pieces > 1 ? 2 : 1;  // records below 64 * chunks are set up in the ring
+        int next = 64;                    // the next unclaimed record
+        int trip = 0;
+        while (__ballot(b < records) != 0ull) {
+            const bool live = b < records;
+            if (COUNT) iterations += (live & (beam != spare)) ? 1 : 0;
             const float px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
             float clearance, height;
             if constexpr (L == 1 || L == 2)
-                probe_flat<L, decltype(reciprocal)::value>(m, rec, px, py, pz, limx, limy, live, &clearance,
-                                                           &height);
+                probe_flat<L, decltype(reciprocal)::value, decltype(padded)::value>(
+                    m, decltype(padded)::value ? rec_pad : rec, px, py, pz, limx, limy, live, &clearance, &height);
             else
                 probe_packed<L>(m, rec, px, py, pz, &clearance, &height);
             const float fl = (pz - height) - 0.03f;
@@ -1156,16 +1279,21 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
             last = advance ? t : last;
             last_floor = advance ? fl : last_floor;
             t = advance ? next_t : t;
+            // an empty asm: the finished lanes' range below then reads these
+            // new values rather than the old ones the compiler knows equal
+            // them there, which kept both alive and cost three copies a trip
+            __asm__ volatile("" : "+v"(t), "+v"(last), "+v"(last_floor));
             const bool finished = live & (!advance | !(t < RACER_RANGE));
+            ++trip;
             if (COUNT && lane == 0) ++trips;
             // the builtin on the bool itself: __ballot's int argument costs a
             // select and a compare to turn the lane mask back into one
             const uint64_t done_mask = __builtin_amdgcn_ballot_w64(finished);
             if (done_mask != 0ull) {
                 const int count = (int)__popcll(done_mask);
-                // The ring slot reused here held chunk `chunks - 2`, whose beams
+                // The ring slot reused here held chunk `chunks - 2`, whose records
                 // are all below `next` and already taken.
-                if (next + count > chunks * 64 && chunks * 64 < beams) {
+                if (next + count > chunks * 64 && chunks < pieces) {
                     setup_chunk(chunks);
                     ++chunks;
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1176,27 +1304,35 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
                 const int claimed = next + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(done_mask >> 32),
                                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)done_mask, 0u));
                 // one exec region: the range (interpolated between the last two
-                // steps on a floor hit), its store, then the next beam (a lane
-                // whose claim is past the share reads a stale ring slot and stays
-                // idle, since b >= beams)
+                // steps on a floor hit), its store, the piece's iteration
+                // maximum, then the next record (a lane whose claim is past the
+                // share reads a stale ring slot and stays idle, since b >= records)
                 if (finished) {
-                    ranges[b] = hit ? racer_min(last + ((t - last) * last_floor) / (last_floor - fl), RACER_RANGE)
-                                    : racer_min(t, RACER_RANGE);
+                    ranges[beam] = hit ? racer_min(last + ((t - last) * last_floor) / (last_floor - fl), RACER_RANGE)
+                                       : racer_min(t, RACER_RANGE);
+                    if (ordered) __hip_atomic_fetch_max(piece_max + (b >> 6), (uint32_t)(trip - started),
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                     b = claimed;
+                    started = trip;
                     take(claimed);
                 }
                 next += count;
             }
         }
     };
-    if (fast_q)
-        march(std::integral_constant<bool, true>{});
+    if (pad_q)
+        march(std::integral_constant<bool, true>{}, std::integral_constant<bool, true>{});
+    else if (fast_q)
+        march(std::integral_constant<bool, true>{}, std::integral_constant<bool, false>{});
     else
-        march(std::integral_constant<bool, false>{});
+        march(std::integral_constant<bool, false>{}, std::integral_constant<bool, false>{});
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // piece p's key for the next launch: the most iterations of its beams
+    // (trips from a beam's take to its end), saturated to a byte
+    if (ordered && lane < pieces) keys[lane] = (uint8_t)min(piece_max[rank], 255u);
     if constexpr (!COOP) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         for (int j = lane; j < beams; j += 64) obs[(int64_t)(4 + lo + j) * obs_cols + car] = ranges[j] / RACER_RANGE;
     }
     if (COUNT) {
@@ -1210,10 +1346,11 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     if constexpr (COOP) {
         __syncthreads();
         const int cars = wg_waves / waves_per_car;
-        const float* all = lds + wg_waves * LIDAR_RING;
+        const float* all = lds + wg_waves * LIDAR_WAVE_LDS;
         for (int idx = (int)threadIdx.x; idx < cars * total_beams; idx += (int)blockDim.x) {
             const int c = idx % cars, j = idx / cars;
-            if (wg_car0 + c < n) obs[(int64_t)(4 + j) * obs_cols + wg_car0 + c] = all[c * total_beams + j] / RACER_RANGE;
+            if (wg_car0 + c < n)
+                obs[(int64_t)(4 + j) * obs_cols + wg_car0 + c] = all[c * (total_beams + 1) + j] / RACER_RANGE;
         }
     }
 }
@@ -1499,14 +1636,16 @@ static void launch_lidar(racer_sim* sim, float* obs, int64_t obs_cols) {
     int wpc = sim->waves_per_car;
     int per = (sim->beams + wpc - 1) / wpc;
     int64_t total = (int64_t)n * wpc;
-    // cooperative stores from workgroups of 8 cars (runs of 32 bytes; 4 cars
-    // measured slower than per-wavefront stores) when their wavefronts fit one
-    // workgroup and their rings plus the cars' ranges leave room for two
-    // workgroups per CU
-    const int coop_waves = LIDAR_COOP_CARS * wpc;
+    // cooperative stores from workgroups of 8 wavefronts (8 / wpc cars, runs of
+    // 4 (8 / wpc) bytes) when the cars' ranges fit their LDS; else, or when the
+    // sim asks for it, each wavefront stores its own. A CU never holds two
+    // workgroups of 16 wavefronts at once (measured), so 8 cars at two
+    // wavefronts each would run the launch in two rounds.
+    const int coop_waves = LIDAR_COOP_WAVES;
+    const int coop_cars = coop_waves / wpc;
     const size_t coop_shmem =
-        ((size_t)coop_waves * LIDAR_RING + (size_t)LIDAR_COOP_CARS * sim->beams) * sizeof(float);
-    if (coop_waves <= 16 && coop_shmem <= LIDAR_COOP_MAX_SHMEM) {
+        ((size_t)coop_waves * LIDAR_WAVE_LDS + (size_t)coop_cars * (sim->beams + 1)) * sizeof(float);
+    if (sim->lidar_coop && coop_cars >= 1 && coop_shmem <= LIDAR_COOP_MAX_SHMEM) {
         dim3 grid((unsigned)((total + coop_waves - 1) / coop_waves)), block(64 * coop_waves);
         if (sim->count_iterations)
             hipLaunchKernelGGL((k_lidar<L, true, true>), grid, block, coop_shmem, sim->dev->stream, sim->map,
@@ -1516,7 +1655,7 @@ static void launch_lidar(racer_sim* sim, float* obs, int64_t obs_cols) {
                                sim->state->ptr, n, obs, obs_cols, sim->beams, per, wpc, sim->dirs, sim->counter);
         return;
     }
-    size_t bytes_per_wave = (size_t)(LIDAR_RING + per) * sizeof(float);
+    size_t bytes_per_wave = (size_t)(LIDAR_WAVE_LDS + per + 1) * sizeof(float);
     int waves = 4 * bytes_per_wave <= 65536 ? 4 : 1;
     dim3 grid((unsigned)((total + waves - 1) / waves)), block(64 * waves);
     size_t shmem = waves * bytes_per_wave;
@@ -1560,7 +1699,7 @@ static size_t step_fused_shmem(const racer_sim* sim, int* pose_at) {
     const int layers = sim->map.layers;
     const int coop_waves = LIDAR_COOP_CARS * sim->waves_per_car;
     if (sim->schedule != 0 || (layers != 1 && layers != 2) || coop_waves > 16) return 0;
-    const size_t lidar = (size_t)coop_waves * LIDAR_RING + (size_t)LIDAR_COOP_CARS * sim->beams;
+    const size_t lidar = (size_t)coop_waves * LIDAR_WAVE_LDS + (size_t)LIDAR_COOP_CARS * (sim->beams + 1);
     const size_t tiles = (size_t)LIDAR_COOP_CARS * (layers == 1 ? phys_tile_floats<1>() : phys_tile_floats<2>());
     const size_t at = lidar > tiles ? lidar : tiles;
     const size_t bytes = (at + 8 * LIDAR_COOP_CARS) * sizeof(float);
@@ -1670,7 +1809,7 @@ size_t racer_batch_fused_shmem(const racer_batch* b, int waves_per_car, int* pos
         if (sim->map.layers != 1 && sim->map.layers != 2) return 0;
         layers = sim->map.layers > layers ? sim->map.layers : layers;
     }
-    const size_t lidar = (size_t)coop_waves * LIDAR_RING + (size_t)LIDAR_COOP_CARS * b->beams;
+    const size_t lidar = (size_t)coop_waves * LIDAR_WAVE_LDS + (size_t)LIDAR_COOP_CARS * (b->beams + 1);
     const size_t tiles = (size_t)LIDAR_COOP_CARS * (layers == 1 ? phys_tile_floats<1>() : phys_tile_floats<2>());
     const size_t at = lidar > tiles ? lidar : tiles;
     const size_t bytes = (at + 8 * LIDAR_COOP_CARS) * sizeof(float);
@@ -1768,7 +1907,7 @@ int racer_launch_batch_lidar(racer_batch* b) {
         const int wpc = b->waves_per_car;
         const int per = (b->beams + wpc - 1) / wpc;
         const racer_sim_table tab = sim_table(b, wpc);
-        const size_t bytes_per_wave = (size_t)(LIDAR_RING + per) * sizeof(float);
+        const size_t bytes_per_wave = (size_t)(LIDAR_WAVE_LDS + per + 1) * sizeof(float);
         const int waves = 4 * bytes_per_wave <= 65536 ? 4 : 1;
         const int64_t total = tab.start[tab.count];
         dim3 grid((unsigned)((total + waves - 1) / waves)), block(64 * waves);
@@ -1986,6 +2125,7 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
     m.route = route;
     m.plane = plane;
     m.packed = nullptr;
+    m.padded = nullptr;
     if (layers <= 2) {
         RACER_TRY_OR(hipMalloc(&sim->packed, (size_t)(plane + 1) * layers * 2 * sizeof(float)), racer_sim_destroy(sim));
         hipLaunchKernelGGL(k_pack, dim3((unsigned)((plane + 1 + 255) / 256)), dim3(256), 0, dev->stream, m.d, plane,
@@ -1995,7 +2135,45 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
             return 1;
         }
         m.packed = sim->packed;
+        // The padded grid: a lidar probe lies within RACER_RANGE (1 + 2^-20) of
+        // its origin, so `pad` cells of RACER_RANGE + 1 m plus 16 leave an
+        // origin box reaching 14 cells past the map on every side. Kept only
+        // while its cell index is exact in float (fewer than 2^24 cells).
+        const double cell = m.cell;
+        const double reach = (double)RACER_RANGE + 1.0;
+        const double padc = m.quotient_ok ? std::ceil(reach / cell) + 16.0 : 0.0;
+        const double pcells = ((double)nx + 2.0 * padc) * ((double)ny + 2.0 * padc + 1.0);
+        if (m.quotient_ok && pcells < 0x1p24) {
+            m.pad = (int)padc;
+            m.pnx = nx + 2 * m.pad;
+            const int64_t cells = (int64_t)m.pnx * (ny + 2 * m.pad);
+            m.pad_base = (uint32_t)(((int64_t)m.pad * m.pnx + m.pad) * 8 * layers);
+            m.pad_bytes = (uint32_t)(cells * 8 * layers);
+            m.pad_x0 = (float)((double)m.x0 - (padc - 2.0) * cell + reach);
+            m.pad_x1 = (float)((double)m.x0 + ((double)nx + padc - 3.0) * cell - reach);
+            m.pad_y0 = (float)((double)m.y0 - (padc - 2.0) * cell + reach);
+            m.pad_y1 = (float)((double)m.y0 + ((double)ny + padc - 3.0) * cell - reach);
+            RACER_TRY_OR(hipMalloc(&sim->padded, (size_t)cells * layers * 2 * sizeof(float)), racer_sim_destroy(sim));
+            hipLaunchKernelGGL(k_pack_padded, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, dev->stream, m.d, nx,
+                               ny, m.pad, layers, sim->padded);
+            if (racer_launch_check("pack")) {
+                racer_sim_destroy(sim);
+                return 1;
+            }
+            m.padded = sim->padded;
+        }
+        // The lidar's one-step nearest cell (racer_nearest_by_reciprocal1) must
+        // give the division's index at every cell boundary of the padded grid
+        // (of the map itself without it).
+        if (m.quotient_ok && racer_nearest_form_bad(m.cell, -m.pad - 2, std::max(nx, ny) + m.pad + 2, 1) != 0) {
+            m.quotient_ok = 0;
+            m.padded = nullptr;
+        }
     }
+    // The lidar's order keys, all 0 (the natural order) until the first launch.
+    RACER_TRY_OR(hipMalloc(&sim->lidar_keys, (size_t)n * RACER_LIDAR_KEYS), racer_sim_destroy(sim));
+    RACER_TRY_OR(hipMemsetAsync(sim->lidar_keys, 0, (size_t)n * RACER_LIDAR_KEYS, dev->stream), racer_sim_destroy(sim));
+    m.lidar_keys = sim->lidar_keys;
     // Beam directions: azimuth = -FOV/2 + b FOV/(BEAMS-1) (racer/sim.mojo:246).
     std::vector<float2> dirs(cfg->beams);
     for (int b = 0; b < cfg->beams; ++b) {
@@ -2026,6 +2204,8 @@ extern "C" void racer_sim_destroy(racer_sim* sim) {
     racer_free(sim->reward);
     racer_free(sim->done);
     if (sim->packed) (void)hipFree(sim->packed);
+    if (sim->padded) (void)hipFree(sim->padded);
+    if (sim->lidar_keys) (void)hipFree(sim->lidar_keys);
     if (sim->dirs) (void)hipFree(sim->dirs);
     if (sim->counter) (void)hipFree(sim->counter);
     racer_device* dev = sim->dev;
@@ -2132,6 +2312,15 @@ extern "C" int racer_sim_set_waves_per_car(racer_sim* sim, int waves) {
 }
 
 extern "C" int racer_sim_waves_per_car(const racer_sim* sim) { return sim->waves_per_car; }
+
+extern "C" int racer_sim_set_lidar_coop(racer_sim* sim, int on) {
+    racer_clear_error();
+    RACER_REQUIRE(sim, "no sim");
+    sim->lidar_coop = on != 0;
+    return 0;
+}
+
+extern "C" int racer_sim_lidar_coop(const racer_sim* sim) { return sim->lidar_coop ? 1 : 0; }
 
 extern "C" int racer_sim_set_fused_step(racer_sim* sim, int on) {
     racer_clear_error();

@@ -117,18 +117,20 @@ extern "C" int racer_sim_autotune(racer_sim* sim, int* chosen) {
     if (make_events(ev)) return 1;
     const bool counting = sim->count_iterations;
     const int waves0 = sim->waves_per_car;
-    const bool fused0 = sim->fused_step;
+    const bool fused0 = sim->fused_step, coop0 = sim->lidar_coop;
     sim->count_iterations = false;
     int best_waves = waves0;
-    bool best_fused = fused0;
+    bool best_fused = fused0, best_coop = coop0;
     float best = 1e30f;
     Saved saved;
     auto run = [&]() -> int {
         if (saved.save(dev, {sim})) return 1;
         for (int w : {1, 2, 4})
-            for (bool f : {false, true}) {
+            for (int shape : {0, 1, 2}) {  // cooperative stores, per-wavefront stores, fused step
+                const bool f = shape == 2, c = shape == 0;
                 sim->waves_per_car = w;
                 sim->fused_step = f;
+                sim->lidar_coop = c;
                 if (f && !racer_sim_fused_step(sim)) continue;
                 float ms = 0.0f;
                 if (time_candidate(dev, saved, ev, [&](racer_graph** g) { return racer_graph_sim_steps(sim, 8, g); },
@@ -138,6 +140,7 @@ extern "C" int racer_sim_autotune(racer_sim* sim, int* chosen) {
                     best = ms;
                     best_waves = w;
                     best_fused = f;
+                    best_coop = c;
                 }
             }
         return 0;
@@ -147,6 +150,7 @@ extern "C" int racer_sim_autotune(racer_sim* sim, int* chosen) {
     sim->count_iterations = counting;
     sim->waves_per_car = failed ? waves0 : best_waves;
     sim->fused_step = failed ? fused0 : best_fused;
+    sim->lidar_coop = failed ? coop0 : best_coop;
     if (failed || restored) return 1;
     RACER_TRY(hipStreamSynchronize(dev->stream));
     if (chosen) *chosen = best_waves;
@@ -165,20 +169,22 @@ extern "C" int racer_runner_autotune(racer_sim* sim, racer_policy* pol, racer_ac
     if (make_events(ev)) return 1;
     const bool counting = sim->count_iterations;
     const int waves0 = sim->waves_per_car, kernel0 = pol->act_kernel;
-    const bool fused0 = sim->fused_step;
+    const bool fused0 = sim->fused_step, coop0 = sim->lidar_coop;
     sim->count_iterations = false;
     int best_waves = waves0, best_kernel = kernel0;
-    bool best_fused = fused0;
+    bool best_fused = fused0, best_coop = coop0;
     float best = 1e30f;
     Saved saved;
     auto run = [&]() -> int {
         if (saved.save(dev, {sim})) return 1;
         for (int k : act_candidates(pol))
             for (int w : {1, 2, 4})
-                for (bool f : {false, true}) {
+                for (int shape : {0, 1, 2}) {  // cooperative stores, per-wavefront stores, fused step
+                    const bool f = shape == 2, c = shape == 0;
                     pol->act_kernel = k;
                     sim->waves_per_car = w;
                     sim->fused_step = f;
+                    sim->lidar_coop = c;
                     if (f && !racer_sim_fused_step(sim)) continue;
                     float ms = 0.0f;
                     if (time_candidate(
@@ -190,6 +196,7 @@ extern "C" int racer_runner_autotune(racer_sim* sim, racer_policy* pol, racer_ac
                         best = ms;
                         best_waves = w;
                         best_fused = f;
+                        best_coop = c;
                         best_kernel = k;
                     }
                 }
@@ -200,6 +207,7 @@ extern "C" int racer_runner_autotune(racer_sim* sim, racer_policy* pol, racer_ac
     sim->count_iterations = counting;
     sim->waves_per_car = failed ? waves0 : best_waves;
     sim->fused_step = failed ? fused0 : best_fused;
+    sim->lidar_coop = failed ? coop0 : best_coop;
     pol->act_kernel = failed ? kernel0 : best_kernel;
     if (failed || restored) return 1;
     RACER_TRY(hipStreamSynchronize(dev->stream));
