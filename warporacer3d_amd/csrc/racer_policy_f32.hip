@@ -219,6 +219,48 @@ __device__ __forceinline__ void staged_rounds(f32x16& acc, float4 (&ring)[RING],
     }
 }
 
+// The 16 results of a lane's accumulators as tanh, then into the [column][row]
+// LDS tile: every evaluation first, in one basic block, so the 16 independent
+// chains interleave (a guarded store after each one put a branch between them
+// and left one wavefront per SIMD waiting on each chain in turn).
+__device__ __forceinline__ void tanh_tile(const f32x16& acc, float* hs, int ldh, int wave, int ib, int jc, int H) {
+    float v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = racer_tanh_select(acc[r]);
+    const int row0 = 64 * wave + ib;
+    if (64 * wave + 64 <= H) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) hs[jc * ldh + row0 + 16 * (r >> 2) + (r & 3)] = v[r];
+    } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = row0 + 16 * (r >> 2) + (r & 3);
+            if (row < H) hs[jc * ldh + row] = v[r];
+        }
+    }
+}
+
+// The workgroup's h rows 0 .. H - 1 of its FT columns from the LDS tile to
+// out [rows][cols] (the activations the backward reads): thread t writes rows
+// t, t + blockDim, ..., each as FT consecutive floats (16-byte stores when
+// aligned), off the products' path at the end of the kernel.
+__device__ __forceinline__ void tile_out(const float* hs, int ldh, float* out, int H, int64_t cols, int64_t c0,
+                                         bool vec) {
+    const int ncol = cols - c0 < FT ? (int)(cols - c0) : FT;
+    for (int row = (int)threadIdx.x; row < H; row += (int)blockDim.x) {
+        float* dst = out + (int64_t)row * cols + c0;
+        if (vec && ncol == FT) {
+#pragma unroll
+            for (int q = 0; q < FT / 4; ++q)
+                *reinterpret_cast<float4*>(dst + 4 * q) = make_float4(
+                    hs[(4 * q) * ldh + row], hs[(4 * q + 1) * ldh + row], hs[(4 * q + 2) * ldh + row],
+                    hs[(4 * q + 3) * ldh + row]);
+        } else {
+            for (int c = 0; c < ncol; ++c) dst[c] = hs[c * ldh + row];
+        }
+    }
+}
+
 // FULL0 / FULL1: whole k quads of the two products when unrolled (0 = loop);
 // NW: wavefronts (64-row groups) of the unrolled shapes, NR0 their staging rounds.
 template <bool SAMPLE, bool VEC, int FULL0, int FULL1, int NW, int NR0>
@@ -297,15 +339,7 @@ __global__ void __launch_bounds__(64 * MAX_RG, 1) k_policy_f32(
             chain<FULL0>(acc, fr, fbase, brow, IN);
         }
     }
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int row = 64 * wave + 16 * (r >> 2) + ib + (r & 3);
-        if (row < H) {
-            const float v = racer_tanh_select(acc[r]);
-            h1s[jc * Ly.ldh + row] = v;
-            if (c0 + jc < cols) h1out[(int64_t)row * cols + c0 + jc] = v;
-        }
-    }
+    tanh_tile(acc, h1s, Ly.ldh, wave, ib, jc, H);
     for (int j = tid; j < FT; j += blockDim.x) h1s[j * Ly.ldh + H] = 1.0f;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -320,15 +354,7 @@ __global__ void __launch_bounds__(64 * MAX_RG, 1) k_policy_f32(
         const uint32_t fbase = (uint32_t)(((wave * Ly.KB1) * 64 + lane) * 16);
         acc = f32x16{};
         chain<FULL1>(acc, fr, fbase, h1s + jc * Ly.ldh, HA);
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int row = 64 * wave + 16 * (r >> 2) + ib + (r & 3);
-            if (row < H) {
-                const float v = racer_tanh_select(acc[r]);
-                h2s[jc * Ly.ldh + row] = v;
-                if (c0 + jc < cols) h2out[(int64_t)row * cols + c0 + jc] = v;
-            }
-        }
+        tanh_tile(acc, h2s, Ly.ldh, wave, ib, jc, H);
     }
     for (int j = tid; j < FT; j += blockDim.x) h2s[j * Ly.ldh + H] = 1.0f;
     __syncthreads();
@@ -353,6 +379,10 @@ __global__ void __launch_bounds__(64 * MAX_RG, 1) k_policy_f32(
         outs[j * FT + c] = a;
         if (c0 + c < cols) o[j * cols + c0 + c] = a;
     }
+    // h1 and h2 stay in LDS to the end (h2 overlays only the input slice)
+    const bool ovec = cols % 4 == 0 && (uintptr_t)h1out % 16 == 0 && (uintptr_t)h2out % 16 == 0;
+    tile_out(h1s, Ly.ldh, h1out, H, cols, c0, ovec);
+    tile_out(h2s, Ly.ldh, h2out, H, cols, c0, ovec);
     if constexpr (SAMPLE) {
         __syncthreads();
         const int64_t col = c0 + tid;
