@@ -628,6 +628,8 @@ def runner_leg(R, dev, arr, args, precision: str, layers: int) -> dict:
         "bound": "hbm", "kernel": sim_kernel, "achieved": lbytes / (sim_us * 1e-6) / 1e9, "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": lbytes / (sim_us * 1e-6) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
         "algorithmic_bytes_per_launch": lbytes, "march_iterations_per_launch": iters,
+        "wavefront_trips_per_launch": float(np.mean(trips)) if trips else None,
+        "lane_utilisation": iters / (64.0 * float(np.mean(trips))) if trips and np.mean(trips) > 0 else None,
         "avg_launch_us": sim_us,
     }
     peak = MFMA_BF16_PEAK_TFLOPS if precision == "bf16" else FP32_PEAK_TFLOPS
