@@ -241,3 +241,32 @@ def test_objects_outlive_the_device_handle():
     del sur, buf
     sim.step()
     del sim
+
+
+@pytest.mark.parametrize("name,beams,waves,coop", [
+    ("ramp10", 2000, 1, True),    # 32 pieces in one share: more than the order covers (natural order)
+    ("ramp10", 1600, 4, True),    # shares of 400 beams, each starting inside a 64-beam chunk
+    ("bridge8", 65, 2, False),    # a one-beam second piece; per-wavefront stores
+    ("ring10", 2, 2, True),       # the fewest beams: one beam a wavefront, 63 spare records each
+    ("bank10", 1081, 4, False),   # shares of 271 beams, 5 pieces each, per-wavefront stores
+])
+def test_lidar_piece_order_edge_shapes(dev, name, beams, waves, coop):
+    """The lidar's piece order (longest first by the previous step's keys,
+    spare records past a piece's end) at share and piece shapes the benchmark
+    never uses: every step bit-identical to the oracle over steps whose keys
+    come from the steps before."""
+    print(f"lidar piece order {name} beams={beams} waves={waves} coop={coop}", flush=True)
+    g, o = _pair(dev, name, 40, 5, beams, 0.1)
+    g.set_waves_per_car(waves)
+    g.set_lidar_coop(coop)
+    g.reset()
+    o.reset()
+    _assert_same(g, o, "reset")
+    rng = np.random.default_rng(beams)
+    for t in range(4):
+        a = rng.uniform(-1, 1, (2, 40)).astype(np.float32)
+        g.actions.write(a)
+        o.actions[...] = a
+        g.step()
+        o.step()
+        _assert_same(g, o, f"step {t}")
