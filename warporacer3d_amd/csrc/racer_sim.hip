@@ -1246,14 +1246,21 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
     // bound is the smaller one on about one iteration in nine), so a
     // shortcut around it only adds its test. The loop is compiled three
     // times, for the reciprocal form of that quotient on the padded grid and
-    // on the packed grid, and for the division (fast_q and pad_q are uniform),
-    // so the choice costs nothing per trip.
+    // on the packed grid, and for the division (fast_q, pad_q and ordered are
+    // uniform; an unordered share, one of more pieces than the order covers,
+    // takes the dividing loop), so the choice costs nothing per trip.
     auto march = [&](auto reciprocal, auto padded) {
         // declared here rather than captured: the compiler then keeps them in
         // scalar registers in every copy of the loop
         int chunks = pieces > 1 ? 2 : 1;  // records below 64 * chunks are set up in the ring
         int next = 64;                    // the next unclaimed record
         int trip = 0;
+        // the claim that needs the next piece set up (none once all are):
+        // one scalar compare a trip
+        int refill_at = chunks < pieces ? 64 * chunks : 0x7fffffff;
+        // the reciprocal loops run for ordered shares only (the caller sends
+        // the rest to the dividing loop), so their key updates need no test
+        const bool record = decltype(reciprocal)::value || ordered;
         while (__ballot(b < records) != 0ull) {
             const bool live = b < records;
             if (COUNT) iterations += (live & (beam != spare)) ? 1 : 0;
@@ -1293,9 +1300,10 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
                 const int count = (int)__popcll(done_mask);
                 // The ring slot reused here held chunk `chunks - 2`, whose records
                 // are all below `next` and already taken.
-                if (next + count > chunks * 64 && chunks < pieces) {
+                if (next + count > refill_at) {
                     setup_chunk(chunks);
                     ++chunks;
+                    refill_at = chunks < pieces ? 64 * chunks : 0x7fffffff;
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1310,7 +1318,7 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
                 if (finished) {
                     ranges[beam] = hit ? racer_min(last + ((t - last) * last_floor) / (last_floor - fl), RACER_RANGE)
                                        : racer_min(t, RACER_RANGE);
-                    if (ordered) __hip_atomic_fetch_max(piece_max + (b >> 6), (uint32_t)(trip - started),
+                    if (record) __hip_atomic_fetch_max(piece_max + (b >> 6), (uint32_t)(trip - started),
                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                     b = claimed;
                     started = trip;
@@ -1320,9 +1328,9 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
             }
         }
     };
-    if (pad_q)
+    if (pad_q && ordered)
         march(std::integral_constant<bool, true>{}, std::integral_constant<bool, true>{});
-    else if (fast_q)
+    else if (fast_q && ordered)
         march(std::integral_constant<bool, true>{}, std::integral_constant<bool, false>{});
     else
         march(std::integral_constant<bool, false>{}, std::integral_constant<bool, false>{});
