@@ -246,7 +246,7 @@ BATCH_CARS = (256, 512, 1024, 2048, 4096, 4096, 8192, 8192)
 STEP_MODES = ("physics and lidar launches", "fused step", "fused step, persistent grid")
 
 
-def batch_leg(R, T, dev, steps: int) -> dict:
+def batch_leg(R, T, dev, steps: int, separate: bool = True) -> dict:
     """Eight uneven instances (flat, ramp, bank, bridge and four random routes;
     256 .. 8192 cars, 1081 beams, one 2 x 256 bf16 policy): Runner steps of all
     of them from one captured graph of batched launches, against the same
@@ -315,6 +315,12 @@ def batch_leg(R, T, dev, steps: int) -> dict:
     dev.synchronize()
     batched_ms = (time.perf_counter() - t0) * 1000.0 / steps
     del graph, batch, sims
+    result = {"instances": [{"track": t, "cars": n} for t, n in zip(BATCH_TRACKS, BATCH_CARS)], "beams": 1081,
+              "cars": total, "steps": steps, "lidar_waves_per_car": wpc, "step_mode": STEP_MODES[mode],
+              "launch_shapes_ms_per_step": modes,
+              "batched_graph": {"ms_per_step": batched_ms, "env_steps_per_s": total * 1000.0 / batched_ms}}
+    if not separate:
+        return result
     # separate: each instance's own graph of Runner steps, replayed in turn
     sims = instances()
     own, t_own = [], 5
@@ -335,12 +341,9 @@ def batch_leg(R, T, dev, steps: int) -> dict:
     dev.synchronize()
     separate_ms = (time.perf_counter() - t0) * 1000.0 / steps
     del own, sims
-    return {"instances": [{"track": t, "cars": n} for t, n in zip(BATCH_TRACKS, BATCH_CARS)], "beams": 1081,
-            "cars": total, "steps": steps, "lidar_waves_per_car": wpc, "step_mode": STEP_MODES[mode],
-            "launch_shapes_ms_per_step": modes,
-            "batched_graph": {"ms_per_step": batched_ms, "env_steps_per_s": total * 1000.0 / batched_ms},
-            "separate_graphs": {"ms_per_step": separate_ms, "env_steps_per_s": total * 1000.0 / separate_ms},
-            "speedup": separate_ms / batched_ms}
+    result["separate_graphs"] = {"ms_per_step": separate_ms, "env_steps_per_s": total * 1000.0 / separate_ms}
+    result["speedup"] = separate_ms / batched_ms
+    return result
 
 
 def surrogate_leg(R, T, dev, transitions: int, epochs: int, cars: int = 16384) -> dict:
@@ -666,6 +669,8 @@ def main() -> int:
     ap.add_argument("--surrogate-epochs", type=int, default=2)
     ap.add_argument("--batch-steps", type=int, default=100,
                     help="timed steps of the eight-instance batched launch (0 skips it)")
+    ap.add_argument("--no-batch-separate", action="store_true",
+                    help="skip the batch leg's comparison with each instance's own graph")
     ap.add_argument("--ppo-iterations", type=int, default=3,
                     help="timed PPO iterations per beam count on the bridge (0 skips the PPO leg)")
     ap.add_argument("--ppo-legs", default="1081,108,108fp32",
@@ -708,7 +713,7 @@ def main() -> int:
         log("surrogate:", surrogate)
     batched = None
     if args.batch_steps > 0:
-        batched = batch_leg(R, T, dev, args.batch_steps)
+        batched = batch_leg(R, T, dev, args.batch_steps, not args.no_batch_separate)
         log("eight-instance batch:", batched)
     ppo = None
     if args.ppo_iterations > 0:
