@@ -49,7 +49,8 @@ def policy_flops(cars: int, beams: int, hidden: int) -> float:
     return 2.0 * cars * (hidden * IN + hidden * HA + 3 * HA)
 
 
-def cpu_baseline(track, beams: int, hidden: int, cars: int, threads: int, steps: int, steps_1core: int) -> dict:
+def cpu_baseline(track, beams: int, hidden: int, cars: int, threads: int, steps: int, steps_1core: int,
+                 track_name: str = "ramp10") -> dict:
     """The oracle's Runner step (float32 policy forward, sample, physics,
     lidar) on the host at the benchmark's own size, timed like the reference's
     `racer benchmark ... cpu` (main.mojo:225-235: a warm-up step, then timed
@@ -84,7 +85,7 @@ def cpu_baseline(track, beams: int, hidden: int, cars: int, threads: int, steps:
             "kind": "port", "value_1core": cars * steps_1core / dt1,
             "ms_per_step": dt * 1000.0 / steps, "ms_per_step_1core": dt1 * 1000.0 / steps_1core,
             "sample": f"the benchmark workload itself ({cars} cars x {beams} beams, 2x{hidden} float32 policy, "
-                      f"ramp10): {steps} Runner steps on {threads} host threads ({dt:.1f} s) and "
+                      f"{track_name}): {steps} Runner steps on {threads} host threads ({dt:.1f} s) and "
                       f"{steps_1core} on 1 ({dt1:.1f} s), after one warm-up step; host has {os.cpu_count()} "
                       f"logical CPUs"}
 
@@ -650,6 +651,68 @@ def runner_leg(R, dev, arr, args, precision: str, layers: int) -> dict:
             "lidar_wavefront_trips": trips}
 
 
+def flat_leg(R, T, dev, args) -> dict:
+    """The reference's plain `racer benchmark` workload (main.mojo:225-235 on
+    sim.mojo's defaults): the flat ribbon (circle of 10 m, 0.025 m grids, one
+    layer), 1024 cars, the default 108 beams, a 2 x 64 float32 tanh policy,
+    1000 steps. Timed three ways on the same box: the library's Runner step
+    (autotuned launch shapes, graph replays), the plain GPU decomposition, and
+    the oracle on the host cores."""
+    from types import SimpleNamespace
+
+    import numpy as np
+
+    arr = T.cached("ring10")
+    cars, beams, hidden, steps = 1024, 108, 64, args.flat_steps
+    cfg = R.Config(beams=beams, hidden=hidden, precision=0, heading_noise=0.1)
+    run = R.Runner(arr, cars, cfg, seed=42, device=dev)
+    for t in range(20):
+        run.step(False, t)
+    dev.synchronize()
+    tune = run.autotune(10)
+    t = 20
+    chunk = math.gcd(steps, 100)
+    graph = run.capture(chunk)
+    graph.launch(t)
+    t += chunk
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps // chunk):
+        graph.launch(t + k * chunk)
+    dev.synchronize()
+    dt = time.perf_counter() - t0
+    t += steps
+    del graph
+    assert np.isfinite(run.sim.obs.read()).all(), "non-finite observations"
+    ms = dt * 1000.0 / steps
+    dev.profile_only(None)
+    dev.profile_reset()
+    dev.profile(True)
+    for k in range(100):
+        run.step(False, t + k)
+    dev.synchronize()
+    dev.profile(False)
+    prof = dev.profile_results()
+    gpu = {"value": cars * 1000.0 / ms, "unit": "env-steps/s", "ms_per_step": ms, "steps": steps,
+           "lidar_waves_per_car": tune["waves_per_car"], "lidar_coop_stores": tune["lidar_coop"],
+           "fused_step": tune["fused_step"], "act_kernel": ACT_NAMES[tune["act_kernel"]],
+           "kernels": {k: {"launches": n, "avg_us": tot / max(n, 1) * 1000.0} for k, (n, tot) in prof.items()}}
+    log("flat benchmark, GPU Runner step:", gpu)
+    plain = plain_baseline(R, dev, arr, SimpleNamespace(cars=cars, beams=beams, hidden=hidden), min(steps, 200))
+    log("flat benchmark, plain GPU decomposition:", plain)
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
+        cpu = cpu_baseline(arr, beams, hidden, cars, threads, args.flat_cpu_steps, args.flat_cpu_steps_1core,
+                           track_name="ring10")
+        log("flat benchmark, oracle on the host:", cpu)
+    return {"workload": f"ring10 flat ribbon R=10 m, 0.025 m grids; {cars} cars x {beams} beams; 2x{hidden} tanh "
+                        "policy, float32 products; heading noise U(-0.1, 0.1)",
+            "gpu": gpu, "plain_gpu": plain, "cpu": cpu,
+            "speedup_vs_plain_gpu": gpu["value"] / plain["value"],
+            "speedup_vs_cpu": gpu["value"] / cpu["value"] if cpu else None}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -664,6 +727,10 @@ def main() -> int:
     ap.add_argument("--cpu-steps-1core", type=int, default=3, help="oracle Runner steps on one thread")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU decomposition baseline")
+    ap.add_argument("--flat-steps", type=int, default=1000,
+                    help="timed steps of the flat 1024-car x 108-beam benchmark workload (0 skips it)")
+    ap.add_argument("--flat-cpu-steps", type=int, default=1000, help="its oracle steps on all host threads")
+    ap.add_argument("--flat-cpu-steps-1core", type=int, default=100, help="its oracle steps on one thread")
     ap.add_argument("--surrogate-transitions", type=int, default=10_000_000,
                     help="transitions recorded for the learned dynamics surrogate (0 skips it)")
     ap.add_argument("--surrogate-epochs", type=int, default=2)
@@ -711,6 +778,9 @@ def main() -> int:
     if args.surrogate_transitions > 0:
         surrogate = surrogate_leg(R, T, dev, args.surrogate_transitions, args.surrogate_epochs)
         log("surrogate:", surrogate)
+    flat = None
+    if args.flat_steps > 0:
+        flat = flat_leg(R, T, dev, args)
     batched = None
     if args.batch_steps > 0:
         batched = batch_leg(R, T, dev, args.batch_steps, not args.no_batch_separate)
@@ -759,7 +829,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "plain_gpu_baseline": plain, "speedup_vs_plain_gpu": value / plain["value"] if plain else None,
         "speedup_bf16_vs_plain_gpu": bf16["value"] / plain["value"] if (plain and bf16) else None,
-        "ppo_iteration": ppo, "batch": batched, "surrogate": surrogate,
+        "ppo_iteration": ppo, "batch": batched, "surrogate": surrogate, "flat_benchmark": flat,
     }
     if parity is not None:
         out["train_parity"] = parity
