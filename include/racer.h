@@ -399,6 +399,12 @@ int64_t racer_sim_last_iterations(racer_sim* sim);
 /* Wavefront march-loop trips of the same counted launch (schedule 0): lane
  * utilisation = iterations / (64 trips). */
 int64_t racer_sim_last_trips(racer_sim* sim);
+/* The lidar's padded probe takes the floor and conversion of its nearest-cell
+ * index as one instruction (v_cvt_flr_i32_f32); this counts the floats x with
+ * |x| < 2^23 (every `stride`-th bit pattern, both signs) where that
+ * instruction and (int)floorf(x) differ, on the device (0 expected). No
+ * reference counterpart. */
+int racer_check_floor_index(racer_device* dev, uint32_t stride, uint64_t* mismatches);
 /* Lidar launch schedule: 0 (default) = one wavefront per car with beam refill;
  * 1 = one thread per (car, beam), beams fastest, the launch shape of the
  * reference's 2-D Device.run (racer/device.mojo:67-79), for timing the plain

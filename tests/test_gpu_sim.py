@@ -221,6 +221,14 @@ def test_lidar_quotient_paths_bit_identical(dev, name):
     assert len(bad) == 0, f"{len(bad)} ranges differ, first (beam, car) {bad[:3].tolist()}"
 
 
+def test_padded_probe_floor_instruction(dev):
+    """The padded lidar probe takes the floor and the integer conversion of
+    its nearest cell as one instruction: for every float x with |x| < 2^23,
+    both signs (2.1 G values), the same integer as (int)floorf(x)."""
+    print("floor-and-convert instruction over |x| < 2^23", flush=True)
+    assert dev.check_floor_index(1) == 0
+
+
 def test_objects_outlive_the_device_handle():
     """Destroying the device handle first (a garbage collector may finalise a
     cycle in any order) leaves the objects made on it working; the device goes

@@ -171,6 +171,7 @@ def lib() -> C.CDLL:
         "racer_profile_only": (i, [vp, C.c_char_p]),
         "racer_sim_schedule": (i, [vp]),
         "racer_sim_last_trips": (i64, [vp]),
+        "racer_check_floor_index": (i, [vp, C.c_uint32, C.POINTER(C.c_uint64)]),
         "racer_graph_runner": (i, [vp, vp, vp, vp, i, i, cp]),
         "racer_graph_rollout": (i, [vp, vp, vp, vp, cp]),
         "racer_graph_launch": (i, [vp, i64]),
@@ -342,6 +343,14 @@ class Device:
 
     def synchronize(self) -> None:
         _check(lib().racer_sync(self.handle))
+
+    def check_floor_index(self, stride: int = 1) -> int:
+        """Floats x with |x| < 2^23 (every stride-th pattern, both signs) where
+        the padded lidar probe's floor-and-convert instruction and
+        (int)floorf(x) differ (racer_check_floor_index)."""
+        bad = C.c_uint64(0)
+        _check(lib().racer_check_floor_index(self.handle, stride, C.byref(bad)))
+        return int(bad.value)
 
     # kernel timing with HIP events on the library's stream
     def profile(self, on: bool = True) -> None:

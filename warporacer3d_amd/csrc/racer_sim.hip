@@ -950,55 +950,34 @@ __device__ __forceinline__ void probe_packed(const racer_dmap& m, __amdgpu_buffe
     }
 }
 
-// Map.probe's (clearance, height) from the packed records with the load
-// always issued: off the map it reads an all -1 record.
-//
-// RECIP: the nearest cell on both axes by racer_nearest_by_reciprocal1, which
-// the map's quotient_ok flag has checked against the division over every cell
-// boundary of the padded index range. Otherwise racer_nearest_index_lim with
-// one branch for the rare division of either axis (a lane whose shortcut holds
-// gets the same value from the division, which is the formula the shortcut
-// reproduces); `live` = false lanes (their results unused) never take it.
-//
-// PAD: the probe lies on the padded grid (the caller has checked its origin),
-// so its record is read there with no on-map test; else a probe off the map
-// reads the all -1 record kept after the packed grid.
-template <int L, bool RECIP, bool PAD>
-__device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_rsrc_t rec, float x, float y, float z,
-                                           float limx, float limy, bool live, float* clearance, float* height) {
-    const float ax = x - m.x0, ay = y - m.y0;
-    float fx, fy;
-    if constexpr (RECIP) {
-        fx = racer_nearest_by_reciprocal1(ax, m.cell, m.inv_cell);
-        fy = racer_nearest_by_reciprocal1(ay, m.cell, m.inv_cell);
-    } else {
-        // limx, limy: racer_nearest_limit of bounds on |x - x0| / cell and |y - y0| / cell
-        const float sx = ax * m.inv_cell + 0.5f, sy = ay * m.inv_cell + 0.5f;
-        fx = floorf(sx);
-        fy = floorf(sy);
-        // non-short-circuit & and |: one exec mask, not a nest of them
-        const bool slow_x = !(racer_abs((sx - fx) - 0.5f) < limx), slow_y = !(racer_abs((sy - fy) - 0.5f) < limy);
-        if (live & (slow_x | slow_y)) {
-            fx = floorf(ax / m.cell + 0.5f);
-            fy = floorf(ay / m.cell + 0.5f);
-        }
-    }
-    uint32_t off;
-    if constexpr (PAD) {
-        // cell fy pnx + fx relative to cell (0, 0), exact in float (the padded
-        // grid has fewer than 2^24 cells) and negative left of or below it;
-        // the byte offset wraps modulo 2^32 to the record's
-        off = (uint32_t)(int)fmaf(fy, (float)m.pnx, fx) * (uint32_t)(8 * L) + m.pad_base;
-    } else {
-        // 0 <= f < n as one unsigned compare of the bit patterns: f is a floor
-        // of a sum with 0.5, never -0.0, so a set sign bit means negative, and
-        // NaN and +inf patterns lie above every finite n
-        const bool on_x = __float_as_uint(fx) < __float_as_uint((float)m.nx);
-        const bool on_y = __float_as_uint(fy) < __float_as_uint((float)m.ny);
-        const bool on = on_x & on_y;
-        // cell = fy nx + fx, exact in float (both below 2^12, the sum below 2^24)
-        off = on ? (uint32_t)fmaf(fy, (float)m.nx, fx) * (uint32_t)(8 * L) : (uint32_t)(m.plane * 8 * L);
-    }
+// (int)floorf(x) in one instruction (v_cvt_flr_i32_f32) where the compiler
+// emits a floor and a conversion; racer_check_floor_index compares the two
+// over every float x with |x| < 2^23 on the device. (The instruction that
+// rounds x + 0.5 down in one step is no substitute for floorf(x + 0.5f): it
+// adds exactly, and the float sum rounds 0.5 - 2^-25 + 0.5 up to 1.)
+__device__ __forceinline__ int floor_index(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// sqrtf for x >= 2^-96 (and for +0, +inf and NaN): the hardware
+// root s is within one ulp of sqrt(x), and of s - ulp, s and s + ulp the
+// correctly rounded root is the one the signs of the residuals x - s' s
+// select (the neighbour test of the correctly rounded float square root,
+// without the rescaling that keeps it exact for subnormal x).
+__device__ __forceinline__ float sqrt_normal(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float lo = __uint_as_float(__float_as_uint(s) - 1u), hi = __uint_as_float(__float_as_uint(s) + 1u);
+    const float r = fmaf(-lo, s, x) <= 0.0f ? lo : s;
+    return fmaf(-hi, s, x) > 0.0f ? hi : r;
+}
+
+// The record at byte `off` of a one- or two-layer grid: the clearance and
+// height of the layer Map.probe chooses for height z (sim.mojo:116-124).
+template <int L>
+__device__ __forceinline__ void probe_record(__amdgpu_buffer_rsrc_t rec, uint32_t off, float z, float* clearance,
+                                             float* height) {
     if constexpr (L == 1) {
         // the stored clearance is -1 unless positive
         float2 r = racer_ld_f32x2(rec, off);
@@ -1017,6 +996,66 @@ __device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_
         *clearance = take1 ? r.z : (take0 ? r.x : -1.0f);
         *height = take1 ? r.w : r.y;
     }
+}
+
+// Map.probe's (clearance, height) from the packed records with the load
+// always issued: off the map it reads an all -1 record.
+//
+// RECIP: the nearest cell on both axes by racer_nearest_by_reciprocal1, which
+// the map's quotient_ok flag has checked against the division over every cell
+// boundary of the padded index range. Otherwise racer_nearest_index_lim with
+// one branch for the rare division of either axis (a lane whose shortcut holds
+// gets the same value from the division, which is the formula the shortcut
+// reproduces); `live` = false lanes (their results unused) never take it.
+//
+// PAD: the probe lies on the padded grid (the caller has checked its origin),
+// so its record is read there with no on-map test; else a probe off the map
+// reads the all -1 record kept after the packed grid.
+template <int L, bool RECIP, bool PAD>
+__device__ __forceinline__ void probe_flat(const racer_dmap& m, __amdgpu_buffer_rsrc_t rec, float x, float y, float z,
+                                           float limx, float limy, bool live, float* clearance, float* height) {
+    const float ax = x - m.x0, ay = y - m.y0;
+    uint32_t off;
+    if constexpr (PAD) {
+        static_assert(RECIP, "the padded grid is read with the reciprocal nearest cell");
+        // racer_nearest_by_reciprocal1 with its floor and the conversion as
+        // one instruction per axis (floor_index: probes on the padded grid
+        // have |q + 0.5| < 2^19); the cell iy pnx + ix relative to cell (0,
+        // 0), negative left of or below it, as one 24-bit multiply-add (the
+        // padded grid has fewer than 2^24 cells and more than 32 rows, so pnx
+        // < 2^19 and |iy pnx| < 2^24); the byte offset wraps modulo 2^32 to
+        // the record's
+        const float q0x = ax * m.inv_cell, q0y = ay * m.inv_cell;
+        const int ix = floor_index(fmaf(fmaf(-q0x, m.cell, ax), m.inv_cell, q0x) + 0.5f);
+        const int iy = floor_index(fmaf(fmaf(-q0y, m.cell, ay), m.inv_cell, q0y) + 0.5f);
+        off = (uint32_t)(__mul24(iy, m.pnx) + ix) * (uint32_t)(8 * L) + m.pad_base;
+    } else {
+        float fx, fy;
+        if constexpr (RECIP) {
+            fx = racer_nearest_by_reciprocal1(ax, m.cell, m.inv_cell);
+            fy = racer_nearest_by_reciprocal1(ay, m.cell, m.inv_cell);
+        } else {
+            // limx, limy: racer_nearest_limit of bounds on |x - x0| / cell and |y - y0| / cell
+            const float sx = ax * m.inv_cell + 0.5f, sy = ay * m.inv_cell + 0.5f;
+            fx = floorf(sx);
+            fy = floorf(sy);
+            // non-short-circuit & and |: one exec mask, not a nest of them
+            const bool slow_x = !(racer_abs((sx - fx) - 0.5f) < limx), slow_y = !(racer_abs((sy - fy) - 0.5f) < limy);
+            if (live & (slow_x | slow_y)) {
+                fx = floorf(ax / m.cell + 0.5f);
+                fy = floorf(ay / m.cell + 0.5f);
+            }
+        }
+        // 0 <= f < n as one unsigned compare of the bit patterns: f is a floor
+        // of a sum with 0.5, never -0.0, so a set sign bit means negative, and
+        // NaN and +inf patterns lie above every finite n
+        const bool on_x = __float_as_uint(fx) < __float_as_uint((float)m.nx);
+        const bool on_y = __float_as_uint(fy) < __float_as_uint((float)m.ny);
+        const bool on = on_x & on_y;
+        // cell = fy nx + fx, exact in float (both below 2^12, the sum below 2^24)
+        off = on ? (uint32_t)fmaf(fy, (float)m.nx, fx) * (uint32_t)(8 * L) : (uint32_t)(m.plane * 8 * L);
+    }
+    probe_record<L>(rec, off, z, clearance, height);
 }
 
 constexpr int LIDAR_RECORD = 5;               // dx, dy, dz, descent, RN(1 / descent)
@@ -1177,8 +1216,12 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
 
     // Set-up of the c-th piece in the order, one beam per lane, into ring half
     // c % 2: field f of record r lives at ring[128 f + r % 128], its beam (relative
-    // to lo, or the spare slot) at ring_beam[r % 128].
-    auto setup_chunk = [&](int c) {
+    // to lo, or the spare slot) at ring_beam[r % 128]. For the reciprocal march
+    // (fast_q) the two square roots are sqrt_normal: with |along| + |across| <=
+    // 2^20 the squared norm lies in [0.99, 2^40 + 1] ((d0, d1) is the unit fan
+    // direction turned by the heading) and the squared horizontal part in
+    // [2^-41, 1], no subnormals there.
+    auto setup_chunk = [&](int c, auto reciprocal) {
         const int piece = ordered ? (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(rank == c)) : c;
         const int k = k0 + piece;
         const int b = k * 64 + lane - lo;  // this lane's beam, relative to lo
@@ -1188,15 +1231,17 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
             float d0 = cs.x * ch + cs.y * (-sh);
             float d1 = cs.x * sh + cs.y * ch;
             float d2 = cs.x * along + cs.y * across;
-            float norm = sqrtf((d0 * d0 + d2 * d2) + (d1 * d1 + 0.0f));
+            const float n2 = (d0 * d0 + d2 * d2) + (d1 * d1 + 0.0f);
+            float norm = decltype(reciprocal)::value ? sqrt_normal(n2) : sqrtf(n2);
             float dx = d0 / norm, dy = d1 / norm, dz = d2 / norm;
-            float horizontal = sqrtf(dx * dx + dy * dy);
+            const float h2 = dx * dx + dy * dy;
+            float horizontal = decltype(reciprocal)::value ? sqrt_normal(h2) : sqrtf(h2);
             float descent = slope15 * horizontal + racer_max(-dz, 0.0f);
             slot[0 * 128] = dx;
             slot[1 * 128] = dy;
             slot[2 * 128] = dz;
             slot[3 * 128] = descent;
-            slot[4 * 128] = fast_q ? 1.0f / descent : 0.0f;
+            slot[4 * 128] = decltype(reciprocal)::value ? 1.0f / descent : 0.0f;
             ring_beam[(c & 1) * 64 + lane] = (uint16_t)b;
         } else {
             // a spare record: its NaN descent starts the march at t = NaN
@@ -1227,15 +1272,9 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
         last = 0.0f;
         last_floor = 0.0f;
     };
-    setup_chunk(0);
-    if (pieces > 1) setup_chunk(1);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     // records: 64 per piece, pieces * 64 in all
     const int records = pieces * 64;
     int b = lane;  // this lane's record
-    take(b);
     uint64_t iterations = 0, trips = 0;
     // The march step below runs on every lane with selects in place of
     // branches (lanes past the share's end compute on their last beam's
@@ -1261,6 +1300,12 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
         // the reciprocal loops run for ordered shares only (the caller sends
         // the rest to the dividing loop), so their key updates need no test
         const bool record = decltype(reciprocal)::value || ordered;
+        setup_chunk(0, reciprocal);
+        if (pieces > 1) setup_chunk(1, reciprocal);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        take(b);
         while (__ballot(b < records) != 0ull) {
             const bool live = b < records;
             if (COUNT) iterations += (live & (beam != spare)) ? 1 : 0;
@@ -1301,7 +1346,7 @@ __device__ __forceinline__ void lidar_wave(const racer_dmap& m, const float* __r
                 // The ring slot reused here held chunk `chunks - 2`, whose records
                 // are all below `next` and already taken.
                 if (next + count > refill_at) {
-                    setup_chunk(chunks);
+                    setup_chunk(chunks, reciprocal);
                     ++chunks;
                     refill_at = chunks < pieces ? 64 * chunks : 0x7fffffff;
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -2146,7 +2191,8 @@ extern "C" int racer_sim_create(racer_device* dev, const float* track, size_t le
         // The padded grid: a lidar probe lies within RACER_RANGE (1 + 2^-20) of
         // its origin, so `pad` cells of RACER_RANGE + 1 m plus 16 leave an
         // origin box reaching 14 cells past the map on every side. Kept only
-        // while its cell index is exact in float (fewer than 2^24 cells).
+        // while its cells number fewer than 2^24 (the probe's index is one
+        // 24-bit multiply-add).
         const double cell = m.cell;
         const double reach = (double)RACER_RANGE + 1.0;
         const double padc = m.quotient_ok ? std::ceil(reach / cell) + 16.0 : 0.0;
@@ -2311,6 +2357,45 @@ extern "C" int racer_sim_count_iterations(racer_sim* sim, int on) {
 }
 extern "C" int64_t racer_sim_last_iterations(racer_sim* sim) { return sim->last_iterations; }
 extern "C" int64_t racer_sim_last_trips(racer_sim* sim) { return sim->last_trips; }
+
+// The floats x with |x| < 2^23 of either sign, every `stride`-th bit pattern
+// of that range: count those where floor_index(x) differs from
+// (int)floorf(x), the index the padded lidar probe takes it for.
+static __global__ void k_check_floor(uint32_t patterns, uint32_t stride, unsigned long long* bad) {
+    uint32_t local = 0;
+    const uint32_t step = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < patterns; i += step) {
+        const uint32_t u = (uint32_t)((uint64_t)i * stride);
+#pragma unroll
+        for (uint32_t sign = 0; sign < 2; ++sign) {
+            const float x = __uint_as_float(u | (sign << 31));
+            local += floor_index(x) != (int)floorf(x) ? 1u : 0u;
+        }
+    }
+    if (local != 0) atomicAdd(bad, (unsigned long long)local);
+}
+
+extern "C" int racer_check_floor_index(racer_device* dev, uint32_t stride, uint64_t* mismatches) {
+    racer_clear_error();
+    RACER_REQUIRE(dev && mismatches && stride >= 1, "racer_check_floor_index: device, output and stride >= 1");
+    RACER_TRY(hipSetDevice(dev->id));
+    // bit patterns below that of 2^23
+    const uint32_t patterns = (uint32_t)((0x4B000000ull + stride - 1) / stride);
+    unsigned long long* bad = nullptr;
+    RACER_TRY(hipMalloc(&bad, sizeof(unsigned long long)));
+    hipError_t e = hipMemsetAsync(bad, 0, sizeof(unsigned long long), dev->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_check_floor, dim3(2048), dim3(256), 0, dev->stream, patterns, stride, bad);
+        e = hipGetLastError();
+    }
+    unsigned long long host = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&host, bad, sizeof(host), hipMemcpyDeviceToHost, dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+    (void)hipFree(bad);
+    RACER_TRY(e);
+    *mismatches = host;
+    return 0;
+}
 
 extern "C" int racer_sim_set_waves_per_car(racer_sim* sim, int waves) {
     racer_clear_error();
