@@ -548,6 +548,13 @@ def runner_leg(R, dev, arr, args, precision: str, layers: int) -> dict:
         run.step(False, t)
     dev.synchronize()
     tune = run.autotune(10)
+    if args.lidar_waves:
+        # a pinned launch shape (profiler passes: the autotuner's timings under a
+        # counter profiler need not pick the timed run's shape)
+        run.sim.set_waves_per_car(args.lidar_waves)
+        run.sim.set_lidar_coop(args.lidar_coop != 0)
+        run.sim.set_fused_step(False)
+        tune = dict(tune, waves_per_car=args.lidar_waves, lidar_coop=args.lidar_coop != 0, fused_step=False)
     t = args.warmup
     log(f"[{precision}] launch shapes (library autotuner): lidar wavefronts per car {tune['waves_per_car']}, "
         f"{'cooperative' if tune['lidar_coop'] else 'per-wavefront'} range stores, "
@@ -727,6 +734,9 @@ def main() -> int:
     ap.add_argument("--cpu-steps-1core", type=int, default=3, help="oracle Runner steps on one thread")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU decomposition baseline")
+    ap.add_argument("--lidar-waves", type=int, default=0,
+                    help="pin the Runner step's lidar to this many wavefronts per car after the autotuner (0: its choice)")
+    ap.add_argument("--lidar-coop", type=int, default=1, help="with --lidar-waves: cooperative range stores (1) or not (0)")
     ap.add_argument("--flat-steps", type=int, default=1000,
                     help="timed steps of the flat 1024-car x 108-beam benchmark workload (0 skips it)")
     ap.add_argument("--flat-cpu-steps", type=int, default=1000, help="its oracle steps on all host threads")
