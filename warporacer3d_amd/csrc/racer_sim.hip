@@ -1691,9 +1691,8 @@ static void launch_lidar(racer_sim* sim, float* obs, int64_t obs_cols) {
     int64_t total = (int64_t)n * wpc;
     // cooperative stores from workgroups of 8 wavefronts (8 / wpc cars, runs of
     // 4 (8 / wpc) bytes) when the cars' ranges fit their LDS; else, or when the
-    // sim asks for it, each wavefront stores its own. A CU never holds two
-    // workgroups of 16 wavefronts at once (measured), so 8 cars at two
-    // wavefronts each would run the launch in two rounds.
+    // sim asks for it, each wavefront stores its own. (Workgroups of 16
+    // wavefronts measured the same, every workgroup being resident either way.)
     const int coop_waves = LIDAR_COOP_WAVES;
     const int coop_cars = coop_waves / wpc;
     const size_t coop_shmem =
