@@ -118,8 +118,11 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const void* __restrict__ Ap, 
 //
 // IO16 (the PPO update's through product): B is already bf16 and C is stored
 // in bf16, rounded as its consumers round it on load; the same bits on half
-// the bytes.
-template <int EPI, bool IO16 = false>
+// the bytes. PAIRS (IO16 with even N and ldb and a 4-byte aligned B): thread t
+// loads columns 2 (t % 32) and 2 (t % 32) + 1 for 8 consecutive k as 4-byte
+// words and splits them into the two columns' 16-byte LDS rows, half the load
+// instructions of one column's 16 k as 2-byte loads; the same LDS contents.
+template <int EPI, bool IO16 = false, bool PAIRS = false>
 __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict__ A, int64_t lda,
                                                       const void* __restrict__ B, int64_t ldb, void* __restrict__ C,
                                                       int64_t ldc, int M, int N, int K, const float* __restrict__ H,
@@ -137,9 +140,12 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
     const int arow = tid >> 2, akc = (tid & 3) * 16;  // A: 16 consecutive k of one row
     const int bcol = tid & 63, bkc = (tid >> 6) * 16;  // B: 16 consecutive k of one column
     constexpr int BSZ = IO16 ? 2 : 4;  // bytes of one B element
+    static_assert(!PAIRS || IO16, "column pairs are 4-byte words of bf16");
+    const int pcol = (tid & 31) * 2, pkc = (tid >> 5) * 8;  // PAIRS: two columns, 8 consecutive k
     uint4 ra[2];
     using BVal = std::conditional_t<IO16, uint16_t, float>;
-    BVal rb[16];
+    BVal rb[PAIRS ? 1 : 16];
+    uint32_t rp[PAIRS ? 8 : 1];
     // buffer loads: out-of-range offsets (rows past M, k past K, columns past N) read zeros
     const auto ares = racer_rsrc(A, (uint64_t)M * lda * 2);
     const auto bres = racer_rsrc(B, ((uint64_t)(K - 1) * ldb + N) * BSZ);
@@ -150,15 +156,26 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
             const int k = k0 + akc + 8 * h;  // rows are zero-padded to lda, a multiple of 8
             ra[h] = racer_ld_b128(ares, (m < M && k < K) ? (uint32_t)(((int64_t)m * lda + k) * 2) : RACER_OOB);
         }
-        const int n = n0 + bcol;
+        if constexpr (PAIRS) {
+            // N even: column n + 1 exists wherever column n does
+            const int n = n0 + pcol;
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const int k = k0 + bkc + e;
-            const uint32_t off = (n < N && k < K) ? (uint32_t)(((int64_t)k * ldb + n) * BSZ) : RACER_OOB;
-            if constexpr (IO16)
-                rb[e] = racer_ld_u16(bres, off);
-            else
-                rb[e] = racer_ld_f32(bres, off);
+            for (int e = 0; e < 8; ++e) {
+                const int k = k0 + pkc + e;
+                const uint32_t off = (n < N && k < K) ? (uint32_t)(((int64_t)k * ldb + n) * 2) : RACER_OOB;
+                rp[e] = __builtin_bit_cast(uint32_t, racer_ld_f32(bres, off));
+            }
+        } else {
+            const int n = n0 + bcol;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int k = k0 + bkc + e;
+                const uint32_t off = (n < N && k < K) ? (uint32_t)(((int64_t)k * ldb + n) * BSZ) : RACER_OOB;
+                if constexpr (IO16)
+                    rb[e] = racer_ld_u16(bres, off);
+                else
+                    rb[e] = racer_ld_f32(bres, off);
+            }
         }
     };
     f32x16 acc;
@@ -179,16 +196,31 @@ __global__ void __launch_bounds__(256) k_gemm_bf16_nn(const uint16_t* __restrict
     for (int k0 = 0; k0 < K; k0 += SK) {
         *reinterpret_cast<uint4*>(&As[arow][akc]) = ra[0];
         *reinterpret_cast<uint4*>(&As[arow][akc + 8]) = ra[1];
-        uint16_t v[16];
+        if constexpr (PAIRS) {
+            // word e holds k = pkc + e of column pcol (low half) and pcol + 1 (high half)
+            uint4 c0, c1;
+            c0.x = (rp[0] & 0xFFFFu) | (rp[1] << 16);
+            c0.y = (rp[2] & 0xFFFFu) | (rp[3] << 16);
+            c0.z = (rp[4] & 0xFFFFu) | (rp[5] << 16);
+            c0.w = (rp[6] & 0xFFFFu) | (rp[7] << 16);
+            c1.x = (rp[0] >> 16) | (rp[1] & 0xFFFF0000u);
+            c1.y = (rp[2] >> 16) | (rp[3] & 0xFFFF0000u);
+            c1.z = (rp[4] >> 16) | (rp[5] & 0xFFFF0000u);
+            c1.w = (rp[6] >> 16) | (rp[7] & 0xFFFF0000u);
+            *reinterpret_cast<uint4*>(&Bs[pcol][pkc]) = c0;
+            *reinterpret_cast<uint4*>(&Bs[pcol + 1][pkc]) = c1;
+        } else {
+            uint16_t v[16];
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            if constexpr (IO16)
-                v[e] = rb[e];
-            else
-                v[e] = to_bf16(rb[e]);
+            for (int e = 0; e < 16; ++e) {
+                if constexpr (IO16)
+                    v[e] = rb[e];
+                else
+                    v[e] = to_bf16(rb[e]);
+            }
+            *reinterpret_cast<uint4*>(&Bs[bcol][bkc]) = *reinterpret_cast<uint4*>(v);
+            *reinterpret_cast<uint4*>(&Bs[bcol][bkc + 8]) = *reinterpret_cast<uint4*>(v + 8);
         }
-        *reinterpret_cast<uint4*>(&Bs[bcol][bkc]) = *reinterpret_cast<uint4*>(v);
-        *reinterpret_cast<uint4*>(&Bs[bcol][bkc + 8]) = *reinterpret_cast<uint4*>(v + 8);
         __syncthreads();
         if (k0 + SK < K) load(k0 + SK);
 #pragma unroll
@@ -420,8 +452,12 @@ int racer_gemm_nn_bf16_mask16(racer_device* dev, const uint16_t* A, int64_t lda,
                   "bf16 product operands exceed the 32-bit offsets of the buffer loads");
     dim3 grid((unsigned)(((N + TN - 1) / TN) * ((M + TM - 1) / TM))), block(256);
     racer_timed timer(dev, "gemm_bf16_mask");
-    hipLaunchKernelGGL((k_gemm_bf16_nn<RACER_EPI_MASK, true>), grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc,
-                       M, N, K, H, ldh);
+    if (N % 2 == 0 && ldb % 2 == 0 && (uintptr_t)B % 4 == 0)
+        hipLaunchKernelGGL((k_gemm_bf16_nn<RACER_EPI_MASK, true, true>), grid, block, 0, dev->stream, A, lda, B, ldb,
+                           C, ldc, M, N, K, H, ldh);
+    else
+        hipLaunchKernelGGL((k_gemm_bf16_nn<RACER_EPI_MASK, true>), grid, block, 0, dev->stream, A, lda, B, ldb, C,
+                           ldc, M, N, K, H, ldh);
     return racer_launch_check("gemm_nn_bf16");
 }
 
