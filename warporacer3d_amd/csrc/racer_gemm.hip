@@ -202,6 +202,78 @@ __global__ void __launch_bounds__(256) k_gemm_nt_f32(const float* __restrict__ A
     }
 }
 
+// The same chunked partial products on gfx950's K = 1 float32 MFMA
+// (v_mfma_f32_16x16x1_4b_f32 forms each output as one correctly rounded fused
+// multiply-add per k, so its chain over a chunk's k in order is the fmaf chain
+// of k_gemm_nt_f32, bit for bit). A workgroup owns a 64 x 64 tile of one
+// chunk's partial, like k_gemm_nt_f32: stages of 32 k of both operands' 64
+// rows go to LDS as [row][k] through 16-byte loads (eight lanes per row's 128
+// bytes), the next stage's loads in flight under this stage's MFMAs.
+// Wavefront w owns rows 16 w .. + 15 of the tile and all 64 columns as the
+// instruction's four 16-column blocks: lane l reads four k of row 16 w + l %
+// 16 and of column l from LDS at once. k past the chunk reads zeros: fma(0, 0,
+// acc) leaves acc as it is, since a chain that starts at +0 never holds -0.
+typedef float nt_f32x16 __attribute__((ext_vector_type(16)));
+constexpr int NT_SK = 32, NT_LD = NT_SK + 4;
+__global__ void __launch_bounds__(256) k_gemm_nt_f32_mfma(const float* __restrict__ A, int64_t lda,
+                                                          const float* __restrict__ B, int64_t ldb,
+                                                          float* __restrict__ out, int64_t out_stride, int64_t ldc,
+                                                          int M, int N, int K, int chunk) {
+    __shared__ __attribute__((aligned(16))) float As[64][NT_LD];
+    __shared__ __attribute__((aligned(16))) float Bs[64][NT_LD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int kbeg = blockIdx.z * chunk, kend = min(K, kbeg + chunk);
+    const auto ares = racer_rsrc(A, ((uint64_t)(M - 1) * lda + K) * 4);
+    const auto bres = racer_rsrc(B, ((uint64_t)(N - 1) * ldb + K) * 4);
+    // thread t stages rows t / 8 and t / 8 + 32, k quad t % 8, of both operands
+    const int srow = tid >> 3, skq = (tid & 7) * 4;
+    float4 ra[2], rb[2];
+    auto load = [&](int k0) {
+        const int k = k0 + skq;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int m = m0 + srow + 32 * h, n = n0 + srow + 32 * h;
+            ra[h] = __builtin_bit_cast(float4, racer_ld_b128(ares, (m < M && k < kend)
+                                                                       ? (uint32_t)(((int64_t)m * lda + k) * 4)
+                                                                       : RACER_OOB));
+            rb[h] = __builtin_bit_cast(float4, racer_ld_b128(bres, (n < N && k < kend)
+                                                                       ? (uint32_t)(((int64_t)n * ldb + k) * 4)
+                                                                       : RACER_OOB));
+        }
+    };
+    nt_f32x16 acc = {};
+    const int ar = 16 * wave + (lane & 15);
+    load(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += NT_SK) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            *reinterpret_cast<float4*>(&As[srow + 32 * h][skq]) = ra[h];
+            *reinterpret_cast<float4*>(&Bs[srow + 32 * h][skq]) = rb[h];
+        }
+        __syncthreads();
+        if (k0 + NT_SK < kend) load(k0 + NT_SK);
+#pragma unroll
+        for (int q = 0; q < NT_SK / 4; ++q) {
+            const float4 a = *reinterpret_cast<const float4*>(&As[ar][4 * q]);
+            const float4 b = *reinterpret_cast<const float4*>(&Bs[lane][4 * q]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.x, b.x, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.y, b.y, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.z, b.z, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.w, b.w, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D of block b = r / 4 (columns 16 b .. + 15): row 4 (lane / 16) + r % 4 of
+    // the wavefront's 16, column lane % 16 of the block's
+    float* dst = out + (int64_t)blockIdx.z * out_stride;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = m0 + 16 * wave + 4 * (lane >> 4) + (r & 3), j = n0 + 16 * (r >> 2) + (lane & 15);
+        if (i < M && j < N) dst[(int64_t)i * ldc + j] = acc[r];
+    }
+}
+
 // C[m, n] = sum over chunks c in order of partial[c][m, n].
 // The same chunked product for at most four rows of A (the output layer's
 // gradient, M = 3): thread (j, n) of a 256-thread workgroup accumulates
@@ -349,6 +421,12 @@ int racer_gemm_nt_f32(racer_device* dev, const float* A, int64_t lda, const floa
         dim3 grid((N + SK_N - 1) / SK_N, 1, chunks), block(256);
         hipLaunchKernelGGL(k_gemm_nt_f32_skinny, grid, block, 0, dev->stream, A, lda, B, ldb, partial, stride,
                            ldc, M, N, K, chunk);
+    } else if (reach && lda % 4 == 0 && ldb % 4 == 0 && K % 4 == 0 && (uintptr_t)A % 16 == 0 &&
+               (uintptr_t)B % 16 == 0) {
+        racer_timed timer(dev, "gemm_nt_f32");
+        dim3 grid((N + 63) / 64, (M + 63) / 64, chunks), block(256);
+        hipLaunchKernelGGL(k_gemm_nt_f32_mfma, grid, block, 0, dev->stream, A, lda, B, ldb, partial, stride, ldc, M,
+                           N, K, chunk);
     } else {
         racer_timed timer(dev, "gemm_nt_f32");
         dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, chunks), block(256);
