@@ -274,6 +274,70 @@ __global__ void __launch_bounds__(256) k_gemm_nt_f32_mfma(const float* __restric
     }
 }
 
+// The backward's masked through products C = (A B) (1 - H^2) on the K = 1
+// float32 MFMA: every output is k_gemm_nn_f32's fmaf chain over k from 0 in
+// order, then the same epilogue, bit for bit. A 64 x 64 tile per workgroup;
+// stages of 32 k of A ([row][k], k contiguous) and of B ([k][column], columns
+// contiguous) go to LDS through 16-byte loads, the next stage's in flight
+// under this stage's MFMAs; wavefront w owns rows 16 w .. + 15 and all 64
+// columns as the instruction's four column blocks, lane l reading four k of
+// row 16 w + l % 16 at once and column l at each k. k past K reads zeros
+// (exact, as in k_gemm_nt_f32_mfma).
+__global__ void __launch_bounds__(256) k_gemm_nn_f32_mask_mfma(const float* __restrict__ A, int64_t lda,
+                                                               const float* __restrict__ B, int64_t ldb,
+                                                               float* __restrict__ C, int64_t ldc, int M, int N,
+                                                               int K, const float* __restrict__ H, int64_t ldh) {
+    __shared__ __attribute__((aligned(16))) float As[64][NT_LD];
+    __shared__ __attribute__((aligned(16))) float Bs[NT_SK][64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const auto ares = racer_rsrc(A, ((uint64_t)(M - 1) * lda + K) * 4);
+    const auto bres = racer_rsrc(B, ((uint64_t)(K - 1) * ldb + N) * 4);
+    // A: rows t / 8 and t / 8 + 32, k quad t % 8; B: k rows t / 16 and t / 16
+    // + 16, column quad t % 16
+    const int arow = tid >> 3, akq = (tid & 7) * 4, bk = tid >> 4, bcq = (tid & 15) * 4;
+    float4 ra[2], rb[2];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int m = m0 + arow + 32 * h, k = k0 + akq;
+            ra[h] = __builtin_bit_cast(float4, racer_ld_b128(ares, (m < M && k < K)
+                                                                       ? (uint32_t)(((int64_t)m * lda + k) * 4)
+                                                                       : RACER_OOB));
+            const int kb = k0 + bk + 16 * h, n = n0 + bcq;
+            rb[h] = __builtin_bit_cast(float4, racer_ld_b128(bres, (kb < K && n < N)
+                                                                       ? (uint32_t)(((int64_t)kb * ldb + n) * 4)
+                                                                       : RACER_OOB));
+        }
+    };
+    nt_f32x16 acc = {};
+    const int ar = 16 * wave + (lane & 15);
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += NT_SK) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            *reinterpret_cast<float4*>(&As[arow + 32 * h][akq]) = ra[h];
+            *reinterpret_cast<float4*>(&Bs[bk + 16 * h][bcq]) = rb[h];
+        }
+        __syncthreads();
+        if (k0 + NT_SK < K) load(k0 + NT_SK);
+#pragma unroll
+        for (int q = 0; q < NT_SK / 4; ++q) {
+            const float4 a = *reinterpret_cast<const float4*>(&As[ar][4 * q]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.x, Bs[4 * q][lane], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.y, Bs[4 * q + 1][lane], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.z, Bs[4 * q + 2][lane], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.w, Bs[4 * q + 3][lane], acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = m0 + 16 * wave + 4 * (lane >> 4) + (r & 3), j = n0 + 16 * (r >> 2) + (lane & 15);
+        if (i < M && j < N) C[(int64_t)i * ldc + j] = epilogue<RACER_EPI_MASK>(acc[r], H, ldh, i, j);
+    }
+}
+
 // C[m, n] = sum over chunks c in order of partial[c][m, n].
 // The same chunked product for at most four rows of A (the output layer's
 // gradient, M = 3): thread (j, n) of a 256-thread workgroup accumulates
@@ -373,6 +437,11 @@ int racer_gemm_nn_f32(racer_device* dev, const float* A, int64_t lda, const floa
     if (epi == RACER_EPI_TANH)
         hipLaunchKernelGGL(k_gemm_nn_f32<RACER_EPI_TANH>, grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc, M, N,
                            K, H, ldh);
+    else if (epi == RACER_EPI_MASK && K % 4 == 0 && N % 4 == 0 && lda % 4 == 0 && ldb % 4 == 0 &&
+             (uintptr_t)A % 16 == 0 && (uintptr_t)B % 16 == 0 && ((uint64_t)(M - 1) * lda + K) * 4 < 0xFFFFFF00ull &&
+             ((uint64_t)(K - 1) * ldb + N) * 4 < 0xFFFFFF00ull)
+        hipLaunchKernelGGL(k_gemm_nn_f32_mask_mfma, dim3((N + 63) / 64, (M + 63) / 64), block, 0, dev->stream, A, lda,
+                           B, ldb, C, ldc, M, N, K, H, ldh);
     else if (epi == RACER_EPI_MASK)
         hipLaunchKernelGGL(k_gemm_nn_f32<RACER_EPI_MASK>, grid, block, 0, dev->stream, A, lda, B, ldb, C, ldc, M, N,
                            K, H, ldh);
